@@ -1,0 +1,662 @@
+"""FP64 numpy restatement of jaxfun's spectral-Galerkin hot path (the oracle).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()``
+and ``bench.py``'s ``cpu_baseline`` leg, never by the product package.
+
+Each function cites the reference file:line (paths relative to the
+reference's ``src/jaxfun``) whose algorithm it restates.  How each part is
+pinned by checks that do not retype the formula (all in ``tests/test_oracle.py``):
+
+* Vandermonde + derivatives (a1/a2): ``numpy.polynomial.{legendre,chebyshev}``
+  ``legvander/chebvander`` + ``legder/chebder`` (what the reference's own
+  ``tests/galerkin/test_approximations.py:48-87`` compares against).
+* Gauss nodes (a3): ``numpy.polynomial.legendre.leggauss``; exactness of the
+  rule on monomials up to degree 2N-1; Chebyshev closed form.
+* Galerkin matrices (a6/a10): the literal shenfun golden diagonals of the
+  reference's ``tests/galerkin/test_inner.py:104-229``; sympy exact integrals;
+  closed forms (Legendre-Dirichlet K = -(4k+6) I).
+* Transforms (a4/a5/a8): forward(backward(c)) == c; exactness on
+  polynomials; Chebyshev backward vs ``chebval``.
+* Banded solve (a7): dense ``numpy.linalg.solve``.
+* Per-axis contraction (a9): ``numpy.einsum``.
+* FD / Kronecker solves (a12/a13): ``scipy.linalg.solve`` on the assembled
+  Kronecker matrix; manufactured solution (1-x^2)(1-y^2).
+* IMEX ARS443 (a14): exact modal decay exp(-nu*lambda*t) of the linear heat
+  equation (third-order convergence), and the tableau's row sums.
+* Variable-coefficient operator / PCG / MLP (config-4 path): dense assembled
+  operator vs the matrix-free one; PCG vs direct solve; MLP gradients vs
+  central finite differences.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import scipy.linalg as sla
+
+# --------------------------------------------------------------------------
+# a3: quadrature
+# --------------------------------------------------------------------------
+
+
+def leggauss(N: int) -> tuple[np.ndarray, np.ndarray]:
+    """Gauss-Legendre nodes (ascending) and weights.
+
+    Restates ``Legendre.quad_points_and_weights`` (galerkin/Legendre.py:126-137)
+    -> ``utils/fastgl.py:561-567`` (nodes ascending, ``GLPair(N, N-i)``).  The
+    reference uses Bogaert's iteration-free formulas; here Newton's method on
+    the three-term recurrence is iterated to convergence in FP64.
+    """
+    if N == 1:
+        return np.zeros(1), np.full(1, 2.0)
+    k = np.arange(N)
+    x = -np.cos(np.pi * (k + 0.75) / (N + 0.5))
+    for _ in range(100):
+        p0 = np.ones_like(x)
+        p1 = x.copy()
+        for n in range(1, N):
+            p0, p1 = p1, ((2 * n + 1) * x * p1 - n * p0) / (n + 1)
+        dp = N * (x * p1 - p0) / (x * x - 1.0)
+        dx = p1 / dp
+        x = x - dx
+        if np.max(np.abs(dx)) < 1e-16:
+            break
+    p0 = np.ones_like(x)
+    p1 = x.copy()
+    for n in range(1, N):
+        p0, p1 = p1, ((2 * n + 1) * x * p1 - n * p0) / (n + 1)
+    omx2 = (1.0 - x) * (1.0 + x)
+    dp = N * (p0 - x * p1) / omx2
+    w = 2.0 / (omx2 * dp * dp)
+    order = np.argsort(x)
+    return x[order], w[order]
+
+
+def chebgauss(N: int) -> tuple[np.ndarray, np.ndarray]:
+    """Gauss-Chebyshev nodes ``-cos((2k+1)pi/2N)`` (ascending), weights pi/N.
+
+    Restates ``Chebyshev.quad_points_and_weights`` (galerkin/Chebyshev.py:149-170).
+    """
+    k = np.arange(N)
+    return np.cos(np.pi + (2 * k + 1) * np.pi / (2 * N)), np.full(N, np.pi / N)
+
+
+def quad(family: str, N: int) -> tuple[np.ndarray, np.ndarray]:
+    return leggauss(N) if family == "legendre" else chebgauss(N)
+
+
+# --------------------------------------------------------------------------
+# a1/a2: Vandermonde matrices and derivative coefficients
+# --------------------------------------------------------------------------
+
+
+def vandermonde(family: str, x, N: int, k: int = 0) -> np.ndarray:
+    """k-th derivative pseudo-Vandermonde ``V[q, n] = d^k/dx^k psi_n(x_q)``.
+
+    Restates ``Legendre.eval_basis_functions`` (galerkin/Legendre.py:163-183),
+    ``Chebyshev.eval_basis_functions`` (galerkin/Chebyshev.py:200-223) and the
+    k-fold derivative of ``OrthogonalSpace.evaluate_basis_derivative``
+    (galerkin/orthogonal.py:168-178), using the differentiated recurrences
+    ``P^(d)_{n+1} = P^(d)_{n-1} + (2n+1) P^(d-1)_n`` (Legendre) and
+    ``T^(d)_{n+1} = 2x T^(d)_n + 2d T^(d-1)_n - T^(d)_{n-1}`` (Chebyshev).
+    """
+    x = np.asarray(x, dtype=np.float64)
+    Q = x.shape[0]
+    V = np.zeros((k + 1, Q, N))
+    V[0, :, 0] = 1.0
+    if N > 1:
+        V[0, :, 1] = x
+    for d in range(1, k + 1):
+        if N > 1:
+            V[d, :, 1] = 1.0 if d == 1 else 0.0
+    if family == "legendre":
+        for n in range(1, N - 1):
+            V[0, :, n + 1] = ((2 * n + 1) * x * V[0, :, n] - n * V[0, :, n - 1]) / (n + 1)
+            for d in range(1, k + 1):
+                V[d, :, n + 1] = V[d, :, n - 1] + (2 * n + 1) * V[d - 1, :, n]
+    elif family == "chebyshev":
+        for n in range(1, N - 1):
+            V[0, :, n + 1] = 2 * x * V[0, :, n] - V[0, :, n - 1]
+            for d in range(1, k + 1):
+                V[d, :, n + 1] = (
+                    2 * x * V[d, :, n] + 2 * d * V[d - 1, :, n] - V[d, :, n - 1]
+                )
+    else:
+        raise ValueError(family)
+    return V[k]
+
+
+def derivative_coeffs(family: str, c: np.ndarray, k: int = 1) -> np.ndarray:
+    """Coefficients of the k-th derivative of a series (same length as c).
+
+    Restates ``Legendre.derivative_coeffs`` (galerkin/Legendre.py:186-216) and
+    ``Chebyshev.derivative_coeffs`` (galerkin/Chebyshev.py:281-315): backward
+    recurrences ``b_n = (2n+1) c_{n+1} + b_{n+2}/(2n+5)*(2n+1)`` (Legendre,
+    written for the raw derivative) and ``b_n = 2(n+1) c_{n+1} + b_{n+2}``,
+    ``b_0 /= 2`` (Chebyshev).
+    """
+    c = np.asarray(c, dtype=np.float64)
+    for _ in range(k):
+        N = c.shape[0]
+        b = np.zeros_like(c)
+        if family == "legendre":
+            # d/dx sum c_n P_n = sum_m b_m P_m,  b_m = (2m+1) sum_{n>m, n-m odd} c_n
+            for m in range(N - 2, -1, -1):
+                b[m] = (2 * m + 1) * (c[m + 1] + (b[m + 2] / (2 * m + 5) if m + 2 < N else 0.0))
+        else:
+            for m in range(N - 2, -1, -1):
+                b[m] = 2 * (m + 1) * c[m + 1] + (b[m + 2] if m + 2 < N else 0.0)
+            b[0] /= 2.0
+        c = b
+    return c
+
+
+def norm_squared(family: str, N: int) -> np.ndarray:
+    """Orthogonal-basis norms ``h_n = (psi_n, psi_n)_w``.
+
+    Legendre ``2/(2n+1)`` (galerkin/Jacobi.py:249-251 with alpha=beta=0);
+    Chebyshev ``pi, pi/2, ...`` (galerkin/Chebyshev.py:331-338).
+    """
+    n = np.arange(N)
+    if family == "legendre":
+        return 2.0 / (2 * n + 1)
+    h = np.full(N, np.pi / 2)
+    h[0] = np.pi
+    return h
+
+
+# --------------------------------------------------------------------------
+# a6: Dirichlet stencil, closed-form matrices
+# --------------------------------------------------------------------------
+
+
+def dirichlet_stencil(N: int, scaling=None) -> np.ndarray:
+    """Stencil ``S`` (N-2, N) of phi_k = (P_k - P_{k+2}) / s_k.
+
+    Restates ``get_stencil_matrix`` LDRD fast case (galerkin/composite.py:786-789)
+    and ``stencil_to_diamatrix`` (galerkin/composite.py:255-269); ``scaling``
+    divides each stencil row (galerkin/composite.py:178-179).
+    """
+    n = N - 2
+    S = np.zeros((n, N))
+    s = np.ones(n) if scaling is None else np.asarray(scaling, dtype=np.float64)
+    S[np.arange(n), np.arange(n)] = 1.0 / s
+    S[np.arange(n), np.arange(n) + 2] = -1.0 / s
+    return S
+
+
+def orth_derivative_matrix(family: str, Nt: int, Nu: int, j: int) -> np.ndarray:
+    """Dense ``(psi_m, psi_n^(j))_w`` for j in (1, 2) in the orthogonal basis.
+
+    Restates ``_dense_derivative_matrix_data`` (galerkin/Legendre.py:20-33,
+    galerkin/Chebyshev.py:22-41).
+    """
+    r = np.arange(Nt)[:, None]
+    c = np.arange(Nu)[None, :]
+    off = c - r
+    mask = (off >= j) & ((off - j) % 2 == 0)
+    if family == "legendre":
+        val = np.full((Nt, Nu), 2.0) if j == 1 else (c * (c + 1) - r * (r + 1)).astype(float)
+    else:
+        val = (np.pi * c + 0 * r).astype(float) if j == 1 else c * (c**2 - r**2) * np.pi / 2
+    return np.where(mask, val, 0.0)
+
+
+def orth_matrix(family: str, N: int, i: int, j: int) -> np.ndarray:
+    """Precomputed orthogonal-basis matrix (psi^(i), psi^(j)) (galerkin/orthogonal.py:479-529)."""
+    if i == 0 and j == 0:
+        return np.diag(norm_squared(family, N))
+    if i == 0:
+        return orth_derivative_matrix(family, N, N, j)
+    if j == 0:
+        return orth_derivative_matrix(family, N, N, i).T
+    raise ValueError("no precomputed matrix")
+
+
+def quad_matrix(family: str, N: int, i: int, j: int, Nq: int | None = None,
+                weight=None) -> np.ndarray:
+    """Quadrature assembly ``z = (P_i^T * w) @ P_j`` in the orthogonal basis.
+
+    Restates the quadrature branch of ``inner_bilinear``
+    (galerkin/inner.py:887-900): test derivative i on rows, trial derivative j
+    on columns; optional pointwise ``weight`` at the quadrature points.
+    """
+    Nq = N if Nq is None else Nq
+    x, w = quad(family, Nq)
+    if weight is not None:
+        w = w * weight(x) if callable(weight) else w * weight
+    Pi = vandermonde(family, x, N, i)
+    Pj = vandermonde(family, x, N, j)
+    return (Pi.T * w[None, :]) @ Pj
+
+
+def galerkin_matrix(family: str, N: int, i: int, j: int, Nq=None, weight=None,
+                    scaling=None) -> np.ndarray:
+    """Dirichlet-composite Galerkin matrix ``S z S^T`` (test i, trial j).
+
+    Restates ``Composite.apply_stencil_galerkin`` (galerkin/composite.py:290-292)
+    applied to the orthogonal matrix (galerkin/inner.py:902-903).
+    """
+    S = dirichlet_stencil(N, scaling)
+    z = quad_matrix(family, N, i, j, Nq, weight) if (Nq or weight is not None) else \
+        orth_matrix(family, N, i, j)
+    return S @ z @ S.T
+
+
+def legendre_dirichlet_stiffness(N: int) -> np.ndarray:
+    """Closed form ``(phi_k, phi_l'') = -(4k+6) delta_kl`` (galerkin/Legendre.py:439-443)."""
+    k = np.arange(N - 2)
+    return np.diag(-(4.0 * k + 6.0))
+
+
+# --------------------------------------------------------------------------
+# a7: banded LU without pivoting (DIA solve)
+# --------------------------------------------------------------------------
+
+
+def banded_lu_solve(A: np.ndarray, b: np.ndarray, p: int, q: int) -> np.ndarray:
+    """Solve ``A x = b`` with a no-pivot banded LU (lower bw p, upper bw q).
+
+    Restates ``DiaMatrix.lu_factor`` no-pivot kernel (la/diamatrix.py:1936-1973)
+    and ``_forward_elimination``/``_backward_substitution``
+    (la/diamatrix.py:2079-2195).  ``b`` may carry trailing batch columns.
+    """
+    n = A.shape[0]
+    LU = np.array(A, dtype=np.float64)
+    for k in range(n - 1):
+        piv = LU[k, k]
+        if piv == 0 or not np.isfinite(piv):
+            raise ValueError("zero pivot")
+        for i in range(k + 1, min(n, k + p + 1)):
+            LU[i, k] /= piv
+            LU[i, k + 1:min(n, k + q + 1)] -= LU[i, k] * LU[k, k + 1:min(n, k + q + 1)]
+    y = np.array(b, dtype=np.float64)
+    for i in range(n):
+        lo = max(0, i - p)
+        if i > lo:
+            y[i] -= LU[i, lo:i] @ y[lo:i]
+    for i in range(n - 1, -1, -1):
+        hi = min(n, i + q + 1)
+        if hi > i + 1:
+            y[i] -= LU[i, i + 1:hi] @ y[i + 1:hi]
+        y[i] /= LU[i, i]
+    return y
+
+
+# --------------------------------------------------------------------------
+# a4/a5/a8/a9: transforms and per-axis contraction
+# --------------------------------------------------------------------------
+
+
+def apply_along(A: np.ndarray, X: np.ndarray, axis: int) -> np.ndarray:
+    """``Y = A`` applied along ``axis`` of X (la/tpmatrix.py:152-156, la/matrix.py:89-118)."""
+    return np.moveaxis(np.tensordot(A, X, axes=([1], [axis])), 0, axis)
+
+
+class Space1D:
+    """A 1D Legendre/Chebyshev space, orthogonal or homogeneous Dirichlet.
+
+    Mirrors ``OrthogonalSpace``/``Composite`` transforms
+    (galerkin/orthogonal.py:214-277, galerkin/composite.py:205-349).
+    """
+
+    def __init__(self, family: str, N: int, dirichlet: bool = False, scaling=None):
+        self.family, self.N, self.dirichlet = family, N, dirichlet
+        self.S = dirichlet_stencil(N, scaling) if dirichlet else np.eye(N)
+        self.dim = self.S.shape[0]
+        self.h = norm_squared(family, N)
+
+    def backward_matrix(self, Nq: int | None = None, k: int = 0) -> np.ndarray:
+        """(Nq, dim): ``u(x_q) = sum_n c_n phi_n^(k)(x_q)`` (orthogonal.py:214-246)."""
+        x, _ = quad(self.family, self.N if Nq is None else Nq)
+        return vandermonde(self.family, x, self.N, k) @ self.S.T
+
+    def scalar_product_matrix(self, Nq: int | None = None) -> np.ndarray:
+        """(dim, Nq): ``(u, phi_n)_w`` by quadrature (orthogonal.py:264-277, composite.py:345-349)."""
+        x, w = quad(self.family, self.N if Nq is None else Nq)
+        return self.S @ (vandermonde(self.family, x, self.N).T * w[None, :])
+
+    def mass(self) -> np.ndarray:
+        """``S diag(h) S^T`` (composite.py:330-337)."""
+        return self.S @ np.diag(self.h) @ self.S.T
+
+    def forward_matrix(self, Nq: int | None = None) -> np.ndarray:
+        """(dim, Nq): orthogonal ``L/h`` (orthogonal.py:256-262) or mass solve (composite.py:339-343)."""
+        L = self.scalar_product_matrix(Nq)
+        if not self.dirichlet:
+            return L / self.h[:, None]
+        return np.linalg.solve(self.mass(), L)
+
+    def forward(self, u: np.ndarray) -> np.ndarray:
+        L = self.scalar_product_matrix(u.shape[0]) @ u
+        if not self.dirichlet:
+            return L / self.h[:, None] if L.ndim > 1 else L / self.h
+        M = self.mass()
+        return banded_lu_solve(M, L, 2, 2)
+
+
+def tp_backward(spaces, c, Nq=None):
+    """TensorProductSpace.backward (galerkin/tensorproductspace.py:330-365)."""
+    for ax, sp in enumerate(spaces):
+        c = apply_along(sp.backward_matrix(None if Nq is None else Nq[ax]), c, ax + c.ndim - len(spaces))
+    return c
+
+
+def tp_forward(spaces, u):
+    """TensorProductSpace.forward (galerkin/tensorproductspace.py:395-417)."""
+    nb = u.ndim - len(spaces)
+    for ax, sp in enumerate(spaces):
+        u = apply_along(sp.forward_matrix(u.shape[nb + ax]), u, nb + ax)
+    return u
+
+
+def tp_scalar_product(spaces, u):
+    """TensorProductSpace.scalar_product (galerkin/tensorproductspace.py:367-393)."""
+    nb = u.ndim - len(spaces)
+    for ax, sp in enumerate(spaces):
+        u = apply_along(sp.scalar_product_matrix(u.shape[nb + ax]), u, nb + ax)
+    return u
+
+
+# --------------------------------------------------------------------------
+# a12/a13: Kronecker and fast-diagonalisation solves
+# --------------------------------------------------------------------------
+
+
+def kron_matrix(terms) -> np.ndarray:
+    """Dense ``sum_k s_k A_k^0 kron A_k^1 [kron A_k^2]`` (la/tpmatrix.py:1357-1402)."""
+    out = None
+    for s, mats in terms:
+        K = mats[0]
+        for m in mats[1:]:
+            K = np.kron(K, m)
+        out = s * K if out is None else out + s * K
+    return out
+
+
+def kron_solve(terms, F: np.ndarray) -> np.ndarray:
+    """Exact solve via the dense Kronecker matrix (la/tpmatrix.py:1052-1086)."""
+    A = kron_matrix(terms)
+    return sla.solve(A, F.ravel()).reshape(F.shape)
+
+
+def fd_factor(A: np.ndarray, B: np.ndarray):
+    """Generalised eigendecomposition ``A V = B V diag(lam)`` for one axis.
+
+    Restates ``tpmats_lu_factor``'s per-axis pair (la/tpmatrix.py:1160-1204):
+    symmetric-definite pairs use ``eigh`` (V^T B V = I, so W = V^T); a
+    non-symmetric pair (Chebyshev-Dirichlet stiffness) uses ``eig`` and
+    ``W = (B V)^{-1}`` so the transform stays exact.  Returns (lam, V, W).
+    """
+    if np.allclose(A, A.T, rtol=1e-13, atol=1e-13 * np.abs(A).max()) and np.allclose(B, B.T):
+        lam, V = sla.eigh(A, B)
+        return lam, V, V.T.copy()
+    lam, V = sla.eig(A, B)
+    if np.max(np.abs(lam.imag)) > 1e-8 * np.max(np.abs(lam.real)):
+        raise ValueError("complex eigenvalues")
+    lam = lam.real
+    V = V.real
+    order = np.argsort(lam)
+    lam, V = lam[order], V[:, order]
+    W = np.linalg.inv(B @ V)
+    return lam, V, W
+
+
+def fd_solve(axes, terms_eig, F: np.ndarray) -> np.ndarray:
+    """``U = V (W F W^T / D) V^T`` (la/tpmatrix.py:545-587).
+
+    ``axes`` is a list of (V, W) per axis; ``terms_eig`` a list of
+    (s_k, [lam_k^axis0, lam_k^axis1, ...]) with 1 standing for the B matrix.
+    """
+    nd = len(axes)
+    Ft = F
+    for i, (V, W) in enumerate(axes):
+        Ft = apply_along(W, Ft, i + F.ndim - nd)
+    D = np.zeros(Ft.shape[F.ndim - nd:])
+    for s, lams in terms_eig:
+        t = np.ones_like(D)
+        for i, lam in enumerate(lams):
+            shape = [1] * nd
+            shape[i] = -1
+            t = t * np.reshape(lam, shape)
+        D = D + s * t
+    U = Ft / D
+    for i, (V, W) in enumerate(axes):
+        U = apply_along(V, U, i + F.ndim - nd)
+    return U
+
+
+def helmholtz_partial_fd(n_modes_N: int, alpha: float, F: np.ndarray) -> np.ndarray:
+    """Legendre-Dirichlet ``(alpha M (x) M - K (x) M - M (x) K) U = F`` by
+    diagonalising axis 0 and banded-solving axis 1 (la/tpmatrix.py:1089-1233
+    for the per-axis eigenpair; la/diamatrix.py:394-504 for the banded solve).
+    """
+    sp = Space1D("legendre", n_modes_N, dirichlet=True)
+    M = sp.mass()
+    K = legendre_dirichlet_stiffness(n_modes_N)
+    lam, V = sla.eigh(K, M)
+    G = V.T @ F
+    U = np.empty_like(G)
+    for i in range(G.shape[0]):
+        Ai = (alpha - lam[i]) * M - K
+        U[i] = banded_lu_solve(Ai, G[i], 2, 2)
+    return V @ U
+
+
+# --------------------------------------------------------------------------
+# a14: IMEX ARS443
+# --------------------------------------------------------------------------
+
+ARS443_EXPLICIT = np.array([
+    [0, 0, 0, 0, 0],
+    [1 / 2, 0, 0, 0, 0],
+    [11 / 18, 1 / 18, 0, 0, 0],
+    [5 / 6, -5 / 6, 1 / 2, 0, 0],
+    [1 / 4, 7 / 4, 3 / 4, -7 / 4, 0],
+])
+ARS443_IMPLICIT = np.array([
+    [0, 0, 0, 0, 0],
+    [0, 1 / 2, 0, 0, 0],
+    [0, 1 / 6, 1 / 2, 0, 0],
+    [0, -1 / 2, 1 / 2, 1 / 2, 0],
+    [0, 3 / 2, -3 / 2, 1 / 2, 1 / 2],
+])
+"""ARS(4,4,3) tableau (integrators/tableau.py:318-349)."""
+
+
+def imex_ars443_step(u, dt, mass_apply, mass_solve, linear_apply, stage_solve,
+                     nonlinear_sp):
+    """One globally stiffly accurate ARS443 step in coefficient space.
+
+    Restates ``IMEXRungeKutta.stage``/``step`` (integrators/imex_rk.py:55-159):
+    rhs_i = M u + dt sum_j (aE_ij N_j + aI_ij L_j); stage 0 (a_00 = 0) is
+    ``M^{-1} rhs``; stages 1..4 solve ``(M - dt a_ii L) U_i = rhs_i``; the last
+    stage is the new state (globally stiffly accurate).
+    """
+    aE, aI = ARS443_EXPLICIT, ARS443_IMPLICIT
+    mu = mass_apply(u)
+    Ns, Ls = [], []
+    U = None
+    for i in range(5):
+        rhs = mu.copy()
+        for j in range(i):
+            if aE[i, j] != 0:
+                rhs = rhs + dt * aE[i, j] * Ns[j]
+            if aI[i, j] != 0:
+                rhs = rhs + dt * aI[i, j] * Ls[j]
+        U = mass_solve(rhs) if aI[i, i] == 0 else stage_solve(rhs)
+        if i < 4:
+            Ns.append(nonlinear_sp(U))
+            Ls.append(linear_apply(U))
+    return U
+
+
+# --------------------------------------------------------------------------
+# Config-3 Burgers pieces (Chebyshev Dirichlet, 3/2 padding)
+# --------------------------------------------------------------------------
+
+
+class Burgers2D:
+    """2D viscous Burgers ``u_t + u (u_x + u_y) = nu Lap u`` on a Chebyshev
+    Dirichlet tensor space, weak form ``(v, u_t) = nu (v, Lap u) - (v, u(u_x+u_y))``.
+
+    Mass/linear operators follow ``BaseIntegrator`` (integrators/base.py:92-113);
+    the nonlinear scalar product is evaluated on a padded mesh and truncated
+    (integrators/base.py:238-248, integrators/nonlinear.py:89-133).
+    """
+
+    def __init__(self, N: int, nu: float, pad: float = 1.5):
+        self.sp = Space1D("chebyshev", N, dirichlet=True)
+        self.N, self.nu = N, nu
+        self.Np = int(pad * N)
+        S = self.sp.S
+        self.M = self.sp.mass()
+        self.K = S @ orth_derivative_matrix("chebyshev", N, N, 2) @ S.T  # (v, u'')
+        self.B0 = self.sp.backward_matrix(self.Np)
+        self.B1 = self.sp.backward_matrix(self.Np, k=1)
+        self.SP = self.sp.scalar_product_matrix(self.Np)
+
+    def mass_apply(self, U):
+        return self.M @ U @ self.M.T
+
+    def linear_apply(self, U):
+        return self.nu * (self.K @ U @ self.M.T + self.M @ U @ self.K.T)
+
+    def nonlinear_sp(self, U):
+        u = self.B0 @ U @ self.B0.T
+        ux = self.B1 @ U @ self.B0.T
+        uy = self.B0 @ U @ self.B1.T
+        return -(self.SP @ (u * (ux + uy)) @ self.SP.T)
+
+    def solver(self, c):
+        """FD solve of ``(M (x) M - c nu (K (x) M + M (x) K)) U = R``."""
+        lam, V, W = fd_factor(self.K, self.M)
+        terms = [(1.0, [np.ones_like(lam), np.ones_like(lam)]),
+                 (-c * self.nu, [lam, np.ones_like(lam)]),
+                 (-c * self.nu, [np.ones_like(lam), lam])]
+        return lambda R: fd_solve([(V, W), (V, W)], terms, R)
+
+
+# --------------------------------------------------------------------------
+# Config-4: variable-coefficient Poisson, PCG, MLP
+# --------------------------------------------------------------------------
+
+
+class VarCoeffPoisson:
+    """``-div(k grad u) = f`` on a Legendre-Dirichlet tensor space, k = exp(g).
+
+    Operator ``A U = sum_d (d_d phi, k d_d phi)`` assembled by quadrature
+    at the Legendre-Gauss points; the same contraction as ``assemble_multivar``'s
+    ``einsum('pi,pk,qj,ql,pq->ikjl')`` (galerkin/inner.py:991-1024) applied
+    matrix-free as ``TensorMatrix`` does (la/tensormatrix.py:54-56).
+    """
+
+    def __init__(self, N: int, Nq: int | None = None):
+        self.sp = Space1D("legendre", N, dirichlet=True)
+        self.N = N
+        self.Nq = N if Nq is None else Nq
+        x, w = leggauss(self.Nq)
+        self.x, self.w = x, w
+        self.B = self.sp.backward_matrix(self.Nq)        # (Q, n)
+        self.D = self.sp.backward_matrix(self.Nq, k=1)   # (Q, n)
+        self.Bw = self.B * w[:, None]
+        self.Dw = self.D * w[:, None]
+
+    def kfield(self, ghat: np.ndarray) -> np.ndarray:
+        """k = exp(g) at the quadrature mesh from low-mode Legendre coefficients."""
+        m = ghat.shape[0]
+        V = vandermonde("legendre", self.x, m)
+        return np.exp(V @ ghat @ V.T)
+
+    def apply(self, kq: np.ndarray, U: np.ndarray) -> np.ndarray:
+        ux = self.D @ U @ self.B.T
+        uy = self.B @ U @ self.D.T
+        return self.Dw.T @ (kq * ux) @ self.Bw + self.Bw.T @ (kq * uy) @ self.Dw
+
+    def dense(self, kq: np.ndarray) -> np.ndarray:
+        n = self.B.shape[1]
+        A = np.zeros((n * n, n * n))
+        E = np.zeros((n, n))
+        for j in range(n * n):
+            E.flat[j] = 1.0
+            A[:, j] = self.apply(kq, E).ravel()
+            E.flat[j] = 0.0
+        return A
+
+    def rhs(self, fhat: np.ndarray) -> np.ndarray:
+        """``b = (phi_k phi_l, f)`` for orthogonal-Legendre forcing coefficients."""
+        S = self.sp.S
+        h = self.sp.h
+        return S @ (h[:, None] * fhat * h[None, :]) @ S.T
+
+
+def pcg(apply, b, precond=None, x0=None, rtol=1e-10, maxiter=5000):
+    """Preconditioned conjugate gradients; stops on ||r|| <= rtol ||b||.
+
+    Returns (x, iterations, final relative residual).
+    """
+    x = np.zeros_like(b) if x0 is None else x0.copy()
+    r = b - apply(x)
+    nb = np.linalg.norm(b)
+    if np.linalg.norm(r) <= rtol * nb:
+        return x, 0, np.linalg.norm(r) / nb
+    z = r if precond is None else precond(r)
+    p = z.copy()
+    rz = np.vdot(r, z)
+    for it in range(1, maxiter + 1):
+        Ap = apply(p)
+        a = rz / np.vdot(p, Ap)
+        x = x + a * p
+        r = r - a * Ap
+        rn = np.linalg.norm(r)
+        if rn <= rtol * nb:
+            return x, it, rn / nb
+        z = r if precond is None else precond(r)
+        rz_new = np.vdot(r, z)
+        p = z + (rz_new / rz) * p
+        rz = rz_new
+    return x, maxiter, np.linalg.norm(r) / nb
+
+
+def mlp_forward(params, X):
+    """tanh MLP: ``h_{l+1} = tanh(h_l W_l + b_l)``, linear last layer.
+
+    Returns the output and the cache of layer inputs/pre-activations.
+    """
+    hs = [X]
+    h = X
+    L = len(params)
+    for li, (W, b) in enumerate(params):
+        z = h @ W + b
+        h = z if li == L - 1 else np.tanh(z)
+        hs.append(h)
+    return h, hs
+
+
+def mlp_backward(params, hs, dY):
+    """Gradients of ``sum(dY * Y)`` w.r.t. every (W, b); returns list of (dW, db)."""
+    grads = [None] * len(params)
+    g = dY
+    L = len(params)
+    for li in range(L - 1, -1, -1):
+        W, b = params[li]
+        if li != L - 1:
+            g = g * (1.0 - hs[li + 1] ** 2)
+        grads[li] = (hs[li].T @ g, g.sum(axis=0))
+        g = g @ W.T
+    return grads
+
+
+def ars443_stage_coeffs():
+    """Distinct implicit diagonal coefficients of ARS443 (integrators/imex_rk.py:39-48)."""
+    return sorted({float(ARS443_IMPLICIT[i, i]) for i in range(5) if ARS443_IMPLICIT[i, i] != 0})
+
+
+def rel_l2(a, b) -> float:
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+__all__ = [name for name in dir() if not name.startswith("_")] + ["math"]

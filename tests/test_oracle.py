@@ -1,0 +1,297 @@
+"""Pins for the CPU oracle: every check here is independent of the oracle's own
+formulas (numpy.polynomial, sympy integrals, the reference's golden vectors,
+brute-force dense solves, exact modal decay)."""
+
+import json
+import os
+
+import numpy as np
+import numpy.polynomial.chebyshev as C
+import numpy.polynomial.legendre as L
+import pytest
+import scipy.linalg as sla
+import sympy as sp
+
+from oracle import spectral as O
+
+HERE = os.path.dirname(__file__)
+
+
+FNS = {"legendre": (L.legval, L.legder), "chebyshev": (C.chebval, C.chebder)}
+
+
+@pytest.mark.parametrize("family", ["legendre", "chebyshev"])
+@pytest.mark.parametrize("k", [0, 1, 2, 3])
+def test_vandermonde_vs_numpy(family, k):
+    # reference tests/galerkin/test_approximations.py:48-87 compares against numpy
+    N = 17
+    x = np.linspace(-1, 1, 29)
+    V = O.vandermonde(family, x, N, k)
+    val, der = FNS[family]
+    ref = np.stack([val(x, der(np.eye(N)[n], k)) for n in range(N)], axis=1)
+    assert np.allclose(V, ref, rtol=1e-12, atol=1e-10 * max(1.0, np.abs(ref).max()))
+
+
+@pytest.mark.parametrize("N", [1, 2, 5, 11, 64, 100, 128, 201, 256])
+def test_leggauss_vs_numpy(N):
+    x, w = O.leggauss(N)
+    xr, wr = L.leggauss(N)
+    assert np.allclose(x, xr, atol=2e-15)
+    # numpy's weights lose ~1e-11 near the endpoints; mpmath pins them below
+    assert np.allclose(w, wr, rtol=1e-10, atol=1e-16)
+    # exact for polynomials of degree <= 2N-1
+    for m in range(0, 2 * N, max(1, N // 3)):
+        exact = 0.0 if m % 2 else 2.0 / (m + 1)
+        assert abs(np.sum(w * x**m) - exact) < 1e-13
+
+
+def test_leggauss_vs_mpmath():
+    mpmath = pytest.importorskip("mpmath")
+    mpmath.mp.dps = 40
+    for N in (64, 128):
+        x, w = O.leggauss(N)
+        for i in (0, 3, N // 2):
+            xi = mpmath.findroot(lambda t: mpmath.legendre(N, t), x[i])
+            wi = 2 / ((1 - xi**2) * mpmath.diff(lambda t: mpmath.legendre(N, t), xi) ** 2)
+            assert abs(x[i] - float(xi)) < 1e-16
+            assert abs(w[i] - float(wi)) < 5e-13 * float(wi)
+
+
+@pytest.mark.parametrize("N", [4, 11, 64, 192])
+def test_chebgauss_exactness(N):
+    x, w = O.chebgauss(N)
+    assert np.all(np.diff(x) > 0)
+    T = C.chebvander(x, N - 1)
+    G = (T.T * w) @ T
+    h = np.full(N, np.pi / 2)
+    h[0] = np.pi
+    assert np.allclose(G, np.diag(h), atol=1e-12)
+
+
+@pytest.mark.parametrize("family", ["legendre", "chebyshev"])
+@pytest.mark.parametrize("k", [1, 2])
+def test_derivative_coeffs(family, k):
+    c = np.random.default_rng(0).standard_normal(13)
+    ref = FNS[family][1](c, k)
+    got = O.derivative_coeffs(family, c, k)
+    assert np.allclose(got[: len(ref)], ref, rtol=1e-12, atol=1e-10)
+    assert np.allclose(got[len(ref):], 0)
+
+
+def test_golden_shenfun_diagonals():
+    g = json.load(open(os.path.join(HERE, "golden", "shenfun_inner_N11.json")))
+    N = 11
+    for family in ("legendre", "chebyshev"):
+        # answer1: off-diagonals of (x u, v) in the orthogonal basis
+        M = O.quad_matrix(family, N, 0, 0, weight=lambda x: x)
+        a1 = np.array(g["answer1_offdiag_x_u_v"][family])
+        assert np.allclose(np.diag(M, 1), a1) and np.allclose(np.diag(M, -1), a1)
+        # answer2: (x u + u', v) on the Dirichlet composite basis
+        S = O.dirichlet_stencil(N)
+        Z = S @ (O.quad_matrix(family, N, 0, 0, weight=lambda x: x)
+                 + O.orth_derivative_matrix(family, N, N, 1)) @ S.T
+        for d, vals in g["answer2_composite_x_u_v_plus_du_v"][family].items():
+            assert np.allclose(np.diag(Z, int(d)), vals, atol=1e-12), (family, d)
+
+
+def test_sympy_exact_legendre_dirichlet_mass_stiffness():
+    N = 7
+    x = sp.Symbol("x")
+    phi = [sp.legendre(k, x) - sp.legendre(k + 2, x) for k in range(N - 2)]
+    Mx = np.array([[float(sp.integrate(phi[i] * phi[j], (x, -1, 1))) for j in range(N - 2)] for i in range(N - 2)])
+    Kx = np.array([[float(sp.integrate(phi[i] * sp.diff(phi[j], x, 2), (x, -1, 1))) for j in range(N - 2)]
+                   for i in range(N - 2)])
+    sp1 = O.Space1D("legendre", N, dirichlet=True)
+    assert np.allclose(sp1.mass(), Mx, atol=1e-14)
+    assert np.allclose(O.galerkin_matrix("legendre", N, 0, 2), Kx, atol=1e-13)
+    assert np.allclose(O.legendre_dirichlet_stiffness(N), Kx, atol=1e-13)
+    # quadrature assembly agrees with the precomputed closed form
+    assert np.allclose(O.galerkin_matrix("legendre", N, 0, 2, Nq=N), Kx, atol=1e-12)
+
+
+def test_sympy_exact_chebyshev_dirichlet_stiffness():
+    N = 7
+    x = sp.Symbol("x")
+    w = 1 / sp.sqrt(1 - x**2)
+    phi = [sp.chebyshevt(k, x) - sp.chebyshevt(k + 2, x) for k in range(N - 2)]
+    # substitute x = cos(t) for the weighted integral
+    t = sp.Symbol("t")
+    def wint(f):
+        return float(sp.integrate(sp.expand(f.subs(x, sp.cos(t))), (t, 0, sp.pi)))
+    Kx = np.array([[wint(phi[i] * sp.diff(phi[j], x, 2)) for j in range(N - 2)] for i in range(N - 2)])
+    assert np.allclose(O.galerkin_matrix("chebyshev", N, 0, 2), Kx, atol=1e-12)
+    assert np.allclose(O.galerkin_matrix("chebyshev", N, 0, 2, Nq=N), Kx, atol=1e-11)
+    assert np.allclose(np.tril(Kx, -1), 0, atol=1e-12)  # upper-triangular, not symmetric
+    del w
+
+
+def test_banded_lu_vs_dense():
+    rng = np.random.default_rng(3)
+    n = 40
+    A = np.zeros((n, n))
+    for d in range(-2, 3):
+        A += np.diag(rng.standard_normal(n - abs(d)), d)
+    A += 8 * np.eye(n)
+    b = rng.standard_normal((n, 3))
+    assert np.allclose(O.banded_lu_solve(A, b, 2, 2), np.linalg.solve(A, b), rtol=1e-12, atol=1e-12)
+    A0 = A.copy()
+    A0[0, 0] = 0.0
+    with pytest.raises(ValueError):
+        O.banded_lu_solve(A0, b, 2, 2)
+
+
+def test_apply_along_vs_einsum():
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((3, 5, 6, 7))
+    A = rng.standard_normal((4, 6))
+    assert np.allclose(O.apply_along(A, X, 2), np.einsum("mk,bikj->bimj", A, X))
+
+
+@pytest.mark.parametrize("family", ["legendre", "chebyshev"])
+@pytest.mark.parametrize("dirichlet", [False, True])
+def test_forward_backward_identity(family, dirichlet):
+    # reference tests/galerkin/test_forward_backward.py:31-104 (identity to ulp)
+    N = 16
+    sp1 = O.Space1D(family, N, dirichlet)
+    c = np.random.default_rng(5).standard_normal((sp1.dim, sp1.dim, sp1.dim))
+    u = O.tp_backward([sp1] * 3, c)
+    assert np.allclose(O.tp_forward([sp1] * 3, u), c, atol=1e-12)
+    # padded backward then truncating forward (3/2 rule) is still the identity
+    up = O.tp_backward([sp1] * 2, c[0], Nq=(24, 24))
+    assert np.allclose(O.tp_forward([sp1] * 2, up), c[0], atol=1e-12)
+    if family == "chebyshev" and not dirichlet:
+        x, _ = O.chebgauss(N)
+        assert np.allclose(sp1.backward_matrix() @ c[0, 0], C.chebval(x, c[0, 0]), atol=1e-12)
+
+
+def _poisson_terms(family, N):
+    sp1 = O.Space1D(family, N, dirichlet=True)
+    M = sp1.mass()
+    K = O.galerkin_matrix(family, N, 0, 2)
+    return sp1, M, K
+
+
+@pytest.mark.parametrize("family", ["legendre", "chebyshev"])
+def test_fd_vs_kron_random_rhs(family):
+    N = 16
+    sp1, M, K = _poisson_terms(family, N)
+    F = np.random.default_rng(6).standard_normal((N - 2, N - 2))
+    terms = [(1.0, [K, M]), (1.0, [M, K])]
+    Uk = O.kron_solve(terms, F)
+    lam, V, W = O.fd_factor(K, M)
+    one = np.ones_like(lam)
+    Uf = O.fd_solve([(V, W), (V, W)], [(1.0, [lam, one]), (1.0, [one, lam])], F)
+    assert O.rel_l2(Uf, Uk) < 1e-12
+    if family == "chebyshev":
+        # the symmetric-eigh FD (what the reference does) is wrong for this
+        # non-symmetric pair on a general right-hand side
+        lam2, V2 = sla.eigh(K, M)
+        one2 = np.ones_like(lam2)
+        Ue = O.fd_solve([(V2, V2.T), (V2, V2.T)], [(1.0, [lam2, one2]), (1.0, [one2, lam2])], F)
+        assert O.rel_l2(Ue, Uk) > 1e-3
+
+
+@pytest.mark.parametrize("family", ["legendre", "chebyshev"])
+def test_poisson_manufactured(family):
+    # examples/poisson2D.py:19-43: ue = (1-x^2)(1-y^2), -Lap ue = 2(1-y^2) + 2(1-x^2)
+    N = 12
+    sp1, M, K = _poisson_terms(family, N)
+    x, _ = O.quad(family, N)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    f = 2 * (1 - Y**2) + 2 * (1 - X**2)
+    b = O.tp_scalar_product([sp1, sp1], f)
+    # (v, Lap u) = -(v, f)  =>  (K (x) M + M (x) K) U = -b
+    U = O.kron_solve([(1.0, [K, M]), (1.0, [M, K])], -b)
+    u = O.tp_backward([sp1, sp1], U)
+    assert np.allclose(u, (1 - X**2) * (1 - Y**2), atol=1e-12)
+
+
+def test_helmholtz_partial_fd_vs_kron():
+    N = 20
+    sp1, M, K = _poisson_terms("legendre", N)
+    F = np.random.default_rng(7).standard_normal((N - 2, N - 2))
+    for alpha in (1e-2, 3.0, 1e4):
+        Uk = O.kron_solve([(alpha, [M, M]), (-1.0, [K, M]), (-1.0, [M, K])], F)
+        assert O.rel_l2(O.helmholtz_partial_fd(N, alpha, F), Uk) < 1e-12
+
+
+def test_ars443_tableau_consistency():
+    aE, aI = O.ARS443_EXPLICIT, O.ARS443_IMPLICIT
+    c = np.array([0, 1 / 2, 2 / 3, 1 / 2, 1])
+    assert np.allclose(aE.sum(1), c) and np.allclose(aI.sum(1), c)
+    assert O.ars443_stage_coeffs() == [0.5]
+
+
+def test_imex_ars443_linear_modal_decay_third_order():
+    N, nu = 12, 0.1
+    bg = O.Burgers2D(N, nu)
+    lam, V, W = O.fd_factor(bg.K, bg.M)
+    # eigenmode U0 = v_i (x) v_j decays as exp(nu (lam_i + lam_j) t)
+    order = np.argsort(np.abs(lam))
+    i, j = order[0], order[1]
+    U0 = np.outer(V[:, i], V[:, j])
+    T = 1.0
+    errs = []
+    for nsteps in (10, 20, 40):
+        dt = T / nsteps
+        solve = bg.solver(0.5 * dt)
+        U = U0.copy()
+        for _ in range(nsteps):
+            U = O.imex_ars443_step(U, dt, bg.mass_apply,
+                                   lambda R: O.kron_solve([(1.0, [bg.M, bg.M])], R),
+                                   bg.linear_apply, solve, lambda U: np.zeros_like(U))
+        errs.append(O.rel_l2(U, np.exp(nu * (lam[i] + lam[j]) * T) * U0))
+    assert errs[0] / errs[1] > 6 and errs[1] / errs[2] > 6, errs
+
+
+def test_varcoeff_operator_vs_dense_einsum():
+    N = 8
+    vc = O.VarCoeffPoisson(N)
+    rng = np.random.default_rng(8)
+    ghat = rng.standard_normal((3, 3)) * 0.3
+    kq = vc.kfield(ghat)
+    # assemble_multivar einsum('pi,pk,qj,ql,pq->ikjl') (galerkin/inner.py:1016-1023)
+    P0, P1 = vc.Dw, vc.D
+    P2, P3 = vc.Bw, vc.B
+    A1 = np.einsum("pi,pk,qj,ql,pq->ikjl", P0, P1, P2, P3, kq)
+    A2 = np.einsum("pi,pk,qj,ql,pq->ikjl", vc.Bw, vc.B, vc.Dw, vc.D, kq)
+    A = A1 + A2
+    U = rng.standard_normal((N - 2, N - 2))
+    assert np.allclose(np.einsum("ikjl,kl->ij", A, U), vc.apply(kq, U), atol=1e-12)
+    # k = 1 reduces to -(K (x) M + M (x) K)
+    M = O.Space1D("legendre", N, True).mass()
+    K = O.legendre_dirichlet_stiffness(N)
+    assert np.allclose(vc.apply(np.ones_like(kq), U), -(K @ U @ M + M @ U @ K), atol=1e-11)
+
+
+def test_pcg_vs_direct():
+    N = 10
+    vc = O.VarCoeffPoisson(N)
+    rng = np.random.default_rng(9)
+    kq = vc.kfield(rng.standard_normal((4, 4)) * 0.5)
+    b = rng.standard_normal((N - 2, N - 2))
+    x, it, res = O.pcg(lambda U: vc.apply(kq, U), b, rtol=1e-12)
+    xd = np.linalg.solve(vc.dense(kq), b.ravel()).reshape(b.shape)
+    assert res <= 1e-12 and O.rel_l2(x, xd) < 1e-9 and it > 0
+
+
+def test_mlp_gradients_finite_difference():
+    rng = np.random.default_rng(10)
+    sizes = [6, 5, 5, 4]
+    params = [(rng.standard_normal((a, b)) * 0.5, rng.standard_normal(b) * 0.1)
+              for a, b in zip(sizes[:-1], sizes[1:])]
+    X = rng.standard_normal((3, 6))
+    dY = rng.standard_normal((3, 4))
+    Y, hs = O.mlp_forward(params, X)
+    g = O.mlp_backward(params, hs, dY)
+    eps = 1e-6
+    for li in range(len(params)):
+        W, b = params[li]
+        for (idx, arr, gd) in [((1, 2), W, g[li][0]), ((0,), b, g[li][1])]:
+            old = arr[idx]
+            arr[idx] = old + eps
+            yp = np.sum(dY * O.mlp_forward(params, X)[0])
+            arr[idx] = old - eps
+            ym = np.sum(dY * O.mlp_forward(params, X)[0])
+            arr[idx] = old
+            assert abs((yp - ym) / (2 * eps) - gd[idx]) < 1e-6
