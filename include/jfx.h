@@ -1,0 +1,124 @@
+/*
+ * jfx.h -- C ABI of libjfx.so, the MI355X (gfx950) kernels behind the
+ * spectral-Galerkin hot path of jaxfun_amd.
+ *
+ * Conventions
+ *   - All pointers are device pointers unless a comment says "host".
+ *   - Arrays are C-order.  A "batch along an axis" is viewed as (P, n, Q):
+ *     P = product of the leading dims, Q = product of the trailing dims.
+ *   - Every entry point is stream-ordered on `stream` (a hipStream_t; NULL =
+ *     the legacy default stream) and returns 0 or a negative status:
+ *       -1 HIP error, -2 bad/unsupported sizes, -3 singular pivot,
+ *       -4 bad argument.  jfx_last_error() returns the message (per thread).
+ *   - No entry point allocates device memory unless its comment says so.
+ *
+ * Each declaration cites the reference interface (paths relative to the
+ * reference's src/jaxfun) that it replaces on the device.
+ */
+#ifndef JFX_H
+#define JFX_H
+
+#include <hip/hip_runtime.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int jfx_version(void);
+const char* jfx_last_error(void);
+int jfx_device_sync(void);
+
+/* ---- a3: quadrature.  family 0 = Legendre, 1 = Chebyshev.  Nodes ascending.
+ * Replaces Legendre.quad_points_and_weights (galerkin/Legendre.py:126-137 ->
+ * utils/fastgl.py:561-567) and Chebyshev.quad_points_and_weights
+ * (galerkin/Chebyshev.py:149-170). */
+int jfx_gauss_nodes(int family, int N, double* x, double* w, hipStream_t stream);
+
+/* ---- a1/a2: V[q, n] = scale * d^k/dx^k psi_n(x[q]), V is (Q, N), k <= 4.
+ * Replaces OrthogonalSpace.vandermonde / evaluate_basis_derivative
+ * (galerkin/orthogonal.py:131-141, 168-178) over eval_basis_functions
+ * (galerkin/Legendre.py:163-183, galerkin/Chebyshev.py:200-223). */
+int jfx_vandermonde_f64(int family, const double* x, int Q, int N, int k, double scale, double* V,
+                        hipStream_t stream);
+
+/* out[c, r] = V[r, c] * rowscale[r] * colscale[c] * s for an (R, C) matrix
+ * (either scale pointer may be NULL).  Builds the quadrature-weighted
+ * transposed Vandermonde (P^T * w) of OrthogonalSpace.scalar_product
+ * (galerkin/orthogonal.py:264-277) and of inner_bilinear's quadrature branch
+ * (galerkin/inner.py:887-900). */
+int jfx_scale_transpose(const double* V, int R, int C, const double* rowscale, const double* colscale,
+                        double s, double* out, hipStream_t stream);
+
+/* ---- a9: per-axis contraction Y[p, m, q] = sum_k A[m, k] X[p, k, q];
+ * A is (M, K) row-major, X is (P, K, Q), Y is (P, M, Q).
+ * Replaces DiaMatrix/Matrix.matvec(axis=i) (la/diamatrix.py:227-349,
+ * la/matrix.py:89-118) as used by TPMatrix._matmul_array
+ * (la/tpmatrix.py:152-156) and by TensorProductSpace.forward/backward/
+ * scalar_product (galerkin/tensorproductspace.py:330-417). */
+int jfx_contract_f64(const double* A, int M, int K, const double* X, double* Y, long long P,
+                     long long Q, hipStream_t stream);
+int jfx_contract_f32(const float* A, int M, int K, const float* X, float* Y, long long P,
+                     long long Q, hipStream_t stream);
+
+/* Split an FP64 (M, K) matrix into `nslices` bf16 slices stored [s][M][K].
+ * mode 0: float split of fp32(A) (for jfx_contract_bf16x);
+ * mode 1: fixed-point digits with power-of-two row scales rowscale[M]
+ *         (for jfx_contract_ozaki).  Same reference interface as above. */
+int jfx_split_matrix(const double* A, int M, int K, int mode, int nslices, unsigned short* slices,
+                     double* rowscale, hipStream_t stream);
+
+/* BF16 float-split contraction on FP32 data: nslices 2 -> 3 products,
+ * 3 -> 6 products, FP32 accumulation.  Same reference interface as above. */
+int jfx_contract_bf16x(const unsigned short* Aslices, int nslices, int M, int K, const float* X,
+                       float* Y, long long P, long long Q, hipStream_t stream);
+
+/* Fixed-point-slice contraction on FP64 data: nslices in {3, 4, 5} 8-bit
+ * digit slices per operand, bf16 MFMA with exact FP32 accumulation
+ * (requires nslices * K <= 1024), levels combined in FP64.
+ * Same reference interface as above. */
+int jfx_contract_ozaki(const unsigned short* Aslices, const double* Arowscale, int nslices, int M,
+                       int K, const double* X, double* Y, long long P, long long Q,
+                       hipStream_t stream);
+
+/* ---- a7: banded LU without pivoting.  band is (n, p+q+1) "by row":
+ * band[i*(p+q+1) + o + p] = A[i, i+o].  status_dev (device int) receives 0,
+ * or 1 + the row of the first zero/non-finite pivot.
+ * Replaces DiaMatrix.lu_factor no-pivot kernel (la/diamatrix.py:394-504,
+ * 1936-1973). */
+int jfx_band_lu_factor(const double* band, int n, int p, int q, double* lu, int* status_dev,
+                       hipStream_t stream);
+/* In-place solve along the middle axis of X (P, n, Q).
+ * Replaces LUFactors.solve (la/diamatrix.py:1880-1925) with
+ * _forward_elimination/_backward_substitution (la/diamatrix.py:2079-2195). */
+int jfx_band_lu_solve(const double* lu, int n, int p, int q, double* X, long long P, long long Q,
+                      hipStream_t stream);
+
+/* ---- a12: Y = X / D with D (size) broadcast over the leading batch: the
+ * eigenvalue division of TPMatricesLUFactors.solve (la/tpmatrix.py:565-579). */
+int jfx_div_bcast_f64(const double* X, const double* D, double* Y, long long total, long long size,
+                      hipStream_t stream);
+int jfx_div_bcast_f32(const float* X, const float* D, float* Y, long long total, long long size,
+                      hipStream_t stream);
+
+/* Y = X * C with C (size) broadcast over the leading batch: the pointwise
+ * coefficient product of a non-separable bilinear form evaluated at the
+ * quadrature mesh (assemble_multivar, galerkin/inner.py:991-1024, applied
+ * matrix-free as TensorMatrix.matvec, la/tensormatrix.py:54-56). */
+int jfx_mul_bcast_f64(const double* X, const double* C, double* Y, long long total, long long size,
+                      hipStream_t stream);
+
+/* y (+)= sum_j coeffs[j] * xs[j], nin <= 8; xs and coeffs are HOST arrays of
+ * device pointers / scalars.  dtype 0 = f64, 1 = f32.  The stage
+ * combinations of IMEXRungeKutta.stage (integrators/imex_rk.py:55-94). */
+int jfx_lincomb(int dtype, int nin, const void* const* xs, const double* coeffs, void* y, long long n,
+                int accumulate, hipStream_t stream);
+
+/* direction 0: f64 -> f32 (round to nearest), 1: f32 -> f64.  The dtype
+ * boundary of the mixed-precision transforms (no reference counterpart:
+ * the reference runs in one dtype). */
+int jfx_cast(int direction, const void* x, void* y, long long n, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* JFX_H */

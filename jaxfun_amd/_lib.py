@@ -1,0 +1,91 @@
+"""ctypes binding of ``libjfx.so``, the HIP kernel library behind every
+device operation of this package.
+
+The library is loaded lazily on first use.  There is no fallback: when the
+library is missing, or no GPU is visible, every device operation raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libjfx.so")
+
+JFX_OK = 0
+JFX_EHIP = -1
+JFX_ESHAPE = -2
+JFX_ESINGULAR = -3
+JFX_EARG = -4
+
+c_int = ctypes.c_int
+c_ll = ctypes.c_longlong
+c_dbl = ctypes.c_double
+c_vp = ctypes.c_void_p
+
+# name -> (restype, argtypes); the single source of truth for the ABI that
+# include/jfx.h declares (tests check both agree).
+SIGNATURES: dict[str, tuple] = {
+    "jfx_version": (c_int, []),
+    "jfx_last_error": (ctypes.c_char_p, []),
+    "jfx_device_sync": (c_int, []),
+    "jfx_gauss_nodes": (c_int, [c_int, c_int, c_vp, c_vp, c_vp]),
+    "jfx_vandermonde_f64": (c_int, [c_int, c_vp, c_int, c_int, c_int, c_dbl, c_vp, c_vp]),
+    "jfx_contract_f64": (c_int, [c_vp, c_int, c_int, c_vp, c_vp, c_ll, c_ll, c_vp]),
+    "jfx_contract_f32": (c_int, [c_vp, c_int, c_int, c_vp, c_vp, c_ll, c_ll, c_vp]),
+    "jfx_split_matrix": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp]),
+    "jfx_contract_bf16x": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_vp, c_ll, c_ll, c_vp]),
+    "jfx_contract_ozaki": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_vp, c_vp, c_ll, c_ll, c_vp]),
+    "jfx_band_lu_factor": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_vp, c_vp]),
+    "jfx_band_lu_solve": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_ll, c_ll, c_vp]),
+    "jfx_div_bcast_f64": (c_int, [c_vp, c_vp, c_vp, c_ll, c_ll, c_vp]),
+    "jfx_div_bcast_f32": (c_int, [c_vp, c_vp, c_vp, c_ll, c_ll, c_vp]),
+    "jfx_mul_bcast_f64": (c_int, [c_vp, c_vp, c_vp, c_ll, c_ll, c_vp]),
+    "jfx_lincomb": (c_int, [c_int, c_int, c_vp, c_vp, c_vp, c_ll, c_int, c_vp]),
+    "jfx_cast": (c_int, [c_int, c_vp, c_vp, c_ll, c_vp]),
+    "jfx_scale_transpose": (c_int, [c_vp, c_int, c_int, c_vp, c_vp, c_dbl, c_vp, c_vp]),
+}
+
+_lib = None
+
+
+class JfxError(RuntimeError):
+    """A HIP-level failure reported by libjfx."""
+
+
+def load():
+    """Load ``libjfx.so`` and bind every declared entry point."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m jaxfun_amd.build` "
+            "(this package has no CPU fallback)"
+        )
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str = "") -> None:
+    """Map a libjfx status code to the exception class the reference raises."""
+    if rc == JFX_OK:
+        return
+    msg = load().jfx_last_error().decode(errors="replace")
+    text = f"{what}: {msg}" if what else msg
+    if rc == JFX_ESINGULAR:
+        raise ValueError(text)
+    if rc in (JFX_ESHAPE, JFX_EARG):
+        raise ValueError(text)
+    raise JfxError(text)
+
+
+def call(name: str, *args) -> None:
+    rc = getattr(load(), name)(*args)
+    check(rc, name)

@@ -1,0 +1,210 @@
+// Banded LU without pivoting (DIA matrices) and batched FP64 solves along an
+// axis, plus the elementwise kernels the solvers and integrators need.
+//
+// Band storage ("by row"): band[i * W + (o + p)] = A[i, i + o] for offsets
+// o in [-p, q], W = p + q + 1.  The factorisation overwrites a copy with L
+// (unit lower, multipliers at o < 0) and U (o >= 0) in the same layout.
+#include "jfx_common.h"
+
+namespace jfx {
+
+// Single-lane Doolittle factorisation of one small banded matrix (setup).
+__global__ void band_lu_factor_kernel(const double* __restrict__ band, int n, int p, int q,
+                                      double* __restrict__ lu, int* __restrict__ status) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int W = p + q + 1;
+  for (long long i = 0; i < (long long)n * W; ++i) lu[i] = band[i];
+  int st = 0;
+  for (int k = 0; k < n - 1 && st == 0; ++k) {
+    double piv = lu[(long long)k * W + p];
+    if (piv == 0.0 || !isfinite(piv)) {
+      st = k + 1;
+      break;
+    }
+    int iend = min(n - 1, k + p);
+    for (int i = k + 1; i <= iend; ++i) {
+      double& lik = lu[(long long)i * W + (k - i + p)];
+      lik /= piv;
+      int jend = min(n - 1, k + q);
+      for (int j = k + 1; j <= jend; ++j)
+        lu[(long long)i * W + (j - i + p)] -= lik * lu[(long long)k * W + (j - k + p)];
+    }
+  }
+  if (st == 0) {
+    double piv = lu[(long long)(n - 1) * W + p];
+    if (piv == 0.0 || !isfinite(piv)) st = n;
+  }
+  *status = st;
+}
+
+// Solve L U x = b in place along axis of length n of a (P, n, Q) batch; one
+// thread per line (coalesced over q).
+__global__ void band_lu_solve_kernel(const double* __restrict__ lu, int n, int p, int q,
+                                     double* __restrict__ X, long long P, long long Q) {
+  long long line = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (line >= P * Q) return;
+  long long pp = line / Q, qq = line - pp * Q;
+  double* x = X + pp * (long long)n * Q + qq;
+  const int W = p + q + 1;
+  for (int i = 1; i < n; ++i) {
+    double s = x[(long long)i * Q];
+    int j0 = max(0, i - p);
+    for (int j = j0; j < i; ++j) s -= lu[(long long)i * W + (j - i + p)] * x[(long long)j * Q];
+    x[(long long)i * Q] = s;
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double s = x[(long long)i * Q];
+    int j1 = min(n - 1, i + q);
+    for (int j = i + 1; j <= j1; ++j) s -= lu[(long long)i * W + (j - i + p)] * x[(long long)j * Q];
+    x[(long long)i * Q] = s / lu[(long long)i * W + p];
+  }
+}
+
+// Y[b, i] = X[b, i] / D[i]  (D broadcast over the leading batch)
+template <typename T>
+__global__ void div_bcast_kernel(const T* __restrict__ X, const T* __restrict__ D, T* __restrict__ Y,
+                                 long long total, long long size) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < total; i += stride) Y[i] = X[i] / D[i % size];
+}
+
+template <typename T>
+__global__ void mul_bcast_kernel(const T* __restrict__ X, const T* __restrict__ C, T* __restrict__ Y,
+                                 long long total, long long size) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < total; i += stride) Y[i] = X[i] * C[i % size];
+}
+
+// y = sum_j c_j x_j (n <= 8 inputs), optionally accumulating into y.
+template <typename T>
+struct LinComb {
+  const T* x[8];
+  double c[8];
+};
+template <typename T>
+__global__ void lincomb_kernel(LinComb<T> lc, int nin, T* __restrict__ y, long long n, int accumulate) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    double s = accumulate ? (double)y[i] : 0.0;
+    for (int j = 0; j < nin; ++j) s += lc.c[j] * (double)lc.x[j][i];
+    y[i] = (T)s;
+  }
+}
+
+template <typename TI, typename TO>
+__global__ void cast_kernel(const TI* __restrict__ x, TO* __restrict__ y, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) y[i] = (TO)x[i];
+}
+
+static unsigned grid_for(long long n, int block) {
+  long long g = (n + block - 1) / block;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+}  // namespace jfx
+
+using namespace jfx;
+
+extern "C" {
+
+int jfx_band_lu_factor(const double* band, int n, int p, int q, double* lu, int* status_dev,
+                       hipStream_t stream) {
+  if (!band || !lu || !status_dev || n <= 0 || p < 0 || q < 0)
+    return set_error("band_lu_factor: bad arguments"), JFX_EARG;
+  hipLaunchKernelGGL(band_lu_factor_kernel, dim3(1), dim3(64), 0, stream, band, n, p, q, lu, status_dev);
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
+}
+
+int jfx_band_lu_solve(const double* lu, int n, int p, int q, double* X, long long P, long long Q,
+                      hipStream_t stream) {
+  if (!lu || !X || n <= 0 || p < 0 || q < 0 || P < 0 || Q <= 0)
+    return set_error("band_lu_solve: bad arguments"), JFX_EARG;
+  long long lines = P * Q;
+  if (lines == 0) return JFX_OK;
+  hipLaunchKernelGGL(band_lu_solve_kernel, dim3((unsigned)((lines + 255) / 256)), dim3(256), 0, stream, lu,
+                     n, p, q, X, P, Q);
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
+}
+
+int jfx_div_bcast_f64(const double* X, const double* D, double* Y, long long total, long long size,
+                      hipStream_t stream) {
+  if (!X || !D || !Y || size <= 0 || total % size) return set_error("div_bcast: bad arguments"), JFX_EARG;
+  hipLaunchKernelGGL(div_bcast_kernel<double>, dim3(grid_for(total, 256)), dim3(256), 0, stream, X, D, Y,
+                     total, size);
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
+}
+
+int jfx_div_bcast_f32(const float* X, const float* D, float* Y, long long total, long long size,
+                      hipStream_t stream) {
+  if (!X || !D || !Y || size <= 0 || total % size) return set_error("div_bcast: bad arguments"), JFX_EARG;
+  hipLaunchKernelGGL(div_bcast_kernel<float>, dim3(grid_for(total, 256)), dim3(256), 0, stream, X, D, Y,
+                     total, size);
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
+}
+
+int jfx_mul_bcast_f64(const double* X, const double* C, double* Y, long long total, long long size,
+                      hipStream_t stream) {
+  if (!X || !C || !Y || size <= 0 || total % size) return set_error("mul_bcast: bad arguments"), JFX_EARG;
+  hipLaunchKernelGGL(mul_bcast_kernel<double>, dim3(grid_for(total, 256)), dim3(256), 0, stream, X, C, Y,
+                     total, size);
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
+}
+
+// direction 0: f64 -> f32 (round to nearest), 1: f32 -> f64
+int jfx_cast(int direction, const void* x, void* y, long long n, hipStream_t stream) {
+  if (!x || !y || n < 0) return set_error("cast: bad arguments"), JFX_EARG;
+  if (n == 0) return JFX_OK;
+  if (direction == 0)
+    hipLaunchKernelGGL((cast_kernel<double, float>), dim3(grid_for(n, 256)), dim3(256), 0, stream,
+                       (const double*)x, (float*)y, n);
+  else if (direction == 1)
+    hipLaunchKernelGGL((cast_kernel<float, double>), dim3(grid_for(n, 256)), dim3(256), 0, stream,
+                       (const float*)x, (double*)y, n);
+  else
+    return set_error("cast: bad direction"), JFX_EARG;
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
+}
+
+// dtype: 0 = f64, 1 = f32
+int jfx_lincomb(int dtype, int nin, const void* const* xs, const double* coeffs, void* y, long long n,
+                int accumulate, hipStream_t stream) {
+  if (nin < 0 || nin > 8 || !y || (nin > 0 && (!xs || !coeffs)))
+    return set_error("lincomb: bad arguments"), JFX_EARG;
+  if (n == 0) return JFX_OK;
+  if (dtype == 0) {
+    LinComb<double> lc{};
+    for (int j = 0; j < nin; ++j) {
+      lc.x[j] = (const double*)xs[j];
+      lc.c[j] = coeffs[j];
+    }
+    hipLaunchKernelGGL(lincomb_kernel<double>, dim3(grid_for(n, 256)), dim3(256), 0, stream, lc, nin,
+                       (double*)y, n, accumulate);
+  } else if (dtype == 1) {
+    LinComb<float> lc{};
+    for (int j = 0; j < nin; ++j) {
+      lc.x[j] = (const float*)xs[j];
+      lc.c[j] = coeffs[j];
+    }
+    hipLaunchKernelGGL(lincomb_kernel<float>, dim3(grid_for(n, 256)), dim3(256), 0, stream, lc, nin,
+                       (float*)y, n, accumulate);
+  } else {
+    return set_error("lincomb: bad dtype"), JFX_EARG;
+  }
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
+}
+
+}  // extern "C"

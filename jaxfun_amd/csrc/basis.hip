@@ -1,0 +1,142 @@
+// Basis evaluation in FP64 on the GPU: Gauss nodes/weights and the
+// (derivative) pseudo-Vandermonde matrices of Legendre / Chebyshev bases.
+#include "jfx_common.h"
+
+namespace jfx {
+
+enum { FAM_LEGENDRE = 0, FAM_CHEBYSHEV = 1 };
+
+// One thread per node; Newton on P_N with the three-term recurrence.
+// Nodes ascending; weights 2 / ((1-x)(1+x) P_N'(x)^2).
+__global__ void gauss_legendre_kernel(int N, double* __restrict__ x, double* __restrict__ w) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double pi = 3.14159265358979323846;
+  // i-th node in ascending order
+  double xi = -cos(pi * (i + 0.75) / (N + 0.5));
+  double p0 = 1.0, p1 = xi;
+  for (int it = 0; it < 100; ++it) {
+    p0 = 1.0;
+    p1 = xi;
+    for (int n = 1; n < N; ++n) {
+      double p2 = ((2 * n + 1) * xi * p1 - n * p0) / (n + 1);
+      p0 = p1;
+      p1 = p2;
+    }
+    double dp = N * (xi * p1 - p0) / (xi * xi - 1.0);
+    double dx = p1 / dp;
+    xi -= dx;
+    if (fabs(dx) < 1e-16) break;
+  }
+  p0 = 1.0;
+  p1 = xi;
+  for (int n = 1; n < N; ++n) {
+    double p2 = ((2 * n + 1) * xi * p1 - n * p0) / (n + 1);
+    p0 = p1;
+    p1 = p2;
+  }
+  double omx2 = (1.0 - xi) * (1.0 + xi);
+  double dp = N * (p0 - xi * p1) / omx2;
+  x[i] = xi;
+  w[i] = 2.0 / (omx2 * dp * dp);
+}
+
+__global__ void gauss_chebyshev_kernel(int N, double* __restrict__ x, double* __restrict__ w) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double pi = 3.14159265358979323846;
+  x[i] = cos(pi + (2 * i + 1) * pi / (2 * N));
+  w[i] = pi / N;
+}
+
+// V[q, n] = d^k/dx^k psi_n(x_q); one thread per point, k <= 4.
+// Differentiated recurrences:
+//   Legendre   P^(d)_{n+1} = P^(d)_{n-1} + (2n+1) P^(d-1)_n  (d >= 1)
+//   Chebyshev  T^(d)_{n+1} = 2x T^(d)_n + 2d T^(d-1)_n - T^(d)_{n-1}
+__global__ void vandermonde_kernel(int family, const double* __restrict__ x, int Q, int N, int k,
+                                   double scale, double* __restrict__ V) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= Q) return;
+  const double X = x[q];
+  double prev[5], cur[5];
+  for (int d = 0; d <= 4; ++d) {
+    prev[d] = (d == 0) ? 1.0 : 0.0;  // psi_0
+    cur[d] = (d == 0) ? X : (d == 1 ? 1.0 : 0.0);  // psi_1
+  }
+  double* row = V + (long long)q * N;
+  row[0] = prev[k] * scale;
+  if (N > 1) row[1] = cur[k] * scale;
+  for (int n = 1; n < N - 1; ++n) {
+    double nxt[5];
+    if (family == FAM_LEGENDRE) {
+      nxt[0] = ((2 * n + 1) * X * cur[0] - n * prev[0]) / (n + 1);
+      for (int d = 1; d <= k; ++d) nxt[d] = prev[d] + (2 * n + 1) * cur[d - 1];
+    } else {
+      nxt[0] = 2.0 * X * cur[0] - prev[0];
+      for (int d = 1; d <= k; ++d) nxt[d] = 2.0 * X * cur[d] + 2.0 * d * cur[d - 1] - prev[d];
+    }
+    for (int d = 0; d <= k; ++d) {
+      prev[d] = cur[d];
+      cur[d] = nxt[d];
+    }
+    row[n + 1] = cur[k] * scale;
+  }
+}
+
+// out[c, r] = V[r, c] * (rs ? rs[r] : 1) * (cs ? cs[c] : 1) * s   (V is R x C)
+__global__ void scale_transpose_kernel(const double* __restrict__ V, int R, int C,
+                                       const double* __restrict__ rs, const double* __restrict__ cs,
+                                       double s, double* __restrict__ out) {
+  __shared__ double tile[32][33];
+  int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+  int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
+  for (int j = ty; j < 32; j += 8) {
+    int r = r0 + j, c = c0 + tx;
+    tile[j][tx] = (r < R && c < C) ? V[(long long)r * C + c] * (rs ? rs[r] : 1.0) : 0.0;
+  }
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8) {
+    int c = c0 + j, r = r0 + tx;
+    if (r < R && c < C) out[(long long)c * R + r] = tile[tx][j] * (cs ? cs[c] : 1.0) * s;
+  }
+}
+
+}  // namespace jfx
+
+using namespace jfx;
+
+extern "C" {
+
+int jfx_gauss_nodes(int family, int N, double* x, double* w, hipStream_t stream) {
+  if (!x || !w || N <= 0) return set_error("gauss_nodes: bad arguments"), JFX_EARG;
+  dim3 g((N + 255) / 256), b(256);
+  if (family == FAM_LEGENDRE)
+    hipLaunchKernelGGL(gauss_legendre_kernel, g, b, 0, stream, N, x, w);
+  else if (family == FAM_CHEBYSHEV)
+    hipLaunchKernelGGL(gauss_chebyshev_kernel, g, b, 0, stream, N, x, w);
+  else
+    return set_error("gauss_nodes: unknown family %d", family), JFX_EARG;
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
+}
+
+int jfx_vandermonde_f64(int family, const double* x, int Q, int N, int k, double scale, double* V,
+                        hipStream_t stream) {
+  if (!x || !V || Q <= 0 || N <= 0 || k < 0 || k > 4 || (family != 0 && family != 1))
+    return set_error("vandermonde: bad arguments"), JFX_EARG;
+  hipLaunchKernelGGL(vandermonde_kernel, dim3((Q + 127) / 128), dim3(128), 0, stream, family, x, Q, N,
+                     k, scale, V);
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
+}
+
+int jfx_scale_transpose(const double* V, int R, int C, const double* rowscale, const double* colscale,
+                        double s, double* out, hipStream_t stream) {
+  if (!V || !out || R <= 0 || C <= 0) return set_error("scale_transpose: bad arguments"), JFX_EARG;
+  dim3 g((C + 31) / 32, (R + 31) / 32), b(32, 8);
+  hipLaunchKernelGGL(scale_transpose_kernel, g, b, 0, stream, V, R, C, rowscale, colscale, s, out);
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+// Shared helpers for the jfx HIP kernels (gfx950 / CDNA4 only).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdio>
+#include <cstring>
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef unsigned short v8u16 __attribute__((ext_vector_type(8)));
+
+// Status codes returned by every C-ABI entry point.
+enum {
+  JFX_OK = 0,
+  JFX_EHIP = -1,       // a HIP runtime call failed
+  JFX_ESHAPE = -2,     // unsupported or inconsistent sizes
+  JFX_ESINGULAR = -3,  // zero / non-finite pivot in a banded factorisation
+  JFX_EARG = -4,       // invalid argument (null pointer, bad enum)
+};
+
+namespace jfx {
+
+void set_error(const char* fmt, ...);
+
+#define JFX_CHECK_HIP(expr)                                              \
+  do {                                                                   \
+    hipError_t _e = (expr);                                              \
+    if (_e != hipSuccess) {                                              \
+      ::jfx::set_error("%s failed: %s", #expr, hipGetErrorString(_e));   \
+      return JFX_EHIP;                                                   \
+    }                                                                    \
+  } while (0)
+
+#define JFX_LAUNCH_CHECK()                                               \
+  do {                                                                   \
+    hipError_t _e = hipGetLastError();                                   \
+    if (_e != hipSuccess) {                                              \
+      ::jfx::set_error("kernel launch failed: %s", hipGetErrorString(_e)); \
+      return JFX_EHIP;                                                   \
+    }                                                                    \
+  } while (0)
+
+// float -> bf16 bits, round to nearest even (finite inputs).
+__device__ __forceinline__ unsigned short f2bf(float f) {
+  unsigned int u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short b) {
+  return __uint_as_float(((unsigned int)b) << 16);
+}
+
+// Map a linear workgroup id so that consecutive logical tiles land on the
+// same XCD (workgroups are dispatched round-robin over the 8 XCDs).
+__device__ __forceinline__ unsigned xcd_remap(unsigned bid, unsigned nblk) {
+  const unsigned nx = 8;
+  if (nblk % nx != 0) return bid;
+  return (bid % nx) * (nblk / nx) + bid / nx;
+}
+
+// Output row of accumulator element r held by `lane` for a 16x16 MFMA block
+// (column = lane % 16).  Measured on gfx950: f32/bf16/f16 16x16 outputs hold
+// rows 4*(lane/16) + r; v_mfma_f64_16x16x4f64 holds rows lane/16 + 4*r.
+__device__ __forceinline__ int mfma16_row_f32(int lane, int r) { return 4 * (lane >> 4) + r; }
+__device__ __forceinline__ int mfma16_row_f64(int lane, int r) { return (lane >> 4) + 4 * r; }
+
+__device__ __forceinline__ v4d mfma_f64(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ v4f mfma_f32(float a, float b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ v4f mfma_bf16(v8u16 a, v8u16 b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a),
+                                                 __builtin_bit_cast(v8bf, b), c, 0, 0, 0);
+}
+
+}  // namespace jfx

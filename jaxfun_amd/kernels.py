@@ -1,0 +1,229 @@
+"""Thin torch-tensor wrappers around the libjfx C ABI.
+
+Every function here launches a hand-written HIP kernel from ``libjfx.so`` on
+the current torch stream; torch provides only device memory and the stream.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+
+import torch
+
+from . import _lib
+
+PRECISIONS = ("f64", "f32", "bf16x3", "bf16x6", "ozaki3", "ozaki4", "ozaki5")
+FAMILY = {"legendre": 0, "chebyshev": 1}
+
+
+def device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("jaxfun_amd needs a ROCm GPU (gfx950); none is visible")
+    _lib.load()
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _p(t: torch.Tensor | None):
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def _s():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _req(t: torch.Tensor, dtype: torch.dtype, name: str) -> None:
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError(f"{name} must be a CUDA/HIP torch tensor")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+
+
+# ---------------------------------------------------------------- basis
+
+
+def gauss_nodes(family: str, N: int) -> tuple[torch.Tensor, torch.Tensor]:
+    dev = device()
+    x = torch.empty(N, dtype=torch.float64, device=dev)
+    w = torch.empty(N, dtype=torch.float64, device=dev)
+    _lib.call("jfx_gauss_nodes", FAMILY[family], N, _p(x), _p(w), _s())
+    return x, w
+
+
+def vandermonde(family: str, x: torch.Tensor, N: int, k: int = 0, scale: float = 1.0) -> torch.Tensor:
+    _req(x, torch.float64, "x")
+    V = torch.empty((x.shape[0], N), dtype=torch.float64, device=x.device)
+    _lib.call("jfx_vandermonde_f64", FAMILY[family], _p(x), x.shape[0], N, k, float(scale), _p(V), _s())
+    return V
+
+
+def scale_transpose(V: torch.Tensor, rowscale: torch.Tensor | None = None,
+                    colscale: torch.Tensor | None = None, s: float = 1.0) -> torch.Tensor:
+    """``out[c, r] = V[r, c] * rowscale[r] * colscale[c] * s``."""
+    _req(V, torch.float64, "V")
+    R, C = V.shape
+    out = torch.empty((C, R), dtype=torch.float64, device=V.device)
+    _lib.call("jfx_scale_transpose", _p(V), R, C, _p(rowscale), _p(colscale), float(s), _p(out), _s())
+    return out
+
+
+# ---------------------------------------------------------------- casts / elementwise
+
+
+def cast(x: torch.Tensor, dtype: torch.dtype) -> torch.Tensor:
+    if x.dtype == dtype:
+        return x
+    x = x.contiguous()
+    y = torch.empty(x.shape, dtype=dtype, device=x.device)
+    if x.dtype == torch.float64 and dtype == torch.float32:
+        _lib.call("jfx_cast", 0, _p(x), _p(y), x.numel(), _s())
+    elif x.dtype == torch.float32 and dtype == torch.float64:
+        _lib.call("jfx_cast", 1, _p(x), _p(y), x.numel(), _s())
+    else:
+        raise TypeError(f"unsupported cast {x.dtype} -> {dtype}")
+    return y
+
+
+def lincomb(terms, out: torch.Tensor | None = None, accumulate: bool = False) -> torch.Tensor:
+    """``out (+)= sum c_j x_j`` for up to 8 same-shape tensors in one pass."""
+    terms = [(float(c), x) for c, x in terms]
+    if not terms and out is None:
+        raise ValueError("lincomb needs at least one term or an output")
+    ref = out if out is not None else terms[0][1]
+    if out is None:
+        out = torch.empty_like(ref)
+    dt = {torch.float64: 0, torch.float32: 1}[out.dtype]
+    for chunk_start in range(0, max(1, len(terms)), 8):
+        chunk = terms[chunk_start:chunk_start + 8]
+        for _, x in chunk:
+            _req(x, out.dtype, "lincomb input")
+            if x.shape != out.shape:
+                raise ValueError("lincomb: shape mismatch")
+        ptrs = (ctypes.c_void_p * 8)(*[x.data_ptr() for _, x in chunk])
+        cs = (ctypes.c_double * 8)(*[c for c, _ in chunk])
+        acc = 1 if (accumulate or chunk_start > 0) else 0
+        _lib.call("jfx_lincomb", dt, len(chunk), ptrs, cs, _p(out), out.numel(), acc, _s())
+    return out
+
+
+def div_bcast(X: torch.Tensor, D: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+    """``out[b, ...] = X[b, ...] / D[...]`` with D broadcast over leading dims."""
+    _req(X, X.dtype, "X")
+    _req(D, X.dtype, "D")
+    out = torch.empty_like(X) if out is None else out
+    name = "jfx_div_bcast_f64" if X.dtype == torch.float64 else "jfx_div_bcast_f32"
+    _lib.call(name, _p(X), _p(D), _p(out), X.numel(), D.numel(), _s())
+    return out
+
+
+# ---------------------------------------------------------------- banded LU
+
+
+def band_lu_factor(band: torch.Tensor, p: int, q: int) -> torch.Tensor:
+    """Factor a (n, p+q+1) by-row band matrix without pivoting.
+
+    Raises ValueError on a zero or non-finite pivot, like
+    ``DiaMatrix.lu_factor`` (la/diamatrix.py:461-471).
+    """
+    _req(band, torch.float64, "band")
+    n = band.shape[0]
+    lu = torch.empty_like(band)
+    st = torch.zeros(1, dtype=torch.int32, device=band.device)
+    _lib.call("jfx_band_lu_factor", _p(band), n, p, q, _p(lu), _p(st), _s())
+    s = int(st.item())
+    if s != 0:
+        raise ValueError(f"Zero or non-finite pivot at row {s - 1} in banded LU factorization")
+    return lu
+
+
+def band_lu_solve(lu: torch.Tensor, p: int, q: int, X: torch.Tensor, axis: int) -> torch.Tensor:
+    """Solve in place along ``axis`` of X with a factored band matrix."""
+    _req(X, torch.float64, "X")
+    n = lu.shape[0]
+    if X.shape[axis] != n:
+        raise ValueError(f"band solve: axis {axis} has length {X.shape[axis]}, matrix is {n}")
+    P = math.prod(X.shape[:axis])
+    Q = math.prod(X.shape[axis + 1:])
+    _lib.call("jfx_band_lu_solve", _p(lu), n, p, q, _p(X), P, Q, _s())
+    return X
+
+
+# ---------------------------------------------------------------- per-axis contraction
+
+
+class AxisOperator:
+    """A constant (M x K) matrix applied along one axis, prepared per precision.
+
+    ``f64`` uses the FP64 matrix; ``f32`` an FP32 copy; ``bf16x3``/``bf16x6``
+    2/3 bf16 float slices; ``ozakiS`` S fixed-point digit slices with
+    power-of-two row scales.  Preparations are cached on the object, like the
+    reference caches factorizations on its matrix objects.
+    """
+
+    def __init__(self, A: torch.Tensor):
+        if A.dtype != torch.float64:
+            A = A.to(torch.float64)
+        self.A64 = A.contiguous()
+        self.M, self.K = self.A64.shape
+        self._prep: dict[str, tuple] = {}
+
+    def prepared(self, prec: str):
+        if prec in self._prep:
+            return self._prep[prec]
+        A = self.A64
+        if prec == "f64":
+            r = (A,)
+        elif prec == "f32":
+            r = (cast(A, torch.float32),)
+        elif prec in ("bf16x3", "bf16x6"):
+            S = 2 if prec == "bf16x3" else 3
+            sl = torch.empty((S, self.M, self.K), dtype=torch.int16, device=A.device)
+            _lib.call("jfx_split_matrix", _p(A), self.M, self.K, 0, S, _p(sl), _p(None), _s())
+            r = (sl, S)
+        elif prec.startswith("ozaki"):
+            S = int(prec[5:])
+            sl = torch.empty((S, self.M, self.K), dtype=torch.int16, device=A.device)
+            rs = torch.empty(self.M, dtype=torch.float64, device=A.device)
+            _lib.call("jfx_split_matrix", _p(A), self.M, self.K, 1, S, _p(sl), _p(rs), _s())
+            r = (sl, rs, S)
+        else:
+            raise ValueError(f"unknown precision {prec!r}; expected one of {PRECISIONS}")
+        self._prep[prec] = r
+        return r
+
+
+def io_dtype(prec: str) -> torch.dtype:
+    return torch.float32 if prec in ("f32", "bf16x3", "bf16x6") else torch.float64
+
+
+def apply_along(op: AxisOperator, X: torch.Tensor, axis: int, prec: str = "f64",
+                out: torch.Tensor | None = None) -> torch.Tensor:
+    """``Y = A`` applied along ``axis`` of X: Y[p, m, q] = sum_k A[m, k] X[p, k, q]."""
+    dt = io_dtype(prec)
+    _req(X, dt, "X")
+    axis = axis % X.ndim
+    K = X.shape[axis]
+    if K != op.K:
+        raise ValueError(f"apply_along: axis {axis} has length {K}, operator expects {op.K}")
+    P = math.prod(X.shape[:axis])
+    Q = math.prod(X.shape[axis + 1:])
+    shape = list(X.shape)
+    shape[axis] = op.M
+    Y = torch.empty(shape, dtype=dt, device=X.device) if out is None else out
+    if out is not None:
+        _req(out, dt, "out")
+        if list(out.shape) != shape:
+            raise ValueError("apply_along: bad out shape")
+    prep = op.prepared(prec)
+    if prec == "f64":
+        _lib.call("jfx_contract_f64", _p(prep[0]), op.M, K, _p(X), _p(Y), P, Q, _s())
+    elif prec == "f32":
+        _lib.call("jfx_contract_f32", _p(prep[0]), op.M, K, _p(X), _p(Y), P, Q, _s())
+    elif prec in ("bf16x3", "bf16x6"):
+        _lib.call("jfx_contract_bf16x", _p(prep[0]), prep[1], op.M, K, _p(X), _p(Y), P, Q, _s())
+    else:
+        _lib.call("jfx_contract_ozaki", _p(prep[0]), _p(prep[1]), prep[2], op.M, K, _p(X), _p(Y), P, Q,
+                  _s())
+    return Y

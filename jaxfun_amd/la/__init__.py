@@ -1,0 +1,26 @@
+"""Linear algebra on the device (reference: la/)."""
+
+from .matrix import BaseMatrix, DiaMatrix, Matrix, SolverNotApplicable
+from .tpmatrix import (
+    TPMatrices,
+    TPMatricesDenseLUFactors,
+    TPMatricesLUFactors,
+    TPMatrix,
+    tpmats_dense_lu_factor,
+    tpmats_lu_factor,
+    tpmats_to_kron,
+)
+
+__all__ = [
+    "BaseMatrix",
+    "DiaMatrix",
+    "Matrix",
+    "SolverNotApplicable",
+    "TPMatrices",
+    "TPMatricesDenseLUFactors",
+    "TPMatricesLUFactors",
+    "TPMatrix",
+    "tpmats_dense_lu_factor",
+    "tpmats_lu_factor",
+    "tpmats_to_kron",
+]

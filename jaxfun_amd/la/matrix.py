@@ -1,0 +1,193 @@
+"""Dense and DIA (banded) 1D matrices (reference: la/matrix.py, la/diamatrix.py).
+
+Both keep the matrix on the device in FP64 (plus a host copy for the one-time
+set-up work the reference also does on the host: eigendecompositions).
+``matvec(x, axis)`` applies the matrix along one axis of a batch with the
+MFMA contraction kernel; ``DiaMatrix.solve`` runs the banded no-pivot LU
+kernels.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .. import kernels as K
+
+
+class SolverNotApplicable(Exception):
+    """A solver strategy cannot be applied (la/matrixprotocol.py:30)."""
+
+
+def _host(a) -> np.ndarray:
+    if isinstance(a, torch.Tensor):
+        return a.detach().to("cpu", torch.float64).numpy()
+    return np.asarray(a, dtype=np.float64)
+
+
+class BaseMatrix:
+    """Common interface (la/matrixprotocol.py:68-242)."""
+
+    is_zero = False
+
+    def __init__(self, data):
+        self._host_data = _host(data).copy()
+        self._dev = None
+        self._op = None
+        self._lu = None
+
+    # ---------------------------------------------------------------- data
+    @property
+    def shape(self) -> tuple[int, int]:
+        return self._host_data.shape
+
+    @property
+    def dtype(self):
+        return torch.float64
+
+    @property
+    def size(self) -> int:
+        return int(self._host_data.size)
+
+    def todense_host(self) -> np.ndarray:
+        return self._host_data
+
+    def todense(self) -> torch.Tensor:
+        if self._dev is None:
+            self._dev = torch.tensor(self._host_data, dtype=torch.float64, device=K.device())
+        return self._dev
+
+    def operator(self) -> K.AxisOperator:
+        if self._op is None:
+            self._op = K.AxisOperator(self.todense())
+        return self._op
+
+    def diagonal_or_none(self) -> np.ndarray | None:
+        A = self._host_data
+        if A.shape[0] != A.shape[1]:
+            return None
+        if np.count_nonzero(A - np.diag(np.diag(A))) == 0:
+            return np.diag(A).copy()
+        return None
+
+    # ---------------------------------------------------------------- algebra
+    def matvec(self, x, axis: int = 0, precision: str = "f64") -> torch.Tensor:
+        """Apply along ``axis`` (la/matrix.py:89-118, la/diamatrix.py:227-349)."""
+        from ..galerkin.orthogonal import as_device
+
+        x = as_device(x, K.io_dtype(precision))
+        return K.apply_along(self.operator(), x, axis % x.ndim, precision)
+
+    def __matmul__(self, other):
+        if isinstance(other, BaseMatrix):
+            return type(self)._combine(self, self._host_data @ other._host_data, other)
+        return self.matvec(other, 0)
+
+    @staticmethod
+    def _combine(a, data, b=None):
+        if isinstance(a, DiaMatrix) and (b is None or isinstance(b, DiaMatrix)):
+            return DiaMatrix.from_dense(data)
+        return Matrix(data)
+
+    def __mul__(self, s):
+        return type(self)._combine(self, self._host_data * float(s))
+
+    __rmul__ = __mul__
+
+    def __add__(self, other):
+        return type(self)._combine(self, self._host_data + other._host_data, other)
+
+    def __sub__(self, other):
+        return type(self)._combine(self, self._host_data - other._host_data, other)
+
+    def __neg__(self):
+        return self * -1.0
+
+    @property
+    def T(self):
+        return type(self)._combine(self, self._host_data.T.copy())
+
+    def to_matrix(self) -> "Matrix":
+        return Matrix(self._host_data)
+
+    def tosparse(self, tol: int = 100) -> "DiaMatrix":
+        return DiaMatrix.from_dense(self._host_data, tol=tol)
+
+    def __repr__(self) -> str:
+        return f"{type(self).__name__}(shape={self.shape})"
+
+
+class Matrix(BaseMatrix):
+    """Dense matrix (la/matrix.py:23-214)."""
+
+    def lu_factor(self):
+        if self._lu is None:
+            self._lu = torch.linalg.lu_factor(self.todense())
+        return self._lu
+
+    def solve(self, b, axis: int = 0):
+        """Dense solve along an axis (la/matrix.py:187-214) through rocSOLVER."""
+        from ..galerkin.orthogonal import as_device
+
+        b = as_device(b)
+        LU, piv = self.lu_factor()
+        x = b.movedim(axis, 0).reshape(b.shape[axis], -1)
+        y = torch.linalg.lu_solve(LU, piv, x)
+        return y.reshape(b.movedim(axis, 0).shape).movedim(0, axis).contiguous()
+
+
+class DiaMatrix(BaseMatrix):
+    """Banded matrix in DIA form (la/diamatrix.py:101-349)."""
+
+    def __init__(self, data, offsets=None):
+        super().__init__(data)
+        A = self._host_data
+        if offsets is None:
+            m, n = A.shape
+            offs = [o for o in range(-(m - 1), n) if np.any(np.diagonal(A, o) != 0)]
+            offsets = tuple(offs) if offs else (0,)
+        self.offsets = tuple(int(o) for o in offsets)
+
+    @classmethod
+    def from_dense(cls, A, tol: int = 100) -> "DiaMatrix":
+        """Keep diagonals with entries above ``tol`` ulps (la/diamatrix.py:158-225)."""
+        A = _host(A).copy()
+        thr = tol * np.spacing(max(1.0, float(np.abs(A).max()) if A.size else 1.0))
+        A[np.abs(A) < thr] = 0.0
+        return cls(A)
+
+    @property
+    def bandwidths(self) -> tuple[int, int]:
+        return max(0, -min(self.offsets)), max(0, max(self.offsets))
+
+    def band(self) -> np.ndarray:
+        p, q = self.bandwidths
+        A = self._host_data
+        n = A.shape[0]
+        out = np.zeros((n, p + q + 1))
+        for i in range(n):
+            for o in range(-p, q + 1):
+                if 0 <= i + o < n:
+                    out[i, o + p] = A[i, i + o]
+        return out
+
+    def lu_factor(self):
+        """No-pivot banded LU on the device (la/diamatrix.py:394-504)."""
+        if self._lu is None:
+            if self.shape[0] != self.shape[1]:
+                raise ValueError("lu_factor needs a square matrix")
+            p, q = self.bandwidths
+            band = torch.tensor(self.band(), dtype=torch.float64, device=K.device())
+            self._lu = K.band_lu_factor(band, p, q)
+        return self._lu
+
+    def solve(self, b, axis: int = 0):
+        """Banded solve along an axis (la/diamatrix.py:592-770, 1880-1925)."""
+        from ..galerkin.orthogonal import as_device
+
+        lu = self.lu_factor()
+        p, q = self.bandwidths
+        x = as_device(b).clone()
+        return K.band_lu_solve(lu, p, q, x, axis % x.ndim)
+
+    lu_solve = solve

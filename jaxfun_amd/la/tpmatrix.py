@@ -1,0 +1,390 @@
+"""Kronecker-product operators and their solvers (reference: la/tpmatrix.py).
+
+``TPMatrix`` applies ``scale * A0 (x) A1 [(x) A2]`` as one contraction per
+axis; ``TPMatrices`` sums such terms.  ``TPMatrices.solve`` follows the
+reference's dispatch (la/tpmatrix.py:386-509): all-dense factors use the dense
+Kronecker LU, otherwise fast diagonalisation.  The per-axis generalised
+eigenproblem is solved once on the host in FP64 (as the reference does with
+``scipy.linalg.eigh``, la/tpmatrix.py:1160-1204); non-symmetric pairs (the
+Chebyshev-Dirichlet stiffness) use ``scipy.linalg.eig`` with the exact
+left transform ``W = (B V)^-1`` instead of ``V^T``, so the solve stays exact.
+The solve itself is four (2D) or six (3D) device contractions and one
+broadcast division.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import scipy.linalg as sla
+import torch
+
+from .. import kernels as K
+from .matrix import BaseMatrix, DiaMatrix, Matrix, SolverNotApplicable
+
+
+def _as_dev(x, dtype=torch.float64):
+    from ..galerkin.orthogonal import as_device
+
+    return as_device(x, dtype)
+
+
+class TPMatrix:
+    """Rank-d separable operator ``scale * kron(A0, A1, ...)`` (la/tpmatrix.py:46-220)."""
+
+    is_zero = False
+
+    def __init__(self, mats, scale=1.0, global_indices=(0, 0)):
+        self.mats = list(mats)
+        self.coefficient = scale
+        self.global_indices = global_indices
+
+    @property
+    def dims(self) -> int:
+        return len(self.mats)
+
+    def __len__(self) -> int:
+        return len(self.mats)
+
+    @property
+    def shape(self) -> tuple[int, int]:
+        return (int(np.prod([m.shape[0] for m in self.mats])), int(np.prod([m.shape[1] for m in self.mats])))
+
+    def scale(self, alpha) -> "TPMatrix":
+        return TPMatrix(self.mats, self.coefficient * alpha, self.global_indices)
+
+    def _matmul_array(self, w, precision: str = "f64") -> torch.Tensor:
+        """A0 along axis 0, A1 along axis 1, ... (la/tpmatrix.py:152-156)."""
+        w = _as_dev(w, K.io_dtype(precision))
+        nb = w.ndim - self.dims
+        for i, m in enumerate(self.mats):
+            w = m.matvec(w, nb + i, precision)
+        if self.coefficient != 1:
+            w = K.lincomb([(float(self.coefficient), w)])
+        return w
+
+    def __call__(self, u, precision: str = "f64"):
+        return self._matmul_array(u, precision)
+
+    __matmul__ = __call__
+
+    def diagonal_or_none(self):
+        diags = [m.diagonal_or_none() for m in self.mats]
+        if any(d is None for d in diags):
+            return None
+        D = diags[0]
+        for d in diags[1:]:
+            D = np.multiply.outer(D, d)
+        return D * float(self.coefficient)
+
+    @property
+    def is_diagonal(self) -> bool:
+        return self.diagonal_or_none() is not None
+
+    def solve(self, rhs):
+        """Factor-wise solve (la/tpmatrix.py:178-213)."""
+        d = self.diagonal_or_none()
+        if d is not None:
+            return _solve_diagonal(d, rhs)
+        x = _as_dev(rhs).clone()
+        nb = x.ndim - self.dims
+        for i, m in enumerate(self.mats):
+            x = m.solve(x, axis=nb + i)
+        return K.lincomb([(1.0 / float(self.coefficient), x)])
+
+    def todense_host(self) -> np.ndarray:
+        return tpmats_to_kron([self])
+
+    def todense(self) -> torch.Tensor:
+        return torch.tensor(self.todense_host(), device=K.device())
+
+    def to_matrix(self) -> Matrix:
+        return Matrix(self.todense_host())
+
+    def __add__(self, other):
+        if isinstance(other, TPMatrix):
+            return TPMatrices([self, other])
+        if isinstance(other, TPMatrices):
+            return TPMatrices([self, *other.tpmats])
+        return NotImplemented
+
+
+def _solve_diagonal(d: np.ndarray, rhs) -> torch.Tensor:
+    x = _as_dev(rhs)
+    D = torch.tensor(np.ascontiguousarray(d), dtype=torch.float64, device=x.device)
+    return K.div_bcast(x, D)
+
+
+class TPMatricesLUFactors:
+    """Fast-diagonalisation solver (la/tpmatrix.py:512-587).
+
+    ``U = V (W F W^T / D) V^T`` (2D; one transform per axis in general) with
+    ``D = sum_k s_k prod_axis lambda``.  ``precision`` selects the contraction
+    arithmetic of the four/six transforms (the division follows the I/O dtype).
+    """
+
+    def __init__(self, eigvecs, lefts, per_term_eigenvalues, scales, shape):
+        self.shape = tuple(shape)
+        self.V = [K.AxisOperator(torch.tensor(v, device=K.device())) for v in eigvecs]
+        self.W = [K.AxisOperator(torch.tensor(w, device=K.device())) for w in lefts]
+        D = np.zeros(self.shape)
+        for evals, s in zip(per_term_eigenvalues, scales):
+            t = np.ones(self.shape)
+            for i, ev in enumerate(evals):
+                sh = [1] * len(self.shape)
+                sh[i] = -1
+                t = t * np.reshape(ev, sh)
+            D = D + float(s) * t
+        self.D_host = D
+        self._D = {}
+        self.eigvecs_host = [np.asarray(v) for v in eigvecs]
+        self.lefts_host = [np.asarray(w) for w in lefts]
+
+    def D(self, dtype) -> torch.Tensor:
+        if dtype not in self._D:
+            self._D[dtype] = torch.tensor(self.D_host, dtype=dtype, device=K.device())
+        return self._D[dtype]
+
+    def solve(self, rhs, precision: str = "f64") -> torch.Tensor:
+        dt = K.io_dtype(precision)
+        F = _as_dev(rhs, dt)
+        nb = F.ndim - len(self.shape)
+        if tuple(F.shape[nb:]) != self.shape:
+            F = F.reshape(F.shape[:nb] + self.shape) if F.numel() % int(np.prod(self.shape)) == 0 else F
+        for i, W in enumerate(self.W):
+            F = K.apply_along(W, F, nb + i, precision)
+        U = K.div_bcast(F, self.D(dt))
+        for i, V in enumerate(self.V):
+            U = K.apply_along(V, U, nb + i, precision)
+        return U
+
+
+class TPMatricesDenseLUFactors:
+    """Dense Kronecker LU (la/tpmatrix.py:1016-1049), factored by rocSOLVER."""
+
+    def __init__(self, A: np.ndarray, shape):
+        self.shape = tuple(shape)
+        self.LU, self.piv = torch.linalg.lu_factor(torch.tensor(A, device=K.device()))
+
+    def solve(self, rhs, precision: str = "f64") -> torch.Tensor:
+        b = _as_dev(rhs)
+        nb = b.ndim - len(self.shape)
+        flat = b.reshape(b.shape[:nb] + (-1,))
+        x = torch.linalg.lu_solve(self.LU, self.piv, flat.reshape(-1, flat.shape[-1]).T.contiguous())
+        return x.T.reshape(b.shape).contiguous()
+
+
+def tpmats_to_kron(tpmats) -> np.ndarray:
+    """Dense sum of Kronecker products (la/tpmatrix.py:1357-1402)."""
+    out = None
+    for tp in tpmats:
+        Kd = tp.mats[0].todense_host()
+        for m in tp.mats[1:]:
+            Kd = np.kron(Kd, m.todense_host())
+        Kd = float(tp.coefficient) * Kd
+        out = Kd if out is None else out + Kd
+    return out
+
+
+def tpmats_dense_lu_factor(A) -> TPMatricesDenseLUFactors:
+    """(la/tpmatrix.py:1052-1086)"""
+    tpmats = [A] if isinstance(A, TPMatrix) else list(A)
+    for tp in tpmats:
+        for m in tp.mats:
+            if not isinstance(m, Matrix):
+                raise TypeError(f"tpmats_dense_lu_factor requires all factor matrices to be Matrix (dense); "
+                                f"got {type(m).__name__}.")
+    shape = tuple(int(m.shape[0]) for m in tpmats[0].mats)
+    return TPMatricesDenseLUFactors(tpmats_to_kron(tpmats), shape)
+
+
+def _is_spd(A: np.ndarray) -> bool:
+    if not np.allclose(A, A.T, rtol=1e-12, atol=1e-14 * max(1.0, np.abs(A).max())):
+        return False
+    try:
+        np.linalg.cholesky(A)
+        return True
+    except np.linalg.LinAlgError:
+        return False
+
+
+def axis_eig(A: np.ndarray, B: np.ndarray):
+    """Generalised eigenpairs ``A V = B V diag(lam)`` and the left transform W
+    with ``W B V = I``: symmetric-definite -> eigh (W = V^T, la/tpmatrix.py:1173);
+    otherwise eig with W = (B V)^-1."""
+    sym = np.allclose(A, A.T, rtol=1e-12, atol=1e-13 * max(1.0, np.abs(A).max()))
+    if sym and _is_spd(B):
+        lam, V = sla.eigh(A, B)
+        return lam, V, V.T.copy()
+    lam, V = sla.eig(A, B)
+    if np.max(np.abs(lam.imag)) > 1e-8 * max(1.0, np.max(np.abs(lam.real))):
+        raise SolverNotApplicable("complex generalised eigenvalues")
+    lam, V = lam.real.copy(), V.real.copy()
+    order = np.argsort(lam)
+    lam, V = lam[order], V[:, order]
+    W = np.linalg.inv(B @ V)
+    return lam, V, W
+
+
+def tpmats_lu_factor(A) -> TPMatricesLUFactors:
+    """Per-axis simultaneous diagonalisation (la/tpmatrix.py:1089-1233)."""
+    tpmats = [A] if isinstance(A, TPMatrix) else list(A)
+    ndim = tpmats[0].dims
+    reps: list[np.ndarray] = []
+
+    def rep(m) -> int:
+        d = m.todense_host()
+        for r, R in enumerate(reps):
+            if R.shape == d.shape and np.allclose(d, R, rtol=1e-12, atol=1e-14 * max(1.0, np.abs(R).max())):
+                return r
+        reps.append(d)
+        return len(reps) - 1
+
+    ids = [[rep(tp.mats[i]) for i in range(ndim)] for tp in tpmats]
+    eigvecs, lefts = [], []
+    evals_of: dict[int, np.ndarray] = {}
+    per_axis_evals: list[dict] = []
+    for i in range(ndim):
+        uniq = list(dict.fromkeys(ids[k][i] for k in range(len(tpmats))))
+        ev: dict[int, np.ndarray] = {}
+        if len(uniq) == 1:
+            Am = reps[uniq[0]]
+            lam, V, W = axis_eig(Am, np.eye(Am.shape[0]))
+            ev[uniq[0]] = lam
+        elif len(uniq) == 2:
+            A0, A1 = reps[uniq[0]], reps[uniq[1]]
+            if _is_spd(A1) or not _is_spd(A0):
+                a, b = uniq
+            else:
+                b, a = uniq
+            lam, V, W = axis_eig(reps[a], reps[b])
+            ev[a] = lam
+            ev[b] = np.ones_like(lam)
+        else:
+            raise SolverNotApplicable(
+                f"Axis {i} has {len(uniq)} distinct factor matrices; simultaneous diagonalization "
+                "requires <= 2 distinct matrices per axis.")
+        eigvecs.append(V)
+        lefts.append(W)
+        per_axis_evals.append(ev)
+    del evals_of
+    per_term = [[per_axis_evals[i][ids[k][i]] for i in range(ndim)] for k in range(len(tpmats))]
+    scales = [tp.coefficient for tp in tpmats]
+    shape = tuple(int(tpmats[0].mats[i].shape[0]) for i in range(ndim))
+    return TPMatricesLUFactors(eigvecs, lefts, per_term, scales, shape)
+
+
+class TPMatrices:
+    """Sum of TPMatrix terms (la/tpmatrix.py:279-509)."""
+
+    is_zero = False
+
+    def __init__(self, tpmats):
+        self.tpmats = list(tpmats)
+        self._lu_cache = None
+        self._kron_cache = None
+
+    def __len__(self) -> int:
+        return len(self.tpmats)
+
+    @property
+    def shape(self):
+        return self.tpmats[0].shape if self.tpmats else (0, 0)
+
+    def _apply_array(self, u, precision: str = "f64") -> torch.Tensor:
+        parts = [tp._matmul_array(u, precision) for tp in self.tpmats]
+        return K.lincomb([(1.0, p) for p in parts])
+
+    def __call__(self, u, precision: str = "f64"):
+        return self._apply_array(u, precision)
+
+    __matmul__ = __call__
+
+    def diagonal_or_none(self):
+        tot = None
+        for tp in self.tpmats:
+            d = tp.diagonal_or_none()
+            if d is None:
+                return None
+            tot = d if tot is None else tot + d
+        return tot
+
+    @property
+    def is_diagonal(self) -> bool:
+        return self.diagonal_or_none() is not None
+
+    def scale(self, alpha) -> "TPMatrices":
+        return TPMatrices([tp.scale(alpha) for tp in self.tpmats])
+
+    def __add__(self, other):
+        if isinstance(other, TPMatrix):
+            return TPMatrices([*self.tpmats, other])
+        if isinstance(other, TPMatrices):
+            return TPMatrices([*self.tpmats, *other.tpmats])
+        return NotImplemented
+
+    def __sub__(self, other):
+        return self + other.scale(-1.0)
+
+    def __mul__(self, s):
+        return self.scale(s)
+
+    __rmul__ = __mul__
+
+    def todense_host(self) -> np.ndarray:
+        return tpmats_to_kron(self.tpmats)
+
+    def todense(self) -> torch.Tensor:
+        return torch.tensor(self.todense_host(), device=K.device())
+
+    def to_matrix(self) -> Matrix:
+        return Matrix(self.todense_host())
+
+    def lu_factor(self):
+        """Dispatch (la/tpmatrix.py:386-427); the Fourier wavenumber solver is not built."""
+        if self._lu_cache is not None:
+            return self._lu_cache
+        if all(isinstance(m, Matrix) for tp in self.tpmats for m in tp.mats):
+            res = tpmats_dense_lu_factor(self.tpmats)
+        else:
+            res = tpmats_lu_factor(self.tpmats)
+        self._lu_cache = res
+        return res
+
+    def fd_factor(self) -> TPMatricesLUFactors:
+        """Force fast diagonalisation whatever the factor storage."""
+        if not isinstance(self._lu_cache, TPMatricesLUFactors):
+            self._fd = getattr(self, "_fd", None) or tpmats_lu_factor(self.tpmats)
+            return self._fd
+        return self._lu_cache
+
+    def solve(self, rhs, *, method: str = "auto", kron_method: str = "auto", auto_threshold: int = 100,
+              precision: str = "f64"):
+        """(la/tpmatrix.py:429-509): ``auto`` tries the factored path and falls
+        back to the Kronecker solve on SolverNotApplicable; ``lu`` forces the
+        factored path; ``kron`` the Kronecker solve; ``fd`` (an addition)
+        forces fast diagonalisation."""
+        d = self.diagonal_or_none()
+        if d is not None:
+            return _solve_diagonal(d, rhs)
+        if method not in ("auto", "lu", "kron", "fd"):
+            raise ValueError(f"'{method}' is not a valid TPSolveMethod")
+        if method == "fd":
+            return self.fd_factor().solve(rhs, precision)
+        if method == "kron":
+            return self._kron_solve(rhs)
+        if method == "lu":
+            return self.lu_factor().solve(rhs, precision)
+        try:
+            return self.lu_factor().solve(rhs, precision)
+        except SolverNotApplicable:
+            return self._kron_solve(rhs)
+
+    def _kron_solve(self, rhs):
+        if self._kron_cache is None:
+            shape = tuple(int(m.shape[0]) for m in self.tpmats[0].mats)
+            self._kron_cache = TPMatricesDenseLUFactors(self.todense_host(), shape)
+        return self._kron_cache.solve(rhs)
+
+
+__all__ = ["TPMatrix", "TPMatrices", "TPMatricesLUFactors", "TPMatricesDenseLUFactors", "DiaMatrix",
+           "BaseMatrix", "tpmats_lu_factor", "tpmats_dense_lu_factor", "tpmats_to_kron", "axis_eig"]

@@ -1,0 +1,130 @@
+"""GPU parity of the low-level kernels (through the C ABI) against the oracle."""
+
+import numpy as np
+import pytest
+
+from oracle import spectral as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(a, dev, dtype=None):
+    import torch
+
+    return torch.tensor(np.asarray(a), device=dev, dtype=dtype or torch.float64)
+
+
+@pytest.mark.parametrize("family", ["legendre", "chebyshev"])
+@pytest.mark.parametrize("N", [1, 2, 8, 64, 128, 192, 256])
+def test_gauss_nodes(dev, family, N):
+    from jaxfun_amd import kernels as K
+
+    x, w = K.gauss_nodes(family, N)
+    xr, wr = O.quad(family, N)
+    assert np.allclose(x.cpu().numpy(), xr, atol=1e-15)
+    assert np.allclose(w.cpu().numpy(), wr, rtol=1e-13)
+
+
+@pytest.mark.parametrize("family", ["legendre", "chebyshev"])
+@pytest.mark.parametrize("k", [0, 1, 2, 3])
+def test_vandermonde(dev, family, k):
+    from jaxfun_amd import kernels as K
+
+    x = np.linspace(-1, 1, 37)
+    V = K.vandermonde(family, _t(x, dev), 40, k).cpu().numpy()
+    Vr = O.vandermonde(family, x, 40, k)
+    assert np.allclose(V, Vr, rtol=1e-12, atol=1e-12 * np.abs(Vr).max())
+
+
+SHAPES = [  # (shape, axis, M)
+    ((3, 128, 128, 128), 1, 128),
+    ((2, 128, 128, 128), 3, 128),
+    ((2, 128, 128, 128), 2, 128),
+    ((5, 62, 62), 1, 64),
+    ((7, 62, 62), 2, 62),
+    ((33, 50), 0, 37),
+    ((4, 256, 130), 1, 256),
+    ((3, 96, 5), 1, 200),
+    ((1, 17), 1, 1),
+]
+
+
+@pytest.mark.parametrize("shape,axis,M", SHAPES)
+def test_contract_f64(dev, shape, axis, M):
+    from jaxfun_amd import kernels as K
+
+    rng = np.random.default_rng(1)
+    X = rng.standard_normal(shape)
+    A = rng.standard_normal((M, shape[axis]))
+    Y = K.apply_along(K.AxisOperator(_t(A, dev)), _t(X, dev), axis, "f64").cpu().numpy()
+    Yr = O.apply_along(A, X, axis)
+    assert O.rel_l2(Y, Yr) < 1e-14
+
+
+# ozakiS drops the digit products of level >= S: error ~ (S+1) 2^-8S sqrt(K)
+@pytest.mark.parametrize("prec,tol", [("f32", 2e-6), ("bf16x3", 2e-5), ("bf16x6", 2e-6),
+                                      ("ozaki3", 5e-7), ("ozaki4", 5e-9), ("ozaki5", 5e-11)])
+@pytest.mark.parametrize("shape,axis,M", SHAPES[:6])
+def test_contract_mixed(dev, prec, tol, shape, axis, M):
+    import torch
+
+    from jaxfun_amd import kernels as K
+
+    rng = np.random.default_rng(2)
+    X = rng.standard_normal(shape)
+    A = rng.standard_normal((M, shape[axis]))
+    dt = K.io_dtype(prec)
+    Xd = _t(X, dev, dt)
+    Y = K.apply_along(K.AxisOperator(_t(A, dev)), Xd, axis, prec).to(torch.float64).cpu().numpy()
+    Yr = O.apply_along(A, Xd.to(torch.float64).cpu().numpy(), axis)
+    err = O.rel_l2(Y, Yr)
+    assert err < tol, (prec, err)
+
+
+def test_ozaki_rejects_inexact_sizes(dev):
+    from jaxfun_amd import kernels as K
+
+    A = _t(np.ones((4, 300)), dev)
+    X = _t(np.ones((2, 300)), dev)
+    with pytest.raises(ValueError):
+        K.apply_along(K.AxisOperator(A), X, 1, "ozaki4")
+
+
+def test_band_lu(dev):
+    from jaxfun_amd import kernels as K
+
+    rng = np.random.default_rng(3)
+    n, p, q = 50, 2, 2
+    A = np.zeros((n, n))
+    for d in range(-p, q + 1):
+        A += np.diag(rng.standard_normal(n - abs(d)), d)
+    A += 10 * np.eye(n)
+    band = np.zeros((n, p + q + 1))
+    for i in range(n):
+        for o in range(-p, q + 1):
+            if 0 <= i + o < n:
+                band[i, o + p] = A[i, i + o]
+    lu = K.band_lu_factor(_t(band, dev), p, q)
+    B = rng.standard_normal((3, n, 7))
+    X = K.band_lu_solve(lu, p, q, _t(B, dev).contiguous(), 1).cpu().numpy()
+    Xr = np.stack([O.banded_lu_solve(A, B[i], p, q) for i in range(3)])
+    assert np.allclose(X, Xr, rtol=1e-12, atol=1e-12)
+    band[0, p] = 0.0
+    with pytest.raises(ValueError):
+        K.band_lu_factor(_t(band, dev), p, q)
+
+
+def test_elementwise(dev):
+    import torch
+
+    from jaxfun_amd import kernels as K
+
+    rng = np.random.default_rng(4)
+    a, b, c = (rng.standard_normal((5, 6)) for _ in range(3))
+    y = K.lincomb([(2.0, _t(a, dev)), (-0.5, _t(b, dev))]).cpu().numpy()
+    assert np.allclose(y, 2 * a - 0.5 * b)
+    D = rng.standard_normal(6) + 3
+    z = K.div_bcast(_t(a, dev), _t(D, dev)).cpu().numpy()
+    assert np.allclose(z, a / D)
+    f = K.cast(_t(c, dev), torch.float32)
+    assert f.dtype == torch.float32 and np.allclose(f.cpu().numpy(), c.astype(np.float32))
