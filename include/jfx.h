@@ -118,6 +118,29 @@ int jfx_lincomb(int dtype, int nin, const void* const* xs, const double* coeffs,
  * the reference runs in one dtype). */
 int jfx_cast(int direction, const void* x, void* y, long long n, hipStream_t stream);
 
+/* ---- a12 (batched): ragged batch of 2D Legendre-Dirichlet Helmholtz solves
+ * (alpha M (x) M - K (x) M - M (x) K) U = F in one persistent launch.
+ * items: device array of nitems work items (jfx_helm_item_size() bytes each:
+ *   {long long foff, voff; double alpha; int n, p, q, a, b, AP, loff, pad}),
+ *   one per (solve, row parity p, column parity q), processed in array order
+ *   from the device counter `counter` (reset by the call itself).
+ * Vd/VTd: FP64 eigenvector tables V_p / V_p^T (AP x AP, zero padded) of the
+ *   parity blocks of K_p V = M_p V diag(lam); Vs/VTs: the same as 3 bf16
+ *   float-split slices each ([3][AP][AP] per table at 3*voff).  lam at loff.
+ * mode 0: FP64 MFMA (fp64 I/O); mode 1: 3-term BF16 split MFMA with FP32
+ *   accumulation/storage and FP64 tridiagonal solves (io_f32 selects float
+ *   or double F/U).  nwaves 4 or 8; scratch holds
+ *   nblocks * jfx_helm_scratch_doubles_per_block() doubles.
+ * Replaces TPMatrices.solve -> tpmats_lu_factor / TPMatricesLUFactors.solve
+ * (la/tpmatrix.py:429-587, 1089-1233) applied once per solve, and the banded
+ * DiaMatrix solve along an axis (la/diamatrix.py:592-770). */
+int jfx_helm_item_size(void);
+int jfx_helm_scratch_doubles_per_block(void);
+int jfx_helm_ragged(int mode, int io_f32, int nwaves, const void* items, int nitems, int* counter,
+                    const double* Vd, const double* VTd, const unsigned short* Vs,
+                    const unsigned short* VTs, const double* lam, const void* F, void* U,
+                    double* scratch, int nblocks, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,242 @@
+"""Ragged batches of 2D Legendre-Dirichlet Helmholtz solves on one GPU.
+
+``HelmholtzBatch(Ns, alphas)`` solves, for every b,
+
+    (alpha_b M (x) M - K (x) M - M (x) K) U_b = F_b,     U_b, F_b of shape (N_b-2, N_b-2)
+
+which is ``inner(v*(alpha*u - Div(Grad(u))))`` on ``TensorProduct(D, D)`` with
+``D = FunctionSpace(N, Legendre, bcs=Dirichlet 0)``, solved by
+``TPMatrices.solve`` in the reference (la/tpmatrix.py:429-587).  All solves run
+in one persistent kernel launch (``jfx_helm_ragged``) fed by a device work
+counter; the launch can be captured in a HIP graph and replayed.
+
+Set-up (once per distinct N, as the reference's ``lu_factor`` caches): the
+parity blocks of (K, M) are diagonalised on the host in FP64 with
+``scipy.linalg.eigh`` (la/tpmatrix.py:1173), uploaded, and split into bf16
+slices on the device.  Work items (solve, row parity, column parity) are
+ordered largest-first so the tail of the queue holds the small items.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import scipy.linalg as sla
+import torch
+
+from . import _lib
+from . import kernels as K
+
+ITEM_DTYPE = np.dtype([("foff", "<i8"), ("voff", "<i8"), ("alpha", "<f8"), ("n", "<i4"), ("p", "<i4"),
+                       ("q", "<i4"), ("a", "<i4"), ("b", "<i4"), ("AP", "<i4"), ("loff", "<i4"),
+                       ("pad", "<i4")])
+
+MODES = {"fp64": 0, "mixed": 1}
+
+
+def _parity_blocks(N: int):
+    """Host FP64 eigenpairs of the parity blocks of the Legendre-Dirichlet (K, M)."""
+    n = N - 2
+    k = np.arange(N)
+    h = 2.0 / (2 * k + 1)
+    j = np.arange(n)
+    # S diag(h) S^T with phi_j = P_j - P_{j+2}: bands 0 and +-2
+    M = np.diag(h[:n] + h[2:n + 2]) - np.diag(h[2:n], 2) - np.diag(h[2:n], -2)
+    Kd = -(4.0 * j + 6.0)
+    out = []
+    for p in (0, 1):
+        Mp = M[p::2, p::2]
+        Kp = np.diag(Kd[p::2])
+        if Mp.shape[0] == 0:
+            out.append((np.zeros(0), np.zeros((0, 0))))
+            continue
+        lam, V = sla.eigh(Kp, Mp)
+        out.append((lam, V))
+    return out
+
+
+class HelmholtzBatch:
+    """Ragged batch of Legendre-Dirichlet Helmholtz solves (see module doc).
+
+    Args:
+        Ns: per-solve number of Legendre modes N_b (unknowns (N_b-2)^2), 4 <= N_b <= 258.
+        alphas: per-solve Helmholtz shifts alpha_b >= 0.
+        mode: ``"fp64"`` (FP64 MFMA) or ``"mixed"`` (3-term bf16 split MFMA,
+            FP32 accumulation, FP64 tridiagonal solves).
+        io_dtype: dtype of F/U (``mixed`` defaults to float32, ``fp64`` to float64).
+        nwaves, blocks_per_cu: launch shape (see :func:`autotune`).
+    """
+
+    def __init__(self, Ns, alphas, mode: str = "mixed", io_dtype=None, nwaves: int = 4,
+                 blocks_per_cu: int = 2):
+        self.dev = K.device()
+        self.Ns = np.asarray(Ns, dtype=np.int64)
+        self.alphas = np.asarray(alphas, dtype=np.float64)
+        if self.Ns.shape != self.alphas.shape or self.Ns.ndim != 1:
+            raise ValueError("Ns and alphas must be 1D arrays of equal length")
+        if np.any(self.Ns < 4) or np.any(self.Ns > 258):
+            raise ValueError("every N must be in [4, 258] (parity blocks of at most 128 unknowns)")
+        if np.any(self.alphas < 0) or not np.all(np.isfinite(self.alphas)):
+            raise ValueError("alphas must be finite and >= 0")
+        if mode not in MODES:
+            raise ValueError(f"mode must be one of {list(MODES)}")
+        self.mode = mode
+        self.io_dtype = io_dtype or (torch.float32 if mode == "mixed" else torch.float64)
+        if mode == "fp64" and self.io_dtype != torch.float64:
+            raise ValueError("fp64 mode needs float64 I/O")
+        self.nwaves = nwaves
+        self.blocks_per_cu = blocks_per_cu
+        ns = self.Ns - 2
+        self.sizes = ns * ns
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+        self.total = int(self.offsets[-1])
+        self._build_tables()
+        self._build_items()
+        cus = torch.cuda.get_device_properties(self.dev).multi_processor_count
+        self.nblocks = int(cus * blocks_per_cu)
+        per = _lib.load().jfx_helm_scratch_doubles_per_block()
+        self.scratch = torch.empty(self.nblocks * per, dtype=torch.float64, device=self.dev)
+        self.counter = torch.zeros(1, dtype=torch.int32, device=self.dev)
+        self._graph = None
+
+    # ------------------------------------------------------------------ set-up
+    def _build_tables(self):
+        uniq = sorted(set(int(N) for N in self.Ns))
+        voffs, loffs = {}, {}
+        Vs, VTs, lams = [], [], []
+        vcur = lcur = 0
+        for N in uniq:
+            for p, (lam, V) in enumerate(_parity_blocks(N)):
+                a = lam.shape[0]
+                AP = max(32, ((a + 31) // 32) * 32)
+                Vp = np.zeros((AP, AP))
+                Vp[:a, :a] = V
+                voffs[(N, p)] = (vcur, AP)
+                loffs[(N, p)] = lcur
+                Vs.append(Vp.ravel())
+                VTs.append(Vp.T.copy().ravel())
+                lams.append(lam)
+                vcur += AP * AP
+                lcur += a
+        self._voffs, self._loffs = voffs, loffs
+        self.Vd = torch.tensor(np.concatenate(Vs), device=self.dev)
+        self.VTd = torch.tensor(np.concatenate(VTs), device=self.dev)
+        self.lam = torch.tensor(np.concatenate(lams), device=self.dev)
+        self.table_bytes = (self.Vd.numel() + self.VTd.numel()) * 8
+        if self.mode == "mixed":
+            self.Vs = torch.empty(3 * vcur, dtype=torch.int16, device=self.dev)
+            self.VTs = torch.empty(3 * vcur, dtype=torch.int16, device=self.dev)
+            for (N, p), (off, AP) in voffs.items():
+                for src, dst in ((self.Vd, self.Vs), (self.VTd, self.VTs)):
+                    _lib.call("jfx_split_matrix", K._p(src[off:off + AP * AP]), AP, AP, 0, 3,
+                              K._p(dst[3 * off:3 * off + 3 * AP * AP]), K._p(None), K._s())
+        else:
+            self.Vs = self.VTs = None
+
+    def _build_items(self):
+        items = []
+        for bi, (N, alpha) in enumerate(zip(self.Ns, self.alphas)):
+            n = int(N) - 2
+            for p in (0, 1):
+                a = (n - p + 1) // 2
+                if a == 0:
+                    continue
+                off, AP = self._voffs[(int(N), p)]
+                for q in (0, 1):
+                    b = (n - q + 1) // 2
+                    if b == 0:
+                        continue
+                    items.append((int(self.offsets[bi]), off, float(alpha), n, p, q, a, b, AP,
+                                  self._loffs[(int(N), p)], 0))
+        arr = np.array(items, dtype=ITEM_DTYPE)
+        cost = arr["a"].astype(np.float64) ** 2 * arr["b"] + 40.0 * arr["a"] * arr["b"]
+        order = np.argsort(-cost, kind="stable")
+        arr = arr[order]
+        if ITEM_DTYPE.itemsize != _lib.load().jfx_helm_item_size():
+            raise RuntimeError("HelmItem layout mismatch between Python and libjfx")
+        self.items_host = arr
+        self.items = torch.tensor(arr.view(np.uint8), device=self.dev)
+        self.nitems = len(arr)
+
+    # ------------------------------------------------------------------ packing
+    def pack(self, arrays) -> torch.Tensor:
+        """Concatenate per-solve (n_b, n_b) arrays into the packed device layout."""
+        flat = np.concatenate([np.asarray(a, dtype=np.float64).ravel() for a in arrays])
+        if flat.size != self.total:
+            raise ValueError("packed size mismatch")
+        t = torch.tensor(flat, device=self.dev)
+        return K.cast(t, self.io_dtype)
+
+    def unpack(self, packed: torch.Tensor) -> list[np.ndarray]:
+        h = packed.detach().to(torch.float64).cpu().numpy()
+        return [h[self.offsets[i]:self.offsets[i + 1]].reshape(int(self.Ns[i]) - 2, int(self.Ns[i]) - 2)
+                for i in range(len(self.Ns))]
+
+    # ------------------------------------------------------------------ solve
+    def _launch(self, F: torch.Tensor, U: torch.Tensor) -> None:
+        _lib.call("jfx_helm_ragged", MODES[self.mode], 1 if self.io_dtype == torch.float32 else 0,
+                  self.nwaves, K._p(self.items), self.nitems, K._p(self.counter), K._p(self.Vd),
+                  K._p(self.VTd), K._p(self.Vs), K._p(self.VTs), K._p(self.lam), K._p(F), K._p(U),
+                  K._p(self.scratch), self.nblocks, K._s())
+
+    def solve(self, F: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        K._req(F, self.io_dtype, "F")
+        if F.numel() != self.total:
+            raise ValueError(f"F has {F.numel()} entries, the batch needs {self.total}")
+        U = torch.empty_like(F) if out is None else out
+        self._launch(F, U)
+        return U
+
+    def capture(self, F: torch.Tensor, U: torch.Tensor) -> None:
+        """Capture the whole batch (counter reset + persistent kernel) in a HIP graph."""
+        K._req(F, self.io_dtype, "F")
+        K._req(U, self.io_dtype, "U")
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            self._launch(F, U)  # warm-up outside capture
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self._launch(F, U)
+        self._graph = g
+
+    def replay(self) -> None:
+        if self._graph is None:
+            raise RuntimeError("call capture() first")
+        self._graph.replay()
+
+    # ------------------------------------------------------------------ accounting
+    def algorithmic_flops(self) -> float:
+        """Sum of 4 n^3 + 14 n^2 per solve (partial diagonalisation count)."""
+        n = (self.Ns - 2).astype(np.float64)
+        return float(np.sum(4 * n**3 + 14 * n**2))
+
+    def algorithmic_bytes(self) -> float:
+        """F read + U written once each."""
+        return float(2 * self.total * torch.tensor([], dtype=self.io_dtype).element_size())
+
+
+def autotune(Ns, alphas, mode: str = "mixed", candidates=((4, 1), (4, 2), (8, 1), (8, 2), (4, 3)),
+             reps: int = 3) -> tuple[int, int]:
+    """Time each (waves per workgroup, workgroups per CU) launch shape on the
+    actual ragged batch and return the fastest."""
+    best, best_t = None, float("inf")
+    F = None
+    for nw, bpc in candidates:
+        hb = HelmholtzBatch(Ns, alphas, mode=mode, nwaves=nw, blocks_per_cu=bpc)
+        if F is None:
+            F = torch.ones(hb.total, dtype=hb.io_dtype, device=hb.dev)
+        U = torch.empty_like(F)
+        hb.solve(F, U)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            hb.solve(F, U)
+        e1.record()
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) / reps
+        if t < best_t:
+            best, best_t = (nw, bpc), t
+    return best

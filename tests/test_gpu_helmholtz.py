@@ -1,0 +1,99 @@
+"""Ragged Helmholtz batch (persistent kernel) vs the oracle."""
+
+import numpy as np
+import pytest
+
+from oracle import spectral as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(Ns, seed=11):
+    from jaxfun_amd import workloads as W
+
+    rng = np.random.default_rng(seed)
+    alphas = 10.0 ** rng.uniform(-2, 4, size=len(Ns))
+    loads = [W.dirichlet_load(W.forcing_coefficients(rng, int(N)), W.legendre_norms(int(N))) for N in Ns]
+    return np.asarray(Ns), alphas, loads
+
+
+def _ref(N, alpha, F):
+    if N <= 40:
+        sp1 = O.Space1D("legendre", N, True)
+        M, K = sp1.mass(), O.legendre_dirichlet_stiffness(N)
+        return O.kron_solve([(alpha, [M, M]), (-1.0, [K, M]), (-1.0, [M, K])], F)
+    return O.helmholtz_partial_fd(N, alpha, F)
+
+
+EDGE_NS = [4, 5, 6, 7, 33, 34, 64, 97, 130, 200, 255, 256, 257, 258]
+
+
+def _residual(N, alpha, U, F):
+    sp1 = O.Space1D("legendre", N, True)
+    M, K = sp1.mass(), O.legendre_dirichlet_stiffness(N)
+    R = alpha * M @ U @ M - K @ U @ M - M @ U @ K - F
+    return np.linalg.norm(R) / np.linalg.norm(F)
+
+
+# two FP64 implementations of the same solve differ by ~cond * eps (cond ~ N^4 / alpha);
+# the FP64 residual is bounded by 10x the oracle's own residual
+@pytest.mark.parametrize("mode,tol,rtol", [("fp64", 1e-10, None), ("mixed", 1e-6, 1e-6)])
+def test_ragged_vs_oracle(dev, mode, tol, rtol):
+    import torch
+
+    from jaxfun_amd.helmholtz import HelmholtzBatch
+
+    Ns, alphas, loads = _case(EDGE_NS)
+    hb = HelmholtzBatch(Ns, alphas, mode=mode)
+    U = hb.unpack(hb.solve(hb.pack(loads)))
+    torch.cuda.synchronize()
+    for N, a, F, Ub in zip(Ns, alphas, loads, U):
+        Ur = _ref(int(N), a, F)
+        err = O.rel_l2(Ub, Ur)
+        assert err < tol, (mode, N, a, err)
+        res = _residual(int(N), a, Ub, F)
+        bound = rtol if rtol is not None else 10 * max(_residual(int(N), a, Ur, F), 1e-15)
+        assert res < bound, (mode, N, a, res, bound)
+
+
+@pytest.mark.parametrize("nwaves,bpc", [(4, 1), (8, 2), (4, 3)])
+def test_launch_shapes_agree(dev, nwaves, bpc):
+    from jaxfun_amd.helmholtz import HelmholtzBatch
+
+    Ns, alphas, loads = _case([32, 77, 128, 256, 190], seed=12)
+    base = HelmholtzBatch(Ns, alphas, mode="mixed", nwaves=4, blocks_per_cu=2)
+    hb = HelmholtzBatch(Ns, alphas, mode="mixed", nwaves=nwaves, blocks_per_cu=bpc)
+    F = base.pack(loads)
+    a = base.solve(F).cpu().numpy()
+    b = hb.solve(F).cpu().numpy()
+    assert np.array_equal(a, b)  # each work item is computed identically
+
+
+def test_graph_replay(dev):
+    import torch
+
+    from jaxfun_amd.helmholtz import HelmholtzBatch
+
+    Ns, alphas, loads = _case([40, 100, 256, 63], seed=13)
+    hb = HelmholtzBatch(Ns, alphas, mode="mixed")
+    F = hb.pack(loads)
+    U = torch.empty_like(F)
+    eager = hb.solve(F).cpu().numpy()
+    hb.capture(F, U)
+    for _ in range(3):
+        U.fill_(0)
+        hb.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(U.cpu().numpy(), eager)
+        hb.solve(F)  # eager work between replays
+
+
+def test_rejects_bad_sizes(dev):
+    from jaxfun_amd.helmholtz import HelmholtzBatch
+
+    with pytest.raises(ValueError):
+        HelmholtzBatch([300], [1.0])
+    with pytest.raises(ValueError):
+        HelmholtzBatch([3], [1.0])
+    with pytest.raises(ValueError):
+        HelmholtzBatch([40], [-1.0])
