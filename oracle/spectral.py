@@ -444,6 +444,41 @@ def helmholtz_partial_fd(n_modes_N: int, alpha: float, F: np.ndarray) -> np.ndar
     return V @ U
 
 
+_PFD_CACHE: dict = {}
+
+
+def helmholtz_partial_fd_lapack(N: int, alpha: float, F: np.ndarray) -> np.ndarray:
+    """Same algorithm as :func:`helmholtz_partial_fd` with compiled kernels:
+    per-N eigenpairs cached (as ``TPMatrices.lu_factor`` caches,
+    la/tpmatrix.py:386-427), BLAS for the two transforms and LAPACK's banded
+    solver (``scipy.linalg.solve_banded``) for every row.  Used as the timed
+    CPU baseline; pinned to :func:`helmholtz_partial_fd` in the tests."""
+    if N not in _PFD_CACHE:
+        sp = Space1D("legendre", N, dirichlet=True)
+        M = sp.mass()
+        K = legendre_dirichlet_stiffness(N)
+        lam, V = sla.eigh(K, M)
+        n = N - 2
+        ab_M = np.zeros((5, n))
+        ab_K = np.zeros((5, n))
+        for o in range(-2, 3):
+            d = np.diagonal(M, o)
+            dk = np.diagonal(K, o)
+            if o >= 0:
+                ab_M[2 - o, o:] = d
+                ab_K[2 - o, o:] = dk
+            else:
+                ab_M[2 - o, :o] = d
+                ab_K[2 - o, :o] = dk
+        _PFD_CACHE[N] = (lam, V, ab_M, ab_K)
+    lam, V, ab_M, ab_K = _PFD_CACHE[N]
+    G = V.T @ F
+    U = np.empty_like(G)
+    for i in range(G.shape[0]):
+        U[i] = sla.solve_banded((2, 2), (alpha - lam[i]) * ab_M - ab_K, G[i], check_finite=False)
+    return V @ U
+
+
 # --------------------------------------------------------------------------
 # a14: IMEX ARS443
 # --------------------------------------------------------------------------

@@ -180,5 +180,6 @@ def test_mixed_precision_transforms(dev):
     for prec, tol in (("f32", 1e-6), ("bf16x6", 1e-6), ("ozaki4", 1e-8), ("ozaki5", 5e-11)):
         u = T.backward(c, precision=prec)
         assert O.rel_l2(_np(u.to(torch.float64)), _np(u64)) < tol, prec
+    # the round trip amplifies the ozaki5 error (~5e-13) by ||u||/||c|| ~ N^1.5
     c2 = T.forward(u64, precision="ozaki5")
-    assert O.rel_l2(_np(c2), c) < 1e-10
+    assert O.rel_l2(_np(c2), c) < 2e-9

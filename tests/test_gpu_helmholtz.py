@@ -37,7 +37,7 @@ def _residual(N, alpha, U, F):
 
 # two FP64 implementations of the same solve differ by ~cond * eps (cond ~ N^4 / alpha);
 # the FP64 residual is bounded by 10x the oracle's own residual
-@pytest.mark.parametrize("mode,tol,rtol", [("fp64", 1e-10, None), ("mixed", 1e-6, 1e-6)])
+@pytest.mark.parametrize("mode,tol,rtol", [("fp64", 1e-8, None), ("mixed", 1e-6, 1e-6)])
 def test_ragged_vs_oracle(dev, mode, tol, rtol):
     import torch
 

@@ -63,7 +63,7 @@ def test_contract_f64(dev, shape, axis, M):
 
 # ozakiS drops the digit products of level >= S: error ~ (S+1) 2^-8S sqrt(K)
 @pytest.mark.parametrize("prec,tol", [("f32", 2e-6), ("bf16x3", 2e-5), ("bf16x6", 2e-6),
-                                      ("ozaki3", 5e-7), ("ozaki4", 5e-9), ("ozaki5", 5e-11)])
+                                      ("ozaki3", 1e-6), ("ozaki4", 5e-9), ("ozaki5", 5e-11)])
 @pytest.mark.parametrize("shape,axis,M", SHAPES[:6])
 def test_contract_mixed(dev, prec, tol, shape, axis, M):
     import torch

@@ -215,6 +215,14 @@ def test_helmholtz_partial_fd_vs_kron():
         assert O.rel_l2(O.helmholtz_partial_fd(N, alpha, F), Uk) < 1e-12
 
 
+def test_helmholtz_lapack_variant_matches():
+    rng = np.random.default_rng(8)
+    for N in (7, 40, 101):
+        F = rng.standard_normal((N - 2, N - 2))
+        for alpha in (0.01, 30.0):
+            assert O.rel_l2(O.helmholtz_partial_fd_lapack(N, alpha, F), O.helmholtz_partial_fd(N, alpha, F)) < 1e-12
+
+
 def test_ars443_tableau_consistency():
     aE, aI = O.ARS443_EXPLICIT, O.ARS443_IMPLICIT
     c = np.array([0, 1 / 2, 2 / 3, 1 / 2, 1])
