@@ -113,6 +113,13 @@ int jfx_mul_bcast_f64(const double* X, const double* C, double* Y, long long tot
 int jfx_lincomb(int dtype, int nin, const void* const* xs, const double* coeffs, void* y, long long n,
                 int accumulate, hipStream_t stream);
 
+/* out = sum_t coeffs[t] * As[t] * Bs[t] pointwise (nterms <= 4; As, Bs,
+ * coeffs are HOST arrays of device pointers / scalars; dtype 0 f64, 1 f32).
+ * The quadratic nonlinear terms evaluated on the padded mesh by the IMEX
+ * integrators' nonlinear evaluator (integrators/nonlinear.py:89-133). */
+int jfx_sum_products(int dtype, int nterms, const void* const* As, const void* const* Bs,
+                     const double* coeffs, void* out, long long n, hipStream_t stream);
+
 /* direction 0: f64 -> f32 (round to nearest), 1: f32 -> f64.  The dtype
  * boundary of the mixed-precision transforms (no reference counterpart:
  * the reference runs in one dtype). */

@@ -44,6 +44,7 @@ SIGNATURES: dict[str, tuple] = {
     "jfx_mul_bcast_f64": (c_int, [c_vp, c_vp, c_vp, c_ll, c_ll, c_vp]),
     "jfx_lincomb": (c_int, [c_int, c_int, c_vp, c_vp, c_vp, c_ll, c_int, c_vp]),
     "jfx_cast": (c_int, [c_int, c_vp, c_vp, c_ll, c_vp]),
+    "jfx_sum_products": (c_int, [c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_ll, c_vp]),
     "jfx_helm_item_size": (c_int, []),
     "jfx_helm_scratch_doubles_per_block": (c_int, []),
     "jfx_helm_ragged": (c_int, [c_int, c_int, c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,

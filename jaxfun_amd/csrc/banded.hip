@@ -77,6 +77,24 @@ __global__ void mul_bcast_kernel(const T* __restrict__ X, const T* __restrict__ 
   for (; i < total; i += stride) Y[i] = X[i] * C[i % size];
 }
 
+// out = sum_t c_t * A_t * B_t  (pointwise, nterms <= 4)
+template <typename T>
+struct SumProd {
+  const T* a[4];
+  const T* b[4];
+  double c[4];
+};
+template <typename T>
+__global__ void sum_products_kernel(SumProd<T> sp, int nterms, T* __restrict__ out, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    T s = 0;
+    for (int t = 0; t < nterms; ++t) s += (T)sp.c[t] * sp.a[t][i] * sp.b[t][i];
+    out[i] = s;
+  }
+}
+
 // y = sum_j c_j x_j (n <= 8 inputs), optionally accumulating into y.
 template <typename T>
 struct LinComb {
@@ -158,6 +176,36 @@ int jfx_mul_bcast_f64(const double* X, const double* C, double* Y, long long tot
   if (!X || !C || !Y || size <= 0 || total % size) return set_error("mul_bcast: bad arguments"), JFX_EARG;
   hipLaunchKernelGGL(mul_bcast_kernel<double>, dim3(grid_for(total, 256)), dim3(256), 0, stream, X, C, Y,
                      total, size);
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
+}
+
+// out = sum_t coeffs[t] * As[t] * Bs[t]; As, Bs, coeffs are HOST arrays (nterms <= 4)
+int jfx_sum_products(int dtype, int nterms, const void* const* As, const void* const* Bs,
+                     const double* coeffs, void* out, long long n, hipStream_t stream) {
+  if (nterms < 1 || nterms > 4 || !As || !Bs || !coeffs || !out) return set_error("sum_products: bad arguments"), JFX_EARG;
+  if (n == 0) return JFX_OK;
+  if (dtype == 0) {
+    SumProd<double> sp{};
+    for (int t = 0; t < nterms; ++t) {
+      sp.a[t] = (const double*)As[t];
+      sp.b[t] = (const double*)Bs[t];
+      sp.c[t] = coeffs[t];
+    }
+    hipLaunchKernelGGL(sum_products_kernel<double>, dim3(grid_for(n, 256)), dim3(256), 0, stream, sp, nterms,
+                       (double*)out, n);
+  } else if (dtype == 1) {
+    SumProd<float> sp{};
+    for (int t = 0; t < nterms; ++t) {
+      sp.a[t] = (const float*)As[t];
+      sp.b[t] = (const float*)Bs[t];
+      sp.c[t] = coeffs[t];
+    }
+    hipLaunchKernelGGL(sum_products_kernel<float>, dim3(grid_for(n, 256)), dim3(256), 0, stream, sp, nterms,
+                       (float*)out, n);
+  } else {
+    return set_error("sum_products: bad dtype"), JFX_EARG;
+  }
   JFX_LAUNCH_CHECK();
   return JFX_OK;
 }

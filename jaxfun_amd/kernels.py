@@ -108,6 +108,22 @@ def lincomb(terms, out: torch.Tensor | None = None, accumulate: bool = False) ->
     return out
 
 
+def sum_products(terms, out: torch.Tensor | None = None) -> torch.Tensor:
+    """``out = sum_t c_t * a_t * b_t`` pointwise for up to 4 (c, a, b) terms."""
+    terms = [(float(c), a, b) for c, a, b in terms]
+    ref = terms[0][1]
+    out = torch.empty_like(ref) if out is None else out
+    for _, a, b in terms:
+        _req(a, out.dtype, "a")
+        _req(b, out.dtype, "b")
+    A = (ctypes.c_void_p * 4)(*[a.data_ptr() for _, a, _ in terms])
+    B = (ctypes.c_void_p * 4)(*[b.data_ptr() for _, _, b in terms])
+    C = (ctypes.c_double * 4)(*[c for c, _, _ in terms])
+    dt = {torch.float64: 0, torch.float32: 1}[out.dtype]
+    _lib.call("jfx_sum_products", dt, len(terms), A, B, C, _p(out), out.numel(), _s())
+    return out
+
+
 def div_bcast(X: torch.Tensor, D: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
     """``out[b, ...] = X[b, ...] / D[...]`` with D broadcast over leading dims."""
     _req(X, X.dtype, "X")
