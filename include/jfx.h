@@ -148,6 +148,41 @@ int jfx_helm_ragged(int mode, int io_f32, int nwaves, const void* items, int nit
                     const unsigned short* VTs, const double* lam, const void* F, void* U,
                     double* scratch, int nblocks, hipStream_t stream);
 
+/* ---- learned preconditioner (config 4): no reference counterpart; the
+ * nearest analogue is the Linear+activation MLP of pinns/module.py:335-380.
+ * C (M x N) = alpha op(A) op(B) + beta C (+ bias[N]) (tanh if act = 1);
+ * op(A) = A (M x K, lda) if ta = 0, else A^T of a stored (K x M, lda);
+ * op(B) = B (K x N, ldb) if tb = 0, else B^T of a stored (N x K, ldb).
+ * Operands are rounded to bf16, products accumulate in FP32 on MFMA. */
+int jfx_gemm_bf16(int ta, int tb, int M, int N, int K, float alpha, const float* A, int lda, const float* B,
+                  int ldb, float beta, float* C, int ldc, const float* bias, int act, hipStream_t stream);
+/* dZ = dY (1 - H^2) */
+int jfx_tanh_backward(const float* dY, const float* H, float* dZ, long long n, hipStream_t stream);
+/* out[j] = sum_i X[i, j] */
+int jfx_colsum(const float* X, float* out, int rows, int cols, hipStream_t stream);
+/* Adam step `step` (>= 1) with bias correction. */
+int jfx_adam(float* p, const float* g, float* m, float* v, long long n, float lr, float b1, float b2, float eps,
+             int step, hipStream_t stream);
+
+/* ---- batched PCG (one SPD system per batch entry, FP64; the operator and
+ * preconditioner applications are contractions above).  The reference has
+ * no PCG; the operator is TensorMatrix's matvec (la/tensormatrix.py:54-56).
+ * out[b] = <x_b, y_b> */
+int jfx_batched_dot(const double* x, const double* y, double* out, int batch, long long n, hipStream_t stream);
+/* x_b += (num_b/den_b) p_b, r_b -= (num_b/den_b) q_b for active b */
+int jfx_pcg_update_xr(double* x, double* r, const double* p, const double* q, const double* num,
+                      const double* den, const int* active, int batch, long long n, hipStream_t stream);
+/* p_b = z_b + (num_b/den_b) p_b for active b */
+int jfx_pcg_update_p(double* p, const double* z, const double* num, const double* den, const int* active,
+                     int batch, long long n, hipStream_t stream);
+/* active_b &= rr_b > tol2 bb_b, counting iters_b for systems active on entry when count != 0 */
+int jfx_pcg_check(const double* rr, const double* bb, double tol2, int* active, int* iters, int batch, int count,
+                  hipStream_t stream);
+/* y_b = s_b x_b */
+int jfx_scale_batch(const double* x, const double* s, double* y, int batch, long long n, hipStream_t stream);
+/* y = exp(scale x): the coefficient field k = exp(g) at the quadrature mesh */
+int jfx_exp(const double* x, double* y, double scale, long long n, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -49,6 +49,18 @@ SIGNATURES: dict[str, tuple] = {
     "jfx_helm_scratch_doubles_per_block": (c_int, []),
     "jfx_helm_ragged": (c_int, [c_int, c_int, c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                                 c_vp, c_vp, c_int, c_vp]),
+    "jfx_gemm_bf16": (c_int, [c_int, c_int, c_int, c_int, c_int, ctypes.c_float, c_vp, c_int, c_vp, c_int,
+                              ctypes.c_float, c_vp, c_int, c_vp, c_int, c_vp]),
+    "jfx_tanh_backward": (c_int, [c_vp, c_vp, c_vp, c_ll, c_vp]),
+    "jfx_colsum": (c_int, [c_vp, c_vp, c_int, c_int, c_vp]),
+    "jfx_adam": (c_int, [c_vp, c_vp, c_vp, c_vp, c_ll, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                         ctypes.c_float, c_int, c_vp]),
+    "jfx_batched_dot": (c_int, [c_vp, c_vp, c_vp, c_int, c_ll, c_vp]),
+    "jfx_pcg_update_xr": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_ll, c_vp]),
+    "jfx_pcg_update_p": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_ll, c_vp]),
+    "jfx_pcg_check": (c_int, [c_vp, c_vp, c_dbl, c_vp, c_vp, c_int, c_int, c_vp]),
+    "jfx_scale_batch": (c_int, [c_vp, c_vp, c_vp, c_int, c_ll, c_vp]),
+    "jfx_exp": (c_int, [c_vp, c_vp, c_dbl, c_ll, c_vp]),
     "jfx_scale_transpose": (c_int, [c_vp, c_int, c_int, c_vp, c_vp, c_dbl, c_vp, c_vp]),
 }
 
