@@ -152,6 +152,99 @@ def bench_poisson_c0(reps: int = 50):
                 batched_4096_solves_per_s=4096 / bt)
 
 
+def bench_burgers_c3(steps: int = 1000, batch: int = 256, N: int = 128):
+    """Config 3: IMEX ARS443 Burgers, Chebyshev-Dirichlet N x N, 3/2 padding,
+    mixed precision vs FP64, drift after `steps` steps."""
+    import sympy as sp
+    import torch
+
+    from jaxfun_amd import Chebyshev, Div, FunctionSpace, Grad, TensorProduct, TestFunction, TrialFunction
+    from jaxfun_amd import workloads as W
+    from jaxfun_amd.galerkin.arguments import t
+    from jaxfun_amd.integrators import ARS443, IMEXRungeKutta
+
+    D = FunctionSpace(N, Chebyshev, {"left": {"D": 0}, "right": {"D": 0}})
+    T = TensorProduct(D, D)
+    v = TestFunction(T)
+    u = TrialFunction(T, transient=True)
+    x, y = T.system.base_scalars()
+    nu, dt = 1e-2, 1e-3
+    weak = v * (u.diff(t) + u * (u.diff(x) + u.diff(y)) - nu * Div(Grad(u)))
+    u0f, _, _ = W.burgers_c3(N, batch)
+    X, Y = [m.cpu().numpy() for m in T.mesh()]
+    U0 = T.forward(torch.tensor(np.stack([u0f(b, X, Y) for b in range(batch)]), device="cuda"))
+    out = {}
+    res = {}
+    for prec in ("f64", "mixed"):
+        integ = IMEXRungeKutta(weak, tableau=ARS443, initial=sp.S.Zero, precision=prec)
+        U = U0 if prec == "f64" else U0.to(torch.float32)
+        U = integ.step(U, dt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps - 1):
+            U = integ.step(U, dt)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        res[prec] = U.to(torch.float64)
+        out[prec] = dict(s_per_step=el / (steps - 1), ic_steps_per_s=batch * (steps - 1) / el,
+                         finite=bool(torch.isfinite(U).all()))
+    out["drift_rel_l2_mixed_vs_f64"] = float(torch.linalg.vector_norm(res["mixed"] - res["f64"]) /
+                                             torch.linalg.vector_norm(res["f64"]))
+    out["config"] = dict(N=N, batch=batch, steps=steps, dt=dt, nu=nu, padding="3/2")
+    return out
+
+
+def bench_learned_pcg_c4(ntrain: int = 20000, neval: int = 1024, chunk: int = 1024, train_steps: int = 1500):
+    """Config 4: variable-coefficient Poisson 64^2, plain PCG vs learned PCG."""
+    import torch
+
+    from jaxfun_amd import learned as Lm
+    from jaxfun_amd import workloads as W
+
+    op = Lm.VarCoeffPoisson2D(64, 16)
+
+    def gen(seeds):
+        G, Fh = zip(*[W.varcoeff_c4(int(s)) for s in seeds])
+        return torch.tensor(np.stack(G), device="cuda"), torch.tensor(np.stack(Fh), device="cuda")
+
+    t0 = time.perf_counter()
+    gh, bs, us, it_plain_train = [], [], [], []
+    for s0 in range(10000, 10000 + ntrain, chunk):
+        g, fh = gen(range(s0, min(s0 + chunk, 10000 + ntrain)))
+        kq = op.kfield(g)
+        b = op.rhs(fh)
+        x, iters = Lm.pcg(lambda U: op.apply(kq, U), b, op.plain_precond(op.kbar(kq)), rtol=1e-10)
+        gh.append(g)
+        bs.append(b)
+        us.append(x)
+        it_plain_train.append(iters)
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+    gh, bs, us = torch.cat(gh), torch.cat(bs), torch.cat(us)
+    lp = Lm.LearnedPreconditioner(op, hidden=512, layers=4)
+    t0 = time.perf_counter()
+    hist = lp.train(gh, bs, us, steps=train_steps, batch=256, lr=1e-3)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    g, fh = gen(range(40000, 40000 + neval))
+    kq = op.kfield(g)
+    b = op.rhs(fh)
+    A = lambda U: op.apply(kq, U)  # noqa: E731
+    xp, itp = Lm.pcg(A, b, op.plain_precond(op.kbar(kq)), rtol=1e-10)
+    s_an = op.kfield(-0.5 * g)
+    xa, ita = Lm.pcg(A, b, op.scaled_precond(s_an), x0=op.scaled_precond(s_an)(b), rtol=1e-10)
+    s_nn = lp.field(g)
+    P = op.scaled_precond(s_nn)
+    xl, itl = Lm.pcg(A, b, P, x0=P(b), rtol=1e-10)
+    res = {k: float(Lm.true_residual(op, kq, x_, b).max().item()) for k, x_ in
+           (("plain", xp), ("analytic_sqrt_k", xa), ("learned", xl))}
+    itp, ita, itl = (t_.double().mean().item() for t_ in (itp, ita, itl))
+    return dict(train_pairs=ntrain, eval_systems=neval, mlp_params=lp.net.n_params(),
+                gen_s=gen_s, train_s=train_s, train_loss_first=hist[0], train_loss_last=hist[-1],
+                mean_iters_plain=itp, mean_iters_learned=itl, mean_iters_scaled_sqrt_k_no_network=ita,
+                iteration_ratio_plain_over_learned=itp / itl, max_true_residual=res)
+
+
 def cpu_baseline(data, budget_s: float = 15.0):
     """The oracle (numpy/scipy FP64 partial diagonalisation) on a bounded
     sample of the same ragged batch, on the host cores."""
@@ -180,6 +273,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--with-burgers", action="store_true", help="also run config 3 (1000 IMEX steps)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 256 MiB device copy 3x (known traffic, for PMC scaling)")
+    ap.add_argument("--with-pcg", action="store_true", help="also run config 4 (train + evaluate)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -188,6 +285,13 @@ def main():
     import torch
 
     torch.cuda.init()
+    if args.copy_calibration:
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")  # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     res, err, data = bench_helmholtz(args.steps, args.warmup)
     mixed, fp64 = res["mixed"], res["fp64"]
     achieved = mixed["io_bytes"] / (mixed["kernel_ms"] * 1e-3) / 1e9
@@ -219,6 +323,10 @@ def main():
     if not args.no_extras:
         line["transform_3d_16x128^3"] = bench_transform_3d()
         line["poisson_chebyshev_N64"] = bench_poisson_c0()
+    if args.with_burgers:
+        line["burgers_c3"] = bench_burgers_c3()
+    if args.with_pcg:
+        line["learned_pcg_c4"] = bench_learned_pcg_c4()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(data)
     print(json.dumps(line), flush=True)

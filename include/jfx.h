@@ -61,6 +61,8 @@ int jfx_contract_f32(const float* A, int M, int K, const float* X, float* Y, lon
                      long long Q, hipStream_t stream);
 
 /* Split an FP64 (M, K) matrix into `nslices` bf16 slices stored [s][M][K].
+ * The contraction kernels read slices with a row stride Kp = 32*ceil(K/32):
+ * split a zero-padded (M, Kp) copy of A to obtain that layout.
  * mode 0: float split of fp32(A) (for jfx_contract_bf16x);
  * mode 1: fixed-point digits with power-of-two row scales rowscale[M]
  *         (for jfx_contract_ozaki).  Same reference interface as above. */

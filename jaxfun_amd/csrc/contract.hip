@@ -159,6 +159,7 @@ __global__ __launch_bounds__(256) void contract_slices(const unsigned short* __r
                                                        const double* __restrict__ Arow,
                                                        const TX* __restrict__ X,
                                                        TX* __restrict__ Y, ColMap cm, int nmt) {
+  const int Kp = (cm.K + KTB - 1) / KTB * KTB;  // slice row stride (zero padded)
   constexpr int WM = TM / 2;
   constexpr int BI = WM / 16;
   constexpr int BJ = 2;
@@ -204,29 +205,17 @@ __global__ __launch_bounds__(256) void contract_slices(const unsigned short* __r
       for (int j = 0; j < BJ; ++j) acc[l][i][j] = v4f{0, 0, 0, 0};
 
   for (int k0 = 0; k0 < K; k0 += KTB) {
-    // A slices: TM x KTB per slice
-    for (int e = tid; e < TM * KTB; e += 256) {
-      int mm = e / KTB, kk = e - mm * KTB;
-      long long m = m0 + mm;
-      int k = k0 + kk;
-      bool ok = (m < M && k < K);
-#pragma unroll
-      for (int s = 0; s < S; ++s)
-        As[s][mm][kk] = ok ? Asl[((long long)s * M + m) * K + k] : (unsigned short)0;
+    // A slices: TM x KTB per slice, 16-byte chunks (slice rows padded to Kp)
+    for (int ch = tid; ch < S * TM * (KTB / 8); ch += 256) {
+      const int sl = ch / (TM * (KTB / 8)), rem = ch - sl * TM * (KTB / 8);
+      const int mm = rem / (KTB / 8), part = rem - mm * (KTB / 8);
+      const long long m = m0 + mm;
+      v8u16 v = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (m < M) v = *(const v8u16*)(Asl + ((long long)sl * M + m) * Kp + k0 + part * 8);
+      *(v8u16*)&As[sl][mm][part * 8] = v;
     }
     // X tile: split each element into S slices, store [s][n][k]
-    for (int e = tid; e < KTB * TN; e += 256) {
-      int kk, nn;
-      if (cm.mode == COLS_QONE) {
-        nn = e / KTB;
-        kk = e - nn * KTB;
-      } else {
-        kk = e / TN;
-        nn = e - kk * TN;
-      }
-      long long c = c0 + nn;
-      int k = k0 + kk;
-      double x = (c < cm.ncols && k < K) ? (double)X[xb[nn] + k * Q] : 0.0;
+    auto put = [&](int nn, int kk, double x) {
       if constexpr (MODE == 0) {
         float r = (float)x;
 #pragma unroll
@@ -243,6 +232,42 @@ __global__ __launch_bounds__(256) void contract_slices(const unsigned short* __r
           Bs[s][nn][kk] = f2bf((float)d);  // |d| <= 128: exact in bf16
           r = (r - d) * 256.0;
         }
+      }
+    };
+    constexpr int VW = 16 / sizeof(TX);  // elements per 16-byte load
+    if (cm.mode == COLS_INTILE && c0 + TN <= cm.ncols && k0 + KTB <= K) {
+      // a k-row of the tile is TN contiguous elements
+      for (int ch = tid; ch < KTB * (TN / VW); ch += 256) {
+        const int kk = ch / (TN / VW), nn = (ch - kk * (TN / VW)) * VW;
+        const TX* src = X + xb[0] + (long long)(k0 + kk) * Q + nn;
+        TX v[VW];
+        *(uint4*)v = *(const uint4*)src;
+#pragma unroll
+        for (int i = 0; i < VW; ++i) put(nn + i, kk, (double)v[i]);
+      }
+    } else if (cm.mode == COLS_QONE && c0 + TN <= cm.ncols && k0 + KTB <= K && (K % VW) == 0) {
+      // a column of the tile is KTB contiguous elements
+      for (int ch = tid; ch < TN * (KTB / VW); ch += 256) {
+        const int nn = ch / (KTB / VW), kk = (ch - nn * (KTB / VW)) * VW;
+        const TX* src = X + xb[nn] + k0 + kk;
+        TX v[VW];
+        *(uint4*)v = *(const uint4*)src;
+#pragma unroll
+        for (int i = 0; i < VW; ++i) put(nn, kk + i, (double)v[i]);
+      }
+    } else {
+      for (int e = tid; e < KTB * TN; e += 256) {
+        int kk, nn;
+        if (cm.mode == COLS_QONE) {
+          nn = e / KTB;
+          kk = e - nn * KTB;
+        } else {
+          kk = e / TN;
+          nn = e - kk * TN;
+        }
+        long long c = c0 + nn;
+        int k = k0 + kk;
+        put(nn, kk, (c < cm.ncols && k < K) ? (double)X[xb[nn] + k * Q] : 0.0);
       }
     }
     __syncthreads();

@@ -185,6 +185,15 @@ class AxisOperator:
         self.M, self.K = self.A64.shape
         self._prep: dict[str, tuple] = {}
 
+    def _padded(self):
+        """Zero-pad K to a multiple of 32 (the slice row stride the kernels read)."""
+        Kp = (self.K + 31) // 32 * 32
+        if Kp == self.K:
+            return self.A64, Kp
+        Ap = torch.zeros((self.M, Kp), dtype=torch.float64, device=self.A64.device)
+        Ap[:, :self.K] = self.A64
+        return Ap, Kp
+
     def prepared(self, prec: str):
         if prec in self._prep:
             return self._prep[prec]
@@ -195,14 +204,16 @@ class AxisOperator:
             r = (cast(A, torch.float32),)
         elif prec in ("bf16x3", "bf16x6"):
             S = 2 if prec == "bf16x3" else 3
-            sl = torch.empty((S, self.M, self.K), dtype=torch.int16, device=A.device)
-            _lib.call("jfx_split_matrix", _p(A), self.M, self.K, 0, S, _p(sl), _p(None), _s())
+            Ap, Kp = self._padded()
+            sl = torch.empty((S, self.M, Kp), dtype=torch.int16, device=A.device)
+            _lib.call("jfx_split_matrix", _p(Ap), self.M, Kp, 0, S, _p(sl), _p(None), _s())
             r = (sl, S)
         elif prec.startswith("ozaki"):
             S = int(prec[5:])
-            sl = torch.empty((S, self.M, self.K), dtype=torch.int16, device=A.device)
+            Ap, Kp = self._padded()
+            sl = torch.empty((S, self.M, Kp), dtype=torch.int16, device=A.device)
             rs = torch.empty(self.M, dtype=torch.float64, device=A.device)
-            _lib.call("jfx_split_matrix", _p(A), self.M, self.K, 1, S, _p(sl), _p(rs), _s())
+            _lib.call("jfx_split_matrix", _p(Ap), self.M, Kp, 1, S, _p(sl), _p(rs), _s())
             r = (sl, rs, S)
         else:
             raise ValueError(f"unknown precision {prec!r}; expected one of {PRECISIONS}")
