@@ -82,6 +82,18 @@ int jfx_contract_ozaki(const unsigned short* Aslices, const double* Arowscale, i
                        int K, const double* X, double* Y, long long P, long long Q,
                        hipStream_t stream);
 
+/* Fused two-axis contraction of 2D slabs: Y[b] = A0 X[b] A1^T (X: B x K0 x K1,
+ * Y: B x M0 x M1, all dims <= 128), optionally followed by Y /= E (epi 1) or
+ * Y *= E (epi 2) with E (M0 x M1).  prec 0: FP64 operators A0d, A1d; prec 1:
+ * 3-term bf16 slices A0s, A1s ([3][M][Kp], Kp = 32*ceil(K/32)) on FP32 data.
+ * Replaces two successive per-axis applications of TensorProductSpace
+ * transforms (galerkin/tensorproductspace.py:330-417) or of the 2D
+ * fast-diagonalisation solve (la/tpmatrix.py:545-587, with the eigenvalue
+ * division as the epilogue). */
+int jfx_plane_contract(int prec, const double* A0d, const double* A1d, const unsigned short* A0s,
+                       const unsigned short* A1s, const void* X, void* Y, const void* E, int epi, long long B,
+                       int M0, int K0, int M1, int K1, hipStream_t stream);
+
 /* ---- a7: banded LU without pivoting.  band is (n, p+q+1) "by row":
  * band[i*(p+q+1) + o + p] = A[i, i+o].  status_dev (device int) receives 0,
  * or 1 + the row of the first zero/non-finite pivot.

@@ -28,6 +28,7 @@ class TensorProductSpace:
         self.name = name
         self.tensorname = "⊗".join(b.name for b in self.basespaces)
         self.precision = precision
+        self.fused = True  # fuse the last two axes of transforms into one kernel
 
     def __len__(self) -> int:
         return len(self.basespaces)
@@ -85,6 +86,12 @@ class TensorProductSpace:
         nb = X.ndim - len(self)
         if nb < 0:
             raise ValueError(f"array of rank {X.ndim} is smaller than the space dimension {len(self)}")
+        d = len(ops)
+        if d >= 2 and self.fused and K.plane_ok(ops[-2], ops[-1], prec):
+            # leading axes one by one, the last two in one fused pass
+            for ax, op in enumerate(ops[:-2]):
+                X = K.apply_along(op, X, nb + ax, prec)
+            return K.plane(ops[-2], ops[-1], X.contiguous(), prec)
         for ax, op in enumerate(ops):
             X = K.apply_along(op, X, nb + ax, prec)
         return X

@@ -254,3 +254,37 @@ def apply_along(op: AxisOperator, X: torch.Tensor, axis: int, prec: str = "f64",
         _lib.call("jfx_contract_ozaki", _p(prep[0]), _p(prep[1]), prep[2], op.M, K, _p(X), _p(Y), P, Q,
                   _s())
     return Y
+
+
+PLANE_PRECISIONS = ("f64", "bf16x6")
+
+
+def plane_ok(op0: AxisOperator, op1: AxisOperator, prec: str) -> bool:
+    return prec in PLANE_PRECISIONS and max(op0.M, op0.K, op1.M, op1.K) <= 128
+
+
+def plane(op0: AxisOperator, op1: AxisOperator, X: torch.Tensor, prec: str = "f64",
+          epi: tuple | None = None) -> torch.Tensor:
+    """``Y[b] = A0 X[b] A1^T`` over the last two axes of X in one kernel
+    (optionally ``/ E`` or ``* E`` with ``epi=("div"|"mul", E)``)."""
+    dt = io_dtype(prec)
+    _req(X, dt, "X")
+    if X.shape[-2] != op0.K or X.shape[-1] != op1.K:
+        raise ValueError("plane: operator/array shape mismatch")
+    B = math.prod(X.shape[:-2])
+    Y = torch.empty(X.shape[:-2] + (op0.M, op1.M), dtype=dt, device=X.device)
+    mode, E = 0, None
+    if epi is not None:
+        mode = {"div": 1, "mul": 2}[epi[0]]
+        E = epi[1]
+        _req(E, dt, "E")
+        if tuple(E.shape) != (op0.M, op1.M):
+            raise ValueError("plane: epilogue shape mismatch")
+    if prec == "f64":
+        _lib.call("jfx_plane_contract", 0, _p(op0.A64), _p(op1.A64), _p(None), _p(None), _p(X), _p(Y), _p(E),
+                  mode, B, op0.M, op0.K, op1.M, op1.K, _s())
+    else:
+        s0, s1 = op0.prepared("bf16x6")[0], op1.prepared("bf16x6")[0]
+        _lib.call("jfx_plane_contract", 1, _p(None), _p(None), _p(s0), _p(s1), _p(X), _p(Y), _p(E),
+                  mode, B, op0.M, op0.K, op1.M, op1.K, _s())
+    return Y

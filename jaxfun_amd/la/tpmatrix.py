@@ -56,6 +56,11 @@ class TPMatrix:
         """A0 along axis 0, A1 along axis 1, ... (la/tpmatrix.py:152-156)."""
         w = _as_dev(w, K.io_dtype(precision))
         nb = w.ndim - self.dims
+        if self.dims == 2 and K.plane_ok(self.mats[0].operator(), self.mats[1].operator(), precision):
+            w = K.plane(self.mats[0].operator(), self.mats[1].operator(), w.contiguous(), precision)
+            if self.coefficient != 1:
+                w = K.lincomb([(float(self.coefficient), w)])
+            return w
         for i, m in enumerate(self.mats):
             w = m.matvec(w, nb + i, precision)
         if self.coefficient != 1:
@@ -150,6 +155,10 @@ class TPMatricesLUFactors:
         nb = F.ndim - len(self.shape)
         if tuple(F.shape[nb:]) != self.shape:
             F = F.reshape(F.shape[:nb] + self.shape) if F.numel() % int(np.prod(self.shape)) == 0 else F
+        if len(self.shape) == 2 and K.plane_ok(self.W[0], self.W[1], precision):
+            F = F.contiguous()
+            Ft = K.plane(self.W[0], self.W[1], F, precision, ("div", self.D(dt)))
+            return K.plane(self.V[0], self.V[1], Ft, precision)
         for i, W in enumerate(self.W):
             F = K.apply_along(W, F, nb + i, precision)
         U = K.div_bcast(F, self.D(dt))

@@ -183,3 +183,15 @@ def test_mixed_precision_transforms(dev):
     # the round trip amplifies the ozaki5 error (~5e-13) by ||u||/||c|| ~ N^1.5
     c2 = T.forward(u64, precision="ozaki5")
     assert O.rel_l2(_np(c2), c) < 2e-9
+
+
+def test_fused_and_unfused_transforms_agree(dev):
+    from jaxfun_amd import FunctionSpace, Legendre, TensorProduct
+
+    V = FunctionSpace(40, Legendre, {"left": {"D": 0}, "right": {"D": 0}})
+    T = TensorProduct(V, V, V)
+    c = np.random.default_rng(9).standard_normal((2, 38, 38, 38))
+    u1 = _np(T.backward(c))
+    T.fused = False
+    u2 = _np(T.backward(c))
+    assert O.rel_l2(u1, u2) < 1e-14

@@ -128,3 +128,29 @@ def test_elementwise(dev):
     assert np.allclose(z, a / D)
     f = K.cast(_t(c, dev), torch.float32)
     assert f.dtype == torch.float32 and np.allclose(f.cpu().numpy(), c.astype(np.float32))
+
+
+@pytest.mark.parametrize("dims", [(128, 128, 128, 128), (62, 64, 62, 64), (96, 30, 17, 128), (1, 5, 7, 3)])
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-14), ("bf16x6", 2e-6)])
+@pytest.mark.parametrize("epi", [None, "div", "mul"])
+def test_plane_contract(dev, dims, prec, tol, epi):
+    import torch
+
+    from jaxfun_amd import kernels as K
+
+    M0, K0, M1, K1 = dims
+    rng = np.random.default_rng(7)
+    A0, A1 = rng.standard_normal((M0, K0)), rng.standard_normal((M1, K1))
+    X = rng.standard_normal((5, K0, K1))
+    E = rng.uniform(1.0, 2.0, size=(M0, M1))
+    dt = K.io_dtype(prec)
+    Xd = _t(X, dev, dt)
+    ep = None if epi is None else (epi, _t(E, dev, dt))
+    Y = K.plane(K.AxisOperator(_t(A0, dev)), K.AxisOperator(_t(A1, dev)), Xd, prec, ep)
+    Xr = Xd.to(torch.float64).cpu().numpy()
+    Yr = np.einsum("mk,bkl,nl->bmn", A0, Xr, A1)
+    if epi == "div":
+        Yr = Yr / E
+    elif epi == "mul":
+        Yr = Yr * E
+    assert O.rel_l2(Y.to(torch.float64).cpu().numpy(), Yr) < tol
