@@ -38,15 +38,16 @@ def bench_helmholtz(steps: int, warmup: int):
     import torch
 
     from jaxfun_amd import workloads as W
-    from jaxfun_amd.helmholtz import HelmholtzBatch
+    from jaxfun_amd.helmholtz import HelmholtzBatch, autotune
 
     Ns, alphas, loads = W.helmholtz_c2()
     res = {}
     outs = {}
-    for mode, nw, bpc in (("mixed", 8, 2), ("fp64", 8, 1)):
-        hb = HelmholtzBatch(Ns, alphas, mode=mode, nwaves=nw, blocks_per_cu=bpc)
+    for mode in ("mixed", "fp64"):
+        hb = HelmholtzBatch(Ns, alphas, mode=mode)
         F = hb.pack(loads)
         U = torch.empty_like(F)
+        autotune(hb, F)
         hb.capture(F, U)
         for _ in range(warmup):
             hb.replay()
@@ -316,7 +317,8 @@ def main():
         "speedup_vs_fp64_gpu": mixed["solves_per_s"] / fp64["solves_per_s"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "helm_ragged_kernel<1,8,float>", "kernel_ms": mixed["kernel_ms"],
+                     "kernel": f"helm_ragged_kernel<1,{mixed['nwaves']},float>", "kernel_ms": mixed["kernel_ms"],
+                     "launch": {"nwaves": mixed["nwaves"], "nblocks": mixed["nblocks"]},
                      "alg_bytes_per_launch": mixed["io_bytes"],
                      "alg_TFLOPs_4n3": mixed["alg_flops"] / (mixed["kernel_ms"] * 1e-3) / 1e12},
     }

@@ -91,11 +91,18 @@ class HelmholtzBatch:
         self.total = int(self.offsets[-1])
         self._build_tables()
         self._build_items()
+        self.counter = torch.zeros(1, dtype=torch.int32, device=self.dev)
+        self.set_launch(nwaves, blocks_per_cu)
+
+    def set_launch(self, nwaves: int, blocks_per_cu: int) -> None:
+        """Choose the launch shape (waves per workgroup, workgroups per CU)."""
+        if nwaves not in (4, 8) or blocks_per_cu < 1:
+            raise ValueError("nwaves must be 4 or 8 and blocks_per_cu >= 1")
+        self.nwaves, self.blocks_per_cu = nwaves, blocks_per_cu
         cus = torch.cuda.get_device_properties(self.dev).multi_processor_count
         self.nblocks = int(cus * blocks_per_cu)
         per = _lib.load().jfx_helm_scratch_doubles_per_block()
         self.scratch = torch.empty(self.nblocks * per, dtype=torch.float64, device=self.dev)
-        self.counter = torch.zeros(1, dtype=torch.int32, device=self.dev)
         self._graph = None
 
     # ------------------------------------------------------------------ set-up
@@ -217,26 +224,24 @@ class HelmholtzBatch:
         return float(2 * self.total * torch.tensor([], dtype=self.io_dtype).element_size())
 
 
-def autotune(Ns, alphas, mode: str = "mixed", candidates=((4, 1), (4, 2), (8, 1), (8, 2), (4, 3)),
-             reps: int = 3) -> tuple[int, int]:
+def autotune(batch: HelmholtzBatch, F: torch.Tensor,
+             candidates=((4, 1), (8, 1), (4, 2), (8, 2)), reps: int = 3) -> tuple[int, int]:
     """Time each (waves per workgroup, workgroups per CU) launch shape on the
-    actual ragged batch and return the fastest."""
+    batch itself, keep the fastest on ``batch`` and return it."""
+    U = torch.empty_like(F)
     best, best_t = None, float("inf")
-    F = None
     for nw, bpc in candidates:
-        hb = HelmholtzBatch(Ns, alphas, mode=mode, nwaves=nw, blocks_per_cu=bpc)
-        if F is None:
-            F = torch.ones(hb.total, dtype=hb.io_dtype, device=hb.dev)
-        U = torch.empty_like(F)
-        hb.solve(F, U)
+        batch.set_launch(nw, bpc)
+        batch.solve(F, U)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
         for _ in range(reps):
-            hb.solve(F, U)
+            batch.solve(F, U)
         e1.record()
         torch.cuda.synchronize()
         t = e0.elapsed_time(e1) / reps
         if t < best_t:
             best, best_t = (nw, bpc), t
+    batch.set_launch(*best)
     return best

@@ -62,7 +62,8 @@ def test_launch_shapes_agree(dev, nwaves, bpc):
 
     Ns, alphas, loads = _case([32, 77, 128, 256, 190], seed=12)
     base = HelmholtzBatch(Ns, alphas, mode="mixed", nwaves=4, blocks_per_cu=2)
-    hb = HelmholtzBatch(Ns, alphas, mode="mixed", nwaves=nwaves, blocks_per_cu=bpc)
+    hb = HelmholtzBatch(Ns, alphas, mode="mixed")
+    hb.set_launch(nwaves, bpc)
     F = base.pack(loads)
     a = base.solve(F).cpu().numpy()
     b = hb.solve(F).cpu().numpy()
