@@ -143,13 +143,15 @@ int jfx_cast(int direction, const void* x, void* y, long long n, hipStream_t str
  * (alpha M (x) M - K (x) M - M (x) K) U = F in one persistent launch.
  * items: device array of nitems work items (jfx_helm_item_size() bytes each:
  *   {long long foff, voff; double alpha; int n, p, q, a, b, AP, loff, pad}),
- *   one per (solve, row parity p, column parity q), processed in array order
- *   from the device counter `counter` (reset by the call itself).
+ *   one per (solve, row parity p) (q and b are ignored: the workgroup runs
+ *   both column parities in turn), processed in array order from the device
+ *   counter `counter` (reset by the call itself).
  * Vd/VTd: FP64 eigenvector tables V_p / V_p^T (AP x AP, zero padded) of the
  *   parity blocks of K_p V = M_p V diag(lam); Vs/VTs: the same as 3 bf16
  *   float-split slices each ([3][AP][AP] per table at 3*voff).  lam at loff.
  * mode 0: FP64 MFMA (fp64 I/O); mode 1: 3-term BF16 split MFMA with FP32
- *   accumulation/storage and FP64 tridiagonal solves (io_f32 selects float
+ *   accumulation/storage and FP64 tridiagonal solves with FP32-stored
+ *   multipliers (io_f32 selects float
  *   or double F/U).  nwaves 4 or 8; scratch holds
  *   nblocks * jfx_helm_scratch_doubles_per_block() doubles.
  * Replaces TPMatrices.solve -> tpmats_lu_factor / TPMatricesLUFactors.solve

@@ -67,30 +67,32 @@ __device__ __forceinline__ double rcp64(double x) {
 template <int MODE>
 struct HelmLDS;
 template <>
-struct HelmLDS<0> {  // FP64: work block 128 x 129 doubles, V slab 16 x 129 doubles
+struct HelmLDS<0> {  // FP64: work block 128 x 129 doubles, V slab 16 x 129 doubles; multipliers in global scratch
   static constexpr int LDW = 129, KS = 16;
-  static constexpr size_t WK = (size_t)HMAXA * LDW * 8, VS = (size_t)KS * (HMAXA + 1) * 8;
+  static constexpr size_t WK = (size_t)HMAXA * LDW * 8, U1 = (size_t)KS * (HMAXA + 1) * 8;
 };
 template <>
-struct HelmLDS<1> {  // bf16 split: work block 128 x 130 floats, 3 bf16 slices of a 128 x 32 V slab
+struct HelmLDS<1> {  // bf16 split: work block 128 x 130 floats; V slab (3 bf16 slices of 128 x 32) shares
+                     // its space with the 128 x 128 FP32 tridiagonal multipliers
   static constexpr int LDW = 130, KS = 32, VLD = 40;
-  static constexpr size_t WK = (size_t)HMAXA * LDW * 4, VS = (size_t)3 * HMAXA * VLD * 2;
+  static constexpr size_t WK = (size_t)HMAXA * LDW * 4, U1 = (size_t)HMAXA * HMAXA * 4;
 };
 constexpr size_t HTAB = 3 * HMAXA * 8;
 
-// One GEMM phase: C (a x b, rows of V) = A (AP x AP table, applied as rows) * Wk
-//   phase 1: A = V^T (tabT), result written back into Wk (column-owned panels)
-//   phase 3: A = V   (tab),  result written to global U
+// One GEMM phase (a x b block):   C = A * Wk   with A an (AP x AP) table applied by rows
+//   phase 1: A = V^T, result back into Wk (each wave owns its column panels)
+//   phase 3: A = V,   result scattered to U
+// The next k-slab of A is prefetched into registers while the current one is multiplied.
 template <int MODE, int NW, typename TIO, bool TO_GLOBAL>
-__device__ __forceinline__ void helm_gemm_phase(const HelmItem& I, const double* __restrict__ tabd,
+__device__ __forceinline__ void helm_gemm_phase(const HelmItem& I, int q, int b, const double* __restrict__ tabd,
                                                 const unsigned short* __restrict__ tabs,
                                                 typename HelmTypes<MODE>::TS* Wk, unsigned char* vslab,
                                                 TIO* __restrict__ Ub) {
   typedef typename HelmTypes<MODE>::TS TS;
-  constexpr int LDW = HelmLDS<MODE>::LDW, KS = HelmLDS<MODE>::KS;
+  constexpr int LDW = HelmLDS<MODE>::LDW, KS = HelmLDS<MODE>::KS, NT = NW * 64;
   constexpr int PPW = 8 / NW;  // column panels per wave (b <= 128 -> <= 8 panels)
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int a = I.a, b = I.b, AP = I.AP, n = I.n;
+  const int a = I.a, AP = I.AP, n = I.n;
   const int npanel = (b + 15) >> 4, nrb = AP >> 4;
   typedef typename std::conditional<MODE == 0, v4d, v4f>::type vacc;
   vacc acc[PPW][8];
@@ -99,23 +101,51 @@ __device__ __forceinline__ void helm_gemm_phase(const HelmItem& I, const double*
 #pragma unroll
     for (int rb = 0; rb < 8; ++rb) acc[pp][rb] = vacc{0, 0, 0, 0};
 
-  for (int k0 = 0; k0 < AP; k0 += KS) {
-    if constexpr (MODE == 0) {
-      double* Vs0 = reinterpret_cast<double*>(vslab);  // [KS][HMAXA + 1]: Vs0[kk][m] = A[m][k0 + kk]
-      for (int e = tid; e < KS * AP; e += NW * 64) {
-        int kk = e / AP, m = e - kk * AP;
-        Vs0[kk * (HMAXA + 1) + m] = tabd[I.voff + (long long)(k0 + kk) * AP + m];
-      }
-    } else {
-      constexpr int VLD = HelmLDS<1>::VLD;
-      unsigned short* Vsl = reinterpret_cast<unsigned short*>(vslab);  // [3][HMAXA][VLD]
-      for (int ch = tid; ch < 3 * AP * 4; ch += NW * 64) {
-        int sl = ch / (AP * 4), rem = ch - sl * AP * 4, m = rem >> 2, part = rem & 3;
-        *(v8u16*)&Vsl[(sl * HMAXA + m) * VLD + part * 8] =
-            *(const v8u16*)(tabs + 3 * I.voff + (long long)sl * AP * AP + (long long)m * AP + k0 + part * 8);
+  // register prefetch of one k-slab
+  constexpr int NPRE = (MODE == 0) ? (KS * HMAXA / NT) : (3 * HMAXA * 4 / NT);
+  typedef typename std::conditional<MODE == 0, double, v8u16>::type PT;
+  PT pre[NPRE];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < NPRE; ++u) {
+      const int e = tid + u * NT;
+      if constexpr (MODE == 0) {
+        if (e < KS * AP) {
+          const int kk = e / AP, m = e - kk * AP;
+          pre[u] = tabd[I.voff + (long long)(k0 + kk) * AP + m];
+        }
+      } else {
+        if (e < 3 * AP * 4) {
+          const int sl = e / (AP * 4), rem = e - sl * AP * 4, m = rem >> 2, part = rem & 3;
+          pre[u] = *(const v8u16*)(tabs + 3 * I.voff + (long long)sl * AP * AP + (long long)m * AP + k0 + part * 8);
+        }
       }
     }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int u = 0; u < NPRE; ++u) {
+      const int e = tid + u * NT;
+      if constexpr (MODE == 0) {
+        if (e < KS * AP) {
+          const int kk = e / AP, m = e - kk * AP;
+          reinterpret_cast<double*>(vslab)[kk * (HMAXA + 1) + m] = pre[u];
+        }
+      } else {
+        constexpr int VLD = HelmLDS<1>::VLD;
+        if (e < 3 * AP * 4) {
+          const int sl = e / (AP * 4), rem = e - sl * AP * 4, m = rem >> 2, part = rem & 3;
+          *(v8u16*)&reinterpret_cast<unsigned short*>(vslab)[(sl * HMAXA + m) * VLD + part * 8] = pre[u];
+        }
+      }
+    }
+  };
+
+  fetch(0);
+  for (int k0 = 0; k0 < AP; k0 += KS) {
+    store();
     __syncthreads();
+    if (k0 + KS < AP) fetch(k0 + KS);
 #pragma unroll
     for (int pp = 0; pp < PPW; ++pp) {
       const int pc = w + pp * NW;
@@ -155,6 +185,7 @@ __device__ __forceinline__ void helm_gemm_phase(const HelmItem& I, const double*
               v8u16 ah = *(const v8u16*)&Vsl[(0 * HMAXA + row) * VLD + ko];
               v8u16 am = *(const v8u16*)&Vsl[(1 * HMAXA + row) * VLD + ko];
               v8u16 al = *(const v8u16*)&Vsl[(2 * HMAXA + row) * VLD + ko];
+              // products ordered small to large
               acc[pp][rb] = mfma_bf16(al, bh, acc[pp][rb]);
               acc[pp][rb] = mfma_bf16(am, bm, acc[pp][rb]);
               acc[pp][rb] = mfma_bf16(ah, bl, acc[pp][rb]);
@@ -168,7 +199,6 @@ __device__ __forceinline__ void helm_gemm_phase(const HelmItem& I, const double*
     }
     __syncthreads();
   }
-  // each wave writes only its own columns
 #pragma unroll
   for (int pp = 0; pp < PPW; ++pp) {
     const int pc = w + pp * NW;
@@ -183,12 +213,38 @@ __device__ __forceinline__ void helm_gemm_phase(const HelmItem& I, const double*
         const int m = rb * 16 + ((MODE == 0) ? mfma16_row_f64(lane, r) : mfma16_row_f32(lane, r));
         if (m >= a) continue;
         if constexpr (TO_GLOBAL)
-          Ub[(long long)(I.p + 2 * m) * n + I.q + 2 * col] = (TIO)acc[pp][rb][r];
+          Ub[(long long)(I.p + 2 * m) * n + q + 2 * col] = (TIO)acc[pp][rb][r];
         else
           Wk[m * LDW + col] = (TS)acc[pp][rb][r];
       }
     }
   }
+}
+
+// Multiplier storage of the row solves: FP32 LDS (mixed) or FP64 global scratch.
+template <int MODE>
+struct CPStore {
+  float* lds;
+  double* glob;
+  __device__ __forceinline__ void put(int j, int r, double v) const {
+    if constexpr (MODE == 1)
+      lds[j * HMAXA + r] = (float)v;
+    else
+      glob[(long long)j * HMAXA + r] = v;
+  }
+  __device__ __forceinline__ double get(int j, int r) const {
+    if constexpr (MODE == 1)
+      return (double)lds[j * HMAXA + r];
+    else
+      return glob[(long long)j * HMAXA + r];
+  }
+};
+
+// Rescale a continuant pair by the same power of two (exact).
+__device__ __forceinline__ void renorm(double& P1, double& P2) {
+  const int e = ilogb(P1);
+  P1 = ldexp(P1, -e);
+  P2 = ldexp(P2, -e);
 }
 
 template <int MODE, int NW, typename TIO>
@@ -199,16 +255,16 @@ __global__ __launch_bounds__(NW * 64) void helm_ragged_kernel(
     const double* __restrict__ lam, const TIO* __restrict__ F, TIO* __restrict__ U,
     double* __restrict__ scratch) {
   typedef typename HelmTypes<MODE>::TS TS;
-  constexpr int LDW = HelmLDS<MODE>::LDW;
+  constexpr int LDW = HelmLDS<MODE>::LDW, NT = NW * 64;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   TS* Wk = reinterpret_cast<TS*>(smem_raw);
-  unsigned char* vslab = smem_raw + HelmLDS<MODE>::WK;
-  double* Hs = reinterpret_cast<double*>(smem_raw + HelmLDS<MODE>::WK + HelmLDS<MODE>::VS);  // h_c + h_{c+2}
+  unsigned char* u1 = smem_raw + HelmLDS<MODE>::WK;
+  double* Hs = reinterpret_cast<double*>(smem_raw + HelmLDS<MODE>::WK + HelmLDS<MODE>::U1);  // h_c + h_{c+2}
   double* H2 = Hs + HMAXA;                                                                 // h_{c+2}
-  double* Dg = H2 + HMAXA;                                                                // 4c + 6
+  double* Dg = H2 + HMAXA;                                                                 // 4c + 6
   __shared__ int s_item;
   const int tid = threadIdx.x;
-  double* cp = scratch + (long long)blockIdx.x * HMAXA * HMAXA;
+  CPStore<MODE> cps{reinterpret_cast<float*>(u1), scratch + (long long)blockIdx.x * HMAXA * HMAXA};
 
   for (;;) {
     if (tid == 0) s_item = atomicAdd(counter, 1);
@@ -216,129 +272,122 @@ __global__ __launch_bounds__(NW * 64) void helm_ragged_kernel(
     const int it = s_item;
     if (it >= nitems) break;
     const HelmItem I = items[it];
-    const int n = I.n, p = I.p, q = I.q, a = I.a, b = I.b;
+    const int n = I.n, p = I.p, a = I.a;
     const TIO* Fb = F + I.foff;
 
-    // ---------------- phase 0: F_pq -> Wk, per-column constants -> LDS
-    for (int e = tid; e < a * b; e += NW * 64) {
-      const int r = e / b, c = e - r * b;
-      Wk[r * LDW + c] = (TS)Fb[(long long)(p + 2 * r) * n + q + 2 * c];
-    }
-    for (int j = tid; j < b; j += NW * 64) {
-      const int c = q + 2 * j;
-      const double hc = 2.0 / (2 * c + 1), hc2 = 2.0 / (2 * c + 5);
-      Hs[j] = hc + hc2;
-      H2[j] = hc2;
-      Dg[j] = 4.0 * c + 6.0;
-    }
-    __syncthreads();
-
-    // ---------------- phase 1: G = V_p^T F_pq (in place, column-owned)
-    helm_gemm_phase<MODE, NW, TIO, false>(I, Vd, VTs, Wk, vslab, nullptr);
-    __syncthreads();
-
-    // ---------------- phase 2: FP64 tridiagonal solves, two lanes per row
-    // row r: T u = g with T = s M_q + diag(4c + 6), s = alpha - lambda_r:
-    //   T[j][j] = s (h_c + h_{c+2}) + 4c + 6,  T[j][j+1] = T[j+1][j] = -s h_{c+2}
-    // lane pair (r, 0) eliminates rows 0..m-1 downwards, (r, 1) rows b-1..m+1
-    // upwards (twisted factorisation); both meet at m = b / 2.
-    for (int t = tid; t < 2 * HMAXA; t += NW * 64) {
-      const int r = t >> 1, side = t & 1;
-      const bool act = r < a;
-      const double s = act ? I.alpha - lam[I.loff + r] : 1.0;
-      TS* row = Wk + (act ? r : 0) * LDW;
-      const int m = b >> 1;
-      double cl = 0.0, dl = 0.0;  // multiplier / rhs of the last eliminated row next to m
-      if (act) {
-        if (side == 0) {
-          double cprev = 0.0, dprev = 0.0, offprev = 0.0;
-          for (int j = 0; j < m; ++j) {
-            const double off = -s * H2[j];
-            const double den = fma(s, Hs[j], Dg[j]) - offprev * cprev;
-            const double inv = rcp64(den);
-            const double cj = off * inv;
-            const double dj = ((double)row[j] - offprev * dprev) * inv;
-            cp[(long long)j * HMAXA + r] = cj;
-            row[j] = (TS)dj;
-            cprev = cj;
-            dprev = dj;
-            offprev = off;
+    // both column parities of this (solve, row parity) in turn: the second
+    // pass re-reads / re-writes the other half of the same lines from L2
+    for (int q = 0; q < 2; ++q) {
+      const int b = (n - q + 1) >> 1;
+      if (b == 0) continue;
+      // ---------------- phase 0: F_pq -> Wk (8 loads in flight per thread), column constants
+      for (int e0 = tid; e0 < a * b; e0 += 8 * NT) {
+        TIO v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int e = e0 + u * NT;
+          if (e < a * b) {
+            const int r = e / b, c = e - r * b;
+            v[u] = Fb[(long long)(p + 2 * r) * n + q + 2 * c];
           }
-          cl = cprev;
-          dl = dprev;
-        } else {
-          double fnext = 0.0, enext = 0.0;
-          for (int j = b - 1; j > m; --j) {
-            const double offdn = -s * H2[j - 1];                    // couples j and j-1
-            const double offup = (j < b - 1) ? -s * H2[j] : 0.0;    // couples j and j+1
-            const double den = fma(s, Hs[j], Dg[j]) - offup * fnext;
-            const double inv = rcp64(den);
-            const double fj = offdn * inv;
-            const double ej = ((double)row[j] - offup * enext) * inv;
-            cp[(long long)j * HMAXA + r] = fj;
-            row[j] = (TS)ej;
-            fnext = fj;
-            enext = ej;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int e = e0 + u * NT;
+          if (e < a * b) {
+            const int r = e / b, c = e - r * b;
+            Wk[r * LDW + c] = (TS)v[u];
           }
-          cl = fnext;
-          dl = enext;
         }
       }
-      // exchange with the partner lane (adjacent lane of the same wave)
-      const double co = __shfl_xor(cl, 1), dO = __shfl_xor(dl, 1);
-      if (act) {
-        const double ct = side == 0 ? cl : co, dt = side == 0 ? dl : dO;  // top part (rows < m)
-        const double fb = side == 0 ? co : cl, eb = side == 0 ? dO : dl;  // bottom part (rows > m)
-        const double am = (m > 0) ? -s * H2[m - 1] : 0.0;
-        const double cm = (m < b - 1) ? -s * H2[m] : 0.0;
-        const double um = ((double)row[m] - am * dt - cm * eb) / (fma(s, Hs[m], Dg[m]) - am * ct - cm * fb);
-        if (side == 0) {
-          double unext = um;
-          int j = m - 1;
-          for (; j >= 7; j -= 8) {  // prefetch 8 multipliers per chunk
+      for (int j = tid; j < b; j += NT) {
+        const int c = q + 2 * j;
+        const double hc = 2.0 / (2 * c + 1), hc2 = 2.0 / (2 * c + 5);
+        Hs[j] = hc + hc2;
+        H2[j] = hc2;
+        Dg[j] = 4.0 * c + 6.0;
+      }
+      __syncthreads();
+
+      // ---------------- phase 1: G = V_p^T F_pq (in place, column-owned)
+      helm_gemm_phase<MODE, NW, TIO, false>(I, q, b, Vd, VTs, Wk, u1, nullptr);
+      __syncthreads();
+
+      // ---------------- phase 2: FP64 tridiagonal solves, two lanes per row
+      // row r: T u = g, T = s M_q + diag(4c + 6), s = alpha - lambda_r, c = q + 2j:
+      //   d_j = T[j][j] = s (h_c + h_{c+2}) + 4c + 6,  o_j = T[j][j+1] = T[j+1][j] = -s h_{c+2}
+      // Twisted elimination: lane (r, 0) sweeps rows 0..m-1 down, lane (r, 1)
+      // rows b-1..m+1 up, m = b / 2.  The pivots are ratios of continuants
+      // P_j = d_j P_{j-1} - o_{j-1}^2 P_{j-2}  (P_{-1} = 1), so the serial
+      // dependence per step is one FMA; reciprocals run off that chain.
+      for (int t = tid; t < 2 * HMAXA; t += NT) {
+        const int r = t >> 1, side = t & 1;
+        const bool act = r < a;
+        const double s = act ? I.alpha - lam[I.loff + r] : 1.0;
+        TS* row = Wk + (act ? r : 0) * LDW;
+        const int m = b >> 1;
+        // both lanes of a pair run the same instruction stream; `side` only
+        // selects the direction: step i visits j = i (down) or j = b-1-i (up)
+        const int len = act ? (side ? b - 1 - m : m) : 0;
+        const int maxlen = b - m - 1 > m ? b - m - 1 : m;
+        double P1 = 1.0, P2 = 0.0, yprev = 0.0, oprev = 0.0, cl = 0.0;
+        for (int i = 0; i < maxlen; ++i) {
+          if (i < len) {
+            const int j = side ? b - 1 - i : i;
+            const int inext = side ? j - 1 : j;  // coupling towards the middle
+            const double on = -s * H2[inext];
+            const double Pj = fma(fma(s, Hs[j], Dg[j]), P1, -(oprev * oprev) * P2);
+            const double rj = P1 * rcp64(Pj);
+            const double yj = fma(-oprev, yprev, (double)row[j]) * rj;
+            cl = on * rj;
+            cps.put(j, r, cl);
+            row[j] = (TS)yj;
+            P2 = P1;
+            P1 = Pj;
+            yprev = yj;
+            oprev = on;
+            if ((i & 7) == 7) renorm(P1, P2);
+          }
+        }
+        const double dl = yprev;
+        // exchange with the partner lane (adjacent lane of the same wave)
+        const double co = __shfl_xor(cl, 1), dO = __shfl_xor(dl, 1);
+        if (act) {
+          const double ct = side == 0 ? cl : co, dt = side == 0 ? dl : dO;  // rows < m
+          const double fb = side == 0 ? co : cl, eb = side == 0 ? dO : dl;  // rows > m
+          const double am = (m > 0) ? -s * H2[m - 1] : 0.0;
+          const double cm = (m < b - 1) ? -s * H2[m] : 0.0;
+          const double um = ((double)row[m] - am * dt - cm * eb) / (fma(s, Hs[m], Dg[m]) - am * ct - cm * fb);
+          // back substitution outwards from m: j = m-1-i (down) or m+1+i (up)
+          double uprev = um;
+          int i = 0;
+          for (; i + 8 <= len; i += 8) {
             double c8[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) c8[u] = cp[(long long)(j - u) * HMAXA + r];
+            for (int u = 0; u < 8; ++u) c8[u] = cps.get(side ? m + 1 + i + u : m - 1 - i - u, r);
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-              const double v = (double)row[j - u] - c8[u] * unext;
-              row[j - u] = (TS)v;
-              unext = v;
-            }
-          }
-          for (; j >= 0; --j) {
-            const double v = (double)row[j] - cp[(long long)j * HMAXA + r] * unext;
-            row[j] = (TS)v;
-            unext = v;
-          }
-          row[m] = (TS)um;
-        } else {
-          double uprev = um;
-          int j = m + 1;
-          for (; j + 7 < b; j += 8) {
-            double f8[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) f8[u] = cp[(long long)(j + u) * HMAXA + r];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-              const double v = (double)row[j + u] - f8[u] * uprev;
-              row[j + u] = (TS)v;
+              const int j = side ? m + 1 + i + u : m - 1 - i - u;
+              const double v = (double)row[j] - c8[u] * uprev;
+              row[j] = (TS)v;
               uprev = v;
             }
           }
-          for (; j < b; ++j) {
-            const double v = (double)row[j] - cp[(long long)j * HMAXA + r] * uprev;
+          for (; i < len; ++i) {
+            const int j = side ? m + 1 + i : m - 1 - i;
+            const double v = (double)row[j] - cps.get(j, r) * uprev;
             row[j] = (TS)v;
             uprev = v;
           }
+          if (side == 0) row[m] = (TS)um;
         }
       }
-    }
-    __syncthreads();
+      __syncthreads();
 
-    // ---------------- phase 3: U_pq = V_p U~ -> global
-    helm_gemm_phase<MODE, NW, TIO, true>(I, VTd, Vs, Wk, vslab, U + I.foff);
-    __syncthreads();
+      // ---------------- phase 3: U_pq = V_p U~ -> global
+      helm_gemm_phase<MODE, NW, TIO, true>(I, q, b, VTd, Vs, Wk, u1, U + I.foff);
+      __syncthreads();
+    }
   }
 }
 
@@ -347,7 +396,7 @@ static int launch_helm(const HelmItem* items, int nitems, int* counter, const do
                        const double* VTd, const unsigned short* Vs, const unsigned short* VTs,
                        const double* lam, const TIO* F, TIO* U, double* scratch, int nblocks,
                        hipStream_t st) {
-  size_t shm = HelmLDS<MODE>::WK + HelmLDS<MODE>::VS + HTAB;
+  size_t shm = HelmLDS<MODE>::WK + HelmLDS<MODE>::U1 + HTAB;
   auto kern = helm_ragged_kernel<MODE, NW, TIO>;
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
   if (e != hipSuccess) return set_error("helm: hipFuncSetAttribute: %s", hipGetErrorString(e)), JFX_EHIP;

@@ -13,8 +13,9 @@ counter; the launch can be captured in a HIP graph and replayed.
 Set-up (once per distinct N, as the reference's ``lu_factor`` caches): the
 parity blocks of (K, M) are diagonalised on the host in FP64 with
 ``scipy.linalg.eigh`` (la/tpmatrix.py:1173), uploaded, and split into bf16
-slices on the device.  Work items (solve, row parity, column parity) are
-ordered largest-first so the tail of the queue holds the small items.
+slices on the device.  Work items (solve, row parity) are ordered
+largest-first so the tail of the queue holds the small items; each item
+processes both column parities in turn.
 """
 
 from __future__ import annotations
@@ -148,14 +149,11 @@ class HelmholtzBatch:
                 if a == 0:
                     continue
                 off, AP = self._voffs[(int(N), p)]
-                for q in (0, 1):
-                    b = (n - q + 1) // 2
-                    if b == 0:
-                        continue
-                    items.append((int(self.offsets[bi]), off, float(alpha), n, p, q, a, b, AP,
-                                  self._loffs[(int(N), p)], 0))
+                # one item per (solve, row parity); the kernel runs both column parities
+                items.append((int(self.offsets[bi]), off, float(alpha), n, p, 0, a, (n + 1) // 2, AP,
+                              self._loffs[(int(N), p)], 0))
         arr = np.array(items, dtype=ITEM_DTYPE)
-        cost = arr["a"].astype(np.float64) ** 2 * arr["b"] + 40.0 * arr["a"] * arr["b"]
+        cost = arr["a"].astype(np.float64) ** 2 * arr["n"] + 40.0 * arr["a"] * arr["n"]
         order = np.argsort(-cost, kind="stable")
         arr = arr[order]
         if ITEM_DTYPE.itemsize != _lib.load().jfx_helm_item_size():
