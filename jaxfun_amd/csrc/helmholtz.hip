@@ -391,6 +391,310 @@ __global__ __launch_bounds__(NW * 64) void helm_ragged_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Compact mixed-precision variant: 4 waves and under 80 KB of LDS, so two
+// workgroups share a CU and one's MFMA phases overlap the other's row solves.
+//   Wk   : 128 x 130 FP32 work block (as above)
+//   slab : 3 bf16 slices x 64 table rows x 32 k, 16-byte chunks swizzled so a
+//          16-lane group of b128 reads covers all banks
+//   H    : h_{q+2j} = 2 / (2(q+2j) + 1), j = 0..b   (FP64)
+// The tridiagonal multipliers live in registers (at most 64 per lane).
+struct HelmCompact {
+  static constexpr int LDW = 130, NT = 256;
+  static constexpr size_t WK = (size_t)HMAXA * LDW * 4;
+  static constexpr size_t SLAB = 3 * 64 * 32 * 2;
+  static constexpr size_t HT = (HMAXA + 2) * 8;
+  static constexpr size_t BYTES = WK + SLAB + HT;
+};
+static_assert(HelmCompact::BYTES + 64 <= 80 * 1024, "two compact workgroups must fit one CU's LDS");
+
+__device__ __forceinline__ int slab_off(int sl, int row, int ck) {  // in bf16 units
+  return ((sl * 64 + row) * 4 + (ck ^ ((row >> 2) & 3))) * 8;
+}
+
+// C = A * Wk for the a x b block; A is the (AP x AP) 3-slice bf16 table `tabs`
+// (row-major, k contiguous).  Wave w owns column panels w and w + 4; the
+// B fragments of a k-slab are split once and reused over both row halves.
+template <typename TIO, bool TO_GLOBAL>
+__device__ __forceinline__ void helmc_gemm(const HelmItem& I, int q, int b, const unsigned short* __restrict__ tabs,
+                                           float* Wk, unsigned short* slab, TIO* __restrict__ Ub) {
+  constexpr int LDW = HelmCompact::LDW, NT = HelmCompact::NT;
+  // opaque copy of the thread id: keeps the compiler from hoisting every
+  // lane-dependent address of the epilogue out of the item loop into registers
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  const int lane = tid & 63, w = tid >> 6;
+  const int a = I.a, AP = I.AP, n = I.n;
+  const int npanel = (b + 15) >> 4, nrb = AP >> 4, nh = (AP + 63) >> 6;
+  const unsigned short* T = tabs + 3 * I.voff;
+  v4f acc[2][8];
+#pragma unroll
+  for (int pp = 0; pp < 2; ++pp)
+#pragma unroll
+    for (int rb = 0; rb < 8; ++rb) acc[pp][rb] = v4f{0, 0, 0, 0};
+
+  v8u16 pre[3];
+  auto fetch = [&](int k0, int h) {
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+      const int e = tid + u * NT, sl = e >> 8, rem = e & 255, row = rem >> 2, ck = rem & 3;
+      const int gr = h * 64 + row;
+      if (gr < AP) pre[u] = *(const v8u16*)(T + (long long)sl * AP * AP + (long long)gr * AP + k0 + ck * 8);
+    }
+  };
+  auto store = [&](int h) {
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+      const int e = tid + u * NT, sl = e >> 8, rem = e & 255, row = rem >> 2, ck = rem & 3;
+      if (h * 64 + row < AP) *(v8u16*)&slab[slab_off(sl, row, ck)] = pre[u];
+    }
+  };
+
+  fetch(0, 0);
+  for (int k0 = 0; k0 < AP; k0 += 32) {
+    v8u16 bh[2], bm[2], bl[2];
+#pragma unroll
+    for (int pp = 0; pp < 2; ++pp) {
+      const int col = (w + 4 * pp) * 16 + (lane & 15);
+      const bool cok = col < b;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int k = k0 + 8 * (lane >> 4) + e;
+        const float x = (cok && k < a) ? Wk[k * LDW + col] : 0.f;
+        unsigned short h, m_, l;
+        split3(x, h, m_, l);
+        bh[pp][e] = h;
+        bm[pp][e] = m_;
+        bl[pp][e] = l;
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (h < nh) {
+        store(h);
+        __syncthreads();
+        {  // prefetch the next slab while this one is multiplied
+          int nk = k0, nhh = h + 1;
+          if (nhh == nh) {
+            nhh = 0;
+            nk += 32;
+          }
+          if (nk < AP) fetch(nk, nhh);
+        }
+#pragma unroll
+        for (int rl = 0; rl < 4; ++rl) {
+          const int rb = h * 4 + rl;
+          if (rb < nrb) {
+            const int row = rl * 16 + (lane & 15), off = slab_off(0, row, lane >> 4);
+            const v8u16 ah = *(const v8u16*)&slab[off];
+            const v8u16 am = *(const v8u16*)&slab[off + 64 * 32];
+            const v8u16 al = *(const v8u16*)&slab[off + 2 * 64 * 32];
+#pragma unroll
+            for (int pp = 0; pp < 2; ++pp) {
+              if (w + 4 * pp < npanel) {
+                // products ordered small to large
+                acc[pp][rb] = mfma_bf16(al, bh[pp], acc[pp][rb]);
+                acc[pp][rb] = mfma_bf16(am, bm[pp], acc[pp][rb]);
+                acc[pp][rb] = mfma_bf16(ah, bl[pp], acc[pp][rb]);
+                acc[pp][rb] = mfma_bf16(am, bh[pp], acc[pp][rb]);
+                acc[pp][rb] = mfma_bf16(ah, bm[pp], acc[pp][rb]);
+                acc[pp][rb] = mfma_bf16(ah, bh[pp], acc[pp][rb]);
+              }
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+  }
+#pragma unroll
+  for (int pp = 0; pp < 2; ++pp) {
+    const int col = (w + 4 * pp) * 16 + (lane & 15);
+    if (col >= b) continue;
+#pragma unroll
+    for (int rb = 0; rb < 8; ++rb) {
+      if (rb >= nrb) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = rb * 16 + mfma16_row_f32(lane, r);
+        if (m >= a) continue;
+        if constexpr (TO_GLOBAL)
+          Ub[(long long)(I.p + 2 * m) * n + q + 2 * col] = (TIO)acc[pp][rb][r];
+        else
+          Wk[m * LDW + col] = acc[pp][rb][r];
+      }
+    }
+  }
+}
+
+template <typename TIO>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void helm_compact_kernel(const HelmItem* __restrict__ items, int nitems,
+                                                           int* __restrict__ counter,
+                                                           const unsigned short* __restrict__ Vs,
+                                                           const unsigned short* __restrict__ VTs,
+                                                           const double* __restrict__ lam,
+                                                           const TIO* __restrict__ F, TIO* __restrict__ U) {
+  constexpr int LDW = HelmCompact::LDW, NT = HelmCompact::NT;
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  float* Wk = reinterpret_cast<float*>(smem_raw);
+  unsigned short* slab = reinterpret_cast<unsigned short*>(smem_raw + HelmCompact::WK);
+  double* H = reinterpret_cast<double*>(smem_raw + HelmCompact::WK + HelmCompact::SLAB);
+  __shared__ int s_item;
+  const int tid = threadIdx.x;
+
+  for (;;) {
+    if (tid == 0) s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int it = s_item;
+    if (it >= nitems) break;
+    const HelmItem I = items[it];
+    const int n = I.n, p = I.p, a = I.a;
+    const TIO* Fb = F + I.foff;
+
+    for (int q = 0; q < 2; ++q) {
+      const int b = (n - q + 1) >> 1;
+      if (b == 0) continue;
+      // ---------------- phase 0: F_pq -> Wk, h table
+      for (int e0 = tid; e0 < a * b; e0 += 8 * NT) {
+        TIO v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int e = e0 + u * NT;
+          if (e < a * b) {
+            const int r = e / b, c = e - r * b;
+            v[u] = Fb[(long long)(p + 2 * r) * n + q + 2 * c];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int e = e0 + u * NT;
+          if (e < a * b) {
+            const int r = e / b, c = e - r * b;
+            Wk[r * LDW + c] = (float)v[u];
+          }
+        }
+      }
+      for (int j = tid; j <= b; j += NT) H[j] = 2.0 / (2 * (q + 2 * j) + 1);
+      __syncthreads();
+
+      // ---------------- phase 1: G = V_p^T F_pq (in place)
+      helmc_gemm<TIO, false>(I, q, b, VTs, Wk, slab, nullptr);
+      __syncthreads();
+
+      // ---------------- phase 2: FP64 row solves, two lanes per row (twisted)
+      // T = s M_q + diag(4c + 6), c = q + 2j:  d_j = s (H_j + H_{j+1}) + 4q + 6 + 8j,
+      // o_j = T[j][j+1] = -s H_{j+1}.  Lane (r, 0) eliminates rows 0..m-1 downwards,
+      // lane (r, 1) rows b-1..m+1 upwards (step i visits j = i or b-1-i); pivots are
+      // continuant ratios P_{j-1} / P_j with P_j = d_j P_{j-1} - o^2 P_{j-2}.
+      // Loads of a chunk of 8 steps are issued before its arithmetic.
+      {
+        const int r = tid >> 1, side = tid & 1;
+        const bool act = r < a;
+        const double s = act ? I.alpha - lam[I.loff + r] : 1.0;
+        float* row = Wk + (act ? r : 0) * LDW;
+        const int m = b >> 1;
+        const int len = act ? (side ? b - 1 - m : m) : 0;
+        const double dq = 4.0 * q + 6.0;
+        float cm[64];
+        double P1 = 1.0, P2 = 0.0, yprev = 0.0, oprev = 0.0, cl = 0.0;
+#pragma unroll
+        for (int ch = 0; ch < 8; ++ch) {
+          if (ch * 8 < len) {
+            float g[8];
+            double h0[8], h1[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              const int i = ch * 8 + u;
+              int j = side ? b - 1 - i : i;
+              j = j < 0 ? 0 : (j > b - 1 ? b - 1 : j);
+              g[u] = row[j];
+              h0[u] = H[j];
+              h1[u] = H[j + 1];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              const int i = ch * 8 + u;
+              if (i < len) {
+                const int j = side ? b - 1 - i : i;
+                const double d = fma(s, h0[u] + h1[u], fma(8.0, (double)j, dq));
+                const double on = -s * (side ? h0[u] : h1[u]);  // coupling towards the middle
+                const double Pj = fma(d, P1, -(oprev * oprev) * P2);
+                const double rj = P1 * rcp64(Pj);
+                const double yj = fma(-oprev, yprev, (double)g[u]) * rj;
+                cl = on * rj;
+                cm[i] = (float)cl;
+                row[j] = (float)yj;
+                P2 = P1;
+                P1 = Pj;
+                yprev = yj;
+                oprev = on;
+              }
+            }
+            renorm(P1, P2);
+          }
+        }
+        const double dl = yprev;
+        const double co = __shfl_xor(cl, 1), dO = __shfl_xor(dl, 1);
+        if (act) {
+          const double ct = side == 0 ? cl : co, dt = side == 0 ? dl : dO;  // rows < m
+          const double fb = side == 0 ? co : cl, eb = side == 0 ? dO : dl;  // rows > m
+          const double am = (m > 0) ? -s * H[m] : 0.0;
+          const double cmm = (m < b - 1) ? -s * H[m + 1] : 0.0;
+          const double dm = fma(s, H[m] + H[m + 1], fma(8.0, (double)m, dq));
+          const double um = ((double)row[m] - am * dt - cmm * eb) / (dm - am * ct - cmm * fb);
+          // back substitution outwards from m, i = len-1 .. 0
+          double uprev = um;
+#pragma unroll
+          for (int ch = 7; ch >= 0; --ch) {
+            if (ch * 8 < len) {
+              float y[8];
+#pragma unroll
+              for (int u = 0; u < 8; ++u) {
+                const int i = ch * 8 + u;
+                int j = side ? b - 1 - i : i;
+                j = j < 0 ? 0 : (j > b - 1 ? b - 1 : j);
+                y[u] = row[j];
+              }
+#pragma unroll
+              for (int u = 7; u >= 0; --u) {
+                const int i = ch * 8 + u;
+                if (i < len) {
+                  const int j = side ? b - 1 - i : i;
+                  const double v = fma(-(double)cm[i], uprev, (double)y[u]);
+                  row[j] = (float)v;
+                  uprev = v;
+                }
+              }
+            }
+          }
+          if (side == 0) row[m] = (float)um;
+        }
+      }
+      __syncthreads();
+
+      // ---------------- phase 3: U_pq = V_p U~ -> global
+      helmc_gemm<TIO, true>(I, q, b, Vs, Wk, slab, U + I.foff);
+      __syncthreads();
+    }
+    __syncthreads();
+  }
+}
+
+template <typename TIO>
+static int launch_helm_compact(const HelmItem* items, int nitems, int* counter, const unsigned short* Vs,
+                               const unsigned short* VTs, const double* lam, const TIO* F, TIO* U, int nblocks,
+                               hipStream_t st) {
+  const size_t shm = HelmCompact::BYTES;
+  auto kern = helm_compact_kernel<TIO>;
+  hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+  if (e != hipSuccess) return set_error("helm: hipFuncSetAttribute: %s", hipGetErrorString(e)), JFX_EHIP;
+  hipLaunchKernelGGL(helm_reset, dim3(1), dim3(64), 0, st, counter);
+  hipLaunchKernelGGL(kern, dim3(nblocks), dim3(HelmCompact::NT), shm, st, items, nitems, counter, Vs, VTs, lam,
+                     F, U);
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
+}
+
 template <int MODE, int NW, typename TIO>
 static int launch_helm(const HelmItem* items, int nitems, int* counter, const double* Vd,
                        const double* VTd, const unsigned short* Vs, const unsigned short* VTs,
@@ -441,15 +745,15 @@ int jfx_helm_ragged(int mode, int io_f32, int nwaves, const void* items, int nit
     if (!Vs || !VTs) return set_error("helm_ragged: mode 1 needs bf16 slice tables"), JFX_EARG;
     if (io_f32) {
       if (nwaves == 4)
-        return launch_helm<1, 4, float>(it, nitems, counter, Vd, VTd, Vs, VTs, lam, (const float*)F,
-                                        (float*)U, scratch, nblocks, stream);
+        return launch_helm_compact<float>(it, nitems, counter, Vs, VTs, lam, (const float*)F, (float*)U, nblocks,
+                                          stream);
       if (nwaves == 8)
         return launch_helm<1, 8, float>(it, nitems, counter, Vd, VTd, Vs, VTs, lam, (const float*)F,
                                         (float*)U, scratch, nblocks, stream);
     } else {
       if (nwaves == 4)
-        return launch_helm<1, 4, double>(it, nitems, counter, Vd, VTd, Vs, VTs, lam, (const double*)F,
-                                         (double*)U, scratch, nblocks, stream);
+        return launch_helm_compact<double>(it, nitems, counter, Vs, VTs, lam, (const double*)F, (double*)U,
+                                           nblocks, stream);
       if (nwaves == 8)
         return launch_helm<1, 8, double>(it, nitems, counter, Vd, VTd, Vs, VTs, lam, (const double*)F,
                                          (double*)U, scratch, nblocks, stream);
