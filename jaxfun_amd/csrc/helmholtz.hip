@@ -46,15 +46,6 @@ __global__ void helm_reset(int* counter) {
   if (threadIdx.x == 0) counter[0] = 0;
 }
 
-// Split an fp32 value into 3 bf16 terms.
-__device__ __forceinline__ void split3(float x, unsigned short& h, unsigned short& m, unsigned short& l) {
-  h = f2bf(x);
-  float r = x - bf2f(h);
-  m = f2bf(r);
-  r -= bf2f(m);
-  l = f2bf(r);
-}
-
 // FP64 reciprocal: hardware estimate refined by two Newton steps (x > 0, normal).
 __device__ __forceinline__ double rcp64(double x) {
   double r = __builtin_amdgcn_rcp(x);
@@ -168,16 +159,13 @@ __device__ __forceinline__ void helm_gemm_phase(const HelmItem& I, int q, int b,
           constexpr int VLD = HelmLDS<1>::VLD;
           const unsigned short* Vsl = reinterpret_cast<const unsigned short*>(vslab);
           v8u16 bh, bm, bl;
+          float xv[8];
 #pragma unroll
           for (int e = 0; e < 8; ++e) {
             const int k = k0 + 8 * (lane >> 4) + e;
-            float x = (cok && k < a) ? (float)Wk[k * LDW + col] : 0.f;
-            unsigned short h, m_, l;
-            split3(x, h, m_, l);
-            bh[e] = h;
-            bm[e] = m_;
-            bl[e] = l;
+            xv[e] = (cok && k < a) ? (float)Wk[k * LDW + col] : 0.f;
           }
+          split3_pk8(xv, bh, bm, bl);
 #pragma unroll
           for (int rb = 0; rb < 8; ++rb) {
             if (rb < nrb) {
@@ -457,16 +445,13 @@ __device__ __forceinline__ void helmc_gemm(const HelmItem& I, int q, int b, cons
     for (int pp = 0; pp < 2; ++pp) {
       const int col = (w + 4 * pp) * 16 + (lane & 15);
       const bool cok = col < b;
+      float xv[8];
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         const int k = k0 + 8 * (lane >> 4) + e;
-        const float x = (cok && k < a) ? Wk[k * LDW + col] : 0.f;
-        unsigned short h, m_, l;
-        split3(x, h, m_, l);
-        bh[pp][e] = h;
-        bm[pp][e] = m_;
-        bl[pp][e] = l;
+        xv[e] = (cok && k < a) ? Wk[k * LDW + col] : 0.f;
       }
+      split3_pk8(xv, bh[pp], bm[pp], bl[pp]);
     }
 #pragma unroll
     for (int h = 0; h < 2; ++h) {

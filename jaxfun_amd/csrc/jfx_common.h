@@ -52,6 +52,34 @@ __device__ __forceinline__ float bf2f(unsigned short b) {
   return __uint_as_float(((unsigned int)b) << 16);
 }
 
+// Split 8 fp32 values into 3 bf16 terms x = h + m + l (each round to nearest
+// even, the remainders exact), packed for an MFMA operand.  Uses the packed
+// hardware conversion (two values per instruction); results equal f2bf-based
+// splitting bit for bit on finite inputs.
+__device__ __forceinline__ void split3_pk8(const float (&x)[8], v8u16& H, v8u16& M, v8u16& L) {
+  typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  u4 hw, mw, lw;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const float x0 = x[2 * e], x1 = x[2 * e + 1];
+    bf2 h = __builtin_convertvector((f2){x0, x1}, bf2);
+    const unsigned hu = __builtin_bit_cast(unsigned, h);
+    const float r0 = x0 - __uint_as_float(hu << 16), r1 = x1 - __uint_as_float(hu & 0xFFFF0000u);
+    bf2 m = __builtin_convertvector((f2){r0, r1}, bf2);
+    const unsigned mu = __builtin_bit_cast(unsigned, m);
+    const float s0 = r0 - __uint_as_float(mu << 16), s1 = r1 - __uint_as_float(mu & 0xFFFF0000u);
+    bf2 l = __builtin_convertvector((f2){s0, s1}, bf2);
+    hw[e] = hu;
+    mw[e] = mu;
+    lw[e] = __builtin_bit_cast(unsigned, l);
+  }
+  H = __builtin_bit_cast(v8u16, hw);
+  M = __builtin_bit_cast(v8u16, mw);
+  L = __builtin_bit_cast(v8u16, lw);
+}
+
 // Map a linear workgroup id so that consecutive logical tiles land on the
 // same XCD (workgroups are dispatched round-robin over the 8 XCDs).
 __device__ __forceinline__ unsigned xcd_remap(unsigned bid, unsigned nblk) {
