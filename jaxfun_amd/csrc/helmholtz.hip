@@ -235,6 +235,39 @@ __device__ __forceinline__ void renorm(double& P1, double& P2) {
   P2 = ldexp(P2, -e);
 }
 
+// Gather the parity block F[p::2, q::2] (a x b) into the LDS work block:
+// lane l of wave w takes columns l and l + 64 of rows w, w + NW, ...; a
+// round's loads are all issued before its LDS stores, and no index needs an
+// integer division.
+template <int NW, int LDW, typename TIO, typename TS>
+__device__ __forceinline__ void helm_gather(const TIO* __restrict__ Fb, int n, int p, int q, int a, int b,
+                                            TS* Wk) {
+  constexpr int R = sizeof(TIO) == 4 ? 16 : 8;  // rows per wave per round
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const bool ok0 = lane < b, ok1 = lane + 64 < b;
+  const long long rs = 2LL * n;  // one parity row
+  for (int r0 = w; r0 < a; r0 += R * NW) {
+    const TIO* src = Fb + (long long)(p + 2 * r0) * n + q + 2 * lane;
+    TIO v0[R], v1[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      if (r0 + i * NW < a) {
+        const TIO* s = src + i * NW * rs;
+        if (ok0) v0[i] = s[0];
+        if (ok1) v1[i] = s[128];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const int r = r0 + i * NW;
+      if (r < a) {
+        if (ok0) Wk[r * LDW + lane] = (TS)v0[i];
+        if (ok1) Wk[r * LDW + lane + 64] = (TS)v1[i];
+      }
+    }
+  }
+}
+
 template <int MODE, int NW, typename TIO>
 __global__ __launch_bounds__(NW * 64) void helm_ragged_kernel(
     const HelmItem* __restrict__ items, int nitems, int* __restrict__ counter,
@@ -250,16 +283,20 @@ __global__ __launch_bounds__(NW * 64) void helm_ragged_kernel(
   double* Hs = reinterpret_cast<double*>(smem_raw + HelmLDS<MODE>::WK + HelmLDS<MODE>::U1);  // h_c + h_{c+2}
   double* H2 = Hs + HMAXA;                                                                 // h_{c+2}
   double* Dg = H2 + HMAXA;                                                                 // 4c + 6
-  __shared__ int s_item;
+  __shared__ int s_nx[2];
   const int tid = threadIdx.x;
   CPStore<MODE> cps{reinterpret_cast<float*>(u1), scratch + (long long)blockIdx.x * HMAXA * HMAXA};
 
-  for (;;) {
-    if (tid == 0) s_item = atomicAdd(counter, 1);
-    __syncthreads();
-    const int it = s_item;
-    if (it >= nitems) break;
+  // the next item's index is claimed at the start of the current one, so the
+  // counter round trip overlaps the work (two slots: slot k is rewritten only
+  // after every thread has passed the barriers of the item in between)
+  if (tid == 0) s_nx[0] = atomicAdd(counter, 1);
+  __syncthreads();
+  int it = s_nx[0], slot = 1;
+  while (it < nitems) {
     const HelmItem I = items[it];
+    int nxt = 0;
+    if (tid == 0) nxt = atomicAdd(counter, 1);
     const int n = I.n, p = I.p, a = I.a;
     const TIO* Fb = F + I.foff;
 
@@ -268,26 +305,8 @@ __global__ __launch_bounds__(NW * 64) void helm_ragged_kernel(
     for (int q = 0; q < 2; ++q) {
       const int b = (n - q + 1) >> 1;
       if (b == 0) continue;
-      // ---------------- phase 0: F_pq -> Wk (8 loads in flight per thread), column constants
-      for (int e0 = tid; e0 < a * b; e0 += 8 * NT) {
-        TIO v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int e = e0 + u * NT;
-          if (e < a * b) {
-            const int r = e / b, c = e - r * b;
-            v[u] = Fb[(long long)(p + 2 * r) * n + q + 2 * c];
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int e = e0 + u * NT;
-          if (e < a * b) {
-            const int r = e / b, c = e - r * b;
-            Wk[r * LDW + c] = (TS)v[u];
-          }
-        }
-      }
+      // ---------------- phase 0: F_pq -> Wk, column constants
+      helm_gather<NW, LDW>(Fb, n, p, q, a, b, Wk);
       for (int j = tid; j < b; j += NT) {
         const int c = q + 2 * j;
         const double hc = 2.0 / (2 * c + 1), hc2 = 2.0 / (2 * c + 5);
@@ -376,6 +395,10 @@ __global__ __launch_bounds__(NW * 64) void helm_ragged_kernel(
       helm_gemm_phase<MODE, NW, TIO, true>(I, q, b, VTd, Vs, Wk, u1, U + I.foff);
       __syncthreads();
     }
+    if (tid == 0) s_nx[slot] = nxt;
+    __syncthreads();
+    it = s_nx[slot];
+    slot ^= 1;
   }
 }
 
@@ -524,15 +547,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   float* Wk = reinterpret_cast<float*>(smem_raw);
   unsigned short* slab = reinterpret_cast<unsigned short*>(smem_raw + HelmCompact::WK);
   double* H = reinterpret_cast<double*>(smem_raw + HelmCompact::WK + HelmCompact::SLAB);
-  __shared__ int s_item;
+  __shared__ int s_nx[2];
   const int tid = threadIdx.x;
 
-  for (;;) {
-    if (tid == 0) s_item = atomicAdd(counter, 1);
-    __syncthreads();
-    const int it = s_item;
-    if (it >= nitems) break;
+  // the next item's index is claimed at the start of the current one, so the
+  // counter round trip overlaps the work (two slots: slot k is rewritten only
+  // after every thread has passed the barriers of the item in between)
+  if (tid == 0) s_nx[0] = atomicAdd(counter, 1);
+  __syncthreads();
+  int it = s_nx[0], slot = 1;
+  while (it < nitems) {
     const HelmItem I = items[it];
+    int nxt = 0;
+    if (tid == 0) nxt = atomicAdd(counter, 1);
     const int n = I.n, p = I.p, a = I.a;
     const TIO* Fb = F + I.foff;
 
@@ -540,25 +567,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       const int b = (n - q + 1) >> 1;
       if (b == 0) continue;
       // ---------------- phase 0: F_pq -> Wk, h table
-      for (int e0 = tid; e0 < a * b; e0 += 8 * NT) {
-        TIO v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int e = e0 + u * NT;
-          if (e < a * b) {
-            const int r = e / b, c = e - r * b;
-            v[u] = Fb[(long long)(p + 2 * r) * n + q + 2 * c];
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int e = e0 + u * NT;
-          if (e < a * b) {
-            const int r = e / b, c = e - r * b;
-            Wk[r * LDW + c] = (float)v[u];
-          }
-        }
-      }
+      helm_gather<4, LDW>(Fb, n, p, q, a, b, Wk);
       for (int j = tid; j <= b; j += NT) H[j] = 2.0 / (2 * (q + 2 * j) + 1);
       __syncthreads();
 
@@ -661,7 +670,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       helmc_gemm<TIO, true>(I, q, b, Vs, Wk, slab, U + I.foff);
       __syncthreads();
     }
+    if (tid == 0) s_nx[slot] = nxt;
     __syncthreads();
+    it = s_nx[slot];
+    slot ^= 1;
   }
 }
 
