@@ -423,47 +423,70 @@ __device__ __forceinline__ int slab_off(int sl, int row, int ck) {  // in bf16 u
   return ((sl * 64 + row) * 4 + (ck ^ ((row >> 2) & 3))) * 8;
 }
 
-// C = A * Wk for the a x b block; A is the (AP x AP) 3-slice bf16 table `tabs`
-// (row-major, k contiguous).  Wave w owns column panels w and w + 4; the
-// B fragments of a k-slab are split once and reused over both row halves.
-template <typename TIO, bool TO_GLOBAL>
-__device__ __forceinline__ void helmc_gemm(const HelmItem& I, int q, int b, const unsigned short* __restrict__ tabs,
-                                           float* Wk, unsigned short* slab, TIO* __restrict__ Ub) {
-  constexpr int LDW = HelmCompact::LDW, NT = HelmCompact::NT;
+// One k-slab of a 3-slice table in flight in registers (3 x 16 bytes per thread).
+struct SlabRegs {
+  v8u16 v[3];
+};
+
+// Issue the loads of slab (k0, half h) of table T (three AP x AP bf16 slices).
+__device__ __forceinline__ void slab_fetch(SlabRegs& P, const unsigned short* __restrict__ T, int AP, int k0, int h,
+                                           int tid) {
+#pragma unroll
+  for (int u = 0; u < 3; ++u) {
+    const int e = tid + u * 256, sl = e >> 8, rem = e & 255, row = rem >> 2, ck = rem & 3;
+    const int gr = h * 64 + row;
+    if (gr < AP) P.v[u] = *(const v8u16*)(T + (long long)sl * AP * AP + (long long)gr * AP + k0 + ck * 8);
+  }
+}
+
+__device__ __forceinline__ void slab_store(const SlabRegs& P, unsigned short* slab, int AP, int h, int tid) {
+#pragma unroll
+  for (int u = 0; u < 3; ++u) {
+    const int e = tid + u * 256, sl = e >> 8, rem = e & 255, row = rem >> 2, ck = rem & 3;
+    if (h * 64 + row < AP) *(v8u16*)&slab[slab_off(sl, row, ck)] = P.v[u];
+  }
+}
+
+// The first two slabs of a GEMM phase, issued one phase ahead.  NH = number of
+// 64-row halves of the table (AP > 64 -> 2); with NH = 1 the two slabs are
+// the table's only k-slabs (AP <= 64).
+template <int NH>
+__device__ __forceinline__ void slab_prefetch2(SlabRegs (&P)[2], const unsigned short* __restrict__ T, int AP,
+                                               int tid) {
+  if constexpr (NH == 2) {
+    slab_fetch(P[0], T, AP, 0, 0, tid);
+    slab_fetch(P[1], T, AP, 0, 1, tid);
+  } else {
+    slab_fetch(P[0], T, AP, 0, 0, tid);
+    if (AP > 32) slab_fetch(P[1], T, AP, 32, 0, tid);
+  }
+}
+
+// C = A * Wk for the a x b block; A is the (AP x AP) 3-slice bf16 table T
+// (row-major, k contiguous) whose first two slabs are already in flight in P.
+// Wave w owns column panels w and w + 4; the B fragments of a k-slab are
+// split once and reused over both row halves.  Each slab is refetched two
+// steps ahead of its use.
+template <typename TIO, bool TO_GLOBAL, int NH>
+__device__ __forceinline__ void helmc_gemm(const HelmItem& I, int q, int b, const unsigned short* __restrict__ T,
+                                           float* Wk, unsigned short* slab, TIO* __restrict__ Ub,
+                                           SlabRegs (&P)[2]) {
+  constexpr int LDW = HelmCompact::LDW;
   // opaque copy of the thread id: keeps the compiler from hoisting every
   // lane-dependent address of the epilogue out of the item loop into registers
   int tid = threadIdx.x;
   asm volatile("" : "+v"(tid));
   const int lane = tid & 63, w = tid >> 6;
   const int a = I.a, AP = I.AP, n = I.n;
-  const int npanel = (b + 15) >> 4, nrb = AP >> 4, nh = (AP + 63) >> 6;
-  const unsigned short* T = tabs + 3 * I.voff;
+  const int npanel = (b + 15) >> 4, nrb = AP >> 4;
   v4f acc[2][8];
 #pragma unroll
   for (int pp = 0; pp < 2; ++pp)
 #pragma unroll
     for (int rb = 0; rb < 8; ++rb) acc[pp][rb] = v4f{0, 0, 0, 0};
 
-  v8u16 pre[3];
-  auto fetch = [&](int k0, int h) {
-#pragma unroll
-    for (int u = 0; u < 3; ++u) {
-      const int e = tid + u * NT, sl = e >> 8, rem = e & 255, row = rem >> 2, ck = rem & 3;
-      const int gr = h * 64 + row;
-      if (gr < AP) pre[u] = *(const v8u16*)(T + (long long)sl * AP * AP + (long long)gr * AP + k0 + ck * 8);
-    }
-  };
-  auto store = [&](int h) {
-#pragma unroll
-    for (int u = 0; u < 3; ++u) {
-      const int e = tid + u * NT, sl = e >> 8, rem = e & 255, row = rem >> 2, ck = rem & 3;
-      if (h * 64 + row < AP) *(v8u16*)&slab[slab_off(sl, row, ck)] = pre[u];
-    }
-  };
-
-  fetch(0, 0);
-  for (int k0 = 0; k0 < AP; k0 += 32) {
-    v8u16 bh[2], bm[2], bl[2];
+  v8u16 bh[2], bm[2], bl[2];
+  auto bfrag = [&](int k0) {
 #pragma unroll
     for (int pp = 0; pp < 2; ++pp) {
       const int col = (w + 4 * pp) * 16 + (lane & 15);
@@ -476,41 +499,55 @@ __device__ __forceinline__ void helmc_gemm(const HelmItem& I, int q, int b, cons
       }
       split3_pk8(xv, bh[pp], bm[pp], bl[pp]);
     }
+  };
+  auto mult = [&](auto hc) {
+    constexpr int h = decltype(hc)::value;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      if (h < nh) {
-        store(h);
+    for (int rl = 0; rl < 4; ++rl) {
+      const int rb = h * 4 + rl;
+      if (rb < nrb) {
+        const int row = rl * 16 + (lane & 15), off = slab_off(0, row, lane >> 4);
+        const v8u16 ah = *(const v8u16*)&slab[off];
+        const v8u16 am = *(const v8u16*)&slab[off + 64 * 32];
+        const v8u16 al = *(const v8u16*)&slab[off + 2 * 64 * 32];
+#pragma unroll
+        for (int pp = 0; pp < 2; ++pp) {
+          if (w + 4 * pp < npanel) {
+            // products ordered small to large
+            acc[pp][rb] = mfma_bf16(al, bh[pp], acc[pp][rb]);
+            acc[pp][rb] = mfma_bf16(am, bm[pp], acc[pp][rb]);
+            acc[pp][rb] = mfma_bf16(ah, bl[pp], acc[pp][rb]);
+            acc[pp][rb] = mfma_bf16(am, bh[pp], acc[pp][rb]);
+            acc[pp][rb] = mfma_bf16(ah, bm[pp], acc[pp][rb]);
+            acc[pp][rb] = mfma_bf16(ah, bh[pp], acc[pp][rb]);
+          }
+        }
+      }
+    }
+  };
+
+  if constexpr (NH == 2) {
+    for (int k0 = 0; k0 < AP; k0 += 32) {
+      bfrag(k0);
+      slab_store(P[0], slab, AP, 0, tid);
+      __syncthreads();
+      if (k0 + 32 < AP) slab_fetch(P[0], T, AP, k0 + 32, 0, tid);
+      mult(std::integral_constant<int, 0>());
+      __syncthreads();
+      slab_store(P[1], slab, AP, 1, tid);
+      __syncthreads();
+      if (k0 + 32 < AP) slab_fetch(P[1], T, AP, k0 + 32, 1, tid);
+      mult(std::integral_constant<int, 1>());
+      __syncthreads();
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      if (32 * s < AP) {
+        bfrag(32 * s);
+        slab_store(P[s], slab, AP, 0, tid);
         __syncthreads();
-        {  // prefetch the next slab while this one is multiplied
-          int nk = k0, nhh = h + 1;
-          if (nhh == nh) {
-            nhh = 0;
-            nk += 32;
-          }
-          if (nk < AP) fetch(nk, nhh);
-        }
-#pragma unroll
-        for (int rl = 0; rl < 4; ++rl) {
-          const int rb = h * 4 + rl;
-          if (rb < nrb) {
-            const int row = rl * 16 + (lane & 15), off = slab_off(0, row, lane >> 4);
-            const v8u16 ah = *(const v8u16*)&slab[off];
-            const v8u16 am = *(const v8u16*)&slab[off + 64 * 32];
-            const v8u16 al = *(const v8u16*)&slab[off + 2 * 64 * 32];
-#pragma unroll
-            for (int pp = 0; pp < 2; ++pp) {
-              if (w + 4 * pp < npanel) {
-                // products ordered small to large
-                acc[pp][rb] = mfma_bf16(al, bh[pp], acc[pp][rb]);
-                acc[pp][rb] = mfma_bf16(am, bm[pp], acc[pp][rb]);
-                acc[pp][rb] = mfma_bf16(ah, bl[pp], acc[pp][rb]);
-                acc[pp][rb] = mfma_bf16(am, bh[pp], acc[pp][rb]);
-                acc[pp][rb] = mfma_bf16(ah, bm[pp], acc[pp][rb]);
-                acc[pp][rb] = mfma_bf16(ah, bh[pp], acc[pp][rb]);
-              }
-            }
-          }
-        }
+        mult(std::integral_constant<int, 0>());
         __syncthreads();
       }
     }
@@ -535,14 +572,136 @@ __device__ __forceinline__ void helmc_gemm(const HelmItem& I, int q, int b, cons
   }
 }
 
-template <typename TIO>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void helm_compact_kernel(const HelmItem* __restrict__ items, int nitems,
-                                                           int* __restrict__ counter,
-                                                           const unsigned short* __restrict__ Vs,
-                                                           const unsigned short* __restrict__ VTs,
-                                                           const double* __restrict__ lam,
-                                                           const TIO* __restrict__ F, TIO* __restrict__ U) {
+// Both column parities of one work item (solve, row parity p).
+template <typename TIO, int NH>
+__device__ __forceinline__ void helmc_item(const HelmItem& I, const unsigned short* __restrict__ Vs,
+                                           const unsigned short* __restrict__ VTs, const double* __restrict__ lam,
+                                           const TIO* __restrict__ F, TIO* __restrict__ U, float* Wk,
+                                           unsigned short* slab, double* H) {
   constexpr int LDW = HelmCompact::LDW, NT = HelmCompact::NT;
+  const int tid = threadIdx.x;
+  const int n = I.n, p = I.p, a = I.a, AP = I.AP;
+  const unsigned short* TV = Vs + 3 * I.voff;    // V_p slices    (phase 3)
+  const unsigned short* TVT = VTs + 3 * I.voff;  // V_p^T slices  (phase 1)
+  SlabRegs P[2];
+  slab_prefetch2<NH>(P, TVT, AP, tid);
+
+  for (int q = 0; q < 2; ++q) {
+    const int b = (n - q + 1) >> 1;
+    if (b == 0) continue;
+    // ---------------- phase 0: F_pq -> Wk, h table
+    helm_gather<4, LDW>(F + I.foff, n, p, q, a, b, Wk);
+    for (int j = tid; j <= b; j += NT) H[j] = 2.0 / (2 * (q + 2 * j) + 1);
+    __syncthreads();
+
+    // ---------------- phase 1: G = V_p^T F_pq (in place)
+    helmc_gemm<TIO, false, NH>(I, q, b, TVT, Wk, slab, nullptr, P);
+    slab_prefetch2<NH>(P, TV, AP, tid);  // phase 3's first slabs fly during the row solves
+    __syncthreads();
+
+    // ---------------- phase 2: FP64 row solves, two lanes per row (twisted)
+    // T = s M_q + diag(4c + 6), c = q + 2j:  d_j = s (H_j + H_{j+1}) + 4q + 6 + 8j,
+    // o_j = T[j][j+1] = -s H_{j+1}.  Lane (r, 0) eliminates rows 0..m-1 downwards,
+    // lane (r, 1) rows b-1..m+1 upwards (step i visits j = i or b-1-i); pivots are
+    // continuant ratios P_{j-1} / P_j with P_j = d_j P_{j-1} - o^2 P_{j-2}.
+    // Loads of a chunk of 8 steps are issued before its arithmetic.
+    {
+      const int r = tid >> 1, side = tid & 1;
+      const bool act = r < a;
+      const double s = act ? I.alpha - lam[I.loff + r] : 1.0;
+      float* row = Wk + (act ? r : 0) * LDW;
+      const int m = b >> 1;
+      const int len = act ? (side ? b - 1 - m : m) : 0;
+      const double dq = 4.0 * q + 6.0;
+      float cm[64];
+      double P1 = 1.0, P2 = 0.0, yprev = 0.0, oprev = 0.0, cl = 0.0;
+#pragma unroll
+      for (int ch = 0; ch < 8; ++ch) {
+        if (ch * 8 < len) {
+          float g[8];
+          double h0[8], h1[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int i = ch * 8 + u;
+            int j = side ? b - 1 - i : i;
+            j = j < 0 ? 0 : (j > b - 1 ? b - 1 : j);
+            g[u] = row[j];
+            h0[u] = H[j];
+            h1[u] = H[j + 1];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int i = ch * 8 + u;
+            if (i < len) {
+              const int j = side ? b - 1 - i : i;
+              const double d = fma(s, h0[u] + h1[u], fma(8.0, (double)j, dq));
+              const double on = -s * (side ? h0[u] : h1[u]);  // coupling towards the middle
+              const double Pj = fma(d, P1, -(oprev * oprev) * P2);
+              const double rj = P1 * rcp64(Pj);
+              const double yj = fma(-oprev, yprev, (double)g[u]) * rj;
+              cl = on * rj;
+              cm[i] = (float)cl;
+              row[j] = (float)yj;
+              P2 = P1;
+              P1 = Pj;
+              yprev = yj;
+              oprev = on;
+            }
+          }
+          renorm(P1, P2);
+        }
+      }
+      const double dl = yprev;
+      const double co = __shfl_xor(cl, 1), dO = __shfl_xor(dl, 1);
+      if (act) {
+        const double ct = side == 0 ? cl : co, dt = side == 0 ? dl : dO;  // rows < m
+        const double fb = side == 0 ? co : cl, eb = side == 0 ? dO : dl;  // rows > m
+        const double am = (m > 0) ? -s * H[m] : 0.0;
+        const double cmm = (m < b - 1) ? -s * H[m + 1] : 0.0;
+        const double dm = fma(s, H[m] + H[m + 1], fma(8.0, (double)m, dq));
+        const double um = ((double)row[m] - am * dt - cmm * eb) / (dm - am * ct - cmm * fb);
+        // back substitution outwards from m, i = len-1 .. 0
+        double uprev = um;
+#pragma unroll
+        for (int ch = 7; ch >= 0; --ch) {
+          if (ch * 8 < len) {
+            float y[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              const int i = ch * 8 + u;
+              int j = side ? b - 1 - i : i;
+              j = j < 0 ? 0 : (j > b - 1 ? b - 1 : j);
+              y[u] = row[j];
+            }
+#pragma unroll
+            for (int u = 7; u >= 0; --u) {
+              const int i = ch * 8 + u;
+              if (i < len) {
+                const int j = side ? b - 1 - i : i;
+                const double v = fma(-(double)cm[i], uprev, (double)y[u]);
+                row[j] = (float)v;
+                uprev = v;
+              }
+            }
+          }
+        }
+        if (side == 0) row[m] = (float)um;
+      }
+    }
+    __syncthreads();
+
+    // ---------------- phase 3: U_pq = V_p U~ -> global
+    helmc_gemm<TIO, true, NH>(I, q, b, TV, Wk, slab, U + I.foff, P);
+    if (q == 0) slab_prefetch2<NH>(P, TVT, AP, tid);  // the next pass's phase 1
+    __syncthreads();
+  }
+}
+
+template <typename TIO>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void helm_compact_kernel(
+    const HelmItem* __restrict__ items, int nitems, int* __restrict__ counter, const unsigned short* __restrict__ Vs,
+    const unsigned short* __restrict__ VTs, const double* __restrict__ lam, const TIO* __restrict__ F,
+    TIO* __restrict__ U) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   float* Wk = reinterpret_cast<float*>(smem_raw);
   unsigned short* slab = reinterpret_cast<unsigned short*>(smem_raw + HelmCompact::WK);
@@ -550,129 +709,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   __shared__ int s_nx[2];
   const int tid = threadIdx.x;
 
-  // the next item's index is claimed at the start of the current one, so the
-  // counter round trip overlaps the work (two slots: slot k is rewritten only
-  // after every thread has passed the barriers of the item in between)
+  // the next item's index is claimed at the start of the current one (two
+  // slots: slot k is rewritten only after every thread has passed the
+  // barriers of the item in between)
   if (tid == 0) s_nx[0] = atomicAdd(counter, 1);
   __syncthreads();
-  int it = s_nx[0], slot = 1;
+  int it = __builtin_amdgcn_readfirstlane(s_nx[0]), slot = 1;
   while (it < nitems) {
     const HelmItem I = items[it];
     int nxt = 0;
     if (tid == 0) nxt = atomicAdd(counter, 1);
-    const int n = I.n, p = I.p, a = I.a;
-    const TIO* Fb = F + I.foff;
-
-    for (int q = 0; q < 2; ++q) {
-      const int b = (n - q + 1) >> 1;
-      if (b == 0) continue;
-      // ---------------- phase 0: F_pq -> Wk, h table
-      helm_gather<4, LDW>(Fb, n, p, q, a, b, Wk);
-      for (int j = tid; j <= b; j += NT) H[j] = 2.0 / (2 * (q + 2 * j) + 1);
-      __syncthreads();
-
-      // ---------------- phase 1: G = V_p^T F_pq (in place)
-      helmc_gemm<TIO, false>(I, q, b, VTs, Wk, slab, nullptr);
-      __syncthreads();
-
-      // ---------------- phase 2: FP64 row solves, two lanes per row (twisted)
-      // T = s M_q + diag(4c + 6), c = q + 2j:  d_j = s (H_j + H_{j+1}) + 4q + 6 + 8j,
-      // o_j = T[j][j+1] = -s H_{j+1}.  Lane (r, 0) eliminates rows 0..m-1 downwards,
-      // lane (r, 1) rows b-1..m+1 upwards (step i visits j = i or b-1-i); pivots are
-      // continuant ratios P_{j-1} / P_j with P_j = d_j P_{j-1} - o^2 P_{j-2}.
-      // Loads of a chunk of 8 steps are issued before its arithmetic.
-      {
-        const int r = tid >> 1, side = tid & 1;
-        const bool act = r < a;
-        const double s = act ? I.alpha - lam[I.loff + r] : 1.0;
-        float* row = Wk + (act ? r : 0) * LDW;
-        const int m = b >> 1;
-        const int len = act ? (side ? b - 1 - m : m) : 0;
-        const double dq = 4.0 * q + 6.0;
-        float cm[64];
-        double P1 = 1.0, P2 = 0.0, yprev = 0.0, oprev = 0.0, cl = 0.0;
-#pragma unroll
-        for (int ch = 0; ch < 8; ++ch) {
-          if (ch * 8 < len) {
-            float g[8];
-            double h0[8], h1[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-              const int i = ch * 8 + u;
-              int j = side ? b - 1 - i : i;
-              j = j < 0 ? 0 : (j > b - 1 ? b - 1 : j);
-              g[u] = row[j];
-              h0[u] = H[j];
-              h1[u] = H[j + 1];
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-              const int i = ch * 8 + u;
-              if (i < len) {
-                const int j = side ? b - 1 - i : i;
-                const double d = fma(s, h0[u] + h1[u], fma(8.0, (double)j, dq));
-                const double on = -s * (side ? h0[u] : h1[u]);  // coupling towards the middle
-                const double Pj = fma(d, P1, -(oprev * oprev) * P2);
-                const double rj = P1 * rcp64(Pj);
-                const double yj = fma(-oprev, yprev, (double)g[u]) * rj;
-                cl = on * rj;
-                cm[i] = (float)cl;
-                row[j] = (float)yj;
-                P2 = P1;
-                P1 = Pj;
-                yprev = yj;
-                oprev = on;
-              }
-            }
-            renorm(P1, P2);
-          }
-        }
-        const double dl = yprev;
-        const double co = __shfl_xor(cl, 1), dO = __shfl_xor(dl, 1);
-        if (act) {
-          const double ct = side == 0 ? cl : co, dt = side == 0 ? dl : dO;  // rows < m
-          const double fb = side == 0 ? co : cl, eb = side == 0 ? dO : dl;  // rows > m
-          const double am = (m > 0) ? -s * H[m] : 0.0;
-          const double cmm = (m < b - 1) ? -s * H[m + 1] : 0.0;
-          const double dm = fma(s, H[m] + H[m + 1], fma(8.0, (double)m, dq));
-          const double um = ((double)row[m] - am * dt - cmm * eb) / (dm - am * ct - cmm * fb);
-          // back substitution outwards from m, i = len-1 .. 0
-          double uprev = um;
-#pragma unroll
-          for (int ch = 7; ch >= 0; --ch) {
-            if (ch * 8 < len) {
-              float y[8];
-#pragma unroll
-              for (int u = 0; u < 8; ++u) {
-                const int i = ch * 8 + u;
-                int j = side ? b - 1 - i : i;
-                j = j < 0 ? 0 : (j > b - 1 ? b - 1 : j);
-                y[u] = row[j];
-              }
-#pragma unroll
-              for (int u = 7; u >= 0; --u) {
-                const int i = ch * 8 + u;
-                if (i < len) {
-                  const int j = side ? b - 1 - i : i;
-                  const double v = fma(-(double)cm[i], uprev, (double)y[u]);
-                  row[j] = (float)v;
-                  uprev = v;
-                }
-              }
-            }
-          }
-          if (side == 0) row[m] = (float)um;
-        }
-      }
-      __syncthreads();
-
-      // ---------------- phase 3: U_pq = V_p U~ -> global
-      helmc_gemm<TIO, true>(I, q, b, Vs, Wk, slab, U + I.foff);
-      __syncthreads();
-    }
+    if (I.AP > 64)
+      helmc_item<TIO, 2>(I, Vs, VTs, lam, F, U, Wk, slab, H);
+    else
+      helmc_item<TIO, 1>(I, Vs, VTs, lam, F, U, Wk, slab, H);
     if (tid == 0) s_nx[slot] = nxt;
     __syncthreads();
-    it = s_nx[slot];
+    it = __builtin_amdgcn_readfirstlane(s_nx[slot]);
     slot ^= 1;
   }
 }
