@@ -466,7 +466,9 @@ __device__ __forceinline__ void slab_prefetch2(SlabRegs (&P)[2], const unsigned 
 // (row-major, k contiguous) whose first two slabs are already in flight in P.
 // Wave w owns column panels w and w + 4; the B fragments of a k-slab are
 // split once and reused over both row halves.  Each slab is refetched two
-// steps ahead of its use.
+// steps ahead of its use, and the A fragments of the next row block are read
+// from LDS before the current block's MFMAs issue.  Rows a..AP-1 of Wk hold
+// zeros and columns >= b only feed discarded outputs, so B is read unmasked.
 template <typename TIO, bool TO_GLOBAL, int NH>
 __device__ __forceinline__ void helmc_gemm(const HelmItem& I, int q, int b, const unsigned short* __restrict__ T,
                                            float* Wk, unsigned short* slab, TIO* __restrict__ Ub,
@@ -476,9 +478,11 @@ __device__ __forceinline__ void helmc_gemm(const HelmItem& I, int q, int b, cons
   // lane-dependent address of the epilogue out of the item loop into registers
   int tid = threadIdx.x;
   asm volatile("" : "+v"(tid));
-  const int lane = tid & 63, w = tid >> 6;
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
   const int a = I.a, AP = I.AP, n = I.n;
   const int npanel = (b + 15) >> 4, nrb = AP >> 4;
+  const int npp = (w + 4 < npanel) ? 2 : (w < npanel ? 1 : 0);  // panels of this wave
   v4f acc[2][8];
 #pragma unroll
   for (int pp = 0; pp < 2; ++pp)
@@ -489,30 +493,33 @@ __device__ __forceinline__ void helmc_gemm(const HelmItem& I, int q, int b, cons
   auto bfrag = [&](int k0) {
 #pragma unroll
     for (int pp = 0; pp < 2; ++pp) {
-      const int col = (w + 4 * pp) * 16 + (lane & 15);
-      const bool cok = col < b;
+      const float* src = Wk + (k0 + 8 * (lane >> 4)) * LDW + (w + 4 * pp) * 16 + (lane & 15);
       float xv[8];
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int k = k0 + 8 * (lane >> 4) + e;
-        xv[e] = (cok && k < a) ? Wk[k * LDW + col] : 0.f;
-      }
+      for (int e = 0; e < 8; ++e) xv[e] = src[e * LDW];
       split3_pk8(xv, bh[pp], bm[pp], bl[pp]);
     }
   };
   auto mult = [&](auto hc) {
     constexpr int h = decltype(hc)::value;
+    const int off0 = slab_off(0, lane & 15, lane >> 4);
+    v8u16 A[2][3];
+    auto aload = [&](v8u16(&dst)[3], int rl) {
+      const int off = off0 + rl * 16 * 4 * 8;  // 16 rows further; the swizzle depends on row bits 2-3 only
+      dst[0] = *(const v8u16*)&slab[off];
+      dst[1] = *(const v8u16*)&slab[off + 64 * 32];
+      dst[2] = *(const v8u16*)&slab[off + 2 * 64 * 32];
+    };
+    aload(A[0], 0);
 #pragma unroll
     for (int rl = 0; rl < 4; ++rl) {
+      if (rl + 1 < 4) aload(A[(rl + 1) & 1], rl + 1);
       const int rb = h * 4 + rl;
+      const v8u16 ah = A[rl & 1][0], am = A[rl & 1][1], al = A[rl & 1][2];
       if (rb < nrb) {
-        const int row = rl * 16 + (lane & 15), off = slab_off(0, row, lane >> 4);
-        const v8u16 ah = *(const v8u16*)&slab[off];
-        const v8u16 am = *(const v8u16*)&slab[off + 64 * 32];
-        const v8u16 al = *(const v8u16*)&slab[off + 2 * 64 * 32];
 #pragma unroll
         for (int pp = 0; pp < 2; ++pp) {
-          if (w + 4 * pp < npanel) {
+          if (pp < npp) {
             // products ordered small to large
             acc[pp][rb] = mfma_bf16(al, bh[pp], acc[pp][rb]);
             acc[pp][rb] = mfma_bf16(am, bm[pp], acc[pp][rb]);
@@ -552,21 +559,34 @@ __device__ __forceinline__ void helmc_gemm(const HelmItem& I, int q, int b, cons
       }
     }
   }
+  // epilogue: output row m = rb*16 + lr + r, lr = 4 (lane / 16)
+  const int lr = 4 * (lane >> 4), lim = a - lr;
 #pragma unroll
   for (int pp = 0; pp < 2; ++pp) {
+    if (pp >= npp) continue;
     const int col = (w + 4 * pp) * 16 + (lane & 15);
-    if (col >= b) continue;
+    const bool cok = col < b;
+    if constexpr (TO_GLOBAL) {
+      TIO* dst = Ub + ((I.p + 2 * lr) * n + q + 2 * col);  // within one solve: 32-bit offsets
+      const int rs = 2 * n;                                  // one parity row
 #pragma unroll
-    for (int rb = 0; rb < 8; ++rb) {
-      if (rb >= nrb) continue;
+      for (int rb = 0; rb < 8; ++rb) {
+        if (rb >= nrb) continue;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int m = rb * 16 + mfma16_row_f32(lane, r);
-        if (m >= a) continue;
-        if constexpr (TO_GLOBAL)
-          Ub[(long long)(I.p + 2 * m) * n + q + 2 * col] = (TIO)acc[pp][rb][r];
-        else
-          Wk[m * LDW + col] = acc[pp][rb][r];
+        for (int r = 0; r < 4; ++r) {
+          if (cok && rb * 16 + r < lim) dst[0] = (TIO)acc[pp][rb][r];
+          dst += rs;
+        }
+        dst += 12 * rs;
+      }
+    } else {
+      float* dst = Wk + lr * LDW + col;
+#pragma unroll
+      for (int rb = 0; rb < 8; ++rb) {
+        if (rb >= nrb) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (cok && rb * 16 + r < lim) dst[(rb * 16 + r) * LDW] = acc[pp][rb][r];
       }
     }
   }
@@ -591,6 +611,7 @@ __device__ __forceinline__ void helmc_item(const HelmItem& I, const unsigned sho
     if (b == 0) continue;
     // ---------------- phase 0: F_pq -> Wk, h table
     helm_gather<4, LDW>(F + I.foff, n, p, q, a, b, Wk);
+    for (int e = tid; e < (AP - a) * 128; e += NT) Wk[(a + (e >> 7)) * LDW + (e & 127)] = 0.f;  // pad rows
     for (int j = tid; j <= b; j += NT) H[j] = 2.0 / (2 * (q + 2 * j) + 1);
     __syncthreads();
 
