@@ -408,13 +408,13 @@ __global__ __launch_bounds__(NW * 64) void helm_ragged_kernel(
 //   Wk   : 128 x 130 FP32 work block (as above)
 //   slab : 3 bf16 slices x 64 table rows x 32 k, 16-byte chunks swizzled so a
 //          16-lane group of b128 reads covers all banks
-//   H    : h_{q+2j} = 2 / (2(q+2j) + 1), j = 0..b   (FP64)
+//   H    : h_{q+2j} = 2 / (2(q+2j) + 1), j = 0..b, and h_{q+2j} + h_{q+2j+2}   (FP64)
 // The tridiagonal multipliers live in registers (at most 64 per lane).
 struct HelmCompact {
   static constexpr int LDW = 130, NT = 256;
   static constexpr size_t WK = (size_t)HMAXA * LDW * 4;
   static constexpr size_t SLAB = 3 * 64 * 32 * 2;
-  static constexpr size_t HT = (HMAXA + 2) * 8;
+  static constexpr size_t HT = 2 * (HMAXA + 2) * 8;  // H_j and H_j + H_{j+1}
   static constexpr size_t BYTES = WK + SLAB + HT;
 };
 static_assert(HelmCompact::BYTES + 64 <= 80 * 1024, "two compact workgroups must fit one CU's LDS");
@@ -599,8 +599,9 @@ __device__ __forceinline__ void helmc_item(const HelmItem& I, const unsigned sho
                                            const TIO* __restrict__ F, TIO* __restrict__ U, float* Wk,
                                            unsigned short* slab, double* H) {
   constexpr int LDW = HelmCompact::LDW, NT = HelmCompact::NT;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, wu = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int n = I.n, p = I.p, a = I.a, AP = I.AP;
+  double* Hs = H + HMAXA + 2;                    // h_c + h_{c+2}
   const unsigned short* TV = Vs + 3 * I.voff;    // V_p slices    (phase 3)
   const unsigned short* TVT = VTs + 3 * I.voff;  // V_p^T slices  (phase 1)
   SlabRegs P[2];
@@ -612,7 +613,11 @@ __device__ __forceinline__ void helmc_item(const HelmItem& I, const unsigned sho
     // ---------------- phase 0: F_pq -> Wk, h table
     helm_gather<4, LDW>(F + I.foff, n, p, q, a, b, Wk);
     for (int e = tid; e < (AP - a) * 128; e += NT) Wk[(a + (e >> 7)) * LDW + (e & 127)] = 0.f;  // pad rows
-    for (int j = tid; j <= b; j += NT) H[j] = 2.0 / (2 * (q + 2 * j) + 1);
+    for (int j = tid; j <= b; j += NT) {
+      const double hj = 2.0 / (2 * (q + 2 * j) + 1);
+      H[j] = hj;
+      Hs[j] = hj + 2.0 / (2 * (q + 2 * j) + 5);
+    }
     __syncthreads();
 
     // ---------------- phase 1: G = V_p^T F_pq (in place)
@@ -621,92 +626,100 @@ __device__ __forceinline__ void helmc_item(const HelmItem& I, const unsigned sho
     __syncthreads();
 
     // ---------------- phase 2: FP64 row solves, two lanes per row (twisted)
-    // T = s M_q + diag(4c + 6), c = q + 2j:  d_j = s (H_j + H_{j+1}) + 4q + 6 + 8j,
-    // o_j = T[j][j+1] = -s H_{j+1}.  Lane (r, 0) eliminates rows 0..m-1 downwards,
-    // lane (r, 1) rows b-1..m+1 upwards (step i visits j = i or b-1-i); pivots are
-    // continuant ratios P_{j-1} / P_j with P_j = d_j P_{j-1} - o^2 P_{j-2}.
-    // Loads of a chunk of 8 steps are issued before its arithmetic.
-    {
-      const int r = tid >> 1, side = tid & 1;
+    // T = s M_q + diag(4c + 6), c = q + 2j:  d_j = s Hs_j + 4q + 6 + 8j, Hs_j = H_j + H_{j+1},
+    // o_j = T[j][j+1] = -s H_{j+1}.  Lane (r, 0) eliminates rows 0..len-1 downwards,
+    // lane (r, 1) rows b-1..b-len upwards (step i visits j = i or b-1-i), len = b/2,
+    // so both lanes run the same number of steps.  Pivots are continuant ratios
+    // P_{j-1} / P_j with P_j = d_j P_{j-1} - o^2 P_{j-2}.  Odd b leaves the middle
+    // row, solved from both sides' last multipliers; even b leaves a 2 x 2 system
+    // between the two sides' last rows.  Full chunks of 8 steps run without
+    // branches so their loads and reciprocals overlap.
+    if (wu * 32 < a) {
+      // opaque lane index: stops per-step offsets (multiples of the lane's
+      // direction) from being hoisted out of the item loop into registers
+      int t2 = tid;
+      asm volatile("" : "+v"(t2));
+      const int r = t2 >> 1, side = t2 & 1;
       const bool act = r < a;
       const double s = act ? I.alpha - lam[I.loff + r] : 1.0;
-      float* row = Wk + (act ? r : 0) * LDW;
-      const int m = b >> 1;
-      const int len = act ? (side ? b - 1 - m : m) : 0;
-      const double dq = 4.0 * q + 6.0;
+      // rows >= a hold the zero padding the GEMMs read: idle lanes work on the slab instead
+      float* row = act ? Wk + r * LDW : reinterpret_cast<float*>(slab);
+      const int len = b >> 1;
+      const int j0 = side ? b - 1 : 0, dj = side ? -1 : 1;
+      const double* Ho = side ? H : H + 1;  // o towards the middle = -s Ho[j]
+      const double dg0 = fma(8.0, (double)j0, 4.0 * q + 6.0), ddg = 8.0 * dj;
       float cm[64];
       double P1 = 1.0, P2 = 0.0, yprev = 0.0, oprev = 0.0, cl = 0.0;
+      auto fwd = [&](int i) {
+        const int j = j0 + dj * i;
+        const double g = row[j], hs = Hs[j], ho = Ho[j];
+        const double d = fma(s, hs, fma((double)i, ddg, dg0));
+        const double on = -s * ho;
+        const double Pj = fma(d, P1, -(oprev * oprev) * P2);
+        const double rj = P1 * rcp64(Pj);
+        const double yj = fma(-oprev, yprev, g) * rj;
+        cl = on * rj;
+        cm[i] = (float)cl;
+        row[j] = (float)yj;
+        P2 = P1;
+        P1 = Pj;
+        yprev = yj;
+        oprev = on;
+      };
 #pragma unroll
       for (int ch = 0; ch < 8; ++ch) {
-        if (ch * 8 < len) {
-          float g[8];
-          double h0[8], h1[8];
+        if (ch * 8 + 8 <= len) {
 #pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            const int i = ch * 8 + u;
-            int j = side ? b - 1 - i : i;
-            j = j < 0 ? 0 : (j > b - 1 ? b - 1 : j);
-            g[u] = row[j];
-            h0[u] = H[j];
-            h1[u] = H[j + 1];
-          }
-#pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            const int i = ch * 8 + u;
-            if (i < len) {
-              const int j = side ? b - 1 - i : i;
-              const double d = fma(s, h0[u] + h1[u], fma(8.0, (double)j, dq));
-              const double on = -s * (side ? h0[u] : h1[u]);  // coupling towards the middle
-              const double Pj = fma(d, P1, -(oprev * oprev) * P2);
-              const double rj = P1 * rcp64(Pj);
-              const double yj = fma(-oprev, yprev, (double)g[u]) * rj;
-              cl = on * rj;
-              cm[i] = (float)cl;
-              row[j] = (float)yj;
-              P2 = P1;
-              P1 = Pj;
-              yprev = yj;
-              oprev = on;
-            }
-          }
+          for (int u = 0; u < 8; ++u) fwd(ch * 8 + u);
           renorm(P1, P2);
+        } else if (ch * 8 < len) {
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (ch * 8 + u < len) fwd(ch * 8 + u);
         }
       }
-      const double dl = yprev;
-      const double co = __shfl_xor(cl, 1), dO = __shfl_xor(dl, 1);
-      if (act) {
-        const double ct = side == 0 ? cl : co, dt = side == 0 ? dl : dO;  // rows < m
-        const double fb = side == 0 ? co : cl, eb = side == 0 ? dO : dl;  // rows > m
+      const double co = __shfl_xor(cl, 1), yo = __shfl_xor(yprev, 1);  // the partner's last step
+      double own;
+      int start;
+      if (b & 1) {
+        const int m = len;
+        const double ct = side == 0 ? cl : co, yt = side == 0 ? yprev : yo;  // rows < m
+        const double cb = side == 0 ? co : cl, yb = side == 0 ? yo : yprev;  // rows > m
         const double am = (m > 0) ? -s * H[m] : 0.0;
         const double cmm = (m < b - 1) ? -s * H[m + 1] : 0.0;
-        const double dm = fma(s, H[m] + H[m + 1], fma(8.0, (double)m, dq));
-        const double um = ((double)row[m] - am * dt - cmm * eb) / (dm - am * ct - cmm * fb);
-        // back substitution outwards from m, i = len-1 .. 0
-        double uprev = um;
+        const double dm = fma(s, Hs[m], fma(8.0, (double)m, 4.0 * q + 6.0));
+        own = ((double)row[m] - am * yt - cmm * yb) / (dm - am * ct - cmm * cb);
+        start = len - 1;
+        if (side == 0) row[m] = (float)own;  // both lanes have read row[m] above
+      } else {
+        // u_t = y_t - c_t u_b (row len-1),  u_b = y_b - c_b u_t (row len)
+        const double ct = side == 0 ? cl : co, yt = side == 0 ? yprev : yo;
+        const double cb = side == 0 ? co : cl, yb = side == 0 ? yo : yprev;
+        const double ut = (yt - ct * yb) / (1.0 - ct * cb);
+        own = side == 0 ? ut : yb - cb * ut;
+        start = len - 2;
+        row[j0 + dj * (len - 1)] = (float)own;
+      }
+      // back substitution outwards, i = start .. 0
+      double uprev = own;
+      auto bwd = [&](int i, float y) {
+        const double v = fma(-(double)cm[i], uprev, (double)y);
+        row[j0 + dj * i] = (float)v;
+        uprev = v;
+      };
 #pragma unroll
-        for (int ch = 7; ch >= 0; --ch) {
-          if (ch * 8 < len) {
-            float y[8];
+      for (int ch = 7; ch >= 0; --ch) {
+        if (ch * 8 + 7 <= start) {
+          float y[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-              const int i = ch * 8 + u;
-              int j = side ? b - 1 - i : i;
-              j = j < 0 ? 0 : (j > b - 1 ? b - 1 : j);
-              y[u] = row[j];
-            }
+          for (int u = 0; u < 8; ++u) y[u] = row[j0 + dj * (ch * 8 + u)];
 #pragma unroll
-            for (int u = 7; u >= 0; --u) {
-              const int i = ch * 8 + u;
-              if (i < len) {
-                const int j = side ? b - 1 - i : i;
-                const double v = fma(-(double)cm[i], uprev, (double)y[u]);
-                row[j] = (float)v;
-                uprev = v;
-              }
-            }
-          }
+          for (int u = 7; u >= 0; --u) bwd(ch * 8 + u, y[u]);
+        } else if (ch * 8 <= start) {
+#pragma unroll
+          for (int u = 7; u >= 0; --u)
+            if (ch * 8 + u <= start) bwd(ch * 8 + u, row[j0 + dj * (ch * 8 + u)]);
         }
-        if (side == 0) row[m] = (float)um;
       }
     }
     __syncthreads();
