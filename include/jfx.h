@@ -28,13 +28,13 @@ int jfx_version(void);
 const char* jfx_last_error(void);
 int jfx_device_sync(void);
 
-/* ---- a3: quadrature.  family 0 = Legendre, 1 = Chebyshev.  Nodes ascending.
+/* ---- quadrature.  family 0 = Legendre, 1 = Chebyshev.  Nodes ascending.
  * Replaces Legendre.quad_points_and_weights (galerkin/Legendre.py:126-137 ->
  * utils/fastgl.py:561-567) and Chebyshev.quad_points_and_weights
  * (galerkin/Chebyshev.py:149-170). */
 int jfx_gauss_nodes(int family, int N, double* x, double* w, hipStream_t stream);
 
-/* ---- a1/a2: V[q, n] = scale * d^k/dx^k psi_n(x[q]), V is (Q, N), k <= 4.
+/* ---- Vandermonde: V[q, n] = scale * d^k/dx^k psi_n(x[q]), V is (Q, N), k <= 4.
  * Replaces OrthogonalSpace.vandermonde / evaluate_basis_derivative
  * (galerkin/orthogonal.py:131-141, 168-178) over eval_basis_functions
  * (galerkin/Legendre.py:163-183, galerkin/Chebyshev.py:200-223). */
@@ -49,7 +49,7 @@ int jfx_vandermonde_f64(int family, const double* x, int Q, int N, int k, double
 int jfx_scale_transpose(const double* V, int R, int C, const double* rowscale, const double* colscale,
                         double s, double* out, hipStream_t stream);
 
-/* ---- a9: per-axis contraction Y[p, m, q] = sum_k A[m, k] X[p, k, q];
+/* ---- per-axis contraction Y[p, m, q] = sum_k A[m, k] X[p, k, q];
  * A is (M, K) row-major, X is (P, K, Q), Y is (P, M, Q).
  * Replaces DiaMatrix/Matrix.matvec(axis=i) (la/diamatrix.py:227-349,
  * la/matrix.py:89-118) as used by TPMatrix._matmul_array
@@ -94,7 +94,7 @@ int jfx_plane_contract(int prec, const double* A0d, const double* A1d, const uns
                        const unsigned short* A1s, const void* X, void* Y, const void* E, int epi, long long B,
                        int M0, int K0, int M1, int K1, hipStream_t stream);
 
-/* ---- a7: banded LU without pivoting.  band is (n, p+q+1) "by row":
+/* ---- banded LU without pivoting.  band is (n, p+q+1) "by row":
  * band[i*(p+q+1) + o + p] = A[i, i+o].  status_dev (device int) receives 0,
  * or 1 + the row of the first zero/non-finite pivot.
  * Replaces DiaMatrix.lu_factor no-pivot kernel (la/diamatrix.py:394-504,
@@ -107,7 +107,7 @@ int jfx_band_lu_factor(const double* band, int n, int p, int q, double* lu, int*
 int jfx_band_lu_solve(const double* lu, int n, int p, int q, double* X, long long P, long long Q,
                       hipStream_t stream);
 
-/* ---- a12: Y = X / D with D (size) broadcast over the leading batch: the
+/* ---- FD eigenvalue division: Y = X / D with D (size) broadcast over the leading batch: the
  * eigenvalue division of TPMatricesLUFactors.solve (la/tpmatrix.py:565-579). */
 int jfx_div_bcast_f64(const double* X, const double* D, double* Y, long long total, long long size,
                       hipStream_t stream);
@@ -139,13 +139,17 @@ int jfx_sum_products(int dtype, int nterms, const void* const* As, const void* c
  * the reference runs in one dtype). */
 int jfx_cast(int direction, const void* x, void* y, long long n, hipStream_t stream);
 
-/* ---- a12 (batched): ragged batch of 2D Legendre-Dirichlet Helmholtz solves
+/* ---- ragged batch of 2D Legendre-Dirichlet Helmholtz solves
  * (alpha M (x) M - K (x) M - M (x) K) U = F in one persistent launch.
  * items: device array of nitems work items (jfx_helm_item_size() bytes each:
  *   {long long foff, voff; double alpha; int n, p, q, a, b, AP, loff, pad}),
  *   one per (solve, row parity p) (q and b are ignored: the workgroup runs
- *   both column parities in turn), processed in array order from the device
- *   counter `counter` (reset by the call itself).
+ *   both column parities in turn).  The items form nqueues (1..16) queues:
+ *   queue g is items[qbounds[g] .. qbounds[g+1]) (qbounds: nqueues + 1 device
+ *   ints, qbounds[0] = 0, qbounds[nqueues] = nitems), claimed in order through
+ *   the device counters[g] (reset by the call itself).  Workgroup w starts on
+ *   queue w mod nqueues and moves to the following queues once it is drained;
+ *   with one queue per XCD, items that share tables are best kept in one queue.
  * Vd/VTd: FP64 eigenvector tables V_p / V_p^T (AP x AP, zero padded) of the
  *   parity blocks of K_p V = M_p V diag(lam); Vs/VTs: the same as 3 bf16
  *   float-split slices each ([3][AP][AP] per table at 3*voff).  lam at loff.
@@ -159,12 +163,12 @@ int jfx_cast(int direction, const void* x, void* y, long long n, hipStream_t str
  * DiaMatrix solve along an axis (la/diamatrix.py:592-770). */
 int jfx_helm_item_size(void);
 int jfx_helm_scratch_doubles_per_block(void);
-int jfx_helm_ragged(int mode, int io_f32, int nwaves, const void* items, int nitems, int* counter,
-                    const double* Vd, const double* VTd, const unsigned short* Vs,
+int jfx_helm_ragged(int mode, int io_f32, int nwaves, const void* items, int nitems, int* counters,
+                    const int* qbounds, int nqueues, const double* Vd, const double* VTd, const unsigned short* Vs,
                     const unsigned short* VTs, const double* lam, const void* F, void* U,
                     double* scratch, int nblocks, hipStream_t stream);
 
-/* ---- learned preconditioner (config 4): no reference counterpart; the
+/* ---- learned preconditioner: no reference counterpart; the
  * nearest analogue is the Linear+activation MLP of pinns/module.py:335-380.
  * C (M x N) = alpha op(A) op(B) + beta C (+ bias[N]) (tanh if act = 1);
  * op(A) = A (M x K, lda) if ta = 0, else A^T of a stored (K x M, lda);

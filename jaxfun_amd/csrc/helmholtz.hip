@@ -11,8 +11,11 @@
 // which is partial fast diagonalisation (la/tpmatrix.py:1089-1233 for the
 // eigenpairs, la/diamatrix.py:394-504 + 2079-2195 for the banded solve).
 //
-// Work items are taken from a device counter in the host's order (largest
-// first); every workgroup loops until the counter passes nitems.
+// Work items sit in nq queues (the host groups items that share eigenvector
+// tables into one queue, largest first within it).  Workgroup g starts on
+// queue g mod nq -- workgroups are dispatched round-robin over the XCDs, so a
+// queue's tables stay in one XCD's L2 -- and moves on to the next queues when
+// its own is drained; every workgroup loops until all queues are empty.
 //   MODE 0: FP64 MFMA, FP64 storage
 //   MODE 1: BF16 3-term float split (6 products) MFMA, FP32 accumulation and
 //           storage, FP64 tridiagonal solve.
@@ -42,9 +45,39 @@ struct HelmTypes<1> {
   typedef float TS;
 };
 
-__global__ void helm_reset(int* counter) {
-  if (threadIdx.x == 0) counter[0] = 0;
+__global__ void helm_reset(int* counters, int nq) {
+  if ((int)threadIdx.x < nq) counters[threadIdx.x] = 0;
 }
+
+// Work queues: queue g holds items[bounds[g] .. bounds[g+1]), claimed through counters[g].
+constexpr int HMAXQ = 16;
+struct HelmQueues {
+  int* counters;
+  const int* bounds;
+  int nq;
+  // Next item index, claimed from queue qi by lane 0 of the calling wave (all
+  // 64 lanes call; every lane gets the result).  Once queue qi is drained the
+  // lanes read all counters at once and the next queue (cyclically) that
+  // still looks non-empty is tried; qi follows.  Returns bounds[nq] when every
+  // queue is empty.  Counters only grow, so the loop ends.
+  __device__ __forceinline__ int claim(int& qi) const {
+    const int lane = threadIdx.x & 63;
+    int c = 0;
+    if (lane == 0) c = atomicAdd(&counters[qi], 1);
+    c = __shfl(c, 0);
+    if (bounds[qi] + c < bounds[qi + 1]) return bounds[qi] + c;
+    for (;;) {
+      const int cg = lane < nq ? __hip_atomic_load(&counters[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+      const unsigned long long m = __ballot(lane < nq && bounds[lane] + cg < bounds[lane + 1]);
+      if (m == 0) return bounds[nq];
+      const unsigned long long after = m & ~((2ull << qi) - 1);  // non-empty queues past qi
+      qi = __ffsll((long long)(after ? after : m)) - 1;
+      if (lane == 0) c = atomicAdd(&counters[qi], 1);
+      c = __shfl(c, 0);
+      if (bounds[qi] + c < bounds[qi + 1]) return bounds[qi] + c;
+    }
+  }
+};
 
 // FP64 reciprocal: hardware estimate refined by two Newton steps (x > 0, normal).
 __device__ __forceinline__ double rcp64(double x) {
@@ -84,7 +117,7 @@ __device__ __forceinline__ void helm_gemm_phase(const HelmItem& I, int q, int b,
   constexpr int PPW = 8 / NW;  // column panels per wave (b <= 128 -> <= 8 panels)
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int a = I.a, AP = I.AP, n = I.n;
-  const int npanel = (b + 15) >> 4, nrb = AP >> 4;
+  const int npanel = (b + 15) >> 4, nrb = (a + 15) >> 4;  // output row blocks
   typedef typename std::conditional<MODE == 0, v4d, v4f>::type vacc;
   vacc acc[PPW][8];
 #pragma unroll
@@ -270,7 +303,7 @@ __device__ __forceinline__ void helm_gather(const TIO* __restrict__ Fb, int n, i
 
 template <int MODE, int NW, typename TIO>
 __global__ __launch_bounds__(NW * 64) void helm_ragged_kernel(
-    const HelmItem* __restrict__ items, int nitems, int* __restrict__ counter,
+    const HelmItem* __restrict__ items, int nitems, HelmQueues Q,
     const double* __restrict__ Vd, const double* __restrict__ VTd,
     const unsigned short* __restrict__ Vs, const unsigned short* __restrict__ VTs,
     const double* __restrict__ lam, const TIO* __restrict__ F, TIO* __restrict__ U,
@@ -290,13 +323,17 @@ __global__ __launch_bounds__(NW * 64) void helm_ragged_kernel(
   // the next item's index is claimed at the start of the current one, so the
   // counter round trip overlaps the work (two slots: slot k is rewritten only
   // after every thread has passed the barriers of the item in between)
-  if (tid == 0) s_nx[0] = atomicAdd(counter, 1);
+  int qi = blockIdx.x % Q.nq;
+  if (tid < 64) {
+    const int first = Q.claim(qi);
+    if (tid == 0) s_nx[0] = first;
+  }
   __syncthreads();
   int it = s_nx[0], slot = 1;
   while (it < nitems) {
     const HelmItem I = items[it];
     int nxt = 0;
-    if (tid == 0) nxt = atomicAdd(counter, 1);
+    if (tid < 64) nxt = Q.claim(qi);
     const int n = I.n, p = I.p, a = I.a;
     const TIO* Fb = F + I.foff;
 
@@ -481,7 +518,7 @@ __device__ __forceinline__ void helmc_gemm(const HelmItem& I, int q, int b, cons
   const int lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
   const int a = I.a, AP = I.AP, n = I.n;
-  const int npanel = (b + 15) >> 4, nrb = AP >> 4;
+  const int npanel = (b + 15) >> 4, nrb = (a + 15) >> 4;  // output row blocks
   const int npp = (w + 4 < npanel) ? 2 : (w < npanel ? 1 : 0);  // panels of this wave
   v4f acc[2][8];
 #pragma unroll
@@ -733,7 +770,7 @@ __device__ __forceinline__ void helmc_item(const HelmItem& I, const unsigned sho
 
 template <typename TIO>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void helm_compact_kernel(
-    const HelmItem* __restrict__ items, int nitems, int* __restrict__ counter, const unsigned short* __restrict__ Vs,
+    const HelmItem* __restrict__ items, int nitems, HelmQueues Q, const unsigned short* __restrict__ Vs,
     const unsigned short* __restrict__ VTs, const double* __restrict__ lam, const TIO* __restrict__ F,
     TIO* __restrict__ U) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
@@ -746,13 +783,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   // the next item's index is claimed at the start of the current one (two
   // slots: slot k is rewritten only after every thread has passed the
   // barriers of the item in between)
-  if (tid == 0) s_nx[0] = atomicAdd(counter, 1);
+  int qi = blockIdx.x % Q.nq;
+  if (tid < 64) {
+    const int first = Q.claim(qi);
+    if (tid == 0) s_nx[0] = first;
+  }
   __syncthreads();
   int it = __builtin_amdgcn_readfirstlane(s_nx[0]), slot = 1;
   while (it < nitems) {
     const HelmItem I = items[it];
     int nxt = 0;
-    if (tid == 0) nxt = atomicAdd(counter, 1);
+    if (tid < 64) nxt = Q.claim(qi);
     if (I.AP > 64)
       helmc_item<TIO, 2>(I, Vs, VTs, lam, F, U, Wk, slab, H);
     else
@@ -765,22 +806,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 }
 
 template <typename TIO>
-static int launch_helm_compact(const HelmItem* items, int nitems, int* counter, const unsigned short* Vs,
+static int launch_helm_compact(const HelmItem* items, int nitems, HelmQueues Q, const unsigned short* Vs,
                                const unsigned short* VTs, const double* lam, const TIO* F, TIO* U, int nblocks,
                                hipStream_t st) {
   const size_t shm = HelmCompact::BYTES;
   auto kern = helm_compact_kernel<TIO>;
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
   if (e != hipSuccess) return set_error("helm: hipFuncSetAttribute: %s", hipGetErrorString(e)), JFX_EHIP;
-  hipLaunchKernelGGL(helm_reset, dim3(1), dim3(64), 0, st, counter);
-  hipLaunchKernelGGL(kern, dim3(nblocks), dim3(HelmCompact::NT), shm, st, items, nitems, counter, Vs, VTs, lam,
+  hipLaunchKernelGGL(helm_reset, dim3(1), dim3(64), 0, st, Q.counters, Q.nq);
+  hipLaunchKernelGGL(kern, dim3(nblocks), dim3(HelmCompact::NT), shm, st, items, nitems, Q, Vs, VTs, lam,
                      F, U);
   JFX_LAUNCH_CHECK();
   return JFX_OK;
 }
 
 template <int MODE, int NW, typename TIO>
-static int launch_helm(const HelmItem* items, int nitems, int* counter, const double* Vd,
+static int launch_helm(const HelmItem* items, int nitems, HelmQueues Q, const double* Vd,
                        const double* VTd, const unsigned short* Vs, const unsigned short* VTs,
                        const double* lam, const TIO* F, TIO* U, double* scratch, int nblocks,
                        hipStream_t st) {
@@ -788,8 +829,8 @@ static int launch_helm(const HelmItem* items, int nitems, int* counter, const do
   auto kern = helm_ragged_kernel<MODE, NW, TIO>;
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
   if (e != hipSuccess) return set_error("helm: hipFuncSetAttribute: %s", hipGetErrorString(e)), JFX_EHIP;
-  hipLaunchKernelGGL(helm_reset, dim3(1), dim3(64), 0, st, counter);
-  hipLaunchKernelGGL(kern, dim3(nblocks), dim3(NW * 64), shm, st, items, nitems, counter, Vd, VTd, Vs, VTs,
+  hipLaunchKernelGGL(helm_reset, dim3(1), dim3(64), 0, st, Q.counters, Q.nq);
+  hipLaunchKernelGGL(kern, dim3(nblocks), dim3(NW * 64), shm, st, items, nitems, Q, Vd, VTd, Vs, VTs,
                      lam, F, U, scratch);
   JFX_LAUNCH_CHECK();
   return JFX_OK;
@@ -809,37 +850,39 @@ int jfx_helm_scratch_doubles_per_block(void) { return HMAXA * HMAXA; }
 // mode 1: BF16 3-term split, FP32 storage (F, U float if io_f32 else double; Vs/VTs used)
 // nwaves: 4 or 8 waves per workgroup; nblocks: persistent grid size
 //   (scratch must hold nblocks * jfx_helm_scratch_doubles_per_block() doubles).
-int jfx_helm_ragged(int mode, int io_f32, int nwaves, const void* items, int nitems, int* counter,
-                    const double* Vd, const double* VTd, const unsigned short* Vs,
+int jfx_helm_ragged(int mode, int io_f32, int nwaves, const void* items, int nitems, int* counters,
+                    const int* qbounds, int nqueues, const double* Vd, const double* VTd, const unsigned short* Vs,
                     const unsigned short* VTs, const double* lam, const void* F, void* U,
                     double* scratch, int nblocks, hipStream_t stream) {
-  if (!items || !counter || !lam || !F || !U || !scratch || nitems < 0 || nblocks <= 0)
+  if (!items || !counters || !qbounds || !lam || !F || !U || !scratch || nitems < 0 || nblocks <= 0)
     return set_error("helm_ragged: bad arguments"), JFX_EARG;
+  if (nqueues < 1 || nqueues > HMAXQ) return set_error("helm_ragged: nqueues must be in [1, 16]"), JFX_EARG;
   if (nitems == 0) return JFX_OK;
   const HelmItem* it = (const HelmItem*)items;
+  const HelmQueues queues{counters, qbounds, nqueues};
   if (mode == 0) {
     if (!Vd || !VTd || io_f32) return set_error("helm_ragged: mode 0 needs fp64 tables and fp64 I/O"), JFX_EARG;
     if (nwaves == 4)
-      return launch_helm<0, 4, double>(it, nitems, counter, Vd, VTd, Vs, VTs, lam, (const double*)F,
+      return launch_helm<0, 4, double>(it, nitems, queues, Vd, VTd, Vs, VTs, lam, (const double*)F,
                                        (double*)U, scratch, nblocks, stream);
     if (nwaves == 8)
-      return launch_helm<0, 8, double>(it, nitems, counter, Vd, VTd, Vs, VTs, lam, (const double*)F,
+      return launch_helm<0, 8, double>(it, nitems, queues, Vd, VTd, Vs, VTs, lam, (const double*)F,
                                        (double*)U, scratch, nblocks, stream);
   } else if (mode == 1) {
     if (!Vs || !VTs) return set_error("helm_ragged: mode 1 needs bf16 slice tables"), JFX_EARG;
     if (io_f32) {
       if (nwaves == 4)
-        return launch_helm_compact<float>(it, nitems, counter, Vs, VTs, lam, (const float*)F, (float*)U, nblocks,
+        return launch_helm_compact<float>(it, nitems, queues, Vs, VTs, lam, (const float*)F, (float*)U, nblocks,
                                           stream);
       if (nwaves == 8)
-        return launch_helm<1, 8, float>(it, nitems, counter, Vd, VTd, Vs, VTs, lam, (const float*)F,
+        return launch_helm<1, 8, float>(it, nitems, queues, Vd, VTd, Vs, VTs, lam, (const float*)F,
                                         (float*)U, scratch, nblocks, stream);
     } else {
       if (nwaves == 4)
-        return launch_helm_compact<double>(it, nitems, counter, Vs, VTs, lam, (const double*)F, (double*)U,
+        return launch_helm_compact<double>(it, nitems, queues, Vs, VTs, lam, (const double*)F, (double*)U,
                                            nblocks, stream);
       if (nwaves == 8)
-        return launch_helm<1, 8, double>(it, nitems, counter, Vd, VTd, Vs, VTs, lam, (const double*)F,
+        return launch_helm<1, 8, double>(it, nitems, queues, Vd, VTd, Vs, VTs, lam, (const double*)F,
                                          (double*)U, scratch, nblocks, stream);
     }
   }

@@ -13,9 +13,10 @@ counter; the launch can be captured in a HIP graph and replayed.
 Set-up (once per distinct N, as the reference's ``lu_factor`` caches): the
 parity blocks of (K, M) are diagonalised on the host in FP64 with
 ``scipy.linalg.eigh`` (la/tpmatrix.py:1173), uploaded, and split into bf16
-slices on the device.  Work items (solve, row parity) are ordered
-largest-first so the tail of the queue holds the small items; each item
-processes both column parities in turn.
+slices on the device.  Work items (solve, row parity) are grouped into one
+queue per XCD by N, so items sharing tables run on the same L2, and ordered
+largest-first within each queue; each item processes both column parities in
+turn.
 """
 
 from __future__ import annotations
@@ -32,6 +33,7 @@ ITEM_DTYPE = np.dtype([("foff", "<i8"), ("voff", "<i8"), ("alpha", "<f8"), ("n",
                        ("pad", "<i4")])
 
 MODES = {"fp64": 0, "mixed": 1}
+NQUEUES = 8  # work queues: one per XCD of an MI355X (workgroups are dispatched round-robin over them)
 
 
 def _parity_blocks(N: int):
@@ -65,10 +67,11 @@ class HelmholtzBatch:
             FP32 accumulation, FP64 tridiagonal solves).
         io_dtype: dtype of F/U (``mixed`` defaults to float32, ``fp64`` to float64).
         nwaves, blocks_per_cu: launch shape (see :func:`autotune`).
+        queues: number of work queues (1..16); items sharing tables share a queue.
     """
 
     def __init__(self, Ns, alphas, mode: str = "mixed", io_dtype=None, nwaves: int = 4,
-                 blocks_per_cu: int = 2):
+                 blocks_per_cu: int = 2, queues: int = NQUEUES):
         self.dev = K.device()
         self.Ns = np.asarray(Ns, dtype=np.int64)
         self.alphas = np.asarray(alphas, dtype=np.float64)
@@ -90,9 +93,11 @@ class HelmholtzBatch:
         self.sizes = ns * ns
         self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
         self.total = int(self.offsets[-1])
+        if not 1 <= queues <= 16:
+            raise ValueError("queues must be in [1, 16]")
+        self.queues = queues
         self._build_tables()
         self._build_items()
-        self.counter = torch.zeros(1, dtype=torch.int32, device=self.dev)
         self.set_launch(nwaves, blocks_per_cu)
 
     def set_launch(self, nwaves: int, blocks_per_cu: int) -> None:
@@ -154,13 +159,35 @@ class HelmholtzBatch:
                               self._loffs[(int(N), p)], 0))
         arr = np.array(items, dtype=ITEM_DTYPE)
         cost = arr["a"].astype(np.float64) ** 2 * arr["n"] + 40.0 * arr["a"] * arr["n"]
-        order = np.argsort(-cost, kind="stable")
+        # One queue per XCD: the items of one N (which share eigenvector tables)
+        # go to one queue, split into pieces only when they alone outweigh a
+        # queue's share; pieces are dealt largest first onto the lightest
+        # queue.  Within a queue, largest items first.
+        nq = min(self.queues, len(arr))
+        load = np.zeros(nq)
+        qof = np.empty(len(arr), dtype=np.int64)
+        Ns = arr["n"]
+        share = cost.sum() / max(nq, 1)
+        pieces = []
+        for v in np.unique(Ns):
+            idx = np.flatnonzero(Ns == v)
+            k = int(min(len(idx), max(1, np.ceil(cost[idx].sum() / share - 1e-9))))
+            pieces += [(float(cost[c].sum()), c) for c in np.array_split(idx, k)]
+        for c, idx in sorted(pieces, key=lambda t: -t[0]):
+            g = int(np.argmin(load))
+            load[g] += c
+            qof[idx] = g
+        order = np.lexsort((-cost, qof))
         arr = arr[order]
+        bounds = np.searchsorted(qof[order], np.arange(nq + 1)).astype(np.int32)
         if ITEM_DTYPE.itemsize != _lib.load().jfx_helm_item_size():
             raise RuntimeError("HelmItem layout mismatch between Python and libjfx")
         self.items_host = arr
         self.items = torch.tensor(arr.view(np.uint8), device=self.dev)
         self.nitems = len(arr)
+        self.nqueues = max(nq, 1)
+        self.qbounds = torch.tensor(bounds if nq else np.zeros(2, np.int32), device=self.dev)
+        self.counter = torch.zeros(self.nqueues, dtype=torch.int32, device=self.dev)
 
     # ------------------------------------------------------------------ packing
     def pack(self, arrays) -> torch.Tensor:
@@ -179,7 +206,8 @@ class HelmholtzBatch:
     # ------------------------------------------------------------------ solve
     def _launch(self, F: torch.Tensor, U: torch.Tensor) -> None:
         _lib.call("jfx_helm_ragged", MODES[self.mode], 1 if self.io_dtype == torch.float32 else 0,
-                  self.nwaves, K._p(self.items), self.nitems, K._p(self.counter), K._p(self.Vd),
+                  self.nwaves, K._p(self.items), self.nitems, K._p(self.counter), K._p(self.qbounds),
+                  self.nqueues, K._p(self.Vd),
                   K._p(self.VTd), K._p(self.Vs), K._p(self.VTs), K._p(self.lam), K._p(F), K._p(U),
                   K._p(self.scratch), self.nblocks, K._s())
 

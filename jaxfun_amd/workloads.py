@@ -77,7 +77,7 @@ def burgers_c3(N: int = 128, batch: int = 256, seed: int = 3, nmodes: int = 8):
 
 
 def varcoeff_c4(seed: int, N: int = 64, gmodes: int = 16):
-    """(ghat, fhat): g low-mode Legendre coefficients ~ N(0,1)(1+|m|)^-2, forcing as config 0."""
+    """(ghat, fhat): g low-mode Legendre coefficients ~ N(0,1)(1+|m|)^-2, forcing as in the Chebyshev Poisson workload."""
     rng = np.random.default_rng(seed)
     m = np.arange(gmodes, dtype=np.float64)
     mag = np.sqrt(m[:, None] ** 2 + m[None, :] ** 2)
