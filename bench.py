@@ -40,7 +40,7 @@ def bench_helmholtz(steps: int, warmup: int):
     from jaxfun_amd import workloads as W
     from jaxfun_amd.helmholtz import HelmholtzBatch, autotune
 
-    Ns, alphas, loads = W.helmholtz_c2()
+    Ns, alphas, loads = W.helmholtz_ragged_batch()
     res = {}
     outs = {}
     for mode in ("mixed", "fp64"):
@@ -65,7 +65,8 @@ def bench_helmholtz(steps: int, warmup: int):
         kms = float(np.mean([a.elapsed_time(b) for a, b in per]))
         res[mode] = dict(ms=wall * 1e3, kernel_ms=kms, solves_per_s=len(Ns) / wall,
                          io_bytes=hb.algorithmic_bytes(), alg_flops=hb.algorithmic_flops(),
-                         nitems=hb.nitems, nblocks=hb.nblocks, nwaves=hb.nwaves)
+                         mfma_flops=hb.mfma_flops(), nitems=hb.nitems, nblocks=hb.nblocks,
+                         nwaves=hb.nwaves, queues=hb.nqueues)
         outs[mode] = U.to(torch.float64)
     err = float(torch.linalg.vector_norm(outs["mixed"] - outs["fp64"]) / torch.linalg.vector_norm(outs["fp64"]))
     return res, err, (Ns, alphas, loads)
@@ -78,7 +79,7 @@ def bench_transform_3d(reps: int = 3):
     from jaxfun_amd import workloads as W
 
     N, B = 128, 16
-    c = torch.tensor(W.legendre_3d_c1(N, B), device="cuda")
+    c = torch.tensor(W.legendre_3d_coefficients(N, B), device="cuda")
     L = Legendre(N)
     T = TensorProduct(L, L, L)
     u64 = T.backward(c)
@@ -104,7 +105,7 @@ def bench_transform_3d(reps: int = 3):
     return out
 
 
-def bench_poisson_c0(reps: int = 50):
+def bench_poisson_chebyshev(reps: int = 50):
     import torch
 
     from jaxfun_amd import Chebyshev, Div, FunctionSpace, Grad, TensorProduct, TestFunction, TrialFunction, inner
@@ -115,7 +116,7 @@ def bench_poisson_c0(reps: int = 50):
     T = TensorProduct(D, D)
     v, u = TestFunction(T), TrialFunction(T)
     A = inner(v * Div(Grad(u)), sparse=True)
-    b = torch.tensor(-W.poisson_c0(N), device="cuda")
+    b = torch.tensor(-W.chebyshev_poisson_load(N), device="cuda")
     fd = A.fd_factor()
 
     def step(rhs):
@@ -153,8 +154,8 @@ def bench_poisson_c0(reps: int = 50):
                 batched_4096_solves_per_s=4096 / bt)
 
 
-def bench_burgers_c3(steps: int = 1000, batch: int = 256, N: int = 128):
-    """Config 3: IMEX ARS443 Burgers, Chebyshev-Dirichlet N x N, 3/2 padding,
+def bench_burgers(steps: int = 1000, batch: int = 256, N: int = 128):
+    """IMEX ARS443 Burgers, Chebyshev-Dirichlet N x N, 3/2 padding,
     mixed precision vs FP64, drift after `steps` steps."""
     import sympy as sp
     import torch
@@ -171,7 +172,7 @@ def bench_burgers_c3(steps: int = 1000, batch: int = 256, N: int = 128):
     x, y = T.system.base_scalars()
     nu, dt = 1e-2, 1e-3
     weak = v * (u.diff(t) + u * (u.diff(x) + u.diff(y)) - nu * Div(Grad(u)))
-    u0f, _, _ = W.burgers_c3(N, batch)
+    u0f, _, _ = W.burgers_initial_conditions(N, batch)
     X, Y = [m.cpu().numpy() for m in T.mesh()]
     U0 = T.forward(torch.tensor(np.stack([u0f(b, X, Y) for b in range(batch)]), device="cuda"))
     out = {}
@@ -195,8 +196,8 @@ def bench_burgers_c3(steps: int = 1000, batch: int = 256, N: int = 128):
     return out
 
 
-def bench_learned_pcg_c4(ntrain: int = 20000, neval: int = 1024, chunk: int = 1024, train_steps: int = 1500):
-    """Config 4: variable-coefficient Poisson 64^2, plain PCG vs learned PCG."""
+def bench_learned_pcg(ntrain: int = 20000, neval: int = 1024, chunk: int = 1024, train_steps: int = 1500):
+    """Variable-coefficient Poisson 64^2: plain PCG vs learned PCG."""
     import torch
 
     from jaxfun_amd import learned as Lm
@@ -205,7 +206,7 @@ def bench_learned_pcg_c4(ntrain: int = 20000, neval: int = 1024, chunk: int = 10
     op = Lm.VarCoeffPoisson2D(64, 16)
 
     def gen(seeds):
-        G, Fh = zip(*[W.varcoeff_c4(int(s)) for s in seeds])
+        G, Fh = zip(*[W.varcoeff_sample(int(s)) for s in seeds])
         return torch.tensor(np.stack(G), device="cuda"), torch.tensor(np.stack(Fh), device="cuda")
 
     t0 = time.perf_counter()
@@ -274,10 +275,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--with-burgers", action="store_true", help="also run config 3 (1000 IMEX steps)")
+    ap.add_argument("--with-burgers", action="store_true", help="also run the 2D Burgers IMEX workload (1000 ARS443 steps)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy 3x (known traffic, for PMC scaling)")
-    ap.add_argument("--with-pcg", action="store_true", help="also run config 4 (train + evaluate)")
+    ap.add_argument("--with-pcg", action="store_true", help="also run the variable-coefficient PCG workload (train + evaluate)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -296,6 +297,15 @@ def main():
     res, err, data = bench_helmholtz(args.steps, args.warmup)
     mixed, fp64 = res["mixed"], res["fp64"]
     achieved = mixed["io_bytes"] / (mixed["kernel_ms"] * 1e-3) / 1e9
+    kname = "helm_compact_kernel<float>" if mixed["nwaves"] == 4 else f"helm_ragged_kernel<1,{mixed['nwaves']},float>"
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "helm_ragged_4096_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tj = json.load(fh)
+        if tj.get("kernel") == kname:
+            traffic = tj["fetch_bytes"] + tj["write_bytes"]
+    mfma_tfs = mixed["mfma_flops"] / (mixed["kernel_ms"] * 1e-3) / 1e12
     line = {
         "metric": "solves/s + 3D transform GB/s at <=1e-6 rel err vs plain FP64 GPU; PCG iters",
         "value": mixed["solves_per_s"],
@@ -308,7 +318,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "bf16x6-split/fp32-acc (fp64 tridiagonal)",
-        "data": "synthetic (seeded config generators)",
+        "data": "synthetic (seeded generators, jaxfun_amd/workloads.py)",
         "config": {"workload": "ragged batch of 4096 2D Legendre-Dirichlet Helmholtz solves, "
                                "N~U{32..256}, alpha=10^U(-2,4), seed 2", "io_dtype": "float32",
                    "parallelism": "single GPU"},
@@ -316,19 +326,20 @@ def main():
         "fp64_path": {"solves_per_s": fp64["solves_per_s"], "ms_per_step": fp64["ms"]},
         "speedup_vs_fp64_gpu": mixed["solves_per_s"] / fp64["solves_per_s"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": f"helm_ragged_kernel<1,{mixed['nwaves']},float>", "kernel_ms": mixed["kernel_ms"],
-                     "launch": {"nwaves": mixed["nwaves"], "nblocks": mixed["nblocks"]},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": kname, "kernel_ms": mixed["kernel_ms"],
+                     "launch": {"nwaves": mixed["nwaves"], "nblocks": mixed["nblocks"], "queues": mixed["queues"]},
                      "alg_bytes_per_launch": mixed["io_bytes"],
-                     "alg_TFLOPs_4n3": mixed["alg_flops"] / (mixed["kernel_ms"] * 1e-3) / 1e12},
+                     "mfma_executed_TFLOPs": mfma_tfs, "mfma_frac_of_bf16_dense_peak": mfma_tfs / BF16_PEAK_TFS,
+                     "alg_TFLOPs_dense_4n3": mixed["alg_flops"] / (mixed["kernel_ms"] * 1e-3) / 1e12},
     }
     if not args.no_extras:
         line["transform_3d_16x128^3"] = bench_transform_3d()
-        line["poisson_chebyshev_N64"] = bench_poisson_c0()
+        line["poisson_chebyshev_N64"] = bench_poisson_chebyshev()
     if args.with_burgers:
-        line["burgers_c3"] = bench_burgers_c3()
+        line["burgers_imex_128x128"] = bench_burgers()
     if args.with_pcg:
-        line["learned_pcg_c4"] = bench_learned_pcg_c4()
+        line["learned_pcg_varcoeff_64"] = bench_learned_pcg()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(data)
     print(json.dumps(line), flush=True)

@@ -245,6 +245,22 @@ class HelmholtzBatch:
         n = (self.Ns - 2).astype(np.float64)
         return float(np.sum(4 * n**3 + 14 * n**2))
 
+    def mfma_flops(self) -> float:
+        """Flops of the MFMA instructions one launch issues: per (item, column
+        parity) two GEMMs of ceil(a/16) x ceil(b/16) output tiles over AP/32
+        k-slabs (bf16 16x16x32, six products each, in mixed mode; FP64 16x16x4
+        over AP/4 k-steps in fp64 mode), padding included."""
+        it = self.items_host
+        tot = 0.0
+        for q in (0, 1):
+            b = (it["n"] - q + 1) // 2
+            tiles = ((it["a"] + 15) // 16) * ((b + 15) // 16)
+            if self.mode == "mixed":
+                tot += float(np.sum(2 * (it["AP"] // 32) * tiles * 6)) * 16 * 16 * 32 * 2
+            else:
+                tot += float(np.sum(2 * (it["AP"] // 4) * tiles)) * 16 * 16 * 4 * 2
+        return tot
+
     def algorithmic_bytes(self) -> float:
         """F read + U written once each."""
         return float(2 * self.total * torch.tensor([], dtype=self.io_dtype).element_size())

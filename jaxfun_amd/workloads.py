@@ -33,13 +33,13 @@ def chebyshev_norms(N: int) -> np.ndarray:
     return h
 
 
-def poisson_c0(N: int = 64, seed: int = 0) -> np.ndarray:
+def chebyshev_poisson_load(N: int = 64, seed: int = 0) -> np.ndarray:
     """Chebyshev-Dirichlet Poisson load vector b (N-2, N-2)."""
     rng = np.random.default_rng(seed)
     return dirichlet_load(forcing_coefficients(rng, N), chebyshev_norms(N))
 
 
-def legendre_3d_c1(N: int = 128, B: int = 16, seed: int = 1) -> np.ndarray:
+def legendre_3d_coefficients(N: int = 128, B: int = 16, seed: int = 1) -> np.ndarray:
     """c_ijk ~ N(0,1) (1+i+j+k)^-3, shape (B, N, N, N), FP64."""
     rng = np.random.default_rng(seed)
     i = np.arange(N, dtype=np.float64)
@@ -50,7 +50,7 @@ def legendre_3d_c1(N: int = 128, B: int = 16, seed: int = 1) -> np.ndarray:
     return out
 
 
-def helmholtz_c2(nsolves: int = 4096, seed: int = 2, Nmin: int = 32, Nmax: int = 256):
+def helmholtz_ragged_batch(nsolves: int = 4096, seed: int = 2, Nmin: int = 32, Nmax: int = 256):
     """(Ns, alphas, loads): N ~ U{Nmin..Nmax}, alpha = 10^U(-2, 4), Legendre loads."""
     rng = np.random.default_rng(seed)
     Ns = rng.integers(Nmin, Nmax + 1, size=nsolves)
@@ -59,7 +59,7 @@ def helmholtz_c2(nsolves: int = 4096, seed: int = 2, Nmin: int = 32, Nmax: int =
     return Ns, alphas, loads
 
 
-def burgers_c3(N: int = 128, batch: int = 256, seed: int = 3, nmodes: int = 8):
+def burgers_initial_conditions(N: int = 128, batch: int = 256, seed: int = 3, nmodes: int = 8):
     """Initial conditions as sums of 8 smooth Dirichlet-compatible modes
     sin(p pi (x+1)/2) sin(q pi (y+1)/2), amplitudes U(-1,1), p, q in 1..4.
     Returns a callable u0(b, X, Y) and the mode table."""
@@ -76,7 +76,7 @@ def burgers_c3(N: int = 128, batch: int = 256, seed: int = 3, nmodes: int = 8):
     return u0, amp, pq
 
 
-def varcoeff_c4(seed: int, N: int = 64, gmodes: int = 16):
+def varcoeff_sample(seed: int, N: int = 64, gmodes: int = 16):
     """(ghat, fhat): g low-mode Legendre coefficients ~ N(0,1)(1+|m|)^-2, forcing as in the Chebyshev Poisson workload."""
     rng = np.random.default_rng(seed)
     m = np.arange(gmodes, dtype=np.float64)

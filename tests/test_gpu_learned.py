@@ -80,7 +80,7 @@ def small_problem(dev):
 
     N = 24
     op = VarCoeffPoisson2D(N, gmodes=8)
-    G, Fh = zip(*[W.varcoeff_c4(100 + s, N=N, gmodes=8) for s in range(6)])
+    G, Fh = zip(*[W.varcoeff_sample(100 + s, N=N, gmodes=8) for s in range(6)])
     ghat = torch.tensor(np.stack(G), device=dev)
     fhat = torch.tensor(np.stack(Fh), device=dev)
     return op, ghat, fhat, G, Fh
