@@ -200,6 +200,7 @@ def bench_learned_pcg(ntrain: int = 20000, neval: int = 1024, chunk: int = 1024,
     """Variable-coefficient Poisson 64^2: plain PCG vs learned PCG."""
     import torch
 
+    from jaxfun_amd import kernels as Kn
     from jaxfun_amd import learned as Lm
     from jaxfun_amd import workloads as W
 
@@ -209,20 +210,31 @@ def bench_learned_pcg(ntrain: int = 20000, neval: int = 1024, chunk: int = 1024,
         G, Fh = zip(*[W.varcoeff_sample(int(s)) for s in seeds])
         return torch.tensor(np.stack(G), device="cuda"), torch.tensor(np.stack(Fh), device="cuda")
 
+    # training pairs from plain PCG: (b, u) for every system, plus the Krylov
+    # pairs (r_k, u - x_k) after k = 2 and 5 iterations (A (u - x_k) = r_k)
+    snaps = (2, 5)
     t0 = time.perf_counter()
-    gh, bs, us, it_plain_train = [], [], [], []
+    gh, bs, us = [], [], []
+    kry = {k: ([], []) for k in snaps}
     for s0 in range(10000, 10000 + ntrain, chunk):
         g, fh = gen(range(s0, min(s0 + chunk, 10000 + ntrain)))
         kq = op.kfield(g)
         b = op.rhs(fh)
-        x, iters = Lm.pcg(lambda U: op.apply(kq, U), b, op.plain_precond(op.kbar(kq)), rtol=1e-10)
+        so = {}
+        x, _ = Lm.pcg(lambda U: op.apply(kq, U), b, op.plain_precond(op.kbar(kq)), rtol=1e-10,
+                      snapshots=snaps, snap_out=so)
         gh.append(g)
         bs.append(b)
         us.append(x)
-        it_plain_train.append(iters)
+        for k in snaps:
+            kry[k][0].append(so[k][1])
+            kry[k][1].append(Kn.lincomb([(1.0, x), (-1.0, so[k][0])]))
     torch.cuda.synchronize()
     gen_s = time.perf_counter() - t0
-    gh, bs, us = torch.cat(gh), torch.cat(bs), torch.cat(us)
+    nsys = sum(t_.shape[0] for t_ in gh)
+    gh = torch.cat(gh * (1 + len(snaps)))
+    bs = torch.cat(bs + [r_ for k in snaps for r_ in kry[k][0]])
+    us = torch.cat(us + [e_ for k in snaps for e_ in kry[k][1]])
     lp = Lm.LearnedPreconditioner(op, hidden=512, layers=4)
     t0 = time.perf_counter()
     hist = lp.train(gh, bs, us, steps=train_steps, batch=256, lr=1e-3)
@@ -238,13 +250,17 @@ def bench_learned_pcg(ntrain: int = 20000, neval: int = 1024, chunk: int = 1024,
     s_nn = lp.field(g)
     P = op.scaled_precond(s_nn)
     xl, itl = Lm.pcg(A, b, P, x0=P(b), rtol=1e-10)
+    _, itl0 = Lm.pcg(A, b, P, rtol=1e-10)
     res = {k: float(Lm.true_residual(op, kq, x_, b).max().item()) for k, x_ in
            (("plain", xp), ("analytic_sqrt_k", xa), ("learned", xl))}
-    itp, ita, itl = (t_.double().mean().item() for t_ in (itp, ita, itl))
-    return dict(train_pairs=ntrain, eval_systems=neval, mlp_params=lp.net.n_params(),
+    itp, ita, itl, itl0 = (t_.double().mean().item() for t_ in (itp, ita, itl, itl0))
+    return dict(train_systems=nsys, train_pairs=int(gh.shape[0]), eval_systems=neval,
+                mlp_params=lp.net.n_params(), loss="energy norm of P_s b - u, sigma = -g/2 + MLP(g)",
                 gen_s=gen_s, train_s=train_s, train_loss_first=hist[0], train_loss_last=hist[-1],
-                mean_iters_plain=itp, mean_iters_learned=itl, mean_iters_scaled_sqrt_k_no_network=ita,
-                iteration_ratio_plain_over_learned=itp / itl, max_true_residual=res)
+                mean_iters_plain=itp, mean_iters_learned=itl, mean_iters_learned_zero_x0=itl0,
+                mean_iters_scaled_sqrt_k_no_network=ita,
+                iteration_ratio_plain_over_learned=itp / itl,
+                iteration_ratio_no_network_over_learned=ita / itl, max_true_residual=res)
 
 
 def cpu_baseline(data, budget_s: float = 15.0):

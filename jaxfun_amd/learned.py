@@ -14,9 +14,11 @@ Preconditioners (all SPD, so PCG stays valid):
              on the quadrature mesh (B: coefficients -> mesh values,
              F: mesh values -> coefficients).  s = k^-1/2 is the classical
              choice; the learned preconditioner takes s = exp(sigma) with
-             sigma's 16 x 16 low Legendre modes predicted from g's by a
-             4-layer 512-wide tanh MLP trained on (b, u) pairs that plain PCG
-             generates, minimising ||P_s b - u||^2 / ||u||^2.  Learned PCG
+             sigma's 16 x 16 low Legendre modes = -g/2 + MLP(g) (a 4-layer
+             512-wide tanh MLP whose last layer starts at zero, so training
+             starts from the classical scaling), trained on (b, u) pairs that
+             plain PCG generates to minimise the energy-norm error of the
+             initial guess, (P_s b - u)^T A (P_s b - u) / u^T A u.  Learned PCG
              starts from x0 = P_s b.
 
 The MLP runs on the project's BF16 MFMA GEMM (FP32 accumulation) with its own
@@ -217,9 +219,12 @@ def batched_dot(x: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
 
 
 def pcg(apply, b: torch.Tensor, precond, x0: torch.Tensor | None = None, rtol: float = 1e-10,
-        maxiter: int = 2000, check_every: int = 1):
+        maxiter: int = 2000, check_every: int = 1, snapshots=(), snap_out: dict | None = None):
     """Batched preconditioned CG; every system stops on ||r|| <= rtol ||b||
-    (recursive residual).  Returns (x, iterations per system)."""
+    (recursive residual).  Returns (x, iterations per system).  For each k in
+    ``snapshots`` the iterate and residual after k iterations are stored in
+    ``snap_out[k] = (x_k, r_k)`` (systems that stopped earlier keep their
+    final values)."""
     B = b.shape[0]
     n = b[0].numel()
     x = torch.zeros_like(b) if x0 is None else x0.clone()
@@ -239,6 +244,8 @@ def pcg(apply, b: torch.Tensor, precond, x0: torch.Tensor | None = None, rtol: f
                   n, K._s())
         rr = batched_dot(r, r)
         _lib.call("jfx_pcg_check", K._p(rr), K._p(bb), rtol * rtol, K._p(active), K._p(iters), B, 1, K._s())
+        if snap_out is not None and it + 1 in snapshots:
+            snap_out[it + 1] = (x.clone(), r.clone())
         if (it + 1) % check_every == 0 and int(active.sum().item()) == 0:
             break
         z = precond(r)
@@ -255,35 +262,51 @@ def true_residual(op: VarCoeffPoisson2D, kq, x, b) -> torch.Tensor:
 
 # ----------------------------------------------------------------------------- learned preconditioner
 class LearnedPreconditioner:
-    """MLP (ghat -> low modes of log s) driving the scaled FD preconditioner."""
+    """MLP (ghat -> correction to the low modes of log s) driving the scaled FD
+    preconditioner.  ``prior`` sets the network-free part, sigma_hat = prior *
+    ghat (-1/2: s = k^-1/2; 0: the network alone); ``loss`` is ``"energy"``
+    (A-norm of the initial-guess error) or ``"l2"``."""
 
-    def __init__(self, op: VarCoeffPoisson2D, hidden: int = 512, layers: int = 4, seed: int = 0):
+    def __init__(self, op: VarCoeffPoisson2D, hidden: int = 512, layers: int = 4, seed: int = 0,
+                 prior: float = -0.5, loss: str = "energy"):
+        if loss not in ("energy", "l2"):
+            raise ValueError("loss must be 'energy' or 'l2'")
         gm2 = op.gm * op.gm
         self.op = op
+        self.prior = float(prior)
+        self.loss = loss
         self.net = MLP([gm2] + [hidden] * (layers - 1) + [gm2], seed=seed)
+        self.net.params[-1][0].zero_()  # the untrained network adds nothing to the prior
 
     def sigma_hat(self, ghat: torch.Tensor):
         X = _f32(ghat.reshape(ghat.shape[0], -1))
         Y, hs = self.net.forward(X)
-        return K.cast(Y, torch.float64).reshape(ghat.shape).contiguous(), hs
+        sh = K.lincomb([(1.0, K.cast(Y, torch.float64).reshape(ghat.shape)), (self.prior, ghat.contiguous())])
+        return sh, hs
 
     def field(self, ghat):
         sh, _ = self.sigma_hat(ghat)
         return self.op.sigma_field(sh)
 
     def loss_and_grad(self, ghat, b, u):
-        """Mean of ||P_s b - u||^2 / ||u||^2 and its MLP gradients."""
+        """Mean of ||P_s b - u||^2 / ||u||^2 over the batch (in the A-norm for
+        ``loss="energy"``) and its MLP gradients."""
         op = self.op
         sh, hs = self.sigma_hat(ghat)
         s = op.sigma_field(sh)
         z, parts = op.scaled_parts(s, b)
         diff = K.lincomb([(1.0, z), (-1.0, u)])
-        uu = batched_dot(u, u)
-        per = batched_dot(diff, diff) / uu
+        if self.loss == "energy":
+            kq = op.kfield(ghat)
+            Md, Mu = op.apply(kq, diff), op.apply(kq, u)  # A symmetric: d(d^T A d)/dz = 2 A d
+        else:
+            Md, Mu = diff, u
+        uu = batched_dot(u, Mu)
+        per = batched_dot(diff, Md) / uu
         Bn = b.shape[0]
         w = (2.0 / (uu * Bn)).contiguous()
         gz = torch.empty_like(diff)
-        _lib.call("jfx_scale_batch", K._p(diff), K._p(w), K._p(gz), Bn, diff[0].numel(), K._s())
+        _lib.call("jfx_scale_batch", K._p(Md), K._p(w), K._p(gz), Bn, diff[0].numel(), K._s())
         gs = op.scaled_backward(s, parts, gz)
         gsig = mul_bcast(gs, s)
         gsh = op._ap(op.VgT, op._ap(op.VgT, gsig, 1), 2)

@@ -155,3 +155,6 @@ def test_learned_training_reduces_loss_and_pcg_converges(small_problem):
     P = op.scaled_precond(s)
     x, iters = Lm.pcg(lambda U: op.apply(kq, U), b, P, x0=P(b), rtol=1e-10)
     assert np.all(Lm.true_residual(op, kq, x, b).cpu().numpy() < 1e-9)
+    # what the trained preconditioner gains over plain PCG (the method without it)
+    _, it_plain = Lm.pcg(lambda U: op.apply(kq, U), b, op.plain_precond(op.kbar(kq)), rtol=1e-10)
+    assert iters.double().mean().item() < it_plain.double().mean().item()
