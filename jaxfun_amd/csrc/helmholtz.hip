@@ -506,10 +506,12 @@ __device__ __forceinline__ void slab_prefetch2(SlabRegs (&P)[2], const unsigned 
 // steps ahead of its use, and the A fragments of the next row block are read
 // from LDS before the current block's MFMAs issue.  Rows a..AP-1 of Wk hold
 // zeros and columns >= b only feed discarded outputs, so B is read unmasked.
-template <typename TIO, bool TO_GLOBAL, int NH>
+// `between` runs after the last k-slab (Wk and P are free from then on) and
+// before the epilogue; `after` runs after it.
+template <typename TIO, bool TO_GLOBAL, int NH, typename Between, typename After>
 __device__ __forceinline__ void helmc_gemm(const HelmItem& I, int q, int b, const unsigned short* __restrict__ T,
                                            float* Wk, unsigned short* slab, TIO* __restrict__ Ub,
-                                           SlabRegs (&P)[2]) {
+                                           SlabRegs (&P)[2], Between&& between, After&& after) {
   constexpr int LDW = HelmCompact::LDW;
   // opaque copy of the thread id: keeps the compiler from hoisting every
   // lane-dependent address of the epilogue out of the item loop into registers
@@ -596,6 +598,7 @@ __device__ __forceinline__ void helmc_gemm(const HelmItem& I, int q, int b, cons
       }
     }
   }
+  between();
   // epilogue: output row m = rb*16 + lr + r, lr = 4 (lane / 16)
   const int lr = 4 * (lane >> 4), lim = a - lr;
 #pragma unroll
@@ -627,29 +630,76 @@ __device__ __forceinline__ void helmc_gemm(const HelmItem& I, int q, int b, cons
       }
     }
   }
+  after();
 }
 
-// Both column parities of one work item (solve, row parity p).
+// Registers of a gather issued one pass ahead: rows w + 4i (i < 32) of the
+// next parity block, columns lane and lane + 64 (FP32 I/O only; with FP64
+// I/O the next block is gathered after the epilogue instead).
+template <typename TIO>
+struct GatherRegs {
+  float v[sizeof(TIO) == 4 ? 32 : 1][2];
+};
+
+template <typename TIO>
+__device__ __forceinline__ void gather_issue(GatherRegs<TIO>& G, const TIO* __restrict__ Fb, int n, int p, int q,
+                                             int a, int b) {
+  if constexpr (sizeof(TIO) == 4) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const bool ok0 = lane < b, ok1 = lane + 64 < b;
+    const TIO* src = Fb + (long long)(p + 2 * w) * n + q + 2 * lane;
+#pragma unroll
+    for (int i = 0; i < 32; ++i) {
+      if (w + 4 * i < a) {
+        const TIO* s = src + (long long)i * 8 * n;  // 4 parity rows further per i
+        if (ok0) G.v[i][0] = s[0];
+        if (ok1) G.v[i][1] = s[128];
+      }
+    }
+  }
+}
+
+template <typename TIO>
+__device__ __forceinline__ void gather_store(const GatherRegs<TIO>& G, const TIO* __restrict__ Fb, int n, int p, int q,
+                                             int a, int b, float* Wk) {
+  constexpr int LDW = HelmCompact::LDW;
+  if constexpr (sizeof(TIO) == 4) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const bool ok0 = lane < b, ok1 = lane + 64 < b;
+#pragma unroll
+    for (int i = 0; i < 32; ++i) {
+      const int r = w + 4 * i;
+      if (r < a) {
+        if (ok0) Wk[r * LDW + lane] = G.v[i][0];
+        if (ok1) Wk[r * LDW + lane + 64] = G.v[i][1];
+      }
+    }
+  } else {
+    helm_gather<4, LDW>(Fb, n, p, q, a, b, Wk);
+  }
+}
+
+// One pass (column parity q) of work item I.  On entry Wk rows < a hold F_pq
+// and P holds phase 1's first slabs.  The next pass -- (In, qn) if has_next --
+// is set up during this pass's last epilogue: its F block is loaded into
+// registers before the output stores and written to Wk after them, and its
+// first table slabs are fetched into P.
 template <typename TIO, int NH>
-__device__ __forceinline__ void helmc_item(const HelmItem& I, const unsigned short* __restrict__ Vs,
+__device__ __forceinline__ void helmc_pass(const HelmItem& I, int q, const HelmItem& In, int qn, bool has_next,
+                                           const unsigned short* __restrict__ Vs,
                                            const unsigned short* __restrict__ VTs, const double* __restrict__ lam,
                                            const TIO* __restrict__ F, TIO* __restrict__ U, float* Wk,
-                                           unsigned short* slab, double* H) {
+                                           unsigned short* slab, double* H, SlabRegs (&P)[2]) {
   constexpr int LDW = HelmCompact::LDW, NT = HelmCompact::NT;
   const int tid = threadIdx.x, wu = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int n = I.n, p = I.p, a = I.a, AP = I.AP;
+  const int a = I.a, AP = I.AP;
   double* Hs = H + HMAXA + 2;                    // h_c + h_{c+2}
   const unsigned short* TV = Vs + 3 * I.voff;    // V_p slices    (phase 3)
   const unsigned short* TVT = VTs + 3 * I.voff;  // V_p^T slices  (phase 1)
-  SlabRegs P[2];
-  slab_prefetch2<NH>(P, TVT, AP, tid);
-
-  for (int q = 0; q < 2; ++q) {
-    const int b = (n - q + 1) >> 1;
-    if (b == 0) continue;
-    // ---------------- phase 0: F_pq -> Wk, h table
-    helm_gather<4, LDW>(F + I.foff, n, p, q, a, b, Wk);
-    for (int e = tid; e < (AP - a) * 128; e += NT) Wk[(a + (e >> 7)) * LDW + (e & 127)] = 0.f;  // pad rows
+  const int b = (I.n - q + 1) >> 1;
+  {
+    // ---------------- phase 0: pad rows, h tables (F_pq is already in Wk)
+    for (int e = tid; e < (AP - a) * 128; e += NT) Wk[(a + (e >> 7)) * LDW + (e & 127)] = 0.f;
     for (int j = tid; j <= b; j += NT) {
       const double hj = 2.0 / (2 * (q + 2 * j) + 1);
       H[j] = hj;
@@ -658,7 +708,7 @@ __device__ __forceinline__ void helmc_item(const HelmItem& I, const unsigned sho
     __syncthreads();
 
     // ---------------- phase 1: G = V_p^T F_pq (in place)
-    helmc_gemm<TIO, false, NH>(I, q, b, TVT, Wk, slab, nullptr, P);
+    helmc_gemm<TIO, false, NH>(I, q, b, TVT, Wk, slab, nullptr, P, [] {}, [] {});
     slab_prefetch2<NH>(P, TV, AP, tid);  // phase 3's first slabs fly during the row solves
     __syncthreads();
 
@@ -761,10 +811,23 @@ __device__ __forceinline__ void helmc_item(const HelmItem& I, const unsigned sho
     }
     __syncthreads();
 
-    // ---------------- phase 3: U_pq = V_p U~ -> global
-    helmc_gemm<TIO, true, NH>(I, q, b, TV, Wk, slab, U + I.foff, P);
-    if (q == 0) slab_prefetch2<NH>(P, TVT, AP, tid);  // the next pass's phase 1
-    __syncthreads();
+    // ---------------- phase 3: U_pq = V_p U~ -> global, next pass set up around the epilogue
+    GatherRegs<TIO> G;
+    const int bn = (In.n - qn + 1) >> 1;
+    helmc_gemm<TIO, true, NH>(
+        I, q, b, TV, Wk, slab, U + I.foff, P,
+        [&] {
+          if (has_next) {
+            gather_issue(G, F + In.foff, In.n, In.p, qn, In.a, bn);
+            if (In.AP > 64)
+              slab_prefetch2<2>(P, VTs + 3 * In.voff, In.AP, tid);
+            else
+              slab_prefetch2<1>(P, VTs + 3 * In.voff, In.AP, tid);
+          }
+        },
+        [&] {
+          if (has_next) gather_store(G, F + In.foff, In.n, In.p, qn, In.a, bn, Wk);
+        });
   }
 }
 
@@ -789,19 +852,39 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     if (tid == 0) s_nx[0] = first;
   }
   __syncthreads();
-  int it = __builtin_amdgcn_readfirstlane(s_nx[0]), slot = 1;
-  while (it < nitems) {
-    const HelmItem I = items[it];
+  const int it0 = __builtin_amdgcn_readfirstlane(s_nx[0]);
+  if (it0 >= nitems) return;
+  HelmItem I = items[it0];
+  helm_gather<4, HelmCompact::LDW>(F + I.foff, I.n, I.p, 0, I.a, (I.n + 1) >> 1, Wk);
+  SlabRegs P[2];
+  if (I.AP > 64)
+    slab_prefetch2<2>(P, VTs + 3 * I.voff, I.AP, tid);
+  else
+    slab_prefetch2<1>(P, VTs + 3 * I.voff, I.AP, tid);
+  int slot = 1;
+  for (;;) {
+    // the next item is claimed now; its index is published after this item's
+    // first pass (two slots: slot k is rewritten only after every thread has
+    // passed the barriers of the passes in between)
     int nxt = 0;
     if (tid < 64) nxt = Q.claim(qi);
     if (I.AP > 64)
-      helmc_item<TIO, 2>(I, Vs, VTs, lam, F, U, Wk, slab, H);
+      helmc_pass<TIO, 2>(I, 0, I, 1, true, Vs, VTs, lam, F, U, Wk, slab, H, P);
     else
-      helmc_item<TIO, 1>(I, Vs, VTs, lam, F, U, Wk, slab, H);
+      helmc_pass<TIO, 1>(I, 0, I, 1, true, Vs, VTs, lam, F, U, Wk, slab, H, P);
     if (tid == 0) s_nx[slot] = nxt;
     __syncthreads();
-    it = __builtin_amdgcn_readfirstlane(s_nx[slot]);
+    const int itn = __builtin_amdgcn_readfirstlane(s_nx[slot]);
     slot ^= 1;
+    const bool has = itn < nitems;
+    const HelmItem In = items[has ? itn : it0];
+    if (I.AP > 64)
+      helmc_pass<TIO, 2>(I, 1, In, 0, has, Vs, VTs, lam, F, U, Wk, slab, H, P);
+    else
+      helmc_pass<TIO, 1>(I, 1, In, 0, has, Vs, VTs, lam, F, U, Wk, slab, H, P);
+    if (!has) break;
+    I = In;
+    __syncthreads();
   }
 }
 
