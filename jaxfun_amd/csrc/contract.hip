@@ -378,12 +378,16 @@ static int make_colmap(int M, int K, long long P, long long Q, ColMap& cm) {
 template <int POL>
 int launch_persist(const void* A, const double* Arow, int M, int K, const void* X, void* Y, long long P,
                    long long Q, hipStream_t st, int* rc);
+template <int POL>
+int launch_ra(const void* A, int M, int K, const void* X, void* Y, long long P, long long Q, hipStream_t st,
+              int* rc);
 
 template <typename T>
 static int launch_plain(const T* A, int M, int K, const T* X, T* Y, long long P, long long Q,
                         hipStream_t st) {
   {
     int rc = 0;
+    if (launch_ra<(sizeof(T) == 8) ? 0 : 1>(A, M, K, X, Y, P, Q, st, &rc)) return rc;
     if (launch_persist<(sizeof(T) == 8) ? 0 : 1>(A, nullptr, M, K, X, Y, P, Q, st, &rc)) return rc;
   }
   ColMap cm;
@@ -409,6 +413,8 @@ static int launch_slices(const unsigned short* Asl, const double* Arow, int M, i
   if (S == 2 || S == 3) {
     constexpr int POL = (MODE == 1) ? 4 : (S == 2 ? 2 : 3);
     int rc = 0;
+    if constexpr (MODE == 0 && S == 3)
+      if (launch_ra<3>(Asl, M, K, X, Y, P, Q, st, &rc)) return rc;
     if ((MODE == 0 || S == 3) && launch_persist<POL>(Asl, Arow, M, K, X, Y, P, Q, st, &rc)) return rc;
   }
   ColMap cm;

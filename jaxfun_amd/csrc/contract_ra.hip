@@ -1,0 +1,435 @@
+// Register-resident-operator contraction for M, K <= 128:
+//     Y[p, m, q] = sum_k A[m, k] X[p, k, q],   X viewed as (P, K, Q)
+// for the two column layouts that cover every axis of a C-order array:
+// Q % 64 == 0 (a 64-column tile is 64 consecutive q of one p) and Q == 1
+// (a tile is 64 consecutive rows p, each K contiguous values).
+//
+// One persistent workgroup of 8 waves per CU.  Wave w keeps rows
+// 16w..16w+15 of the operator in registers for the whole launch (FP64/FP32:
+// one element per MFMA k-step; bf16x6: three bf16 slices per 32-k step), so
+// the only LDS operand traffic is the X tile.  Each 64-column tile of X (all
+// K rows) is staged through LDS once -- already split into three bf16 terms
+// for bf16x6 -- while the next tile is loaded into registers, so one barrier
+// pair covers a whole tile (up to 128 MFMAs per wave).  With Q == 1 the output
+// tile is transposed through LDS and written as one contiguous block.
+//   POL 0: FP64 v_mfma_f64_16x16x4f64       POL 1: FP32 v_mfma_f32_16x16x4f32
+//   POL 3: bf16 3-term float split, 6 products, FP32 accumulation (FP32 I/O)
+#include <type_traits>
+
+#include "jfx_common.h"
+
+namespace jfx {
+
+constexpr int RTN = 64, RNT = 512, RMAX = 128;
+
+template <int POL>
+struct RT;
+template <>
+struct RT<0> {
+  typedef double TX;
+  typedef v4d V;
+};
+template <>
+struct RT<1> {
+  typedef float TX;
+  typedef v4f V;
+};
+template <>
+struct RT<3> {
+  typedef float TX;
+  typedef v4f V;
+};
+
+template <int POL>
+struct RLds {
+  typedef typename RT<POL>::TX TX;
+  // plain: Bs[k][c], row stride 80 elements; bf16x6: Bsl[3][c][k], row stride 136 bf16
+  static constexpr int BLD = (POL == 3) ? 136 : 80;
+  static constexpr size_t B = (POL == 3) ? (size_t)3 * RTN * BLD * 2 : (size_t)RMAX * BLD * sizeof(TX);
+  // Q == 1 output staging Cs[c][m], row stride 132 elements
+  static constexpr int CLD = RMAX + 4;
+  static constexpr size_t C = (size_t)RTN * CLD * sizeof(TX);
+  static constexpr size_t TOTAL = B + C;
+};
+
+struct RMap {
+  long long P, Q, ncols, ntiles, nqt;  // nqt: 64-column tiles per p (Q % 64 == 0)
+  int M, K, qone;
+};
+
+// NS: k-steps run unconditionally (4 deep for FP64/FP32, 32 deep for bf16x6);
+// the operator registers and the staged rows past K are zero.
+template <int POL, int NS>
+__global__ __launch_bounds__(RNT) void contract_ra_kernel(const void* __restrict__ Aptr,
+                                                          const typename RT<POL>::TX* __restrict__ X,
+                                                          typename RT<POL>::TX* __restrict__ Y, RMap rm) {
+  typedef typename RT<POL>::TX TX;
+  typedef typename RT<POL>::V V;
+  constexpr int BLD = RLds<POL>::BLD, CLD = RLds<POL>::CLD;
+  constexpr int EPC = 16 / sizeof(TX);            // elements per 16-byte chunk
+  constexpr int NCH = RMAX * RTN / EPC / RNT;     // plain: chunks per thread per tile
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  TX* Bs = reinterpret_cast<TX*>(smem);
+  unsigned short* Bsl = reinterpret_cast<unsigned short*>(smem);
+  TX* Cs = reinterpret_cast<TX*>(smem + RLds<POL>::B);
+
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int M = rm.M, K = rm.K;
+  const long long Q = rm.Q;
+  const bool wact = 16 * w < M;  // this wave owns output rows
+
+  // ---- operator rows of this wave, resident in registers
+  typedef typename std::conditional<POL == 3, v8u16, TX>::type AR;
+  AR ar[POL == 3 ? 3 * NS : NS];
+  {
+    const int row = 16 * w + (lane & 15);
+    if constexpr (POL == 3) {
+      const unsigned short* Asl = reinterpret_cast<const unsigned short*>(Aptr);
+      const int Kp = (K + 31) & ~31;
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+#pragma unroll
+        for (int sl = 0; sl < 3; ++sl) {
+          v8u16 v = {0, 0, 0, 0, 0, 0, 0, 0};
+          if (row < M && 32 * s < Kp) v = *(const v8u16*)(Asl + ((long long)sl * M + row) * Kp + 32 * s + 8 * (lane >> 4));
+          ar[3 * s + sl] = v;
+        }
+    } else {
+      const TX* A = reinterpret_cast<const TX*>(Aptr);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        const int k = 4 * s + (lane >> 4);
+        ar[s] = (row < M && k < K) ? A[(long long)row * K + k] : TX(0);
+      }
+    }
+  }
+
+  // ---- X tile loads (registers) and LDS staging
+  // plain: NCH 16-byte chunks per thread; bf16x6: 2 groups of 8 consecutive k of one column
+  typedef typename std::conditional<POL == 3, float, TX>::type RV;
+  constexpr int NREG = (POL == 3) ? 16 : NCH * EPC;
+  RV xr[NREG];
+  auto tile_cols = [&](long long t, long long& xbase, long long& c0, int& ncol) {
+    if (rm.qone) {
+      c0 = t * RTN;
+      xbase = c0 * K;
+      const long long left = rm.ncols - c0;
+      ncol = left < RTN ? (int)left : RTN;
+    } else {
+      const long long p = t / rm.nqt, q0 = (t - p * rm.nqt) * RTN;
+      c0 = q0;
+      xbase = p * K * Q + q0;
+      ncol = RTN;
+    }
+  };
+  auto load_tile = [&](long long t) {
+    long long xbase, c0;
+    int ncol;
+    tile_cols(t, xbase, c0, ncol);
+    if constexpr (POL == 3) {
+#pragma unroll
+      for (int g = 0; g < 2; ++g) {
+        const int gi = tid + RNT * g;
+        if (rm.qone) {
+          const int c = gi >> 4, k8 = (gi & 15) * 8;
+          const bool ok = c < ncol && k8 < K;
+          const float* src = X + xbase + (long long)c * K + k8;
+          float4 v0 = ok ? *(const float4*)src : float4{0, 0, 0, 0};
+          float4 v1 = ok ? *(const float4*)(src + 4) : float4{0, 0, 0, 0};
+          xr[8 * g + 0] = v0.x; xr[8 * g + 1] = v0.y; xr[8 * g + 2] = v0.z; xr[8 * g + 3] = v0.w;
+          xr[8 * g + 4] = v1.x; xr[8 * g + 5] = v1.y; xr[8 * g + 6] = v1.z; xr[8 * g + 7] = v1.w;
+        } else {
+          const int c = gi & 63, k8 = (gi >> 6) * 8;
+#pragma unroll
+          for (int e = 0; e < 8; ++e) xr[8 * g + e] = (k8 + e < K) ? X[xbase + (long long)(k8 + e) * Q + c] : 0.f;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < NCH; ++i) {
+        const int j = tid + RNT * i;
+        TX v[EPC];
+        if (rm.qone) {
+          const int e0 = j * EPC, c = e0 / K;
+          const bool ok = c < ncol;
+          if (ok) {
+            const TX* src = X + xbase + e0;
+            if constexpr (EPC == 2) {
+              const double2 d = *(const double2*)src;
+              v[0] = d.x;
+              v[1] = d.y;
+            } else {
+              const float4 d = *(const float4*)src;
+              v[0] = d.x; v[1] = d.y; v[2] = d.z; v[3] = d.w;
+            }
+          } else {
+#pragma unroll
+            for (int u = 0; u < EPC; ++u) v[u] = TX(0);
+          }
+        } else {
+          constexpr int CPR = RTN / EPC;
+          const int k = j / CPR, col = (j - k * CPR) * EPC;
+          if (k < K) {
+            const TX* src = X + xbase + (long long)k * Q + col;
+            if constexpr (EPC == 2) {
+              const double2 d = *(const double2*)src;
+              v[0] = d.x;
+              v[1] = d.y;
+            } else {
+              const float4 d = *(const float4*)src;
+              v[0] = d.x; v[1] = d.y; v[2] = d.z; v[3] = d.w;
+            }
+          } else {
+#pragma unroll
+            for (int u = 0; u < EPC; ++u) v[u] = TX(0);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < EPC; ++u) xr[EPC * i + u] = v[u];
+      }
+    }
+  };
+  auto store_tile = [&]() {
+    if constexpr (POL == 3) {
+#pragma unroll
+      for (int g = 0; g < 2; ++g) {
+        const int gi = tid + RNT * g;
+        const int c = rm.qone ? (gi >> 4) : (gi & 63), k8 = rm.qone ? (gi & 15) * 8 : (gi >> 6) * 8;
+        float xv[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) xv[e] = xr[8 * g + e];
+        v8u16 h, m_, l;
+        split3_pk8(xv, h, m_, l);
+        *(v8u16*)&Bsl[(0 * RTN + c) * BLD + k8] = h;
+        *(v8u16*)&Bsl[(1 * RTN + c) * BLD + k8] = m_;
+        *(v8u16*)&Bsl[(2 * RTN + c) * BLD + k8] = l;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < NCH; ++i) {
+        const int j = tid + RNT * i;
+        if (rm.qone) {
+          const int e0 = j * EPC, c = e0 / K, k = e0 - c * K;
+          if (c < RTN) {
+#pragma unroll
+            for (int u = 0; u < EPC; ++u) Bs[(k + u) * BLD + c] = xr[EPC * i + u];
+          }
+        } else {
+          constexpr int CPR = RTN / EPC;
+          const int k = j / CPR, col = (j - k * CPR) * EPC;
+#pragma unroll
+          for (int u = 0; u < EPC; ++u) Bs[k * BLD + col + u] = xr[EPC * i + u];
+        }
+      }
+    }
+  };
+
+  if constexpr (POL != 3) {
+    // with Q == 1 only rows < K are staged: the rest of the k-steps reads zeros
+    for (int e = tid; e < RMAX * BLD; e += RNT) Bs[e] = TX(0);
+  }
+  typedef V AccT[4];
+  auto compute = [&](AccT& acc) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[j] = V{0, 0, 0, 0};
+      if (wact) {
+        // B fragments of step s + 1 are read from LDS before step s's MFMAs issue
+        if constexpr (POL == 3) {
+          v8u16 bf[2][4][3];
+          auto bload = [&](int s, v8u16(&dst)[4][3]) {
+  #pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const int off = (16 * j + (lane & 15)) * BLD + 32 * s + 8 * (lane >> 4);
+              dst[j][0] = *(const v8u16*)&Bsl[off];
+              dst[j][1] = *(const v8u16*)&Bsl[RTN * BLD + off];
+              dst[j][2] = *(const v8u16*)&Bsl[2 * RTN * BLD + off];
+            }
+          };
+          bload(0, bf[0]);
+  #pragma unroll
+          for (int s = 0; s < NS; ++s) {
+            if (s + 1 < NS) bload(s + 1, bf[(s + 1) & 1]);
+            __builtin_amdgcn_sched_barrier(0);  // the next step's reads issue before this step's MFMAs
+            const v8u16 ah = ar[3 * s], am = ar[3 * s + 1], al = ar[3 * s + 2];
+  #pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const v8u16 bh = bf[s & 1][j][0], bm = bf[s & 1][j][1], bl = bf[s & 1][j][2];
+              // products ordered small to large
+              acc[j] = mfma_bf16(al, bh, acc[j]);
+              acc[j] = mfma_bf16(am, bm, acc[j]);
+              acc[j] = mfma_bf16(ah, bl, acc[j]);
+              acc[j] = mfma_bf16(am, bh, acc[j]);
+              acc[j] = mfma_bf16(ah, bm, acc[j]);
+              acc[j] = mfma_bf16(ah, bh, acc[j]);
+            }
+          }
+        } else {
+          // B fragments are read T k-steps at a time, so one LDS round trip
+          // covers 4T MFMAs; a running offset keeps one address register
+          constexpr int T = 4;
+          int bo = (lane >> 4) * BLD + (lane & 15);
+  #pragma unroll
+          for (int s0 = 0; s0 < NS; s0 += T) {
+            TX bv[T][4];
+  #pragma unroll
+            for (int u = 0; u < T; ++u)
+  #pragma unroll
+              for (int j = 0; j < 4; ++j) bv[u][j] = Bs[bo + u * 4 * BLD + 16 * j];
+            __builtin_amdgcn_sched_barrier(0);  // all T steps' reads issue before their MFMAs
+            bo += T * 4 * BLD;
+            asm volatile("" : "+v"(bo));
+  #pragma unroll
+            for (int u = 0; u < T; ++u)
+  #pragma unroll
+              for (int j = 0; j < 4; ++j) {
+                if constexpr (POL == 0)
+                  acc[j] = mfma_f64(ar[s0 + u], bv[u][j], acc[j]);
+                else
+                  acc[j] = mfma_f32(ar[s0 + u], bv[u][j], acc[j]);
+              }
+          }
+        }
+      }
+  };
+  auto orow = [&](int r) { return 16 * w + ((POL == 0) ? mfma16_row_f64(lane, r) : mfma16_row_f32(lane, r)); };
+  // Q == 1: the output tile goes to LDS first (transposed), read back after a barrier
+  auto epi_stage = [&](const AccT& acc) {
+    if (rm.qone && wact) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Cs[(16 * j + (lane & 15)) * CLD + orow(r)] = (TX)acc[j][r];
+    }
+  };
+  auto epi_out = [&](long long tt, const AccT& acc) {
+    long long xbase, c0;
+    int ncol;
+    tile_cols(tt, xbase, c0, ncol);
+    if (!rm.qone) {
+      if (wact) {
+        TX* dst = Y + (xbase / ((long long)K * Q)) * (long long)M * Q + c0 + (lane & 15);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int m = orow(r);
+            if (m < M) dst[(long long)m * Q + 16 * j] = (TX)acc[j][r];
+          }
+      }
+    } else {
+      // the (ncol x M) output block is contiguous: 16-byte chunks, linear order
+      TX* dst = Y + c0 * M;
+      const int nchunk = ncol * M / EPC;
+      for (int ch = tid; ch < nchunk; ch += RNT) {
+        const int e0 = ch * EPC, c = e0 / M, m = e0 - c * M;
+        if constexpr (EPC == 2) {
+          *(double2*)(dst + e0) = double2{Cs[c * CLD + m], Cs[c * CLD + m + 1]};
+        } else {
+          *(float4*)(dst + e0) = float4{Cs[c * CLD + m], Cs[c * CLD + m + 1], Cs[c * CLD + m + 2], Cs[c * CLD + m + 3]};
+        }
+      }
+    }
+  };
+
+  // Pipeline per tile: the next tile is staged and the one after it loaded
+  // before this tile's output stores are issued, so the wait for the staged
+  // loads never includes stores issued just before it.
+  long long t = blockIdx.x;
+  if (t >= rm.ntiles) return;
+  load_tile(t);
+  __syncthreads();  // the zero fill above is complete
+  store_tile();
+  __syncthreads();
+  if (t + gridDim.x < rm.ntiles) load_tile(t + gridDim.x);
+  AccT acc;
+  compute(acc);
+  for (;;) {
+    const long long tn = t + gridDim.x;
+    epi_stage(acc);
+    __syncthreads();  // tile t's LDS reads are done, its Q == 1 output is staged
+    if (tn < rm.ntiles) store_tile();
+    epi_out(t, acc);
+    __syncthreads();  // tile tn is staged, the staged output has been read
+    if (tn >= rm.ntiles) break;
+    if (tn + gridDim.x < rm.ntiles) load_tile(tn + gridDim.x);
+    compute(acc);
+    t = tn;
+  }
+}
+
+static long long ra_grid() {
+  static long long g = 0;
+  if (!g) {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
+      ncu = 256;
+    g = ncu;
+  }
+  return g;
+}
+
+// Returns 1 (and the status in *rc) if the register-resident kernel handled
+// the call, 0 if the shape or alignment needs the general kernels.
+template <int POL>
+int launch_ra(const void* A, int M, int K, const void* X, void* Y, long long P, long long Q, hipStream_t st,
+              int* rc) {
+  typedef typename RT<POL>::TX TX;
+  constexpr int EPC = 16 / sizeof(TX);
+  if (M <= 0 || K <= 0 || M > RMAX || K > RMAX || P <= 0 || Q <= 0) return 0;
+  if (((uintptr_t)X & 15) || ((uintptr_t)Y & 15)) return 0;
+  RMap rm;
+  rm.P = P;
+  rm.Q = Q;
+  rm.ncols = P * Q;
+  rm.M = M;
+  rm.K = K;
+  if (Q == 1) {
+    // 16-byte chunks never straddle rows (K, M multiples of the chunk) and the
+    // staged rows cover whole MFMA k-steps (K % 4 == 0; bf16: K % 8 == 0 with
+    // the rest of the 32-step zero-filled)
+    if (K % (POL == 3 ? 8 : 4) || M % EPC) return 0;
+    rm.qone = 1;
+    rm.nqt = 0;
+    rm.ntiles = (P + RTN - 1) / RTN;
+  } else if (Q % RTN == 0) {
+    rm.qone = 0;
+    rm.nqt = Q / RTN;
+    rm.ntiles = P * rm.nqt;
+  } else {
+    return 0;
+  }
+  typedef void (*KernT)(const void*, const TX*, TX*, RMap);
+  KernT kern;
+  if constexpr (POL == 3) {
+    const int kst = (K + 31) / 32;  // 32-deep k-steps needed
+    kern = kst <= 1 ? contract_ra_kernel<POL, 1> : kst <= 2 ? contract_ra_kernel<POL, 2> : contract_ra_kernel<POL, 4>;
+  } else {
+    const int kst = (K + 3) / 4;
+    kern = kst <= 8 ? contract_ra_kernel<POL, 8> : kst <= 16 ? contract_ra_kernel<POL, 16> : contract_ra_kernel<POL, 32>;
+  }
+  const size_t shm = RLds<POL>::TOTAL;
+  hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+  if (e != hipSuccess) {
+    set_error("contract_ra: hipFuncSetAttribute: %s", hipGetErrorString(e));
+    *rc = JFX_EHIP;
+    return 1;
+  }
+  long long grid = ra_grid();
+  if (grid > rm.ntiles) grid = rm.ntiles;
+  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(RNT), shm, st, A, (const TX*)X, (TX*)Y, rm);
+  e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("contract_ra launch: %s", hipGetErrorString(e));
+    *rc = JFX_EHIP;
+    return 1;
+  }
+  *rc = JFX_OK;
+  return 1;
+}
+
+template int launch_ra<0>(const void*, int, int, const void*, void*, long long, long long, hipStream_t, int*);
+template int launch_ra<1>(const void*, int, int, const void*, void*, long long, long long, hipStream_t, int*);
+template int launch_ra<3>(const void*, int, int, const void*, void*, long long, long long, hipStream_t, int*);
+
+}  // namespace jfx

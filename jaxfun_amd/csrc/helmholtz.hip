@@ -818,11 +818,11 @@ __device__ __forceinline__ void helmc_pass(const HelmItem& I, int q, const HelmI
         I, q, b, TV, Wk, slab, U + I.foff, P,
         [&] {
           if (has_next) {
-            gather_issue(G, F + In.foff, In.n, In.p, qn, In.a, bn);
             if (In.AP > 64)
               slab_prefetch2<2>(P, VTs + 3 * In.voff, In.AP, tid);
             else
               slab_prefetch2<1>(P, VTs + 3 * In.voff, In.AP, tid);
+            gather_issue(G, F + In.foff, In.n, In.p, qn, In.a, bn);
           }
         },
         [&] {

@@ -28,7 +28,10 @@ class TensorProductSpace:
         self.name = name
         self.tensorname = "⊗".join(b.name for b in self.basespaces)
         self.precision = precision
-        self.fused = True  # fuse the last two axes of transforms into one kernel
+        # fuse the last two axes into one kernel for batches of fewer planes than
+        # CUs (one launch instead of two); larger batches keep every CU busy with
+        # the per-axis kernels, which run closer to the MFMA / HBM roofs
+        self.fused = True
 
     def __len__(self) -> int:
         return len(self.basespaces)
@@ -87,7 +90,10 @@ class TensorProductSpace:
         if nb < 0:
             raise ValueError(f"array of rank {X.ndim} is smaller than the space dimension {len(self)}")
         d = len(ops)
-        if d >= 2 and self.fused and K.plane_ok(ops[-2], ops[-1], prec):
+        nplanes = 1
+        for n_ in X.shape[:-2]:
+            nplanes *= n_
+        if d >= 2 and self.fused and nplanes < K.num_cus() and K.plane_ok(ops[-2], ops[-1], prec):
             # leading axes one by one, the last two in one fused pass
             for ax, op in enumerate(ops[:-2]):
                 X = K.apply_along(op, X, nb + ax, prec)

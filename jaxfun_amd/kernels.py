@@ -259,6 +259,17 @@ def apply_along(op: AxisOperator, X: torch.Tensor, axis: int, prec: str = "f64",
 PLANE_PRECISIONS = ("f64", "bf16x6")
 
 
+_NUM_CUS = None
+
+
+def num_cus() -> int:
+    """Compute units of the current device (cached)."""
+    global _NUM_CUS
+    if _NUM_CUS is None:
+        _NUM_CUS = torch.cuda.get_device_properties(device()).multi_processor_count
+    return _NUM_CUS
+
+
 def plane_ok(op0: AxisOperator, op1: AxisOperator, prec: str) -> bool:
     return prec in PLANE_PRECISIONS and max(op0.M, op0.K, op1.M, op1.K) <= 128
 

@@ -47,9 +47,19 @@ SHAPES = [  # (shape, axis, M)
     ((3, 96, 5), 1, 200),
     ((1, 17), 1, 1),
 ]
+# shapes of the register-resident-operator kernel (M, K <= 128; Q == 1 or Q % 64 == 0):
+# partial last tiles, M and K below 128, row counts that leave waves idle
+RA_SHAPES = [
+    ((100, 64), 1, 64),
+    ((3, 60, 64), 1, 36),
+    ((130, 120), 1, 124),
+    ((2, 96, 128), 1, 96),
+    ((5, 72), 1, 40),
+    ((2, 32, 192), 1, 16),
+]
 
 
-@pytest.mark.parametrize("shape,axis,M", SHAPES)
+@pytest.mark.parametrize("shape,axis,M", SHAPES + RA_SHAPES)
 def test_contract_f64(dev, shape, axis, M):
     from jaxfun_amd import kernels as K
 
@@ -64,7 +74,7 @@ def test_contract_f64(dev, shape, axis, M):
 # ozakiS drops the digit products of level >= S: error ~ (S+1) 2^-8S sqrt(K)
 @pytest.mark.parametrize("prec,tol", [("f32", 2e-6), ("bf16x3", 2e-5), ("bf16x6", 2e-6),
                                       ("ozaki3", 1e-6), ("ozaki4", 5e-9), ("ozaki5", 5e-11)])
-@pytest.mark.parametrize("shape,axis,M", SHAPES[:6])
+@pytest.mark.parametrize("shape,axis,M", SHAPES[:6] + RA_SHAPES)
 def test_contract_mixed(dev, prec, tol, shape, axis, M):
     import torch
 
