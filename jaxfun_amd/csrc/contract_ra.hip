@@ -40,16 +40,14 @@ struct RT<3> {
   typedef v4f V;
 };
 
+// Two X-tile buffers: plain Bs[k][c] (row stride 72 doubles / 80 floats),
+// bf16x6 Bsl[3][c][k] (row stride 136 bf16).
 template <int POL>
 struct RLds {
   typedef typename RT<POL>::TX TX;
-  // plain: Bs[k][c], row stride 80 elements; bf16x6: Bsl[3][c][k], row stride 136 bf16
-  static constexpr int BLD = (POL == 3) ? 136 : 80;
+  static constexpr int BLD = (POL == 3) ? 136 : (POL == 0 ? 72 : 80);
   static constexpr size_t B = (POL == 3) ? (size_t)3 * RTN * BLD * 2 : (size_t)RMAX * BLD * sizeof(TX);
-  // Q == 1 output staging Cs[c][m], row stride 132 elements
-  static constexpr int CLD = RMAX + 4;
-  static constexpr size_t C = (size_t)RTN * CLD * sizeof(TX);
-  static constexpr size_t TOTAL = B + C;
+  static constexpr size_t TOTAL = 2 * B;
 };
 
 struct RMap {
@@ -65,13 +63,13 @@ __global__ __launch_bounds__(RNT) void contract_ra_kernel(const void* __restrict
                                                           typename RT<POL>::TX* __restrict__ Y, RMap rm) {
   typedef typename RT<POL>::TX TX;
   typedef typename RT<POL>::V V;
-  constexpr int BLD = RLds<POL>::BLD, CLD = RLds<POL>::CLD;
+  constexpr int BLD = RLds<POL>::BLD;
   constexpr int EPC = 16 / sizeof(TX);            // elements per 16-byte chunk
   constexpr int NCH = RMAX * RTN / EPC / RNT;     // plain: chunks per thread per tile
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-  TX* Bs = reinterpret_cast<TX*>(smem);
-  unsigned short* Bsl = reinterpret_cast<unsigned short*>(smem);
-  TX* Cs = reinterpret_cast<TX*>(smem + RLds<POL>::B);
+  TX* Bs0 = reinterpret_cast<TX*>(smem);  // buffer b at Bs0 + b * BSZ
+  unsigned short* Bsl0 = reinterpret_cast<unsigned short*>(smem);
+  constexpr int BSZ = (int)(RLds<POL>::B / sizeof(TX)), BSL = (int)(RLds<POL>::B / 2);
 
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -190,7 +188,9 @@ __global__ __launch_bounds__(RNT) void contract_ra_kernel(const void* __restrict
       }
     }
   };
-  auto store_tile = [&]() {
+  auto store_tile = [&](int buf) {
+    TX* Bs = Bs0 + buf * BSZ;
+    unsigned short* Bsl = Bsl0 + buf * BSL;
     if constexpr (POL == 3) {
 #pragma unroll
       for (int g = 0; g < 2; ++g) {
@@ -227,10 +227,12 @@ __global__ __launch_bounds__(RNT) void contract_ra_kernel(const void* __restrict
 
   if constexpr (POL != 3) {
     // with Q == 1 only rows < K are staged: the rest of the k-steps reads zeros
-    for (int e = tid; e < RMAX * BLD; e += RNT) Bs[e] = TX(0);
+    for (int e = tid; e < 2 * BSZ; e += RNT) Bs0[e] = TX(0);
   }
   typedef V AccT[4];
-  auto compute = [&](AccT& acc) {
+  auto compute = [&](AccT& acc, int buf) {
+    const TX* Bs = Bs0 + buf * BSZ;
+    const unsigned short* Bsl = Bsl0 + buf * BSL;
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[j] = V{0, 0, 0, 0};
       if (wact) {
@@ -293,68 +295,64 @@ __global__ __launch_bounds__(RNT) void contract_ra_kernel(const void* __restrict
       }
   };
   auto orow = [&](int r) { return 16 * w + ((POL == 0) ? mfma16_row_f64(lane, r) : mfma16_row_f32(lane, r)); };
-  // Q == 1: the output tile goes to LDS first (transposed), read back after a barrier
-  auto epi_stage = [&](const AccT& acc) {
-    if (rm.qone && wact) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) Cs[(16 * j + (lane & 15)) * CLD + orow(r)] = (TX)acc[j][r];
-    }
-  };
   auto epi_out = [&](long long tt, const AccT& acc) {
     long long xbase, c0;
     int ncol;
     tile_cols(tt, xbase, c0, ncol);
+    if (!wact) return;
     if (!rm.qone) {
-      if (wact) {
-        TX* dst = Y + (xbase / ((long long)K * Q)) * (long long)M * Q + c0 + (lane & 15);
+      TX* dst = Y + (xbase / ((long long)K * Q)) * (long long)M * Q + c0 + (lane & 15);
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < 4; ++j)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int m = orow(r);
-            if (m < M) dst[(long long)m * Q + 16 * j] = (TX)acc[j][r];
-          }
-      }
+        for (int r = 0; r < 4; ++r) {
+          const int m = orow(r);
+          if (m < M) dst[(long long)m * Q + 16 * j] = (TX)acc[j][r];
+        }
     } else {
-      // the (ncol x M) output block is contiguous: 16-byte chunks, linear order
-      TX* dst = Y + c0 * M;
-      const int nchunk = ncol * M / EPC;
-      for (int ch = tid; ch < nchunk; ch += RNT) {
-        const int e0 = ch * EPC, c = e0 / M, m = e0 - c * M;
-        if constexpr (EPC == 2) {
-          *(double2*)(dst + e0) = double2{Cs[c * CLD + m], Cs[c * CLD + m + 1]};
-        } else {
-          *(float4*)(dst + e0) = float4{Cs[c * CLD + m], Cs[c * CLD + m + 1], Cs[c * CLD + m + 2], Cs[c * CLD + m + 3]};
+      // output column c is the contiguous M-vector Y[(c0 + c) M ...]; with FP32
+      // accumulators a lane holds 4 consecutive rows of it (one 16-byte store)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int c = 16 * j + (lane & 15);
+        if (c < ncol) {
+          TX* dst = Y + (c0 + c) * M;
+          if constexpr (POL == 0) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int m = orow(r);
+              if (m < M) dst[m] = acc[j][r];
+            }
+          } else {
+            const int m = orow(0);  // rows m .. m+3, M % 4 == 0
+            if (m < M) *(float4*)(dst + m) = float4{acc[j][0], acc[j][1], acc[j][2], acc[j][3]};
+          }
         }
       }
     }
   };
 
-  // Pipeline per tile: the next tile is staged and the one after it loaded
-  // before this tile's output stores are issued, so the wait for the staged
-  // loads never includes stores issued just before it.
+  // Pipeline: tile i is multiplied from LDS buffer i % 2 while tile i + 1 is
+  // staged into the other buffer from registers loaded one tile earlier; one
+  // barrier per tile separates the staging of a buffer from its readers.
   long long t = blockIdx.x;
   if (t >= rm.ntiles) return;
   load_tile(t);
   __syncthreads();  // the zero fill above is complete
-  store_tile();
-  __syncthreads();
+  store_tile(0);
   if (t + gridDim.x < rm.ntiles) load_tile(t + gridDim.x);
+  int buf = 0;
   AccT acc;
-  compute(acc);
   for (;;) {
+    __syncthreads();  // buffer buf is staged; the other buffer's readers are done
+    compute(acc, buf);
     const long long tn = t + gridDim.x;
-    epi_stage(acc);
-    __syncthreads();  // tile t's LDS reads are done, its Q == 1 output is staged
-    if (tn < rm.ntiles) store_tile();
+    if (tn < rm.ntiles) store_tile(buf ^ 1);
     epi_out(t, acc);
-    __syncthreads();  // tile tn is staged, the staged output has been read
     if (tn >= rm.ntiles) break;
     if (tn + gridDim.x < rm.ntiles) load_tile(tn + gridDim.x);
-    compute(acc);
     t = tn;
+    buf ^= 1;
   }
 }
 
