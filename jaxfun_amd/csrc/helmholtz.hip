@@ -647,14 +647,17 @@ __device__ __forceinline__ void gather_issue(GatherRegs<TIO>& G, const TIO* __re
   if constexpr (sizeof(TIO) == 4) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const bool ok0 = lane < b, ok1 = lane + 64 < b;
-    const TIO* src = Fb + (long long)(p + 2 * w) * n + q + 2 * lane;
+    // running element offset (an integer keeps the global address space)
+    int off = (p + 2 * w) * n + q + 2 * lane;  // < n^2 <= 65536
+    const int rs = 8 * n;                      // 4 parity rows further per i
 #pragma unroll
     for (int i = 0; i < 32; ++i) {
       if (w + 4 * i < a) {
-        const TIO* s = src + (long long)i * 8 * n;  // 4 parity rows further per i
-        if (ok0) G.v[i][0] = s[0];
-        if (ok1) G.v[i][1] = s[128];
+        if (ok0) G.v[i][0] = Fb[off];
+        if (ok1) G.v[i][1] = Fb[off + 128];
       }
+      off += rs;
+      asm volatile("" : "+v"(off));
     }
   }
 }
@@ -666,13 +669,15 @@ __device__ __forceinline__ void gather_store(const GatherRegs<TIO>& G, const TIO
   if constexpr (sizeof(TIO) == 4) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const bool ok0 = lane < b, ok1 = lane + 64 < b;
+    int o = w * LDW + lane;  // running LDS offset of row w + 4i (one register, not 32)
 #pragma unroll
     for (int i = 0; i < 32; ++i) {
-      const int r = w + 4 * i;
-      if (r < a) {
-        if (ok0) Wk[r * LDW + lane] = G.v[i][0];
-        if (ok1) Wk[r * LDW + lane + 64] = G.v[i][1];
+      if (w + 4 * i < a) {
+        if (ok0) Wk[o] = G.v[i][0];
+        if (ok1) Wk[o + 64] = G.v[i][1];
       }
+      o += 4 * LDW;
+      asm volatile("" : "+v"(o));
     }
   } else {
     helm_gather<4, LDW>(Fb, n, p, q, a, b, Wk);
@@ -737,9 +742,8 @@ __device__ __forceinline__ void helmc_pass(const HelmItem& I, int q, const HelmI
       const double dg0 = fma(8.0, (double)j0, 4.0 * q + 6.0), ddg = 8.0 * dj;
       float cm[64];
       double P1 = 1.0, P2 = 0.0, yprev = 0.0, oprev = 0.0, cl = 0.0;
-      auto fwd = [&](int i) {
+      auto fwd_v = [&](int i, double g, double hs, double ho) {
         const int j = j0 + dj * i;
-        const double g = row[j], hs = Hs[j], ho = Ho[j];
         const double d = fma(s, hs, fma((double)i, ddg, dg0));
         const double on = -s * ho;
         const double Pj = fma(d, P1, -(oprev * oprev) * P2);
@@ -753,11 +757,27 @@ __device__ __forceinline__ void helmc_pass(const HelmItem& I, int q, const HelmI
         yprev = yj;
         oprev = on;
       };
+      auto fwd = [&](int i) {
+        const int j = j0 + dj * i;
+        fwd_v(i, row[j], Hs[j], Ho[j]);
+      };
 #pragma unroll
       for (int ch = 0; ch < 8; ++ch) {
         if (ch * 8 + 8 <= len) {
+          // the chunk's loads all issue before its arithmetic (each step reads
+          // its own row entry before writing it)
+          float g[8];
+          double hs[8], ho[8];
 #pragma unroll
-          for (int u = 0; u < 8; ++u) fwd(ch * 8 + u);
+          for (int u = 0; u < 8; ++u) {
+            const int j = j0 + dj * (ch * 8 + u);
+            g[u] = row[j];
+            hs[u] = Hs[j];
+            ho[u] = Ho[j];
+          }
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int u = 0; u < 8; ++u) fwd_v(ch * 8 + u, g[u], hs[u], ho[u]);
           renorm(P1, P2);
         } else if (ch * 8 < len) {
 #pragma unroll
@@ -800,6 +820,7 @@ __device__ __forceinline__ void helmc_pass(const HelmItem& I, int q, const HelmI
           float y[8];
 #pragma unroll
           for (int u = 0; u < 8; ++u) y[u] = row[j0 + dj * (ch * 8 + u)];
+          __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
           for (int u = 7; u >= 0; --u) bwd(ch * 8 + u, y[u]);
         } else if (ch * 8 <= start) {
