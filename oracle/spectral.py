@@ -7,23 +7,23 @@ Each function cites the reference file:line (paths relative to the
 reference's ``src/jaxfun``) whose algorithm it restates.  How each part is
 pinned by checks that do not retype the formula (all in ``tests/test_oracle.py``):
 
-* Vandermonde + derivatives (a1/a2): ``numpy.polynomial.{legendre,chebyshev}``
+* Vandermonde + derivatives: ``numpy.polynomial.{legendre,chebyshev}``
   ``legvander/chebvander`` + ``legder/chebder`` (what the reference's own
   ``tests/galerkin/test_approximations.py:48-87`` compares against).
-* Gauss nodes (a3): ``numpy.polynomial.legendre.leggauss``; exactness of the
+* Gauss nodes: ``numpy.polynomial.legendre.leggauss``; exactness of the
   rule on monomials up to degree 2N-1; Chebyshev closed form.
-* Galerkin matrices (a6/a10): the literal shenfun golden diagonals of the
+* Galerkin matrices: the literal shenfun golden diagonals of the
   reference's ``tests/galerkin/test_inner.py:104-229``; sympy exact integrals;
   closed forms (Legendre-Dirichlet K = -(4k+6) I).
-* Transforms (a4/a5/a8): forward(backward(c)) == c; exactness on
+* Transforms: forward(backward(c)) == c; exactness on
   polynomials; Chebyshev backward vs ``chebval``.
-* Banded solve (a7): dense ``numpy.linalg.solve``.
-* Per-axis contraction (a9): ``numpy.einsum``.
-* FD / Kronecker solves (a12/a13): ``scipy.linalg.solve`` on the assembled
+* Banded solve: dense ``numpy.linalg.solve``.
+* Per-axis contraction: ``numpy.einsum``.
+* FD / Kronecker solves: ``scipy.linalg.solve`` on the assembled
   Kronecker matrix; manufactured solution (1-x^2)(1-y^2).
-* IMEX ARS443 (a14): exact modal decay exp(-nu*lambda*t) of the linear heat
+* IMEX ARS443: exact modal decay exp(-nu*lambda*t) of the linear heat
   equation (third-order convergence), and the tableau's row sums.
-* Variable-coefficient operator / PCG / MLP (config-4 path): dense assembled
+* Variable-coefficient operator / PCG / MLP: dense assembled
   operator vs the matrix-free one; PCG vs direct solve; MLP gradients vs
   central finite differences.
 """
@@ -36,7 +36,7 @@ import numpy as np
 import scipy.linalg as sla
 
 # --------------------------------------------------------------------------
-# a3: quadrature
+# quadrature
 # --------------------------------------------------------------------------
 
 
@@ -87,7 +87,7 @@ def quad(family: str, N: int) -> tuple[np.ndarray, np.ndarray]:
 
 
 # --------------------------------------------------------------------------
-# a1/a2: Vandermonde matrices and derivative coefficients
+# Vandermonde matrices and derivative coefficients
 # --------------------------------------------------------------------------
 
 
@@ -167,7 +167,7 @@ def norm_squared(family: str, N: int) -> np.ndarray:
 
 
 # --------------------------------------------------------------------------
-# a6: Dirichlet stencil, closed-form matrices
+# Dirichlet stencil, closed-form matrices
 # --------------------------------------------------------------------------
 
 
@@ -251,7 +251,7 @@ def legendre_dirichlet_stiffness(N: int) -> np.ndarray:
 
 
 # --------------------------------------------------------------------------
-# a7: banded LU without pivoting (DIA solve)
+# banded LU without pivoting (DIA solve)
 # --------------------------------------------------------------------------
 
 
@@ -285,7 +285,7 @@ def banded_lu_solve(A: np.ndarray, b: np.ndarray, p: int, q: int) -> np.ndarray:
 
 
 # --------------------------------------------------------------------------
-# a4/a5/a8/a9: transforms and per-axis contraction
+# transforms and per-axis contraction
 # --------------------------------------------------------------------------
 
 
@@ -360,7 +360,7 @@ def tp_scalar_product(spaces, u):
 
 
 # --------------------------------------------------------------------------
-# a12/a13: Kronecker and fast-diagonalisation solves
+# Kronecker and fast-diagonalisation solves
 # --------------------------------------------------------------------------
 
 
@@ -480,7 +480,7 @@ def helmholtz_partial_fd_lapack(N: int, alpha: float, F: np.ndarray) -> np.ndarr
 
 
 # --------------------------------------------------------------------------
-# a14: IMEX ARS443
+# IMEX ARS443
 # --------------------------------------------------------------------------
 
 ARS443_EXPLICIT = np.array([
@@ -528,7 +528,7 @@ def imex_ars443_step(u, dt, mass_apply, mass_solve, linear_apply, stage_solve,
 
 
 # --------------------------------------------------------------------------
-# Config-3 Burgers pieces (Chebyshev Dirichlet, 3/2 padding)
+# Burgers pieces (Chebyshev Dirichlet, 3/2 padding)
 # --------------------------------------------------------------------------
 
 
@@ -574,7 +574,7 @@ class Burgers2D:
 
 
 # --------------------------------------------------------------------------
-# Config-4: variable-coefficient Poisson, PCG, MLP
+# Variable-coefficient Poisson, PCG, MLP
 # --------------------------------------------------------------------------
 
 

@@ -1,4 +1,4 @@
-"""IMEX ARS443 Burgers (config-3 path) vs the oracle's coefficient-space restatement."""
+"""IMEX ARS443 Burgers (2D Chebyshev-Dirichlet, 3/2 dealiasing) vs the oracle's coefficient-space restatement."""
 
 import numpy as np
 import pytest

@@ -1,4 +1,4 @@
-"""Config-4 path: variable-coefficient operator, batched PCG, MLP kernels and
+"""Variable-coefficient operator, batched PCG, MLP kernels and
 the learned preconditioner, against the FP64 oracle."""
 
 import numpy as np
