@@ -121,6 +121,14 @@ int jfx_div_bcast_f32(const float* X, const float* D, float* Y, long long total,
 int jfx_mul_bcast_f64(const double* X, const double* C, double* Y, long long total, long long size,
                       hipStream_t stream);
 
+/* Y = X + alpha * D with D (size) broadcast over the leading batch: adds the
+ * padded boundary-lifting coefficients (or their values on a mesh) of an
+ * inhomogeneous Dirichlet space, DirectSum.to_orthogonal / from_orthogonal
+ * (galerkin/composite.py:588-602) and DirectSumTPS.to_orthogonal /
+ * from_orthogonal (galerkin/tensorproductspace.py:806-851). */
+int jfx_axpy_bcast_f64(const double* X, const double* D, double alpha, double* Y, long long total,
+                       long long size, hipStream_t stream);
+
 /* y (+)= sum_j coeffs[j] * xs[j], nin <= 8; xs and coeffs are HOST arrays of
  * device pointers / scalars.  dtype 0 = f64, 1 = f32.  The stage
  * combinations of IMEXRungeKutta.stage (integrators/imex_rk.py:55-94). */

@@ -42,6 +42,7 @@ SIGNATURES: dict[str, tuple] = {
     "jfx_div_bcast_f64": (c_int, [c_vp, c_vp, c_vp, c_ll, c_ll, c_vp]),
     "jfx_div_bcast_f32": (c_int, [c_vp, c_vp, c_vp, c_ll, c_ll, c_vp]),
     "jfx_mul_bcast_f64": (c_int, [c_vp, c_vp, c_vp, c_ll, c_ll, c_vp]),
+    "jfx_axpy_bcast_f64": (c_int, [c_vp, c_vp, c_dbl, c_vp, c_ll, c_ll, c_vp]),
     "jfx_lincomb": (c_int, [c_int, c_int, c_vp, c_vp, c_vp, c_ll, c_int, c_vp]),
     "jfx_cast": (c_int, [c_int, c_vp, c_vp, c_ll, c_vp]),
     "jfx_sum_products": (c_int, [c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_ll, c_vp]),

@@ -77,6 +77,14 @@ __global__ void mul_bcast_kernel(const T* __restrict__ X, const T* __restrict__ 
   for (; i < total; i += stride) Y[i] = X[i] * C[i % size];
 }
 
+// Y[b, i] = X[b, i] + alpha * D[i]  (D broadcast over the leading batch)
+__global__ void axpy_bcast_kernel(const double* __restrict__ X, const double* __restrict__ D, double alpha,
+                                  double* __restrict__ Y, long long total, long long size) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < total; i += stride) Y[i] = X[i] + alpha * D[i % size];
+}
+
 // out = sum_t c_t * A_t * B_t  (pointwise, nterms <= 4)
 template <typename T>
 struct SumProd {
@@ -175,6 +183,15 @@ int jfx_mul_bcast_f64(const double* X, const double* C, double* Y, long long tot
                       hipStream_t stream) {
   if (!X || !C || !Y || size <= 0 || total % size) return set_error("mul_bcast: bad arguments"), JFX_EARG;
   hipLaunchKernelGGL(mul_bcast_kernel<double>, dim3(grid_for(total, 256)), dim3(256), 0, stream, X, C, Y,
+                     total, size);
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
+}
+
+int jfx_axpy_bcast_f64(const double* X, const double* D, double alpha, double* Y, long long total,
+                       long long size, hipStream_t stream) {
+  if (!X || !D || !Y || size <= 0 || total % size) return set_error("axpy_bcast: bad arguments"), JFX_EARG;
+  hipLaunchKernelGGL(axpy_bcast_kernel, dim3(grid_for(total, 256)), dim3(256), 0, stream, X, D, alpha, Y,
                      total, size);
   JFX_LAUNCH_CHECK();
   return JFX_OK;

@@ -43,7 +43,10 @@ class Chebyshev(OrthogonalSpace):
         if (i, j) not in ((0, 0), (0, 1), (1, 0), (0, 2), (2, 0)):
             return None
         if i == 0 and j == 0:
-            return DiaMatrix.from_dense(np.diag(self.norm_squared()[: min(self.N, u.N)]))
+            m = min(self.N, u.N)
+            A = np.zeros((self.N, u.N))
+            A[np.arange(m), np.arange(m)] = self.norm_squared()[:m]
+            return DiaMatrix.from_dense(A)
         if i in (1, 2) and j == 0:
             m = u._matrices(j, (self, i), q)
             return None if m is None else m.T

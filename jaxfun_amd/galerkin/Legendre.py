@@ -39,7 +39,10 @@ class Legendre(OrthogonalSpace):
         if not isinstance(u, Legendre) or q != 0:
             return None
         if i == 0 and j == 0:
-            return DiaMatrix.from_dense(np.diag(self.norm_squared()[: min(self.N, u.N)]))
+            m = min(self.N, u.N)
+            A = np.zeros((self.N, u.N))
+            A[np.arange(m), np.arange(m)] = self.norm_squared()[:m]
+            return DiaMatrix.from_dense(A)
         if i == 0 and j in (1, 2):
             if j >= u.N:
                 return None

@@ -2,15 +2,15 @@
 
 from .arguments import TestFunction, TrialFunction
 from .Chebyshev import CGComposite, Chebyshev
-from .composite import BoundaryConditions, Composite
+from .composite import BCGeneric, BoundaryConditions, Composite, DirectSum, get_bc_basis
 from .functionspace import FunctionSpace
 from .inner import InnerKind, TensorMatrix, inner, project
 from .Legendre import Legendre, LGComposite
 from .orthogonal import MeshKind, OrthogonalSpace
-from .tensorproductspace import TensorProduct, TensorProductSpace
+from .tensorproductspace import DirectSumTPS, TensorProduct, TensorProductSpace
 
 __all__ = [
-    "BoundaryConditions", "CGComposite", "Chebyshev", "Composite", "FunctionSpace", "InnerKind",
+    "BCGeneric", "BoundaryConditions", "CGComposite", "DirectSum", "DirectSumTPS", "get_bc_basis", "Chebyshev", "Composite", "FunctionSpace", "InnerKind",
     "LGComposite", "Legendre", "MeshKind", "OrthogonalSpace", "TensorMatrix", "TensorProduct",
     "TensorProductSpace", "TestFunction", "TrialFunction", "inner", "project",
 ]

@@ -1,9 +1,11 @@
 """Boundary-constrained composite bases (reference: galerkin/composite.py).
 
-Covers homogeneous Dirichlet conditions on Legendre / Chebyshev bases, the
+Covers Dirichlet conditions on Legendre / Chebyshev bases: the homogeneous
 stencil ``phi_k = (psi_k - psi_{k+2}) / s_k`` (galerkin/composite.py:786-789)
-and its transforms.  Inhomogeneous lifting (BCGeneric / DirectSum) is not on
-this build's path and raises NotImplementedError.
+with its transforms, and inhomogeneous values through the lifting space
+``BCGeneric`` and the direct sum ``DirectSum = Composite (+) BCGeneric``
+(galerkin/composite.py:411-638).  The lift is a constant coefficient vector
+added on the device (``kernels.add_bcast``) to every transform.
 """
 
 from __future__ import annotations
@@ -162,3 +164,273 @@ class Composite(OrthogonalSpace):
 
     def get_orthogonal(self):
         return self.orthogonal
+
+    def __add__(self, b: "BCGeneric") -> "DirectSum":
+        """Direct sum self (+) b (galerkin/composite.py:405-407)."""
+        return DirectSum(self, b)
+
+
+# ----------------------------------------------------------------------------- lifting
+def _boundary_value(family: str, j: int, side: str) -> float:
+    """psi_j at the reference-domain end: 1 at x = +1, (-1)^j at x = -1 for
+    Legendre and Chebyshev (the ``bnd_values`` of galerkin/Jacobi.py and
+    galerkin/Chebyshev.py for k = 0)."""
+    if family not in ("legendre", "chebyshev"):
+        raise NotImplementedError(f"boundary values of the {family!r} family")
+    return 1.0 if side == "R" else float((-1) ** j)
+
+
+def get_bc_basis(bcs, orthogonal) -> np.ndarray:
+    """Lifting basis (num_bcs, num_bcs) (galerkin/composite.py:835-895).
+
+    Row r holds the orthogonal coefficients of the function that takes the
+    value 1 in boundary condition r and 0 in the others: with the boundary
+    matrix ``A[r, j] = psi_j(x_r)`` over the first num_bcs orthogonal
+    functions, the rows are those of ``(A^-1)^T``.  Dirichlet conditions
+    only (the conditions of the composite bases built here); for
+    ``{left: D, right: D}`` the rows are (psi_0 -+ psi_1) / 2.
+    """
+    bcs = bcs if isinstance(bcs, BoundaryConditions) else BoundaryConditions(bcs)
+    names = bcs.orderednames()
+    if any(nm[1:] != "D" for nm in names):
+        raise NotImplementedError(f"lifting of {names}: only Dirichlet conditions are built")
+    nb = len(names)
+    A = np.array([[_boundary_value(orthogonal.family, j, nm[0]) for j in range(nb)] for nm in names])
+    return np.linalg.inv(A).T.copy()
+
+
+def _as_number(v):
+    e = sp.sympify(v)
+    if e.free_symbols:
+        raise ValueError(f"boundary value {v} depends on {e.free_symbols}; a 1D space needs numbers")
+    return float(e)
+
+
+class BCGeneric(Composite):
+    """Space of the boundary lifting functions only (galerkin/composite.py:410-475).
+
+    ``dim`` = number of boundary conditions, ``num_dofs`` = 0.  Its stencil
+    ``S`` (num_bcs, num_bcs) comes from :func:`get_bc_basis`; the orthogonal
+    space has num_bcs modes.
+    """
+
+    is_orthogonal = False
+
+    def __init__(self, N: int, orthogonal: type, bcs, domain=None, name: str = "BCGeneric",
+                 fun_str: str = "B", system=None, stencil=None, num_quad_points: int = 0, **kw):
+        self.bcs = BoundaryConditions(bcs)
+        nb = self.bcs.num_bcs()
+        nq = int(num_quad_points) if num_quad_points else nb
+        OrthogonalSpace.__init__(self, nb, domain=domain, system=system, name=name, fun_str=fun_str,
+                                 num_quad_points=nq)
+        self.bcs = BoundaryConditions(bcs)
+        self.orthogonal = orthogonal(nb, domain=domain, system=system, num_quad_points=nq)
+        self.family = self.orthogonal.family
+        self.S = get_bc_basis(self.bcs, self.orthogonal)
+        self.scaling = sp.S.One
+        self.stencil = None
+        self._mass_lu = None
+
+    @property
+    def dim(self) -> int:
+        return self.S.shape[0]
+
+    @property
+    def num_dofs(self) -> int:
+        return 0
+
+    def stencil_matrix(self) -> np.ndarray:
+        return self.S
+
+    def bnd_vals(self) -> np.ndarray:
+        """Ordered boundary values as numbers (galerkin/composite.py:459-466)."""
+        return np.array([_as_number(v) for v in self.bcs.orderedvals()], dtype=np.float64)
+
+    def get_homogeneous(self):
+        raise NotImplementedError("BCGeneric does not support get_homogeneous()")
+
+
+class DirectSum:
+    """V = Composite (+) BCGeneric: homogeneous unknowns plus a fixed lift
+    carrying the boundary values (galerkin/composite.py:490-638).
+
+    Coefficient arrays hold the homogeneous part only (``dim`` = dim of the
+    Composite).  ``to_orthogonal`` adds the lift's padded orthogonal
+    coefficients; ``backward``/``evaluate_mesh``/``backward_primitive`` add
+    the lift's values (or derivatives) on the requested mesh, cached per mesh;
+    ``forward`` is the orthogonal projection followed by ``from_orthogonal``;
+    ``scalar_product`` uses the homogeneous test functions only.  Leading
+    batch dimensions are allowed; the lift is broadcast over them.
+    """
+
+    is_transient = False
+    is_orthogonal = False
+
+    def __init__(self, a: Composite, b: BCGeneric) -> None:
+        if not isinstance(b, BCGeneric):
+            raise TypeError("DirectSum needs a BCGeneric second summand")
+        self.basespaces = (a, b)
+        self.bcs = b.bcs
+        self.name = a.name + "\u2295" + b.name
+        self._ops: dict = {}
+
+    def __getitem__(self, i: int):
+        return self.basespaces[i]
+
+    def __len__(self) -> int:
+        return len(self.basespaces)
+
+    def __iter__(self):
+        return iter(self.basespaces)
+
+    def __repr__(self) -> str:
+        return f"DirectSum({self[0]!r}, bcs={dict(self.bcs)})"
+
+    # ------------------------------------------------------------ delegated geometry
+    @property
+    def system(self):
+        return self[0].system
+
+    @property
+    def N(self) -> int:
+        return self[0].N
+
+    @property
+    def family(self) -> str:
+        return self[0].family
+
+    @property
+    def domain(self):
+        return self[0].domain
+
+    @property
+    def domain_factor(self) -> float:
+        return self[0].domain_factor
+
+    @property
+    def num_quad_points(self) -> int:
+        return self[0].num_quad_points
+
+    @property
+    def orthogonal(self):
+        return self[0].orthogonal
+
+    @property
+    def dim(self) -> int:
+        return self[0].dim
+
+    @property
+    def num_dofs(self) -> int:
+        return self[0].num_dofs
+
+    @property
+    def dims(self) -> int:
+        return 1
+
+    @property
+    def shape(self) -> tuple[int, ...]:
+        return (self.num_quad_points,)
+
+    def map_reference_domain(self, x):
+        return self[0].map_reference_domain(x)
+
+    def map_true_domain(self, X):
+        return self[0].map_true_domain(X)
+
+    def quad_points_and_weights(self, N: int | None = None):
+        return self[0].quad_points_and_weights(N)
+
+    def mesh(self, kind: str = "quadrature", N: int | None = None):
+        return self[0].mesh(kind, N)
+
+    def get_homogeneous(self) -> Composite:
+        return self[0]
+
+    def get_orthogonal(self):
+        return self[0].get_orthogonal()
+
+    def bnd_vals(self) -> np.ndarray:
+        return self[1].bnd_vals()
+
+    def with_system(self, system) -> "DirectSum":
+        """Copy with both summands on ``system`` (device operators shared)."""
+        parts = []
+        for s in self.basespaces:
+            c = copy.copy(s)
+            c._ops = s._ops
+            c.system = system
+            c.orthogonal = copy.copy(s.orthogonal)
+            c.orthogonal._ops = s.orthogonal._ops
+            c.orthogonal.system = system
+            parts.append(c)
+        out = DirectSum(parts[0], parts[1])
+        out._ops = self._ops
+        return out
+
+    # ------------------------------------------------------------ lift
+    def lift_coefficients(self) -> torch.Tensor:
+        """(N,): the lift in the orthogonal basis, ``pad(bnd_vals @ S_b)``
+        (galerkin/composite.py:588-593)."""
+        if "lift" not in self._ops:
+            cb = self.bnd_vals() @ self[1].S
+            full = np.zeros(self.N)
+            full[: cb.shape[0]] = cb
+            self._ops["lift"] = torch.tensor(full, dtype=torch.float64, device=K.device())
+        return self._ops["lift"]
+
+    def lift_values(self, N: int | None = None, k: int = 0, kind: str = "quadrature") -> torch.Tensor:
+        """(Q,): k-th derivative of the lift on the mesh of ``kind`` with N points."""
+        N = self.num_quad_points if N is None else int(N)
+        key = ("liftv", N, k, kind)
+        if key not in self._ops:
+            op = self.orthogonal.backward_operator(N, k, kind)
+            self._ops[key] = K.apply_along(op, self.lift_coefficients().reshape(1, -1), 1,
+                                           "f64").reshape(-1).contiguous()
+        return self._ops[key]
+
+    @staticmethod
+    def _f64(precision: str) -> None:
+        if precision != "f64":
+            raise ValueError("inhomogeneous (lifted) spaces transform in f64 only")
+
+    # ------------------------------------------------------------ transforms
+    def to_orthogonal(self, c):
+        """Homogeneous part mapped by the stencil plus the lift (composite.py:588-593)."""
+        from .orthogonal import as_device
+
+        return K.add_bcast(self[0].to_orthogonal(as_device(c)).contiguous(), self.lift_coefficients())
+
+    def from_orthogonal(self, a):
+        """(S S^T)^-1 S (a - lift) (galerkin/composite.py:595-601)."""
+        from .orthogonal import as_device
+
+        a = as_device(a)
+        return self[0].from_orthogonal(K.add_bcast(a, self.lift_coefficients(), -1.0))
+
+    def backward(self, c, N: int | None = None, precision: str = "f64"):
+        """orthogonal.backward(to_orthogonal(c)) (composite.py:613-616), as the
+        homogeneous backward plus the lift's values on the mesh."""
+        self._f64(precision)
+        return K.add_bcast(self[0].backward(c, N).contiguous(), self.lift_values(N))
+
+    def backward_primitive(self, c, k: int = 0, N: int | None = None, precision: str = "f64"):
+        """k-th derivative at the mesh (galerkin/composite.py:618-626)."""
+        self._f64(precision)
+        return K.add_bcast(self[0].backward_primitive(c, k, N).contiguous(), self.lift_values(N, k))
+
+    def evaluate_mesh(self, c, kind: str = "quadrature", N: int | None = None):
+        """Series values on a quadrature or uniform mesh (composite.py:606-611)."""
+        return K.add_bcast(self[0].evaluate_mesh(c, kind, N).contiguous(), self.lift_values(N, 0, kind))
+
+    def evaluate(self, x, c):
+        """Series values at arbitrary points x of the true domain (composite.py:603-605)."""
+        return self.orthogonal.evaluate(x, self.to_orthogonal(c))
+
+    def forward(self, u, precision: str = "f64"):
+        """from_orthogonal(orthogonal.forward(u)) (galerkin/composite.py:628-631)."""
+        self._f64(precision)
+        return self.from_orthogonal(self.orthogonal.forward(u))
+
+    def scalar_product(self, u, precision: str = "f64"):
+        """(u, phi_i) with the homogeneous test functions (composite.py:633-636)."""
+        return self[0].scalar_product(u, precision)

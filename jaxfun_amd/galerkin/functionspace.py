@@ -3,16 +3,16 @@
 from __future__ import annotations
 
 from .Chebyshev import CGComposite, Chebyshev
-from .composite import BoundaryConditions, Composite
+from .composite import BCGeneric, BoundaryConditions, Composite, DirectSum
 from .Legendre import Legendre, LGComposite
 
 
 def FunctionSpace(N: int, space: type, bcs=None, domain=None, system=None, name: str | None = None,
                   fun_str: str | None = None, **kw):
-    """Return ``space(N)`` or its homogeneous-Dirichlet composite.
-
-    Same signature as the reference; inhomogeneous boundary values (the
-    reference's DirectSum lifting) raise NotImplementedError.
+    """Return ``space(N)``, its homogeneous-Dirichlet composite, or for
+    inhomogeneous boundary values ``DirectSum(composite, BCGeneric)``
+    (galerkin/functionspace.py:130-173; the lifting space takes the name and
+    function string with a ``_b`` suffix and the composite's name gains ``0``).
     """
     if name is not None:
         kw["name"] = name
@@ -21,8 +21,16 @@ def FunctionSpace(N: int, space: type, bcs=None, domain=None, system=None, name:
     if bcs is None:
         return space(N, domain=domain, system=system, **kw)
     bcs = BoundaryConditions(bcs, domain=domain)
-    if not bcs.is_homogeneous():
-        raise NotImplementedError("inhomogeneous boundary conditions (DirectSum lifting) are not built")
     comp = LGComposite if issubclass(space, Legendre) else CGComposite if issubclass(space, Chebyshev) \
         else Composite
-    return comp(N, space, bcs=bcs, domain=domain, system=system, **kw)
+    C = comp(N, space, bcs=bcs.get_homogeneous(), domain=domain, system=system, **kw)
+    if bcs.is_homogeneous():
+        return C
+    kwb = {}
+    if "name" in kw:
+        kwb["name"] = kw["name"] + "_b"
+    if "fun_str" in kw:
+        kwb["fun_str"] = kw["fun_str"] + "_b"
+    B = BCGeneric(bcs.num_bcs(), space, bcs=bcs, domain=domain, system=system, num_quad_points=N, **kwb)
+    C.name += "0"
+    return DirectSum(C, B)

@@ -20,7 +20,7 @@ from .. import kernels as K
 from ..la import DiaMatrix, Matrix, TPMatrices, TPMatrix
 from ..utils.common import lambdify
 from .arguments import ArgumentTag, functionspace_of, get_arg
-from .composite import Composite
+from .composite import BCGeneric, Composite
 from .orthogonal import MeshKind
 from .tensorproductspace import TensorProductSpace
 
@@ -196,6 +196,14 @@ def pointwise_mul(X: torch.Tensor, C: torch.Tensor) -> torch.Tensor:
 
 
 # ----------------------------------------------------------------------------- inner
+def _lift_of(space):
+    """Fixed coefficients of a lifting trial space (1D BCGeneric: its boundary
+    values; a tensor product with a lifting factor: its ``bndvals``), else None."""
+    if isinstance(space, BCGeneric):
+        return torch.tensor(space.bnd_vals(), dtype=torch.float64, device=K.device())
+    return getattr(space, "bnd_coefficients", None)
+
+
 def _coerce_kind(kind):
     if kind is None or isinstance(kind, InnerKind):
         return kind
@@ -233,10 +241,18 @@ def inner(expr, sparse: bool = False, sparse_tol: int = 1000, num_quad_points=No
     multivar = []    # TensorMatrix terms
     bvec = None
 
+    def accumulate(b, c, X):
+        return K.lincomb([(c, X)]) if b is None else K.lincomb([(1.0, b), (c, X)])
+
     for t in terms:
         if t.trial is not None:
             Uspace = functionspace_of(t.trial)
             uspaces = Uspace.basespaces if isinstance(Uspace, TensorProductSpace) else [Uspace]
+            # a lifting part of a DirectSum / DirectSumTPS trial function has
+            # fixed coefficients: its term moves to the right-hand side with the
+            # sign of a linear form in a system (galerkin/inner.py:362-428,
+            # 493-562, 445-490)
+            lift = _lift_of(Uspace)
             sep = _separate(t.coeff, symbols)
             if sep is None:
                 if d != 2:
@@ -247,6 +263,10 @@ def inner(expr, sparse: bool = False, sparse_tol: int = 1000, num_quad_points=No
                     i = t.dv.get(symbols[ax], 0)
                     j = t.du.get(symbols[ax], 0)
                     ops.append((uspaces[ax].backward_operator(Nq[ax], j), test_operator(spaces[ax], i, Nq[ax])))
+                if lift is not None:
+                    X = TensorMatrix([(coef, ops)], tuple(s.dim for s in spaces))(lift)
+                    bvec = accumulate(bvec, -1.0, X.contiguous())
+                    continue
                 multivar.append((coef, ops))
                 continue
             c = float(sep.get("coeff", 1))
@@ -257,6 +277,12 @@ def inner(expr, sparse: bool = False, sparse_tol: int = 1000, num_quad_points=No
                 j = t.du.get(s, 0)
                 cf = sp.sympify(sep.get(s, sp.S.One))
                 mats.append(bilinear_1d(spaces[ax], i, uspaces[ax], j, cf, Nq[ax], use_precomputed_matrices))
+            if lift is not None:
+                X = lift
+                for ax, m in enumerate(mats):
+                    X = m.matvec(X.contiguous(), ax)
+                bvec = accumulate(bvec, -c, X.contiguous())
+                continue
             amats.append((c, mats))
         else:
             coef = _mesh_values(spaces, symbols, t.coeff, Nq)
@@ -265,7 +291,7 @@ def inner(expr, sparse: bool = False, sparse_tol: int = 1000, num_quad_points=No
                 i = t.dv.get(symbols[ax], 0)
                 X = K.apply_along(test_operator(spaces[ax], i, Nq[ax]), X.contiguous(), ax, "f64")
             sign = -1.0 if has_bilinear else 1.0
-            bvec = K.lincomb([(sign, X)]) if bvec is None else K.lincomb([(1.0, bvec), (sign, X)])
+            bvec = accumulate(bvec, sign, X.contiguous())
 
     A = None
     if amats:

@@ -211,6 +211,15 @@ class OrthogonalSpace:
     def evaluate_mesh(self, c, kind: str = MeshKind.QUADRATURE, N: int | None = None):
         return _apply_1d(self.backward_operator(N, 0, kind), c, "f64")
 
+    def evaluate(self, x, c):
+        """Series values at arbitrary true-domain points x (galerkin/orthogonal.py:102-129):
+        the basis evaluated at the points, contracted with the last axis of c."""
+        xs = as_device(x)
+        X = self.map_reference_domain(xs.reshape(-1)).contiguous()
+        V = self._apply_stencil_cols(K.vandermonde(self.family, X, self.N, 0))
+        out = _apply_1d(K.AxisOperator(V.contiguous()), c, "f64")
+        return out.reshape(tuple(out.shape[:-1]) + tuple(xs.shape))
+
     def mass_matrix(self):
         from ..la import DiaMatrix
 

@@ -10,11 +10,16 @@ selects the contraction arithmetic: ``f64`` (default), ``f32``, ``bf16x3``,
 from __future__ import annotations
 
 import copy
+import itertools
 
+import numpy as np
+import sympy as sp
 import torch
 
 from .. import kernels as K
 from ..coordinates import CartCoordSys, default_system
+from ..utils.common import lambdify
+from .composite import BCGeneric, BoundaryConditions, DirectSum
 from .orthogonal import MeshKind, OrthogonalSpace, as_device
 
 
@@ -157,24 +162,212 @@ class TensorProductSpace:
                                   self.name + "o", self.precision)
 
 
+def _copy_space(s, system):
+    """Copy of a 1D space on ``system``; device operators are shared, not copied."""
+    if isinstance(s, DirectSum):
+        return s.with_system(system)
+    c = copy.copy(s)
+    c._ops = s._ops
+    c.system = system
+    if hasattr(c, "orthogonal") and c.orthogonal is not s:
+        c.orthogonal = copy.copy(s.orthogonal)
+        c.orthogonal._ops = s.orthogonal._ops
+        c.orthogonal.system = c.system
+    elif getattr(s, "orthogonal", None) is s:
+        c.orthogonal = c
+    return c
+
+
 def TensorProduct(*basespaces, system: CartCoordSys | None = None, name: str = "T",
-                  precision: str = "f64") -> TensorProductSpace:
+                  precision: str = "f64"):
     """Factory (tensorproductspace.py:507-557): copies the 1D spaces and
-    assigns each the sub-system of its axis."""
+    assigns each the sub-system of its axis; a product with a DirectSum
+    factor is a :class:`DirectSumTPS`."""
     system = default_system(len(basespaces)) if system is None else system
-    spaces = []
-    for i, s in enumerate(basespaces):
-        c = copy.copy(s)
-        c._ops = s._ops  # device operators are shared, not copied
-        c.system = system.sub_system(i)
-        if hasattr(c, "orthogonal") and c.orthogonal is not s:
-            c.orthogonal = copy.copy(s.orthogonal)
-            c.orthogonal._ops = s.orthogonal._ops
-            c.orthogonal.system = c.system
-        elif getattr(s, "orthogonal", None) is s:
-            c.orthogonal = c
-        spaces.append(c)
+    spaces = [_copy_space(s, system.sub_system(i)) for i, s in enumerate(basespaces)]
+    if any(isinstance(s, DirectSum) for s in spaces):
+        return DirectSumTPS(spaces, system, name, precision)
     return TensorProductSpace(spaces, system, name, precision)
 
 
-__all__ = ["TensorProductSpace", "TensorProduct", "torch"]
+# ----------------------------------------------------------------------------- lifting
+def _project_1d(expr, V) -> torch.Tensor:
+    """Coefficients of a SymPy expression of V's coordinate: V.forward of its
+    values on V's quadrature mesh (project1D, galerkin/inner.py:1027-1047)."""
+    e = sp.sympify(expr)
+    pts = V.mesh().cpu().numpy()
+    if e.free_symbols:
+        vals = lambdify(V.system.base_scalars(), e)(pts)
+    else:
+        vals = float(e)
+    vals = np.broadcast_to(np.asarray(vals, dtype=np.float64), pts.shape).copy()
+    return V.forward(torch.tensor(vals, dtype=torch.float64, device=K.device()))
+
+
+def _lifting_bcs(bcspace: BCGeneric, values) -> BCGeneric:
+    """Copy of a lifting space with new ordered Dirichlet values."""
+    b = copy.copy(bcspace)
+    names = bcspace.bcs.orderednames()
+    bcs = {"left": {}, "right": {}}
+    for nm, v in zip(names, values):
+        bcs["left" if nm[0] == "L" else "right"][nm[1:]] = v
+    b.bcs = BoundaryConditions(bcs)
+    return b
+
+
+class DirectSumTPS(TensorProductSpace):
+    """2D tensor product with one or two DirectSum factors
+    (galerkin/tensorproductspace.py:560-851).
+
+    The trial space splits into the tensor products of every combination of
+    homogeneous and lifting summands (``tpspaces``).  Each product with a
+    lifting factor carries a fixed coefficient array (``bndvals``): the
+    boundary functions of the other coordinate projected onto the other
+    axis's space; with two DirectSums the boundary functions are themselves
+    lifted by their corner values and the product of both lifting spaces
+    holds the four corner values.  Their sum, padded into the orthogonal
+    basis, is a constant lift (``lift_coefficients``) that ``to_orthogonal``
+    adds and ``from_orthogonal`` removes; ``backward``/``evaluate_mesh``/
+    ``backward_primitive`` add its values on the requested mesh (cached).
+    Coefficient arrays hold the homogeneous part only and may carry leading
+    batch dimensions.
+    """
+
+    def __init__(self, basespaces, system: CartCoordSys | None = None, name: str = "DSTPS",
+                 precision: str = "f64"):
+        super().__init__(basespaces, system, name, precision)
+        if precision != "f64":
+            raise ValueError("inhomogeneous (lifted) tensor-product spaces transform in f64 only")
+        if len(self.basespaces) != 2:
+            raise NotImplementedError("DirectSumTPS is built for 2D tensor products")
+        self._ops: dict = {}
+        ds = [i for i, s in enumerate(self.basespaces) if isinstance(s, DirectSum)]
+        two = len(ds) == 2
+        # boundary functions of each DirectSum evaluated at the other axis's ends
+        projected: dict[int, list] = {}
+        if two:
+            for ax in (0, 1):
+                other = self.basespaces[1 - ax]
+                s_other = other.system.base_scalars()[0]
+                ends = (other.domain.lower, other.domain.upper)
+                rows = []
+                for g in self.basespaces[ax][1].bcs.orderedvals():
+                    g = sp.sympify(g)
+                    rows.append([_num_or_expr(g.subs(s_other, z)) for z in ends])
+                projected[ax] = rows
+        self.tpspaces: dict[tuple, TensorProductSpace] = {}
+        choices = [list(s.basespaces) if isinstance(s, DirectSum) else [s] for s in self.basespaces]
+        for i, combo in enumerate(itertools.product(*choices)):
+            self.tpspaces[combo] = TensorProductSpace(combo, self.system, f"{name}{i}", precision)
+        self.bndvals: dict[tuple, torch.Tensor] = {}
+        for combo, tps in self.tpspaces.items():
+            bc_axes = [a for a, s in enumerate(combo) if isinstance(s, BCGeneric)]
+            if not bc_axes:
+                continue
+            if len(bc_axes) == 2:
+                # corner values u(x_j, y_k) = g_j(y_k)
+                arr = torch.tensor(np.array(projected[0], dtype=np.float64), device=K.device())
+            else:
+                a = bc_axes[0]
+                o = 1 - a
+                other = combo[o]
+                rows = []
+                for j, g in enumerate(combo[a].bcs.orderedvals()):
+                    V = other
+                    if two:
+                        V = DirectSum(other, _lifting_bcs(self.basespaces[o][1], projected[a][j]))
+                    rows.append(_project_1d(g, V))
+                arr = torch.stack(rows)
+                if a == 1:
+                    arr = arr.T
+            arr = arr.contiguous()
+            self.bndvals[combo] = arr
+            tps.bnd_coefficients = arr
+        self.orthogonal = self.get_orthogonal()
+
+    # ------------------------------------------------------------ parts
+    def get_homogeneous(self) -> TensorProductSpace:
+        key = tuple(s[0] if isinstance(s, DirectSum) else s for s in self.basespaces)
+        return self.tpspaces[key]
+
+    def get_orthogonal(self) -> TensorProductSpace:
+        return TensorProductSpace([s.get_orthogonal() for s in self.basespaces], self.system,
+                                  self.name + "o", self.precision)
+
+    @property
+    def dim(self) -> int:
+        return self.get_homogeneous().dim
+
+    @property
+    def num_dofs(self) -> tuple[int, ...]:
+        return self.get_homogeneous().num_dofs
+
+    # ------------------------------------------------------------ lift
+    def lift_coefficients(self) -> torch.Tensor:
+        """(N0, N1): sum over the lifting products of their orthogonal
+        coefficients, zero-padded (tensorproductspace.py:806-821)."""
+        if "lift" not in self._ops:
+            shape = tuple(s.N for s in self.basespaces)
+            tot = torch.zeros(shape, dtype=torch.float64, device=K.device())
+            for combo, arr in self.bndvals.items():
+                a = self.tpspaces[combo].to_orthogonal(arr)
+                tot[: a.shape[0], : a.shape[1]] += a
+            self._ops["lift"] = tot.contiguous()
+        return self._ops["lift"]
+
+    def lift_values(self, N=None, k=(0, 0), kind: str = MeshKind.QUADRATURE) -> torch.Tensor:
+        N = tuple(s.num_quad_points if N is None or N[i] is None else N[i]
+                  for i, s in enumerate(self.basespaces))
+        key = ("liftv", N, tuple(k), kind)
+        if key not in self._ops:
+            ops = [s.orthogonal.backward_operator(N[i], k[i], kind) for i, s in enumerate(self.basespaces)]
+            X = self.lift_coefficients()
+            for ax, op in enumerate(ops):
+                X = K.apply_along(op, X.contiguous(), ax, "f64")
+            self._ops[key] = X.contiguous()
+        return self._ops[key]
+
+    @staticmethod
+    def _f64(precision) -> None:
+        if precision not in (None, "f64"):
+            raise ValueError("inhomogeneous (lifted) tensor-product spaces transform in f64 only")
+
+    # ------------------------------------------------------------ transforms
+    def to_orthogonal(self, c):
+        return K.add_bcast(self.get_homogeneous().to_orthogonal(c).contiguous(), self.lift_coefficients())
+
+    def from_orthogonal(self, a):
+        a = as_device(a)
+        return self.get_homogeneous().from_orthogonal(K.add_bcast(a, self.lift_coefficients(), -1.0))
+
+    def backward(self, c, N=None, precision: str | None = None):
+        """orthogonal.backward(to_orthogonal(c)) (tensorproductspace.py:771-776),
+        as the homogeneous backward plus the lift's values on the mesh."""
+        self._f64(precision)
+        u = self.get_homogeneous().backward(c, N).contiguous()
+        return K.add_bcast(u, self.lift_values(tuple(u.shape[-2:])))
+
+    def backward_primitive(self, c, k, N=None, precision: str | None = None):
+        self._f64(precision)
+        u = self.get_homogeneous().backward_primitive(c, k, N).contiguous()
+        return K.add_bcast(u, self.lift_values(tuple(u.shape[-2:]), k))
+
+    def evaluate_mesh(self, c, kind: str = MeshKind.QUADRATURE, N=None):
+        u = self.get_homogeneous().evaluate_mesh(c, kind, N).contiguous()
+        return K.add_bcast(u, self.lift_values(tuple(u.shape[-2:]), (0, 0), kind))
+
+    def forward(self, u, precision: str | None = None):
+        """from_orthogonal(orthogonal.forward(u)) (tensorproductspace.py:778-780)."""
+        self._f64(precision)
+        return self.from_orthogonal(self.orthogonal.forward(u))
+
+    def scalar_product(self, u, precision: str | None = None):
+        raise RuntimeError("Scalar product requires homogeneous test space (call on get_homogeneous())")
+
+
+def _num_or_expr(e):
+    e = sp.sympify(e)
+    return float(e) if not e.free_symbols else e
+
+
+__all__ = ["TensorProductSpace", "TensorProduct", "DirectSumTPS", "torch"]

@@ -134,6 +134,20 @@ def div_bcast(X: torch.Tensor, D: torch.Tensor, out: torch.Tensor | None = None)
     return out
 
 
+def add_bcast(X: torch.Tensor, D: torch.Tensor, alpha: float = 1.0,
+              out: torch.Tensor | None = None) -> torch.Tensor:
+    """``out[b, ...] = X[b, ...] + alpha * D[...]`` (FP64) with D broadcast over leading dims."""
+    _req(X, torch.float64, "X")
+    _req(D, torch.float64, "D")
+    if tuple(X.shape[X.ndim - D.ndim:]) != tuple(D.shape):
+        raise ValueError(f"add_bcast: trailing shape {tuple(X.shape)} does not end with {tuple(D.shape)}")
+    out = torch.empty_like(X) if out is None else out
+    if X.numel() == 0:
+        return out
+    _lib.call("jfx_axpy_bcast_f64", _p(X), _p(D), float(alpha), _p(out), X.numel(), D.numel(), _s())
+    return out
+
+
 # ---------------------------------------------------------------- banded LU
 
 

@@ -26,6 +26,13 @@ pinned by checks that do not retype the formula (all in ``tests/test_oracle.py``
 * Variable-coefficient operator / PCG / MLP: dense assembled
   operator vs the matrix-free one; PCG vs direct solve; MLP gradients vs
   central finite differences.
+* Inhomogeneous Dirichlet lifting: the lifting functions' end values by
+  ``numpy.polynomial`` evaluation; the boundary values the reference's own
+  tests pin (``tests/galerkin/test_composite.py:66-80``,
+  ``test_additional_coverage.py:101-114``); exact recovery of polynomial
+  manufactured solutions (1D and 2D, one or two lifted axes, domains other
+  than [-1, 1]); the accuracy bound of the reference's
+  ``examples/helmholtz2D.py`` for exp(-(x^2+y^2)) on (0, 1)^2.
 """
 
 from __future__ import annotations
@@ -681,6 +688,243 @@ def mlp_backward(params, hs, dY):
         grads[li] = (hs[li].T @ g, g.sum(axis=0))
         g = g @ W.T
     return grads
+
+
+# --------------------------------------------------------------------------
+# Inhomogeneous Dirichlet lifting (DirectSum / DirectSumTPS)
+# --------------------------------------------------------------------------
+
+
+def lift_basis(family: str) -> np.ndarray:
+    """(2, 2) lifting basis of ``{left: D, right: D}`` in the orthogonal basis.
+
+    Restates ``get_bc_basis`` (galerkin/composite.py:835-895): with the
+    boundary matrix ``A[r, j] = psi_j(x_r)`` (x_r = -1 left, +1 right, j < 2)
+    the rows of ``(A^-1)^T`` are the coefficients of the two functions that
+    take the value 1 at one end and 0 at the other.
+    """
+    A = vandermonde(family, np.array([-1.0, 1.0]), 2)
+    return np.linalg.inv(A).T
+
+
+class Axis1D:
+    """One axis on a true domain (a, b): ``kind`` "orth" (orthogonal basis),
+    "dir" (homogeneous Dirichlet composite with row scaling, galerkin/composite.py:121-408)
+    or "lift" (DirectSum of that composite and the lifting basis with numeric
+    boundary values ``bvals``, galerkin/composite.py:490-638).
+    Coefficient and value arrays run along their LAST axis.
+    """
+
+    def __init__(self, family: str, N: int, kind: str = "dir", domain=(-1.0, 1.0), scaling=None,
+                 bvals=None):
+        self.family, self.N, self.kind = family, N, kind
+        self.a, self.b = float(domain[0]), float(domain[1])
+        self.df = 2.0 / (self.b - self.a)
+        self.S = np.eye(N) if kind == "orth" else dirichlet_stencil(N, scaling)
+        self.dim = self.S.shape[0]
+        self.h = norm_squared(family, N)
+        self.Sb = lift_basis(family)
+        self.bvals = None if bvals is None else np.asarray(bvals, dtype=np.float64)
+
+    def ref(self, x):
+        return -1.0 + (np.asarray(x, dtype=np.float64) - self.a) * self.df
+
+    def mesh(self, Nq: int | None = None) -> np.ndarray:
+        x, _ = quad(self.family, self.N if Nq is None else Nq)
+        return self.a + (x + 1.0) / self.df
+
+    def lift(self) -> np.ndarray:
+        """Padded orthogonal coefficients of the lift (galerkin/composite.py:588-593)."""
+        out = np.zeros(self.N)
+        if self.kind == "lift":
+            out[:2] = self.bvals @ self.Sb
+        return out
+
+    def to_orthogonal(self, c):
+        """``c S + lift`` (galerkin/composite.py:276-279, 588-593)."""
+        return np.asarray(c) @ self.S + self.lift()
+
+    def from_orthogonal(self, a):
+        """``(S S^T)^-1 S (a - lift)`` (galerkin/composite.py:281-284, 595-601)."""
+        r = (np.asarray(a) - self.lift()) @ self.S.T
+        return np.linalg.solve(self.S @ self.S.T, r.reshape(-1, self.dim).T).T.reshape(r.shape)
+
+    def basis(self, x, k: int = 0) -> np.ndarray:
+        """(P, N) k-th derivative of the orthogonal basis at true points x."""
+        return vandermonde(self.family, self.ref(x), self.N, k) * self.df ** k
+
+    def values(self, c, x, k: int = 0):
+        """Series (k-th derivative) at true points x along the last axis."""
+        return self.to_orthogonal(c) @ self.basis(x, k).T
+
+    def orth_forward(self, u):
+        """Orthogonal projection ``(V^T w u) / h`` of values at the Gauss mesh
+        (galerkin/orthogonal.py:256-277; the domain factor cancels)."""
+        u = np.asarray(u, dtype=np.float64)
+        x, w = quad(self.family, u.shape[-1])
+        return (u * w) @ vandermonde(self.family, x, self.N) / self.h
+
+    def forward(self, u):
+        """orth: projection; dir: mass solve (galerkin/composite.py:339-343);
+        lift: from_orthogonal(projection) (galerkin/composite.py:628-631)."""
+        if self.kind == "orth":
+            return self.orth_forward(u)
+        if self.kind == "lift":
+            return self.from_orthogonal(self.orth_forward(u))
+        u = np.asarray(u, dtype=np.float64)
+        x, w = quad(self.family, u.shape[-1])
+        L = (u * w) @ vandermonde(self.family, x, self.N) @ self.S.T
+        M = self.S @ np.diag(self.h) @ self.S.T
+        return np.linalg.solve(M, L.reshape(-1, self.dim).T).T.reshape(L.shape)
+
+    def project(self, g):
+        """project1D (galerkin/inner.py:1027-1047): forward of g on the Gauss mesh."""
+        xm = self.mesh()
+        vals = g(xm) if callable(g) else np.full(xm.shape, float(g))
+        return self.forward(np.broadcast_to(vals, xm.shape).astype(np.float64))
+
+    def with_bvals(self, bvals) -> "Axis1D":
+        out = Axis1D.__new__(Axis1D)
+        out.__dict__.update(self.__dict__)
+        out.kind = "lift"
+        out.bvals = np.asarray(bvals, dtype=np.float64)
+        return out
+
+    def trial_basis(self, part: str) -> np.ndarray:
+        """Orthogonal coefficients (rows) of the homogeneous ("h") or lifting ("b") functions."""
+        if part == "h":
+            return self.S
+        B = np.zeros((2, self.N))
+        B[:, :2] = self.Sb
+        return B
+
+
+class Lifted2D:
+    """DirectSumTPS restated for two axes (galerkin/tensorproductspace.py:560-851).
+
+    ``bcs0``/``bcs1`` are the (left, right) boundary functions of the lifted
+    axes: callables of the OTHER coordinate, or numbers.  ``bndvals`` maps
+    (part0, part1), part in {"h", "b"}, to the fixed coefficient arrays of the
+    products with a lifting factor: boundary functions projected onto the
+    other axis (itself lifted by the corner values when both axes are lifted),
+    and the corner values for the product of both lifting bases.
+    """
+
+    def __init__(self, ax0: Axis1D, ax1: Axis1D, bcs0=None, bcs1=None):
+        self.axes = (ax0, ax1)
+        self.bcs = (bcs0, bcs1)
+        lifted = [ax.kind == "lift" for ax in self.axes]
+        two = all(lifted)
+
+        def at(g, z):
+            return float(g(np.array([z]))[0]) if callable(g) else float(g)
+
+        corners = {}
+        if two:
+            for a in (0, 1):
+                o = self.axes[1 - a]
+                corners[a] = [[at(g, o.a), at(g, o.b)] for g in self.bcs[a]]
+        self.bndvals = {}
+        parts = [("h", "b") if lf else ("h",) for lf in lifted]
+        for p0 in parts[0]:
+            for p1 in parts[1]:
+                if (p0, p1) == ("h", "h"):
+                    continue
+                if (p0, p1) == ("b", "b"):
+                    self.bndvals[(p0, p1)] = np.array(corners[0])
+                    continue
+                a = 0 if p0 == "b" else 1
+                o = self.axes[1 - a]
+                rows = []
+                for j, g in enumerate(self.bcs[a]):
+                    other = o.with_bvals(corners[a][j]) if two else _homogeneous(o)
+                    rows.append(other.project(g))
+                arr = np.array(rows)
+                self.bndvals[(p0, p1)] = arr if a == 0 else arr.T
+
+    def lift(self) -> np.ndarray:
+        """Sum of the lifting products in the orthogonal basis (tensorproductspace.py:806-821)."""
+        out = np.zeros((self.axes[0].N, self.axes[1].N))
+        for (p0, p1), arr in self.bndvals.items():
+            out += self.axes[0].trial_basis(p0).T @ arr @ self.axes[1].trial_basis(p1)
+        return out
+
+    def to_orthogonal(self, c):
+        return self.axes[0].S.T @ c @ self.axes[1].S + self.lift()
+
+    def from_orthogonal(self, a):
+        r = a - self.lift()
+        S0, S1 = self.axes[0].S, self.axes[1].S
+        return np.linalg.solve(S0 @ S0.T, S0 @ r @ S1.T @ np.linalg.inv(S1 @ S1.T))
+
+    def values(self, c, x, y, k=(0, 0)):
+        """u^(k) on the grid x (axis 0) times y (axis 1) of true points."""
+        return self.axes[0].basis(x, k[0]) @ self.to_orthogonal(c) @ self.axes[1].basis(y, k[1]).T
+
+    def forward(self, u):
+        """from_orthogonal of the orthogonal projection (tensorproductspace.py:778-780)."""
+        a = self.axes[1].orth_forward(self.axes[0].orth_forward(u.T).T)
+        return self.from_orthogonal(a)
+
+
+def _homogeneous(ax: Axis1D) -> Axis1D:
+    out = Axis1D.__new__(Axis1D)
+    out.__dict__.update(ax.__dict__)
+    if ax.kind == "lift":
+        out.kind = "dir"
+    return out
+
+
+def _axis_matrices(ax: Axis1D, part: str, Nq: int | None = None):
+    """Mass and (v, u'') matrices on the true domain between the homogeneous
+    test functions and the trial functions of ``part``, by Gauss quadrature
+    (the quadrature branch of inner_bilinear, galerkin/inner.py:887-909)."""
+    x, w = quad(ax.family, ax.N if Nq is None else Nq)
+    V = vandermonde(ax.family, x, ax.N)
+    V2 = vandermonde(ax.family, x, ax.N, 2)
+    T = ax.S @ (V.T * w[None, :])
+    B = ax.trial_basis(part)
+    M = T @ V @ B.T / ax.df
+    K = T @ V2 @ B.T * ax.df
+    return M, K
+
+
+def lifted_helmholtz_1d(ax: Axis1D, alpha: float, f, Nq: int | None = None):
+    """Solve ``(v, u'' + alpha u) = (v, f)`` for u = u_h + lift on one lifted
+    axis: the lifting term moves to the right-hand side with the sign of a
+    linear form (galerkin/inner.py:362-386).  Returns homogeneous coefficients."""
+    x, w = quad(ax.family, ax.N if Nq is None else Nq)
+    fq = f(ax.a + (x + 1.0) / ax.df) if callable(f) else np.full(x.shape, float(f))
+    b = ax.S @ (vandermonde(ax.family, x, ax.N).T @ (w * fq)) / ax.df
+    Mh, Kh = _axis_matrices(ax, "h", Nq)
+    A = Kh + alpha * Mh
+    if ax.kind == "lift":
+        Mb, Kb = _axis_matrices(ax, "b", Nq)
+        b = b - (Kb + alpha * Mb) @ ax.bvals
+    return np.linalg.solve(A, b)
+
+
+def lifted_helmholtz_2d(L: Lifted2D, alpha: float, fgrid: np.ndarray, Nq=None):
+    """Solve ``(v, Lap u + alpha u) = (v, f)`` for u = u_h + lift on a
+    DirectSumTPS (galerkin/inner.py:493-562: every product with a lifting
+    factor is applied to its ``bndvals`` and subtracted from b).  ``fgrid``
+    holds f on the Gauss mesh of both axes.  Returns (coefficients, A, b)."""
+    ax0, ax1 = L.axes
+    Nq = (ax0.N, ax1.N) if Nq is None else Nq
+    x0, w0 = quad(ax0.family, Nq[0])
+    x1, w1 = quad(ax1.family, Nq[1])
+    T0 = ax0.S @ (vandermonde(ax0.family, x0, ax0.N).T * w0[None, :]) / ax0.df
+    T1 = ax1.S @ (vandermonde(ax1.family, x1, ax1.N).T * w1[None, :]) / ax1.df
+    b = T0 @ fgrid @ T1.T
+    M0, K0 = _axis_matrices(ax0, "h", Nq[0])
+    M1, K1 = _axis_matrices(ax1, "h", Nq[1])
+    A = np.kron(K0, M1) + np.kron(M0, K1) + alpha * np.kron(M0, M1)
+    for (p0, p1), arr in L.bndvals.items():
+        m0, k0 = _axis_matrices(ax0, p0, Nq[0])
+        m1, k1 = _axis_matrices(ax1, p1, Nq[1])
+        b = b - (k0 @ arr @ m1.T + m0 @ arr @ k1.T + alpha * m0 @ arr @ m1.T)
+    u = np.linalg.solve(A, b.ravel()).reshape(ax0.dim, ax1.dim)
+    return u, A, b
 
 
 def ars443_stage_coeffs():

@@ -303,3 +303,120 @@ def test_mlp_gradients_finite_difference():
             ym = np.sum(dY * O.mlp_forward(params, X)[0])
             arr[idx] = old
             assert abs((yp - ym) / (2 * eps) - gd[idx]) < 1e-6
+
+
+# ---------------------------------------------------------------- inhomogeneous Dirichlet lifting
+
+
+@pytest.mark.parametrize("family", ["legendre", "chebyshev"])
+def test_lift_basis_end_values(family):
+    # each lifting function is 1 at its own end and 0 at the other
+    val = FNS[family][0]
+    S = O.lift_basis(family)
+    ends = np.array([-1.0, 1.0])
+    for r in range(2):
+        assert np.allclose(val(ends, S[r]), np.eye(2)[r], atol=1e-15)
+    assert np.allclose(S, [[0.5, -0.5], [0.5, 0.5]], atol=1e-15)
+
+
+@pytest.mark.parametrize("family", ["legendre", "chebyshev"])
+def test_directsum_boundary_values(family):
+    # reference tests/galerkin/test_composite.py:66-80 and
+    # test_additional_coverage.py:101-114: N = 6, bcs (1, 2), zero coefficients
+    ax = O.Axis1D(family, 6, "lift", bvals=[1.0, 2.0])
+    val = FNS[family][0]
+    assert np.allclose(val(np.array([-1.0, 1.0]), ax.to_orthogonal(np.zeros(4))), [1.0, 2.0], atol=1e-15)
+    # any homogeneous part vanishes at the ends
+    c = np.random.default_rng(0).standard_normal(4)
+    assert np.allclose(val(np.array([-1.0, 1.0]), ax.to_orthogonal(c)), [1.0, 2.0], atol=1e-14)
+
+
+@pytest.mark.parametrize("family", ["legendre", "chebyshev"])
+def test_lifted_helmholtz_1d_polynomial_exact(family):
+    # u = 1 + 2x + x^2 - x^3 on (0, 2) lies in the discrete space: the Galerkin
+    # solution reproduces it to rounding
+    a, b, alpha = 0.0, 2.0, 3.0
+    N = 12
+    ue = np.polynomial.Polynomial([1.0, 2.0, 1.0, -1.0])
+    ax = O.Axis1D(family, N, "lift", (a, b), scaling=np.arange(N - 2) + 1.0, bvals=[ue(a), ue(b)])
+    c = O.lifted_helmholtz_1d(ax, alpha, lambda x: ue.deriv(2)(x) + alpha * ue(x))
+    xs = np.linspace(a, b, 11)
+    got = FNS[family][0](-1.0 + (xs - a) * 2.0 / (b - a), ax.to_orthogonal(c))
+    assert np.abs(got - ue(xs)).max() < 1e-12
+    # forward of the values on the Gauss mesh returns the coefficients
+    assert np.abs(ax.forward(ax.values(c, ax.mesh())) - c).max() < 1e-12
+
+
+def _poly2d():
+    x, y = sp.symbols("x y")
+    ue = 1 + x * y + x**2 * y - 2 * y**3 + x**3
+    lap = sp.diff(ue, x, 2) + sp.diff(ue, y, 2)
+    return x, y, ue, lap
+
+
+@pytest.mark.parametrize("family", ["legendre", "chebyshev"])
+@pytest.mark.parametrize("kinds", [("lift", "lift"), ("lift", "dir"), ("dir", "lift")])
+def test_lifted_helmholtz_2d_polynomial_exact(family, kinds):
+    x, y, ue, lap = _poly2d()
+    dom = ((-1.0, 2.0), (0.0, 1.0))
+    alpha = 2.0
+    f_ue = sp.lambdify((x, y), ue)
+    f_rhs = sp.lambdify((x, y), lap + alpha * ue)
+    if kinds[1] == "dir":
+        # homogeneous second axis: multiply by y (1 - y) so the solution vanishes there
+        ue = ue * y * (1 - y)
+        f_ue = sp.lambdify((x, y), ue)
+        f_rhs = sp.lambdify((x, y), sp.diff(ue, x, 2) + sp.diff(ue, y, 2) + alpha * ue)
+    if kinds[0] == "dir":
+        ue = ue * (x + 1) * (2 - x)
+        f_ue = sp.lambdify((x, y), ue)
+        f_rhs = sp.lambdify((x, y), sp.diff(ue, x, 2) + sp.diff(ue, y, 2) + alpha * ue)
+    ax0 = O.Axis1D(family, 14, kinds[0], dom[0])
+    ax1 = O.Axis1D(family, 12, kinds[1], dom[1])
+    bcs0 = (lambda s: f_ue(dom[0][0], s), lambda s: f_ue(dom[0][1], s)) if kinds[0] == "lift" else None
+    bcs1 = (lambda s: f_ue(s, dom[1][0]), lambda s: f_ue(s, dom[1][1])) if kinds[1] == "lift" else None
+    Lf = O.Lifted2D(ax0, ax1, bcs0, bcs1)
+    X, Y = ax0.mesh(), ax1.mesh()
+    F = f_rhs(X[:, None], Y[None, :]) + 0.0 * X[:, None]
+    u, _, _ = O.lifted_helmholtz_2d(Lf, alpha, F)
+    xs, ys = np.linspace(*dom[0], 9), np.linspace(*dom[1], 8)
+    assert np.abs(Lf.values(u, xs, ys) - f_ue(xs[:, None], ys[None, :])).max() < 1e-11
+
+
+@pytest.mark.parametrize("family", ["legendre", "chebyshev"])
+def test_lifted_2d_forward_backward(family):
+    x, y, ue, _ = _poly2d()
+    f_ue = sp.lambdify((x, y), ue)
+    for kinds in (("lift", "lift"), ("orth", "lift"), ("lift", "dir")):
+        ax0 = O.Axis1D(family, 10, kinds[0], (-2.0, 2.0))
+        ax1 = O.Axis1D(family, 9, kinds[1], (0.0, 1.0))
+        bcs0 = (lambda s: f_ue(-2.0, s), lambda s: f_ue(2.0, s)) if kinds[0] == "lift" else None
+        bcs1 = (lambda s: f_ue(s, 0.0), lambda s: f_ue(s, 1.0)) if kinds[1] == "lift" else None
+        Lf = O.Lifted2D(ax0, ax1, bcs0, bcs1)
+        c = np.random.default_rng(2).standard_normal((ax0.dim, ax1.dim))
+        vals = Lf.values(c, ax0.mesh(), ax1.mesh())
+        assert np.abs(Lf.forward(vals) - c).max() < 1e-12
+        # the lifted function takes the (projected) boundary data on the edges
+        if kinds[1] == "lift":
+            edge = Lf.values(c, ax0.mesh(), np.array([0.0]))[:, 0]
+            assert np.abs(edge - f_ue(ax0.mesh(), 0.0)).max() < 1e-12
+
+
+def test_helmholtz2d_example_accuracy():
+    # reference examples/helmholtz2D.py: ue = exp(-(x^2+y^2)) on (0, 1)^2,
+    # Legendre N = 50, scaling n + 1; error / 100 < ulp(1000) on a 100^2 mesh
+    x, y = sp.symbols("x y")
+    ue = sp.exp(-(x**2 + y**2))
+    f_ue = sp.lambdify((x, y), ue)
+    f_rhs = sp.lambdify((x, y), sp.diff(ue, x, 2) + sp.diff(ue, y, 2) + ue)
+    N = 50
+    sc = np.arange(N - 2) + 1.0
+    ax0 = O.Axis1D("legendre", N, "lift", (0.0, 1.0), scaling=sc)
+    ax1 = O.Axis1D("legendre", N, "lift", (0.0, 1.0), scaling=sc)
+    Lf = O.Lifted2D(ax0, ax1, (lambda s: f_ue(0.0, s), lambda s: f_ue(1.0, s)),
+                    (lambda s: f_ue(s, 0.0), lambda s: f_ue(s, 1.0)))
+    X, Y = ax0.mesh(), ax1.mesh()
+    u, _, _ = O.lifted_helmholtz_2d(Lf, 1.0, f_rhs(X[:, None], Y[None, :]))
+    xu = np.linspace(0.0, 1.0, 100)
+    err = np.linalg.norm(Lf.values(u, xu, xu) - f_ue(xu[:, None], xu[None, :])) / 100
+    assert err < np.spacing(1000.0), err
