@@ -41,6 +41,17 @@ int jfx_gauss_nodes(int family, int N, double* x, double* w, hipStream_t stream)
 int jfx_vandermonde_f64(int family, const double* x, int Q, int N, int k, double scale, double* V,
                         hipStream_t stream);
 
+/* Fourier basis (complex, re/im interleaved): element (q, j) of
+ * (i k_j s)^d exp(i k_j x_q) * post for the N-mode space (FFT wavenumber
+ * order; Nyquist mode zeroed for odd d and even N) at x_q = 2 pi q / Q
+ * (grid != 0, exact phase reduction) or x[q]; out (Q, N), or its conjugate
+ * transpose (N, Q) when adjoint != 0.  Replaces
+ * Fourier.eval_basis_functions / evaluate_basis_derivative and the FFTs of
+ * Fourier.backward / forward / scalar_product (galerkin/Fourier.py:92-180),
+ * which become contractions with these matrices. */
+int jfx_fourier_basis(const double* x, int Q, int N, int d, double s, int grid, int adjoint, double post,
+                      double* out, hipStream_t stream);
+
 /* out[c, r] = V[r, c] * rowscale[r] * colscale[c] * s for an (R, C) matrix
  * (either scale pointer may be NULL).  Builds the quadrature-weighted
  * transposed Vandermonde (P^T * w) of OrthogonalSpace.scalar_product
@@ -107,6 +118,20 @@ int jfx_band_lu_factor(const double* band, int n, int p, int q, double* lu, int*
 int jfx_band_lu_solve(const double* lu, int n, int p, int q, double* X, long long P, long long Q,
                       hipStream_t stream);
 
+/* nsys independent band matrices, band (nsys, n, p+q+1) "by row"; status_dev
+ * receives 0 or 1 + the row of a zero / non-finite pivot in a failing system.
+ * Replaces the per-wavenumber DIA LU of TPMatricesWavenumberSolver
+ * (la/tpmatrix.py:735-775, vmapped _lu_banded_no_pivot_kernel). */
+int jfx_band_lu_factor_batched(const double* band, int nsys, int n, int p, int q, double* lu, int* status_dev,
+                               hipStream_t stream);
+/* In-place solve along the middle axis of X (P, n, Q) where line (pp, qq)
+ * uses factor (pp * (Q / qdiv) + qq / qdiv) mod nsys (P * Q / qdiv a multiple
+ * of nsys, so leading batch dimensions repeat the systems; qdiv = 2 for the
+ * re/im pairs of complex data).  Replaces
+ * TPMatricesWavenumberSolver.solve (la/tpmatrix.py:590-690, 800-1011). */
+int jfx_band_lu_solve_batched(const double* lu, int nsys, int n, int p, int q, double* X, long long P, long long Q,
+                              int qdiv, hipStream_t stream);
+
 /* ---- FD eigenvalue division: Y = X / D with D (size) broadcast over the leading batch: the
  * eigenvalue division of TPMatricesLUFactors.solve (la/tpmatrix.py:565-579). */
 int jfx_div_bcast_f64(const double* X, const double* D, double* Y, long long total, long long size,
@@ -128,6 +153,16 @@ int jfx_mul_bcast_f64(const double* X, const double* C, double* Y, long long tot
  * from_orthogonal (galerkin/tensorproductspace.py:806-851). */
 int jfx_axpy_bcast_f64(const double* X, const double* D, double alpha, double* Y, long long total,
                        long long size, hipStream_t stream);
+
+/* Complex result of a complex operator A = Ar + i Ai applied along an axis:
+ * Z (P, 2M, Qr) is the real contraction of the stacked operator [Ar; Ai]
+ * with the real view of X (Qr = 2Q, re/im interleaved, for complex X; Qr = Q
+ * for real X); Y (P, M, Q) complex (re/im interleaved) = Z_top + i Z_bottom.
+ * The Fourier transforms' complex exponential series, evaluated and projected
+ * by contraction (Fourier.backward / forward / scalar_product,
+ * galerkin/Fourier.py:127-180). */
+int jfx_cplx_combine(const double* Z, double* Y, long long P, long long M, long long Q, int real_input,
+                     hipStream_t stream);
 
 /* y (+)= sum_j coeffs[j] * xs[j], nin <= 8; xs and coeffs are HOST arrays of
  * device pointers / scalars.  dtype 0 = f64, 1 = f32.  The stage

@@ -8,6 +8,7 @@ from . import kernels  # noqa: F401
 from .coordinates import x, y, z  # noqa: F401
 from .galerkin import (  # noqa: F401
     Chebyshev,
+    Fourier,
     FunctionSpace,
     Legendre,
     TensorProduct,

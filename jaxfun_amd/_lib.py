@@ -43,6 +43,10 @@ SIGNATURES: dict[str, tuple] = {
     "jfx_div_bcast_f32": (c_int, [c_vp, c_vp, c_vp, c_ll, c_ll, c_vp]),
     "jfx_mul_bcast_f64": (c_int, [c_vp, c_vp, c_vp, c_ll, c_ll, c_vp]),
     "jfx_axpy_bcast_f64": (c_int, [c_vp, c_vp, c_dbl, c_vp, c_ll, c_ll, c_vp]),
+    "jfx_cplx_combine": (c_int, [c_vp, c_vp, c_ll, c_ll, c_ll, c_int, c_vp]),
+    "jfx_fourier_basis": (c_int, [c_vp, c_int, c_int, c_int, c_dbl, c_int, c_int, c_dbl, c_vp, c_vp]),
+    "jfx_band_lu_factor_batched": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp]),
+    "jfx_band_lu_solve_batched": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_vp, c_ll, c_ll, c_int, c_vp]),
     "jfx_lincomb": (c_int, [c_int, c_int, c_vp, c_vp, c_vp, c_ll, c_int, c_vp]),
     "jfx_cast": (c_int, [c_int, c_vp, c_vp, c_ll, c_vp]),
     "jfx_sum_products": (c_int, [c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_ll, c_vp]),

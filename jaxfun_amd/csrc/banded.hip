@@ -8,19 +8,15 @@
 
 namespace jfx {
 
-// Single-lane Doolittle factorisation of one small banded matrix (setup).
-__global__ void band_lu_factor_kernel(const double* __restrict__ band, int n, int p, int q,
-                                      double* __restrict__ lu, int* __restrict__ status) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Doolittle factorisation of one banded matrix by one lane; returns 0 or
+// 1 + the row of the first zero / non-finite pivot.
+__device__ int band_lu_factor_one(const double* __restrict__ band, int n, int p, int q,
+                                  double* __restrict__ lu) {
   const int W = p + q + 1;
   for (long long i = 0; i < (long long)n * W; ++i) lu[i] = band[i];
-  int st = 0;
-  for (int k = 0; k < n - 1 && st == 0; ++k) {
+  for (int k = 0; k < n - 1; ++k) {
     double piv = lu[(long long)k * W + p];
-    if (piv == 0.0 || !isfinite(piv)) {
-      st = k + 1;
-      break;
-    }
+    if (piv == 0.0 || !isfinite(piv)) return k + 1;
     int iend = min(n - 1, k + p);
     for (int i = k + 1; i <= iend; ++i) {
       double& lik = lu[(long long)i * W + (k - i + p)];
@@ -30,21 +26,31 @@ __global__ void band_lu_factor_kernel(const double* __restrict__ band, int n, in
         lu[(long long)i * W + (j - i + p)] -= lik * lu[(long long)k * W + (j - k + p)];
     }
   }
-  if (st == 0) {
-    double piv = lu[(long long)(n - 1) * W + p];
-    if (piv == 0.0 || !isfinite(piv)) st = n;
-  }
-  *status = st;
+  double piv = lu[(long long)(n - 1) * W + p];
+  return (piv == 0.0 || !isfinite(piv)) ? n : 0;
 }
 
-// Solve L U x = b in place along axis of length n of a (P, n, Q) batch; one
-// thread per line (coalesced over q).
-__global__ void band_lu_solve_kernel(const double* __restrict__ lu, int n, int p, int q,
-                                     double* __restrict__ X, long long P, long long Q) {
-  long long line = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (line >= P * Q) return;
-  long long pp = line / Q, qq = line - pp * Q;
-  double* x = X + pp * (long long)n * Q + qq;
+// One small banded matrix (setup).
+__global__ void band_lu_factor_kernel(const double* __restrict__ band, int n, int p, int q,
+                                      double* __restrict__ lu, int* __restrict__ status) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  *status = band_lu_factor_one(band, n, p, q, lu);
+}
+
+// nsys independent banded matrices (band (nsys, n, W)), one lane each; the
+// first failure's 1 + row lands in *status.
+__global__ void band_lu_factor_batched_kernel(const double* __restrict__ band, int nsys, int n, int p, int q,
+                                              double* __restrict__ lu, int* __restrict__ status) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsys) return;
+  const long long off = (long long)s * n * (p + q + 1);
+  const int st = band_lu_factor_one(band + off, n, p, q, lu + off);
+  if (st) atomicCAS(status, 0, st);
+}
+
+// Forward / back substitution of one line x (stride Q) with one factor.
+__device__ void band_lu_solve_one(const double* __restrict__ lu, int n, int p, int q, double* __restrict__ x,
+                                  long long Q) {
   const int W = p + q + 1;
   for (int i = 1; i < n; ++i) {
     double s = x[(long long)i * Q];
@@ -58,6 +64,29 @@ __global__ void band_lu_solve_kernel(const double* __restrict__ lu, int n, int p
     for (int j = i + 1; j <= j1; ++j) s -= lu[(long long)i * W + (j - i + p)] * x[(long long)j * Q];
     x[(long long)i * Q] = s / lu[(long long)i * W + p];
   }
+}
+
+// Solve L U x = b in place along axis of length n of a (P, n, Q) batch; one
+// thread per line (coalesced over q).
+__global__ void band_lu_solve_kernel(const double* __restrict__ lu, int n, int p, int q,
+                                     double* __restrict__ X, long long P, long long Q) {
+  long long line = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (line >= P * Q) return;
+  long long pp = line / Q, qq = line - pp * Q;
+  band_lu_solve_one(lu, n, p, q, X + pp * (long long)n * Q + qq, Q);
+}
+
+// As above with one factor per line group: line (pp, qq) uses system
+// (pp * (Q / qdiv) + qq / qdiv) mod nsys (qdiv = 2 keeps the re/im pair of a
+// complex value on one system; the modulus repeats the systems over leading
+// batch dimensions).
+__global__ void band_lu_solve_batched_kernel(const double* __restrict__ lu, int nsys, int n, int p, int q,
+                                             double* __restrict__ X, long long P, long long Q, int qdiv) {
+  long long line = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (line >= P * Q) return;
+  long long pp = line / Q, qq = line - pp * Q;
+  const long long sys = (pp * (Q / qdiv) + qq / qdiv) % nsys;
+  band_lu_solve_one(lu + sys * (long long)n * (p + q + 1), n, p, q, X + pp * (long long)n * Q + qq, Q);
 }
 
 // Y[b, i] = X[b, i] / D[i]  (D broadcast over the leading batch)
@@ -83,6 +112,32 @@ __global__ void axpy_bcast_kernel(const double* __restrict__ X, const double* __
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long stride = (long long)gridDim.x * blockDim.x;
   for (; i < total; i += stride) Y[i] = X[i] + alpha * D[i % size];
+}
+
+// Complex result of a stacked real contraction.  Z = [Ar; Ai] applied along
+// an axis: Z (P, 2M, Qr) with Qr = 2Q (complex input, re/im interleaved along
+// the last axis) or Q (real input); Y (P, M, Q) complex, re/im interleaved.
+__global__ void cplx_combine_kernel(const double* __restrict__ Z, double* __restrict__ Y, long long P,
+                                    long long M, long long Q, int real_input) {
+  const long long total = P * M * Q;
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < total; i += stride) {
+    const long long q = i % Q, pm = i / Q, m = pm % M, p = pm / M;
+    double re, im;
+    if (real_input) {
+      const double* zr = Z + (p * 2 * M + m) * Q + q;
+      re = zr[0];
+      im = zr[M * Q];
+    } else {
+      const double* zr = Z + (p * 2 * M + m) * 2 * Q + 2 * q;
+      const double* zi = zr + M * 2 * Q;
+      re = zr[0] - zi[1];
+      im = zr[1] + zi[0];
+    }
+    Y[2 * i] = re;
+    Y[2 * i + 1] = im;
+  }
 }
 
 // out = sum_t c_t * A_t * B_t  (pointwise, nterms <= 4)
@@ -161,6 +216,31 @@ int jfx_band_lu_solve(const double* lu, int n, int p, int q, double* X, long lon
   return JFX_OK;
 }
 
+int jfx_band_lu_factor_batched(const double* band, int nsys, int n, int p, int q, double* lu, int* status_dev,
+                               hipStream_t stream) {
+  if (!band || !lu || !status_dev || nsys <= 0 || n <= 0 || p < 0 || q < 0)
+    return set_error("band_lu_factor_batched: bad arguments"), JFX_EARG;
+  hipLaunchKernelGGL(band_lu_factor_batched_kernel, dim3((unsigned)((nsys + 63) / 64)), dim3(64), 0, stream, band,
+                     nsys, n, p, q, lu, status_dev);
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
+}
+
+int jfx_band_lu_solve_batched(const double* lu, int nsys, int n, int p, int q, double* X, long long P, long long Q,
+                              int qdiv, hipStream_t stream) {
+  if (!lu || !X || nsys <= 0 || n <= 0 || p < 0 || q < 0 || P < 0 || Q <= 0 || qdiv < 1 || Q % qdiv ||
+      (P * (Q / qdiv)) % nsys)
+    return set_error("band_lu_solve_batched: bad arguments (%d systems for P=%lld, Q=%lld, qdiv=%d)", nsys, P, Q,
+                     qdiv),
+           JFX_EARG;
+  long long lines = P * Q;
+  if (lines == 0) return JFX_OK;
+  hipLaunchKernelGGL(band_lu_solve_batched_kernel, dim3((unsigned)((lines + 255) / 256)), dim3(256), 0, stream,
+                     lu, nsys, n, p, q, X, P, Q, qdiv);
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
+}
+
 int jfx_div_bcast_f64(const double* X, const double* D, double* Y, long long total, long long size,
                       hipStream_t stream) {
   if (!X || !D || !Y || size <= 0 || total % size) return set_error("div_bcast: bad arguments"), JFX_EARG;
@@ -193,6 +273,17 @@ int jfx_axpy_bcast_f64(const double* X, const double* D, double alpha, double* Y
   if (!X || !D || !Y || size <= 0 || total % size) return set_error("axpy_bcast: bad arguments"), JFX_EARG;
   hipLaunchKernelGGL(axpy_bcast_kernel, dim3(grid_for(total, 256)), dim3(256), 0, stream, X, D, alpha, Y,
                      total, size);
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
+}
+
+int jfx_cplx_combine(const double* Z, double* Y, long long P, long long M, long long Q, int real_input,
+                     hipStream_t stream) {
+  if (!Z || !Y || P < 0 || M < 0 || Q < 0 || (real_input != 0 && real_input != 1))
+    return set_error("cplx_combine: bad arguments"), JFX_EARG;
+  if (P * M * Q == 0) return JFX_OK;
+  hipLaunchKernelGGL(cplx_combine_kernel, dim3(grid_for(P * M * Q, 256)), dim3(256), 0, stream, Z, Y, P, M, Q,
+                     real_input);
   JFX_LAUNCH_CHECK();
   return JFX_OK;
 }

@@ -105,6 +105,53 @@ __global__ void scale_transpose_kernel(const double* __restrict__ V, int R, int 
 
 using namespace jfx;
 
+// Complex exponential basis of the N-mode Fourier space, wavenumbers in FFT
+// order (k_j = j for j < (N+1)/2, j - N otherwise; for odd derivative orders
+// of even N the Nyquist mode is zeroed).  Element (q, j):
+//     (i k_j s)^d exp(i k_j x_q) * post,
+// with x_q = 2 pi q / Q (grid != 0; the phase k_j q is reduced exactly mod Q)
+// or x_q = x[q].  adjoint == 0 writes out (Q, N); adjoint == 1 writes the
+// conjugate transpose (N, Q).  out is complex, re/im interleaved.
+__global__ void fourier_basis_kernel(const double* __restrict__ x, int Q, int N, int d, double s, int grid,
+                                     int adjoint, double post, double* __restrict__ out) {
+  const long long total = (long long)Q * N;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  int q, j;
+  if (adjoint) {
+    j = (int)(idx / Q);
+    q = (int)(idx - (long long)j * Q);
+  } else {
+    q = (int)(idx / N);
+    j = (int)(idx - (long long)q * N);
+  }
+  const int kj = j < (N + 1) / 2 ? j : j - N;
+  double kd = (double)kj;
+  if ((d & 1) && (N % 2 == 0) && j == N / 2) kd = 0.0;
+  double ang;
+  if (grid) {
+    long long r = ((long long)kj * q) % Q;
+    if (r < 0) r += Q;
+    ang = 2.0 * M_PI * (double)r / (double)Q;
+  } else {
+    ang = (double)kj * x[q];
+  }
+  double sn, cs;
+  sincos(ang, &sn, &cs);
+  double mag = post;
+  for (int e = 0; e < d; ++e) mag *= kd * s;
+  double fr = 0.0, fi = 0.0;
+  switch (d & 3) {
+    case 0: fr = mag; break;
+    case 1: fi = mag; break;
+    case 2: fr = -mag; break;
+    default: fi = -mag; break;
+  }
+  const double re = fr * cs - fi * sn, im = fr * sn + fi * cs;
+  out[2 * idx] = re;
+  out[2 * idx + 1] = adjoint ? -im : im;
+}
+
 extern "C" {
 
 int jfx_gauss_nodes(int family, int N, double* x, double* w, hipStream_t stream) {
@@ -126,6 +173,16 @@ int jfx_vandermonde_f64(int family, const double* x, int Q, int N, int k, double
     return set_error("vandermonde: bad arguments"), JFX_EARG;
   hipLaunchKernelGGL(vandermonde_kernel, dim3((Q + 127) / 128), dim3(128), 0, stream, family, x, Q, N,
                      k, scale, V);
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
+}
+
+int jfx_fourier_basis(const double* x, int Q, int N, int d, double s, int grid, int adjoint, double post,
+                      double* out, hipStream_t stream) {
+  if (!out || Q <= 0 || N <= 0 || d < 0 || d > 8 || (!grid && !x)) return set_error("fourier_basis: bad arguments"), JFX_EARG;
+  const long long total = (long long)Q * N;
+  hipLaunchKernelGGL(fourier_basis_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, x, Q, N, d, s,
+                     grid, adjoint, post, out);
   JFX_LAUNCH_CHECK();
   return JFX_OK;
 }

@@ -115,6 +115,9 @@ def bilinear_1d(v, i: int, u, j: int, coeff, Nq: int, use_precomputed: bool):
                 if Su is not None:
                     A = A @ Su.T
                 return type(zo)._combine(zo, A)
+    if getattr(v, "is_fourier", False) or getattr(u, "is_fourier", False):
+        raise NotImplementedError("Fourier-axis matrices are built from their closed forms only "
+                                  "(constant coefficients, use_precomputed_matrices=True)")
     x, w = _quad_nodes(v, Nq)
     df = float(v.orthogonal.domain_factor)
     wq = w * (df ** (i + j - 1))
@@ -132,7 +135,10 @@ def bilinear_1d(v, i: int, u, j: int, coeff, Nq: int, use_precomputed: bool):
 
 
 def test_operator(v, i: int, Nq: int) -> K.AxisOperator:
-    """(dim, Q): ``b_k = sum_q w_q df^(i-1) f(x_q) phi_k^(i)(x_q)`` (inner.py:914-984)."""
+    """(dim, Q): ``b_k = sum_q w_q df^(i-1) f(x_q) phi_k^(i)(x_q)`` (inner.py:914-984);
+    complex conjugated test functions on a Fourier axis."""
+    if getattr(v, "is_fourier", False):
+        return v.test_operator(i, Nq)
     key = ("test", Nq, i)
     if key not in v._ops:
         x, w = _quad_nodes(v, Nq)

@@ -214,11 +214,11 @@ class OrthogonalSpace:
     def evaluate(self, x, c):
         """Series values at arbitrary true-domain points x (galerkin/orthogonal.py:102-129):
         the basis evaluated at the points, contracted with the last axis of c."""
-        xs = as_device(x)
-        X = self.map_reference_domain(xs.reshape(-1)).contiguous()
+        shp = tuple(x.shape) if isinstance(x, torch.Tensor) else np.shape(x)
+        X = self.map_reference_domain(as_device(x).reshape(-1)).contiguous()
         V = self._apply_stencil_cols(K.vandermonde(self.family, X, self.N, 0))
         out = _apply_1d(K.AxisOperator(V.contiguous()), c, "f64")
-        return out.reshape(tuple(out.shape[:-1]) + tuple(xs.shape))
+        return out.reshape(tuple(out.shape[:-1]) + tuple(shp))
 
     def mass_matrix(self):
         from ..la import DiaMatrix
@@ -267,7 +267,13 @@ def _len(u) -> int:
 
 
 def as_device(a, dtype=torch.float64) -> torch.Tensor:
-    """Accept numpy arrays / torch tensors; return a contiguous device tensor."""
+    """Accept numpy arrays / torch tensors; return a contiguous device tensor.
+    Complex input stays complex128 (complex data is transformed in FP64)."""
+    if isinstance(a, torch.Tensor) and a.is_complex():
+        t = a if a.is_cuda else a.to(K.device())
+        return t.to(torch.complex128).contiguous()
+    if not isinstance(a, torch.Tensor) and np.iscomplexobj(a):
+        return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.complex128, device=K.device()).contiguous()
     if isinstance(a, torch.Tensor):
         t = a if a.is_cuda else a.to(K.device())
         if t.dtype != dtype:

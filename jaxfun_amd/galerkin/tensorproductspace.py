@@ -98,7 +98,8 @@ class TensorProductSpace:
         nplanes = 1
         for n_ in X.shape[:-2]:
             nplanes *= n_
-        if d >= 2 and self.fused and nplanes < K.num_cus() and K.plane_ok(ops[-2], ops[-1], prec):
+        if d >= 2 and self.fused and not X.is_complex() and nplanes < K.num_cus() and \
+                K.plane_ok(ops[-2], ops[-1], prec):
             # leading axes one by one, the last two in one fused pass
             for ax, op in enumerate(ops[:-2]):
                 X = K.apply_along(op, X, nb + ax, prec)
@@ -308,7 +309,9 @@ class DirectSumTPS(TensorProductSpace):
         coefficients, zero-padded (tensorproductspace.py:806-821)."""
         if "lift" not in self._ops:
             shape = tuple(s.N for s in self.basespaces)
-            tot = torch.zeros(shape, dtype=torch.float64, device=K.device())
+            cplx = any(a.is_complex() for a in self.bndvals.values()) or \
+                any(getattr(s, "is_fourier", False) for s in self.basespaces)
+            tot = torch.zeros(shape, dtype=torch.complex128 if cplx else torch.float64, device=K.device())
             for combo, arr in self.bndvals.items():
                 a = self.tpspaces[combo].to_orthogonal(arr)
                 tot[: a.shape[0], : a.shape[1]] += a

@@ -87,8 +87,15 @@ def cast(x: torch.Tensor, dtype: torch.dtype) -> torch.Tensor:
 
 
 def lincomb(terms, out: torch.Tensor | None = None, accumulate: bool = False) -> torch.Tensor:
-    """``out (+)= sum c_j x_j`` for up to 8 same-shape tensors in one pass."""
+    """``out (+)= sum c_j x_j`` for up to 8 same-shape tensors in one pass
+    (real coefficients; complex tensors through their real views)."""
     terms = [(float(c), x) for c, x in terms]
+    if any(x.is_complex() for _, x in terms) or (out is not None and out.is_complex()):
+        terms = [(c, x if x.is_complex() else to_complex(x.contiguous())) for c, x in terms]
+        ref = out if out is not None else terms[0][1]
+        res = torch.empty_like(ref) if out is None else out
+        lincomb([(c, torch.view_as_real(x)) for c, x in terms], torch.view_as_real(res), accumulate)
+        return res
     if not terms and out is None:
         raise ValueError("lincomb needs at least one term or an output")
     ref = out if out is not None else terms[0][1]
@@ -136,7 +143,16 @@ def div_bcast(X: torch.Tensor, D: torch.Tensor, out: torch.Tensor | None = None)
 
 def add_bcast(X: torch.Tensor, D: torch.Tensor, alpha: float = 1.0,
               out: torch.Tensor | None = None) -> torch.Tensor:
-    """``out[b, ...] = X[b, ...] + alpha * D[...]`` (FP64) with D broadcast over leading dims."""
+    """``out[b, ...] = X[b, ...] + alpha * D[...]`` (FP64, real or complex) with D
+    broadcast over leading dims; a real X with a complex D gives a complex result."""
+    if X.is_complex() or D.is_complex():
+        if not X.is_complex():
+            X = to_complex(X)
+        if not D.is_complex():
+            D = to_complex(D)
+        out = torch.empty_like(X) if out is None else out
+        add_bcast(torch.view_as_real(X), torch.view_as_real(D), alpha, torch.view_as_real(out))
+        return out
     _req(X, torch.float64, "X")
     _req(D, torch.float64, "D")
     if tuple(X.shape[X.ndim - D.ndim:]) != tuple(D.shape):
@@ -146,6 +162,18 @@ def add_bcast(X: torch.Tensor, D: torch.Tensor, alpha: float = 1.0,
         return out
     _lib.call("jfx_axpy_bcast_f64", _p(X), _p(D), float(alpha), _p(out), X.numel(), D.numel(), _s())
     return out
+
+
+def to_complex(X: torch.Tensor) -> torch.Tensor:
+    """Real FP64 array as complex128 (imaginary part zero) through the
+    complex-combine kernel's real-input form."""
+    _req(X, torch.float64, "X")
+    Y = torch.empty(X.shape, dtype=torch.complex128, device=X.device)
+    n = X.numel()
+    if n:
+        Z = torch.cat([X.reshape(1, 1, n), torch.zeros_like(X).reshape(1, 1, n)], 1)
+        _lib.call("jfx_cplx_combine", _p(Z), _p(Y), 1, 1, n, 1, _s())
+    return Y
 
 
 # ---------------------------------------------------------------- banded LU
@@ -168,8 +196,48 @@ def band_lu_factor(band: torch.Tensor, p: int, q: int) -> torch.Tensor:
     return lu
 
 
+def band_lu_factor_batched(band: torch.Tensor, p: int, q: int) -> torch.Tensor:
+    """Factor nsys band matrices (nsys, n, p+q+1) without pivoting, one lane each."""
+    _req(band, torch.float64, "band")
+    nsys, n = band.shape[0], band.shape[1]
+    lu = torch.empty_like(band)
+    st = torch.zeros(1, dtype=torch.int32, device=band.device)
+    _lib.call("jfx_band_lu_factor_batched", _p(band), nsys, n, p, q, _p(lu), _p(st), _s())
+    s = int(st.item())
+    if s != 0:
+        raise ValueError(f"Zero or non-finite pivot at row {s - 1} in a batched banded LU factorization")
+    return lu
+
+
+def band_lu_solve_batched(lu: torch.Tensor, p: int, q: int, X: torch.Tensor, axis: int) -> torch.Tensor:
+    """Solve in place along ``axis`` of X; the systems run over the other axes
+    in C order (one factor per line, the re/im pair of complex X sharing one)."""
+    nsys, n = lu.shape[0], lu.shape[1]
+    axis = axis % X.ndim
+    if X.shape[axis] != n:
+        raise ValueError(f"band solve: axis {axis} has length {X.shape[axis]}, matrices are {n}")
+    if (X.numel() // n) % nsys:
+        raise ValueError(f"band solve: {X.numel() // n} lines is not a multiple of {nsys} systems")
+    qdiv = 1
+    Xr = X
+    if X.is_complex():
+        _req(X, torch.complex128, "X")
+        Xr = torch.view_as_real(X)
+        qdiv = 2
+    _req(Xr, torch.float64, "X")
+    P = math.prod(X.shape[:axis])
+    Q = math.prod(X.shape[axis + 1:]) * qdiv
+    _lib.call("jfx_band_lu_solve_batched", _p(lu), nsys, n, p, q, _p(Xr), P, Q, qdiv, _s())
+    return X
+
+
 def band_lu_solve(lu: torch.Tensor, p: int, q: int, X: torch.Tensor, axis: int) -> torch.Tensor:
-    """Solve in place along ``axis`` of X with a factored band matrix."""
+    """Solve in place along ``axis`` of X with a factored band matrix (a complex
+    X is solved through its real view: the same real factor for re and im)."""
+    if X.is_complex():
+        _req(X, torch.complex128, "X")
+        band_lu_solve(lu, p, q, torch.view_as_real(X), axis % X.ndim)
+        return X
     _req(X, torch.float64, "X")
     n = lu.shape[0]
     if X.shape[axis] != n:
@@ -239,9 +307,70 @@ def io_dtype(prec: str) -> torch.dtype:
     return torch.float32 if prec in ("f32", "bf16x3", "bf16x6") else torch.float64
 
 
+class ComplexAxisOperator:
+    """A constant complex (M x K) matrix applied along one axis (FP64).
+
+    Kept as the stacked real (2M x K) operator [Re A; Im A]: one real MFMA
+    contraction over the real view of X, then ``jfx_cplx_combine`` forms
+    ``Re + i Im`` of the result.
+    """
+
+    is_complex = True
+
+    def __init__(self, A: torch.Tensor):
+        A = A.to(torch.complex128)
+        self.M, self.K = A.shape
+        self.A = A.contiguous()
+        self.stacked = AxisOperator(torch.cat([A.real, A.imag], 0).contiguous())
+
+    def prepared(self, prec: str):
+        if prec != "f64":
+            raise ValueError("complex operators are applied in f64 only")
+        return self.stacked.prepared(prec)
+
+
+def _apply_complex(op: ComplexAxisOperator, X: torch.Tensor, axis: int) -> torch.Tensor:
+    axis = axis % X.ndim
+    if X.shape[axis] != op.K:
+        raise ValueError(f"apply_along: axis {axis} has length {X.shape[axis]}, operator expects {op.K}")
+    P = math.prod(X.shape[:axis])
+    Q = math.prod(X.shape[axis + 1:])
+    shape = list(X.shape)
+    shape[axis] = op.M
+    Y = torch.empty(shape, dtype=torch.complex128, device=X.device)
+    if X.is_complex():
+        _req(X, torch.complex128, "X")
+        Z = apply_along(op.stacked, torch.view_as_real(X).reshape(P, op.K, 2 * Q), 1)
+        _lib.call("jfx_cplx_combine", _p(Z), _p(Y), P, op.M, Q, 0, _s())
+    else:
+        _req(X, torch.float64, "X")
+        Z = apply_along(op.stacked, X.reshape(P, op.K, Q), 1)
+        _lib.call("jfx_cplx_combine", _p(Z), _p(Y), P, op.M, Q, 1, _s())
+    return Y
+
+
 def apply_along(op: AxisOperator, X: torch.Tensor, axis: int, prec: str = "f64",
                 out: torch.Tensor | None = None) -> torch.Tensor:
-    """``Y = A`` applied along ``axis`` of X: Y[p, m, q] = sum_k A[m, k] X[p, k, q]."""
+    """``Y = A`` applied along ``axis`` of X: Y[p, m, q] = sum_k A[m, k] X[p, k, q].
+
+    A complex X (FP64 only) is contracted through its real view (the re/im
+    pair rides along the trailing axis); a :class:`ComplexAxisOperator`
+    gives a complex result.
+    """
+    if getattr(op, "is_complex", False):
+        if prec != "f64" or out is not None:
+            raise ValueError("complex operators are applied in f64 without an out= tensor")
+        return _apply_complex(op, X, axis)
+    if X.is_complex():
+        if prec != "f64":
+            raise ValueError("complex arrays are transformed in f64 only")
+        _req(X, torch.complex128, "X")
+        axis = axis % X.ndim
+        shape = list(X.shape)
+        shape[axis] = op.M
+        Y = torch.empty(shape, dtype=torch.complex128, device=X.device) if out is None else out
+        apply_along(op, torch.view_as_real(X), axis, "f64", out=torch.view_as_real(Y))
+        return Y
     dt = io_dtype(prec)
     _req(X, dt, "X")
     axis = axis % X.ndim
@@ -285,6 +414,8 @@ def num_cus() -> int:
 
 
 def plane_ok(op0: AxisOperator, op1: AxisOperator, prec: str) -> bool:
+    if getattr(op0, "is_complex", False) or getattr(op1, "is_complex", False):
+        return False
     return prec in PLANE_PRECISIONS and max(op0.M, op0.K, op1.M, op1.K) <= 128
 
 

@@ -5,10 +5,12 @@ from .tpmatrix import (
     TPMatrices,
     TPMatricesDenseLUFactors,
     TPMatricesLUFactors,
+    TPMatricesWavenumberSolver,
     TPMatrix,
     tpmats_dense_lu_factor,
     tpmats_lu_factor,
     tpmats_to_kron,
+    tpmats_wavenumber_factor,
 )
 
 __all__ = [
@@ -19,8 +21,10 @@ __all__ = [
     "TPMatrices",
     "TPMatricesDenseLUFactors",
     "TPMatricesLUFactors",
+    "TPMatricesWavenumberSolver",
     "TPMatrix",
     "tpmats_dense_lu_factor",
     "tpmats_lu_factor",
     "tpmats_to_kron",
+    "tpmats_wavenumber_factor",
 ]

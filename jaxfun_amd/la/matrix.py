@@ -131,6 +131,8 @@ class Matrix(BaseMatrix):
 
         b = as_device(b)
         LU, piv = self.lu_factor()
+        if b.is_complex():
+            LU = LU.to(torch.complex128)
         x = b.movedim(axis, 0).reshape(b.shape[axis], -1)
         y = torch.linalg.lu_solve(LU, piv, x)
         return y.reshape(b.movedim(axis, 0).shape).movedim(0, axis).contiguous()

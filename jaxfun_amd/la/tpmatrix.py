@@ -56,7 +56,8 @@ class TPMatrix:
         """A0 along axis 0, A1 along axis 1, ... (la/tpmatrix.py:152-156)."""
         w = _as_dev(w, K.io_dtype(precision))
         nb = w.ndim - self.dims
-        if self.dims == 2 and K.plane_ok(self.mats[0].operator(), self.mats[1].operator(), precision):
+        if self.dims == 2 and not w.is_complex() and \
+                K.plane_ok(self.mats[0].operator(), self.mats[1].operator(), precision):
             w = K.plane(self.mats[0].operator(), self.mats[1].operator(), w.contiguous(), precision)
             if self.coefficient != 1:
                 w = K.lincomb([(float(self.coefficient), w)])
@@ -115,6 +116,11 @@ class TPMatrix:
 
 def _solve_diagonal(d: np.ndarray, rhs) -> torch.Tensor:
     x = _as_dev(rhs)
+    if x.is_complex():
+        # the real view divides re and im by the same entry
+        D2 = np.repeat(np.asarray(d, dtype=np.float64)[..., None], 2, axis=-1)
+        D = torch.tensor(np.ascontiguousarray(D2), dtype=torch.float64, device=x.device)
+        return torch.view_as_complex(K.div_bcast(torch.view_as_real(x), D))
     D = torch.tensor(np.ascontiguousarray(d), dtype=torch.float64, device=x.device)
     return K.div_bcast(x, D)
 
@@ -146,7 +152,12 @@ class TPMatricesLUFactors:
 
     def D(self, dtype) -> torch.Tensor:
         if dtype not in self._D:
-            self._D[dtype] = torch.tensor(self.D_host, dtype=dtype, device=K.device())
+            if dtype == torch.complex128:
+                # D for the real view of complex data: each entry twice (re, im)
+                D2 = np.repeat(self.D_host[..., None], 2, axis=-1)
+                self._D[dtype] = torch.tensor(np.ascontiguousarray(D2), dtype=torch.float64, device=K.device())
+            else:
+                self._D[dtype] = torch.tensor(self.D_host, dtype=dtype, device=K.device())
         return self._D[dtype]
 
     def solve(self, rhs, precision: str = "f64") -> torch.Tensor:
@@ -155,6 +166,13 @@ class TPMatricesLUFactors:
         nb = F.ndim - len(self.shape)
         if tuple(F.shape[nb:]) != self.shape:
             F = F.reshape(F.shape[:nb] + self.shape) if F.numel() % int(np.prod(self.shape)) == 0 else F
+        if F.is_complex():
+            for i, W in enumerate(self.W):
+                F = K.apply_along(W, F, nb + i, "f64")
+            U = torch.view_as_complex(K.div_bcast(torch.view_as_real(F), self.D(torch.complex128)))
+            for i, V in enumerate(self.V):
+                U = K.apply_along(V, U, nb + i, "f64")
+            return U
         if len(self.shape) == 2 and K.plane_ok(self.W[0], self.W[1], precision):
             F = F.contiguous()
             Ft = K.plane(self.W[0], self.W[1], F, precision, ("div", self.D(dt)))
@@ -178,8 +196,90 @@ class TPMatricesDenseLUFactors:
         b = _as_dev(rhs)
         nb = b.ndim - len(self.shape)
         flat = b.reshape(b.shape[:nb] + (-1,))
-        x = torch.linalg.lu_solve(self.LU, self.piv, flat.reshape(-1, flat.shape[-1]).T.contiguous())
+        LU = self.LU
+        if b.is_complex() and not LU.is_complex():
+            LU = self.LU.to(torch.complex128)
+        x = torch.linalg.lu_solve(LU, self.piv, flat.reshape(-1, flat.shape[-1]).T.contiguous())
         return x.T.reshape(b.shape).contiguous()
+
+
+class TPMatricesWavenumberSolver:
+    """Per-wavenumber banded solver for Fourier x polynomial systems
+    (la/tpmatrix.py:590-1011).
+
+    Every axis but ``poly_axis`` is diagonal in every term (Fourier), so for
+    each combination of the diagonal axes' indices (flattened in C order,
+    ``nsys`` systems) the polynomial-axis operator is the banded matrix
+    ``B_k = sum_i s_i (prod_a F_i^(a)[k_a]) P_i``.  All ``B_k`` are factored
+    at once on the device (``jfx_band_lu_factor_batched``, no pivoting, one
+    lane per system) and a solve is one ``jfx_band_lu_solve_batched`` launch
+    along the polynomial axis (the re/im parts of complex data share a factor).
+    """
+
+    def __init__(self, poly_axis: int, shape, band: np.ndarray, p: int, q: int):
+        self.poly_axis = int(poly_axis)
+        self.shape = tuple(shape)
+        self.p, self.q = int(p), int(q)
+        self.nsys = band.shape[0]
+        self.lu = K.band_lu_factor_batched(torch.tensor(np.ascontiguousarray(band), device=K.device()), p, q)
+
+    def solve(self, rhs, precision: str = "f64") -> torch.Tensor:
+        x = _as_dev(rhs).clone()
+        nb = x.ndim - len(self.shape)
+        if tuple(x.shape[nb:]) != self.shape:
+            raise ValueError(f"rhs shape {tuple(x.shape)} does not end with {self.shape}")
+        return K.band_lu_solve_batched(self.lu, self.p, self.q, x, nb + self.poly_axis)
+
+
+def tpmats_wavenumber_factor(A) -> TPMatricesWavenumberSolver:
+    """Factor a Fourier x polynomial sum of Kronecker products per wavenumber
+    (la/tpmatrix.py:1236-1354).  Raises SolverNotApplicable unless exactly one
+    axis is non-diagonal; complex wavenumber weights (odd derivative orders
+    on a Fourier axis) are not built and raise SolverNotApplicable too."""
+    if isinstance(A, TPMatrices):
+        tpmats = list(A.tpmats)
+    elif isinstance(A, list):
+        tpmats = A
+    elif isinstance(A, TPMatrix):
+        tpmats = [A]
+    else:
+        raise TypeError(f"tpmats_wavenumber_factor expects a list[TPMatrix] or TPMatrices, got {type(A).__name__!r}.")
+    ndim = tpmats[0].dims
+    diag_axes = [a for a in range(ndim) if all(tp.mats[a].diagonal_or_none() is not None for tp in tpmats)]
+    poly_axes = [a for a in range(ndim) if a not in diag_axes]
+    if len(poly_axes) != 1:
+        raise SolverNotApplicable(
+            f"tpmats_wavenumber_factor requires exactly 1 polynomial (non-diagonal) axis; "
+            f"found {len(poly_axes)}: {poly_axes}. Use tpmats_lu_factor for fully-symmetric problems.")
+    pa = poly_axes[0]
+    shape = tuple(int(tpmats[0].mats[a].shape[0]) for a in range(ndim))
+    n = shape[pa]
+    # W[i, k]: scale_i * prod over the diagonal axes (C order) of their diagonals
+    W = []
+    for tp in tpmats:
+        w = np.array([complex(tp.coefficient)])
+        for a in diag_axes:
+            w = np.outer(w, tp.mats[a].diagonal_or_none()).ravel()
+        W.append(w)
+    W = np.array(W)
+    if np.abs(W.imag).max() > 0:
+        raise SolverNotApplicable("complex wavenumber weights")
+    W = W.real
+    dense = [tp.mats[pa].todense_host() for tp in tpmats]
+    nz = np.zeros((n, n), dtype=bool)
+    for P in dense:
+        nz |= P != 0
+    rows, cols = np.nonzero(nz)
+    p = int(max(0, (rows - cols).max())) if rows.size else 0
+    q = int(max(0, (cols - rows).max())) if rows.size else 0
+    Wd = p + q + 1
+    bands = np.zeros((len(tpmats), n, Wd))
+    for t, P in enumerate(dense):
+        for o in range(-p, q + 1):
+            i = np.arange(max(0, -o), min(n, n - o))
+            bands[t, i, o + p] = P[i, i + o]
+    band = np.einsum("tk,tnw->knw", W, bands)
+    return TPMatricesWavenumberSolver(pa, shape, band, p, q)
 
 
 def tpmats_to_kron(tpmats) -> np.ndarray:
@@ -349,13 +449,18 @@ class TPMatrices:
         return Matrix(self.todense_host())
 
     def lu_factor(self):
-        """Dispatch (la/tpmatrix.py:386-427); the Fourier wavenumber solver is not built."""
+        """Dispatch (la/tpmatrix.py:386-427): all-dense factors -> Kronecker LU;
+        otherwise the wavenumber solver when exactly one axis is not
+        diagonal, else fast diagonalisation."""
         if self._lu_cache is not None:
             return self._lu_cache
         if all(isinstance(m, Matrix) for tp in self.tpmats for m in tp.mats):
             res = tpmats_dense_lu_factor(self.tpmats)
         else:
-            res = tpmats_lu_factor(self.tpmats)
+            try:
+                res = tpmats_wavenumber_factor(self.tpmats)
+            except SolverNotApplicable:
+                res = tpmats_lu_factor(self.tpmats)
         self._lu_cache = res
         return res
 
@@ -395,5 +500,6 @@ class TPMatrices:
         return self._kron_cache.solve(rhs)
 
 
-__all__ = ["TPMatrix", "TPMatrices", "TPMatricesLUFactors", "TPMatricesDenseLUFactors", "DiaMatrix",
+__all__ = ["TPMatrix", "TPMatrices", "TPMatricesLUFactors", "TPMatricesDenseLUFactors",
+           "TPMatricesWavenumberSolver", "tpmats_wavenumber_factor", "DiaMatrix",
            "BaseMatrix", "tpmats_lu_factor", "tpmats_dense_lu_factor", "tpmats_to_kron", "axis_eig"]

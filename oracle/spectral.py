@@ -26,6 +26,12 @@ pinned by checks that do not retype the formula (all in ``tests/test_oracle.py``
 * Variable-coefficient operator / PCG / MLP: dense assembled
   operator vs the matrix-free one; PCG vs direct solve; MLP gradients vs
   central finite differences.
+* Fourier transforms: ``numpy.fft`` is the transform itself (the reference
+  calls ``jnp.fft``); pinned further by exactness on trigonometric
+  polynomials, the reference's truncation/padding test
+  (``tests/galerkin/test_fourier.py``) and the manufactured solutions of
+  ``tests/la/test_tpmatrices_solvers.py:50-160`` (per-wavenumber solve vs
+  the Kronecker solve and vs cos(2x)(1-y^2)).
 * Inhomogeneous Dirichlet lifting: the lifting functions' end values by
   ``numpy.polynomial`` evaluation; the boundary values the reference's own
   tests pin (``tests/galerkin/test_composite.py:66-80``,
@@ -927,14 +933,101 @@ def lifted_helmholtz_2d(L: Lifted2D, alpha: float, fgrid: np.ndarray, Nq=None):
     return u, A, b
 
 
+# --------------------------------------------------------------------------
+# Fourier basis and the per-wavenumber solver
+# --------------------------------------------------------------------------
+
+
+def fourier_wavenumbers(N: int, eliminate_highest_freq: bool = False) -> np.ndarray:
+    """FFT-ordered wavenumbers (galerkin/Fourier.py:15-20) = numpy.fft.fftfreq(N) * N."""
+    k = np.rint(np.fft.fftfreq(N) * N).astype(int)
+    if eliminate_highest_freq and N % 2 == 0:
+        k[N // 2] = 0
+    return k
+
+
+class Fourier1D:
+    """Fourier transforms restated with numpy.fft (galerkin/Fourier.py:119-235).
+
+    Coefficients along the LAST axis; the reference domain is [0, 2 pi), the
+    true domain (a, b) maps onto it with df = 2 pi / (b - a).
+    """
+
+    def __init__(self, N: int, domain=(0.0, 2 * np.pi)):
+        self.N = N
+        self.a, self.b = float(domain[0]), float(domain[1])
+        self.df = 2 * np.pi / (self.b - self.a)
+
+    def k(self, eliminate: bool = False) -> np.ndarray:
+        return fourier_wavenumbers(self.N, eliminate)
+
+    def mesh(self, Nq: int | None = None) -> np.ndarray:
+        Nq = self.N if Nq is None else Nq
+        return self.a + (self.b - self.a) * np.arange(Nq) / Nq
+
+    def backward(self, c, Nq: int | None = None, d: int = 0):
+        """ifft(norm="forward") with zero padding in the middle (Fourier.py:127-149);
+        derivative d by (i k df)^d on the coefficients (Fourier.py:206-219)."""
+        c = np.asarray(c, dtype=np.complex128)
+        if d:
+            c = c * (1j * self.k(d % 2 == 1) * self.df) ** d
+        Nq = self.N if Nq is None else Nq
+        if Nq > self.N:
+            h = self.N // 2
+            pad = np.zeros(c.shape[:-1] + (Nq - self.N,), dtype=np.complex128)
+            c = np.concatenate([c[..., :h], pad, c[..., h:]], axis=-1)
+        return np.fft.ifft(c, axis=-1, norm="forward")
+
+    def forward(self, u):
+        """fft(norm="forward"), truncated to the space's wavenumbers (Fourier.py:166-180)."""
+        u = np.asarray(u)
+        out = np.fft.fft(u, axis=-1, norm="forward")
+        if u.shape[-1] > self.N:
+            out = out[..., self.k() % u.shape[-1]]
+        return out
+
+    def scalar_product(self, u):
+        """forward * 2 pi / df (galerkin/Fourier.py:151-164)."""
+        return self.forward(u) * 2 * np.pi / self.df
+
+    def evaluate(self, x, c):
+        X = (np.asarray(x, dtype=np.float64) - self.a) * self.df
+        return np.exp(1j * np.multiply.outer(X, self.k())) @ np.asarray(c, dtype=np.complex128)
+
+    def matrix(self, i: int, j: int) -> np.ndarray:
+        """diag((i k)^j (-i k)^i 2 pi) df^(i+j-1) (Fourier.py:237-264, orthogonal.py:479-529)."""
+        k = self.k().astype(np.complex128)
+        return np.diag((1j * k) ** j * (-1j * k) ** i * 2 * np.pi * self.df ** (i + j - 1))
+
+
+def wavenumber_solve(terms, F: np.ndarray, poly_axis: int) -> np.ndarray:
+    """Solve ``sum_t s_t A_t^(0) (x) A_t^(1) u = F`` where every factor on the
+    other axis is diagonal: one dense solve of the polynomial-axis matrix
+    ``B_k = sum_t s_t d_t[k] P_t`` per wavenumber k (la/tpmatrix.py:590-1011,
+    1236-1354, with numpy.linalg.solve in place of the banded LU)."""
+    other = 1 - poly_axis
+    n = F.shape[poly_axis]
+    U = np.zeros(F.shape, dtype=np.result_type(F, np.complex128))
+    for k in range(F.shape[other]):
+        B = np.zeros((n, n), dtype=np.complex128)
+        for s, mats in terms:
+            B = B + s * np.diag(mats[other])[k] * mats[poly_axis]
+        if poly_axis == 1:
+            U[k, :] = np.linalg.solve(B, F[k, :])
+        else:
+            U[:, k] = np.linalg.solve(B, F[:, k])
+    return U
+
+
 def ars443_stage_coeffs():
     """Distinct implicit diagonal coefficients of ARS443 (integrators/imex_rk.py:39-48)."""
     return sorted({float(ARS443_IMPLICIT[i, i]) for i in range(5) if ARS443_IMPLICIT[i, i] != 0})
 
 
 def rel_l2(a, b) -> float:
-    a = np.asarray(a, dtype=np.float64)
-    b = np.asarray(b, dtype=np.float64)
+    dt = np.complex128 if (np.iscomplexobj(a) or np.iscomplexobj(b)) else np.float64
+    a = np.asarray(a, dtype=dt)
+    b = np.asarray(b, dtype=dt)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 

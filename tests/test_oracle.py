@@ -420,3 +420,72 @@ def test_helmholtz2d_example_accuracy():
     xu = np.linspace(0.0, 1.0, 100)
     err = np.linalg.norm(Lf.values(u, xu, xu) - f_ue(xu[:, None], xu[None, :])) / 100
     assert err < np.spacing(1000.0), err
+
+
+# ---------------------------------------------------------------- Fourier
+
+
+def test_fourier_trig_polynomial_exact():
+    # reference tests/galerkin/test_fourier.py: N = 8, c[1] = 1, c[2] = 2
+    F = O.Fourier1D(8)
+    c = np.zeros(8, dtype=complex)
+    c[1], c[2] = 1.0, 2.0
+    x = F.mesh()
+    u = F.backward(c)
+    assert np.allclose(u, np.exp(1j * x) + 2 * np.exp(2j * x), atol=1e-14)
+    up = F.backward(c, 12)
+    x12 = F.mesh(12)
+    assert up.shape == (12,) and np.allclose(up, np.exp(1j * x12) + 2 * np.exp(2j * x12), atol=1e-14)
+    assert np.abs(F.forward(u) - c).max() < 1e-14 and np.abs(F.forward(up) - c).max() < 1e-14
+    # first and second derivatives of the series, on a domain of length 3
+    G = O.Fourier1D(8, (0.0, 3.0))
+    s = 2 * np.pi / 3.0
+    xg = G.mesh()
+    assert np.allclose(G.backward(c, d=1), 1j * s * np.exp(1j * s * xg) + 4j * s * np.exp(2j * s * xg), atol=1e-13)
+    assert np.allclose(G.backward(c, d=2), -(s**2) * np.exp(1j * s * xg) - 8 * s**2 * np.exp(2j * s * xg), atol=1e-12)
+    assert np.allclose(G.evaluate(xg, c), G.backward(c), atol=1e-14)
+    # the Nyquist mode (k = -N/2) is real on the grid and dropped by odd derivatives
+    cn = np.zeros(8, dtype=complex)
+    cn[4] = 1.0
+    assert np.allclose(F.backward(cn), np.cos(4 * x), atol=1e-14)
+    assert np.abs(F.backward(cn, d=1)).max() == 0.0
+
+
+def _fourier_legendre_poisson(N):
+    F = O.Fourier1D(N)
+    L = O.Space1D("legendre", N, dirichlet=True)
+    Af = [F.matrix(0, 2), F.matrix(0, 0)]
+    M = L.mass()
+    Kl = O.galerkin_matrix("legendre", N, 0, 2)
+    terms = [(1.0, [Af[0], M]), (1.0, [Af[1], Kl])]
+    return F, L, terms
+
+
+def test_wavenumber_solve_vs_kron():
+    # reference tests/la/test_tpmatrices_solvers.py:126-133 (agreement with the Kronecker solve)
+    N = 16
+    F, L, terms = _fourier_legendre_poisson(N)
+    rng = np.random.default_rng(0)
+    B = rng.standard_normal((N, N - 2)) + 1j * rng.standard_normal((N, N - 2))
+    # the k = 0 row of (v, u_xx) vanishes; K keeps that block non-singular
+    U = O.wavenumber_solve(terms, B, 1)
+    Ak = sum(s * np.kron(m[0], m[1]) for s, m in terms)
+    Ur = np.linalg.solve(Ak, B.ravel()).reshape(B.shape)
+    assert np.abs(U - Ur).max() < 1e-12 * max(1.0, np.abs(Ur).max())
+
+
+def test_fourier_legendre_poisson_manufactured():
+    # reference tests/la/test_tpmatrices_solvers.py:136-146: ue = cos(2x)(1-y^2)
+    N = 16
+    F, L, terms = _fourier_legendre_poisson(N)
+    x = F.mesh()
+    y, wy = O.leggauss(N)
+    lap = (-4 * np.cos(2 * x)[:, None]) * (1 - y**2)[None, :] + np.cos(2 * x)[:, None] * (-2.0)
+    b = F.scalar_product(lap.T).T
+    b = (b * wy[None, :]) @ O.vandermonde("legendre", y, N) @ L.S.T
+    U = O.wavenumber_solve(terms, b, 1)
+    xs = np.linspace(0, 2 * np.pi, 40, endpoint=False)
+    ys = np.linspace(-1, 1, 40)
+    uj = F.evaluate(xs, (U @ L.S @ O.vandermonde("legendre", ys, N).T))
+    ue = np.cos(2 * xs)[:, None] * (1 - ys**2)[None, :]
+    assert np.linalg.norm(uj - ue) / 40 < np.sqrt(np.spacing(10.0))
