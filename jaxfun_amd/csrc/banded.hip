@@ -10,7 +10,7 @@ namespace jfx {
 
 // Doolittle factorisation of one banded matrix by one lane; returns 0 or
 // 1 + the row of the first zero / non-finite pivot.
-__device__ int band_lu_factor_one(const double* __restrict__ band, int n, int p, int q,
+__device__ __forceinline__ int band_lu_factor_one(const double* __restrict__ band, int n, int p, int q,
                                   double* __restrict__ lu) {
   const int W = p + q + 1;
   for (long long i = 0; i < (long long)n * W; ++i) lu[i] = band[i];
@@ -49,9 +49,88 @@ __global__ void band_lu_factor_batched_kernel(const double* __restrict__ band, i
 }
 
 // Forward / back substitution of one line x (stride Q) with one factor.
-__device__ void band_lu_solve_one(const double* __restrict__ lu, int n, int p, int q, double* __restrict__ x,
+// For bandwidths <= RW the last solved values stay in a register window, so
+// the recurrence carries no memory round trip (the right-hand side and factor
+// loads do not depend on it and are issued ahead).
+constexpr int RW = 8;
+__device__ __forceinline__ void band_lu_solve_one(const double* __restrict__ lu, int n, int p, int q, double* __restrict__ x,
                                   long long Q) {
   const int W = p + q + 1;
+  if (p <= RW && q <= RW) {
+    // rows go in chunks of RC: the chunk's right-hand sides and factor rows
+    // are loaded together (one memory latency per chunk), then the chunk's
+    // recurrence runs from registers
+    constexpr int RC = 8;
+    double w[RW];
+#pragma unroll
+    for (int t = 0; t < RW; ++t) w[t] = 0.0;
+    for (int i0 = 0; i0 < n; i0 += RC) {
+      double xv[RC], cf[RC][RW];
+#pragma unroll
+      for (int c = 0; c < RC; ++c) {
+        const int i = i0 + c;
+        xv[c] = 0.0;
+#pragma unroll
+        for (int t = 0; t < RW; ++t) cf[c][t] = 0.0;
+        if (i < n) {
+          const double* row = lu + (long long)i * W;
+          xv[c] = x[(long long)i * Q];
+#pragma unroll
+          for (int t = 0; t < RW; ++t)
+            if (t < p && i - 1 - t >= 0) cf[c][t] = row[p - 1 - t];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < RC; ++c) {
+        const int i = i0 + c;
+        if (i < n) {
+          double s = xv[c];
+#pragma unroll
+          for (int t = 0; t < RW; ++t) s -= cf[c][t] * w[t];
+#pragma unroll
+          for (int t = RW - 1; t > 0; --t) w[t] = w[t - 1];
+          w[0] = s;
+          x[(long long)i * Q] = s;
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < RW; ++t) w[t] = 0.0;
+    for (int i0 = n - 1; i0 >= 0; i0 -= RC) {
+      double xv[RC], dg[RC], cf[RC][RW];
+#pragma unroll
+      for (int c = 0; c < RC; ++c) {
+        const int i = i0 - c;
+        xv[c] = 0.0;
+        dg[c] = 1.0;
+#pragma unroll
+        for (int t = 0; t < RW; ++t) cf[c][t] = 0.0;
+        if (i >= 0) {
+          const double* row = lu + (long long)i * W;
+          xv[c] = x[(long long)i * Q];
+          dg[c] = row[p];
+#pragma unroll
+          for (int t = 0; t < RW; ++t)
+            if (t < q && i + 1 + t < n) cf[c][t] = row[p + 1 + t];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < RC; ++c) {
+        const int i = i0 - c;
+        if (i >= 0) {
+          double s = xv[c];
+#pragma unroll
+          for (int t = 0; t < RW; ++t) s -= cf[c][t] * w[t];
+          s /= dg[c];
+#pragma unroll
+          for (int t = RW - 1; t > 0; --t) w[t] = w[t - 1];
+          w[0] = s;
+          x[(long long)i * Q] = s;
+        }
+      }
+    }
+    return;
+  }
   for (int i = 1; i < n; ++i) {
     double s = x[(long long)i * Q];
     int j0 = max(0, i - p);
@@ -68,7 +147,7 @@ __device__ void band_lu_solve_one(const double* __restrict__ lu, int n, int p, i
 
 // Solve L U x = b in place along axis of length n of a (P, n, Q) batch; one
 // thread per line (coalesced over q).
-__global__ void band_lu_solve_kernel(const double* __restrict__ lu, int n, int p, int q,
+__global__ __launch_bounds__(256) void band_lu_solve_kernel(const double* __restrict__ lu, int n, int p, int q,
                                      double* __restrict__ X, long long P, long long Q) {
   long long line = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (line >= P * Q) return;
@@ -80,7 +159,7 @@ __global__ void band_lu_solve_kernel(const double* __restrict__ lu, int n, int p
 // (pp * (Q / qdiv) + qq / qdiv) mod nsys (qdiv = 2 keeps the re/im pair of a
 // complex value on one system; the modulus repeats the systems over leading
 // batch dimensions).
-__global__ void band_lu_solve_batched_kernel(const double* __restrict__ lu, int nsys, int n, int p, int q,
+__global__ __launch_bounds__(256) void band_lu_solve_batched_kernel(const double* __restrict__ lu, int nsys, int n, int p, int q,
                                              double* __restrict__ X, long long P, long long Q, int qdiv) {
   long long line = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (line >= P * Q) return;
