@@ -1,6 +1,7 @@
 """Test and trial functions as SymPy applied functions (reference:
 galerkin/arguments.py:342-460).  Each carries its function space and an
-argument tag (0 = test, 1 = trial) as class attributes."""
+argument tag (0 = test, 1 = trial, 2 = function with known coefficients) as
+class attributes."""
 
 from __future__ import annotations
 
@@ -10,6 +11,86 @@ import sympy as sp
 class ArgumentTag:
     TEST = 0
     TRIAL = 1
+    JAXFUNC = 2
+
+
+class _Data:
+    """Identity-compared holder for a device array inside SymPy metadata."""
+
+    def __init__(self, value):
+        self.value = value
+
+
+# JAXFunction instances by their Dummy argument, so that SymPy can rebuild
+# one from its arguments (``func(*args)``) without losing its coefficients
+_JAXF_DATA: dict = {}
+
+
+class JAXFunction(sp.Function):
+    """A function of V with explicit expansion coefficients
+    (galerkin/arguments.py:709-900): ``u(x) = sum_i array_i phi_i(x)``.
+
+    ``array`` is a coefficient array (or a SymPy expression, projected onto
+    V).  In a form, a JAXFunction that appears linearly (to the first power,
+    possibly differentiated, in a term without a trial function) contributes
+    ``A @ array`` with the Galerkin matrix A of that term; anywhere else
+    (products with the trial function, powers, ...) it is evaluated on the
+    quadrature mesh as a variable coefficient.
+    """
+
+    argument = ArgumentTag.JAXFUNC
+
+    def __new__(cls, array, V=None, name=None, *rest):
+        if isinstance(V, sp.Basic) or V is None:
+            # SymPy rebuilding the object from its arguments
+            args = (array, V, name) + rest if V is not None else (array,)
+            args = tuple(a for a in args if a is not None)
+            data = _JAXF_DATA[args[-1]]
+            obj = sp.Function.__new__(cls, *args)
+            obj.array, obj.functionspace, obj.name = data
+            return obj
+        from .inner import project
+        from .orthogonal import as_device
+
+        if isinstance(array, sp.Basic):
+            array = project(array, V)
+        d = sp.Dummy()
+        name = name if name is not None else "JAXFunction"
+        arr = as_device(array)
+        _JAXF_DATA[d] = (arr, V, name)
+        obj = sp.Function.__new__(cls, *(tuple(V.system.base_scalars()) + (d,)))
+        obj.array, obj.functionspace, obj.name = arr, V, name
+        return obj
+
+    def get_array(self):
+        return self.array
+
+    def backward(self, N=None):
+        return self.functionspace.backward(self.array, N)
+
+    def __call__(self, *x):
+        """Value at a point of a 1D space (galerkin/arguments.py:850-880)."""
+        import numpy as np
+
+        return self.functionspace.evaluate(np.array(x, dtype=float).reshape(-1), self.array)[0]
+
+    def doit(self, **hints):
+        return self
+
+    def _sympystr(self, printer):
+        return self.name
+
+    def __matmul__(self, a):
+        return self.array @ a
+
+    def __rmatmul__(self, a):
+        return a @ self.array
+
+
+def is_jaxfunction(e) -> bool:
+    if isinstance(e, sp.Derivative):
+        e = e.expr
+    return isinstance(e, JAXFunction)
 
 
 t = sp.Symbol("t", real=True)
@@ -65,4 +146,6 @@ def get_arg(e) -> int | None:
 def functionspace_of(e):
     if isinstance(e, sp.Derivative):
         e = e.expr
+    if isinstance(e, JAXFunction):
+        return e.functionspace
     return getattr(getattr(e, "func", None), "functionspace", None)

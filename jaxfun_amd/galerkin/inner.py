@@ -19,7 +19,7 @@ import torch
 from .. import kernels as K
 from ..la import DiaMatrix, Matrix, TPMatrices, TPMatrix
 from ..utils.common import lambdify
-from .arguments import ArgumentTag, functionspace_of, get_arg
+from .arguments import ArgumentTag, JAXFunction, functionspace_of, get_arg
 from .composite import BCGeneric, Composite
 from .orthogonal import MeshKind
 from .tensorproductspace import TensorProductSpace
@@ -33,10 +33,17 @@ class InnerKind(str, enum.Enum):
 
 # ----------------------------------------------------------------------------- splitting
 class _Term:
-    """One additive term: test derivative orders, trial derivative orders, coefficient."""
+    """One additive term: test derivative orders, trial derivative orders,
+    coefficient, and a JAXFunction factor that appears linearly (``jf``, with
+    derivative orders ``dj``) in a term without a trial function."""
 
-    def __init__(self, test, trial, dv, du, coeff):
+    def __init__(self, test, trial, dv, du, coeff, jf=None, dj=None):
         self.test, self.trial, self.dv, self.du, self.coeff = test, trial, dv, du, coeff
+        self.jf, self.dj = jf, (dj or {})
+
+
+def _has_jaxfunction(e) -> bool:
+    return bool(sp.sympify(e).atoms(JAXFunction))
 
 
 def _deriv_orders(f) -> dict:
@@ -56,6 +63,7 @@ def split_terms(expr) -> list[_Term]:
         test = trial = None
         dv, du = {}, {}
         coeff = sp.S.One
+        jfs = []
         for f in sp.Mul.make_args(term):
             tag = get_arg(f)
             if tag == ArgumentTag.TEST:
@@ -66,16 +74,29 @@ def split_terms(expr) -> list[_Term]:
                 if trial is not None:
                     raise NotImplementedError("nonlinear in the trial function")
                 trial, du = f, _deriv_orders(f)
+            elif tag == ArgumentTag.JAXFUNC:
+                jfs.append(f)
             else:
                 coeff = coeff * f
         if test is None:
             raise AssertionError("No TestFunction found in expression")
+        if trial is None and len(jfs) == 1 and not _has_jaxfunction(coeff):
+            # linear in a function with known coefficients: A @ array
+            f = jfs[0]
+            terms.append(_Term(test, None, dv, {}, coeff, f.expr if isinstance(f, sp.Derivative) else f,
+                               _deriv_orders(f)))
+            continue
+        for f in jfs:
+            coeff = coeff * f
         terms.append(_Term(test, trial, dv, du, coeff))
     return terms
 
 
 def _separate(coeff, symbols):
-    """{axis: factor, 'coeff': scalar} or None when not separable."""
+    """{axis: factor, 'coeff': scalar} or None when not separable (a
+    coefficient holding a JAXFunction is evaluated on the mesh)."""
+    if _has_jaxfunction(coeff):
+        return None
     if not coeff.free_symbols:
         return {"coeff": coeff}
     sep = sp.separatevars(coeff, list(symbols), dict=True, force=True)
@@ -94,11 +115,12 @@ def _stencil_apply(space, z: torch.Tensor, axis: int) -> torch.Tensor:
     return z if S is None else K.apply_along(S, z.contiguous(), axis, "f64")
 
 
-def bilinear_1d(v, i: int, u, j: int, coeff, Nq: int, use_precomputed: bool):
+def bilinear_1d(v, i: int, u, j: int, coeff, Nq: int, use_precomputed: bool, weight=None):
     """One separated factor (galerkin/inner.py:809-911): matrix (test row,
-    trial column) of ``(psi^(i), c psi^(j))`` with stencils applied."""
+    trial column) of ``(psi^(i), c psi^(j))`` with stencils applied;
+    ``weight`` (values at the Nq quadrature points) replaces c."""
     scale = 1.0
-    if not coeff.free_symbols:
+    if weight is None and not coeff.free_symbols:
         scale = float(coeff)
         if use_precomputed:
             if isinstance(v, Composite):
@@ -121,7 +143,11 @@ def bilinear_1d(v, i: int, u, j: int, coeff, Nq: int, use_precomputed: bool):
     x, w = _quad_nodes(v, Nq)
     df = float(v.orthogonal.domain_factor)
     wq = w * (df ** (i + j - 1))
-    if coeff.free_symbols:
+    if weight is not None:
+        if weight.is_complex():
+            raise NotImplementedError("complex coefficients on a polynomial axis")
+        wq = wq * weight
+    elif coeff.free_symbols:
         wq = wq * v.eval_coefficient(coeff, Nq)
     else:
         wq = wq * scale
@@ -149,15 +175,46 @@ def test_operator(v, i: int, Nq: int) -> K.AxisOperator:
     return v._ops[key]
 
 
+def _jaxfunction_values(f, symbols, Nq) -> np.ndarray:
+    """Values (or derivative values) of a JAXFunction on the quadrature mesh of
+    the form's test spaces (its space must share that mesh)."""
+    orders = _deriv_orders(f)
+    jf = f.expr if isinstance(f, sp.Derivative) else f
+    V = jf.functionspace
+    vsyms = tuple(V.system.base_scalars())
+    k = [orders.get(s_, 0) for s_ in vsyms]
+    if len(vsyms) == 1:
+        vals = V.backward_primitive(jf.array, k[0], Nq[0])
+    else:
+        vals = V.backward_primitive(jf.array, tuple(k), tuple(Nq))
+    return vals.cpu().numpy()
+
+
 def _mesh_values(spaces, symbols, expr, Nq) -> torch.Tensor:
     grids = [s.mesh(MeshKind.QUADRATURE, Nq[i]).cpu().numpy() for i, s in enumerate(spaces)]
     mesh = np.meshgrid(*grids, indexing="ij")
+    expr = sp.sympify(expr)
+    extra_syms, extra_vals = [], []
+    if _has_jaxfunction(expr):
+        atoms = [a for a in expr.atoms(sp.Derivative) if isinstance(a.expr, JAXFunction)]
+        atoms += list(expr.atoms(JAXFunction))
+        for a in atoms:
+            z = sp.Dummy()
+            expr = expr.xreplace({a: z})
+            extra_syms.append(z)
+            extra_vals.append(_jaxfunction_values(a, symbols, Nq))
     if expr.free_symbols:
-        vals = lambdify(tuple(symbols), expr)(*mesh)
+        vals = lambdify(tuple(symbols) + tuple(extra_syms), expr)(*mesh, *extra_vals)
     else:
-        vals = float(expr)
-    vals = np.broadcast_to(np.asarray(vals, dtype=np.float64), mesh[0].shape).copy()
-    return torch.tensor(vals, dtype=torch.float64, device=K.device())
+        vals = complex(expr) if expr.has(sp.I) else float(expr)
+    vals = np.asarray(vals)
+    dt = np.complex128 if np.iscomplexobj(vals) else np.float64
+    vals = np.broadcast_to(vals.astype(dt), mesh[0].shape).copy()
+    if dt == np.complex128 and np.abs(vals.imag).max() == 0:
+        vals = vals.real.copy()
+        dt = np.float64
+    return torch.tensor(vals, dtype=torch.complex128 if dt == np.complex128 else torch.float64,
+                        device=K.device())
 
 
 # ----------------------------------------------------------------------------- variable coefficients
@@ -166,33 +223,82 @@ class TensorMatrix:
 
     Each term applies ``T_0^(i0) (c * (B_0^(j0) U B_1^(j1)^T)) T_1^(i1)^T``:
     backward contractions to the quadrature mesh, a pointwise product with the
-    coefficient sampled there, and test-space contractions back.
+    coefficient sampled there, and test-space contractions back.  Separable
+    terms of the same form (``separable``, TPMatrix objects) are added to the
+    result.  ``solve`` assembles the dense (n0 n1 x n0 n1) matrix on the
+    device by applying the operator to the identity batch and factors it
+    (la/tensormatrix.py:76-84).
     """
 
-    def __init__(self, terms, shape):
-        self.terms = terms  # list of (coef_mesh, [(B_op, T_op) per axis])
-        self.shape_dofs = shape
+    def __init__(self, terms, shape, separable=()):
+        self.terms = list(terms)  # list of (coef_mesh, [(B_op, T_op) per axis])
+        self.shape_dofs = tuple(shape)
+        self.separable = list(separable)
+        self._lu = None
+
+    @property
+    def shape(self) -> tuple[int, int]:
+        n = int(np.prod(self.shape_dofs))
+        return (n, n)
 
     def __call__(self, U, precision: str = "f64"):
         from .orthogonal import as_device
 
-        U = as_device(U)
+        U = as_device(getattr(U, "array", U))
         nb = U.ndim - len(self.shape_dofs)
         out = None
         for coef, ops in self.terms:
             X = U
             for ax, (B, _) in enumerate(ops):
-                X = K.apply_along(B, X, nb + ax, "f64")
-            X = pointwise_mul(X, coef)
+                X = K.apply_along(B, X.contiguous(), nb + ax, "f64")
+            X = pointwise_mul(X.contiguous(), coef)
             for ax, (_, T) in enumerate(ops):
-                X = K.apply_along(T, X, nb + ax, "f64")
+                X = K.apply_along(T, X.contiguous(), nb + ax, "f64")
+            out = X if out is None else K.lincomb([(1.0, out), (1.0, X)])
+        for tp in self.separable:
+            X = tp(U)
             out = X if out is None else K.lincomb([(1.0, out), (1.0, X)])
         return out
 
     __matmul__ = __call__
 
     def __add__(self, other):
-        return TensorMatrix(self.terms + other.terms, self.shape_dofs)
+        from ..la import TPMatrices, TPMatrix
+
+        if isinstance(other, TensorMatrix):
+            return TensorMatrix(self.terms + other.terms, self.shape_dofs, self.separable + other.separable)
+        if isinstance(other, TPMatrix):
+            return TensorMatrix(self.terms, self.shape_dofs, self.separable + [other])
+        if isinstance(other, TPMatrices):
+            return TensorMatrix(self.terms, self.shape_dofs, self.separable + list(other.tpmats))
+        return NotImplemented
+
+    __radd__ = __add__
+
+    def todense(self) -> torch.Tensor:
+        """The (n0 n1 x n0 n1) matrix, column j = the operator applied to e_j."""
+        n = self.shape[0]
+        E = torch.eye(n, dtype=torch.float64, device=K.device()).reshape((n,) + self.shape_dofs)
+        Y = self(E)
+        return Y.reshape(n, n).T.contiguous()
+
+    def todense_host(self) -> np.ndarray:
+        return self.todense().cpu().numpy()
+
+    def solve(self, rhs, axis: int = 0):
+        """Dense LU solve (la/tensormatrix.py:76-80); rocSOLVER factorisation."""
+        from .orthogonal import as_device
+
+        if self._lu is None:
+            self._lu = torch.linalg.lu_factor(self.todense())
+        b = as_device(rhs)
+        LU, piv = self._lu
+        if b.is_complex():
+            LU = LU.to(torch.complex128)
+        nb = b.ndim - len(self.shape_dofs)
+        flat = b.reshape((-1, self.shape[0])).T.contiguous()
+        x = torch.linalg.lu_solve(LU, piv, flat)
+        return x.T.reshape(b.shape).contiguous()
 
 
 def pointwise_mul(X: torch.Tensor, C: torch.Tensor) -> torch.Tensor:
@@ -202,6 +308,24 @@ def pointwise_mul(X: torch.Tensor, C: torch.Tensor) -> torch.Tensor:
 
 
 # ----------------------------------------------------------------------------- inner
+def _jaxfunction_parts(jf):
+    """(space, fixed coefficients) pairs that make up a JAXFunction: the
+    homogeneous part with its array, and for a lifted space every lifting part
+    with its boundary coefficients (the expansion of JAXFunction.doit over a
+    DirectSum / DirectSumTPS, galerkin/arguments.py:785-800)."""
+    from .composite import DirectSum
+    from .tensorproductspace import DirectSumTPS
+
+    V = jf.functionspace
+    if isinstance(V, DirectSum):
+        return [(V[0], jf.array), (V[1], torch.tensor(V[1].bnd_vals(), dtype=torch.float64, device=K.device()))]
+    if isinstance(V, DirectSumTPS):
+        out = [(V.get_homogeneous(), jf.array)]
+        out += [(V.tpspaces[f], arr) for f, arr in V.bndvals.items()]
+        return out
+    return [(V, jf.array)]
+
+
 def _lift_of(space):
     """Fixed coefficients of a lifting trial space (1D BCGeneric: its boundary
     values; a tensor product with a lifting factor: its ``bndvals``), else None."""
@@ -260,9 +384,17 @@ def inner(expr, sparse: bool = False, sparse_tol: int = 1000, num_quad_points=No
             # 493-562, 445-490)
             lift = _lift_of(Uspace)
             sep = _separate(t.coeff, symbols)
+            if sep is None and d == 1:
+                # one axis: the coefficient's values weight the quadrature
+                wq = _mesh_values(spaces, symbols, t.coeff, Nq)
+                m = bilinear_1d(spaces[0], t.dv.get(symbols[0], 0), uspaces[0], t.du.get(symbols[0], 0),
+                                sp.S.One, Nq[0], False, weight=wq)
+                if lift is not None:
+                    bvec = accumulate(bvec, -1.0, m.matvec(lift.contiguous(), 0).contiguous())
+                    continue
+                amats.append((1.0, [m]))
+                continue
             if sep is None:
-                if d != 2:
-                    raise NotImplementedError("non-separable coefficients are built for 2D spaces")
                 coef = _mesh_values(spaces, symbols, t.coeff, Nq)
                 ops = []
                 for ax in range(d):
@@ -290,6 +422,29 @@ def inner(expr, sparse: bool = False, sparse_tol: int = 1000, num_quad_points=No
                 bvec = accumulate(bvec, -c, X.contiguous())
                 continue
             amats.append((c, mats))
+        elif t.jf is not None:
+            # A @ array for the linear JAXFunction (its lifting parts included)
+            sign = -1.0 if has_bilinear else 1.0
+            sep = _separate(t.coeff, symbols)
+            for P, arr in _jaxfunction_parts(t.jf):
+                pspaces = P.basespaces if isinstance(P, TensorProductSpace) else [P]
+                if sep is None:
+                    coef = _mesh_values(spaces, symbols, t.coeff, Nq)
+                    ops = [(pspaces[ax].backward_operator(Nq[ax], t.dj.get(symbols[ax], 0)),
+                            test_operator(spaces[ax], t.dv.get(symbols[ax], 0), Nq[ax])) for ax in range(d)]
+                    X = TensorMatrix([(coef, ops)], tuple(s_.dim for s_ in spaces))(arr)
+                    bvec = accumulate(bvec, sign, X.contiguous())
+                    continue
+                c = complex(sep.get("coeff", 1))
+                X = arr
+                for ax in range(d):
+                    s_ = symbols[ax]
+                    m = bilinear_1d(spaces[ax], t.dv.get(s_, 0), pspaces[ax], t.dj.get(s_, 0),
+                                    sp.sympify(sep.get(s_, sp.S.One)), Nq[ax], use_precomputed_matrices)
+                    X = m.matvec(X.contiguous(), ax)
+                if c.imag:
+                    raise NotImplementedError("complex scalar coefficients")
+                bvec = accumulate(bvec, sign * c.real, X.contiguous())
         else:
             coef = _mesh_values(spaces, symbols, t.coeff, Nq)
             X = coef
@@ -315,10 +470,8 @@ def inner(expr, sparse: bool = False, sparse_tol: int = 1000, num_quad_points=No
                 tps.append(TPMatrix(mats, c))
             A = tps[0] if len(tps) == 1 else TPMatrices(tps)
     if multivar:
-        TM = TensorMatrix(multivar, tuple(s.dim for s in spaces))
-        if A is not None:
-            raise NotImplementedError("mixing separable and non-separable bilinear terms")
-        A = TM
+        tps = [] if A is None else (list(A.tpmats) if isinstance(A, TPMatrices) else [A])
+        A = TensorMatrix(multivar, tuple(s.dim for s in spaces), tps)
 
     if A is None and bvec is None:
         raise ValueError("No bilinear or linear forms found in expression")

@@ -25,6 +25,13 @@ def _host(a) -> np.ndarray:
     return np.asarray(a, dtype=np.float64)
 
 
+def _coefficients(x):
+    """The coefficient array of a JAXFunction, else x itself."""
+    from ..galerkin.arguments import JAXFunction
+
+    return x.array if isinstance(x, JAXFunction) else x
+
+
 class BaseMatrix:
     """Common interface (la/matrixprotocol.py:68-242)."""
 
@@ -81,7 +88,14 @@ class BaseMatrix:
     def __matmul__(self, other):
         if isinstance(other, BaseMatrix):
             return type(self)._combine(self, self._host_data @ other._host_data, other)
-        return self.matvec(other, 0)
+        return self.matvec(_coefficients(other), 0)
+
+    def __rmatmul__(self, other):
+        """``x @ A``: A^T applied along the last axis of x."""
+        from ..galerkin.orthogonal import as_device
+
+        x = as_device(_coefficients(other))
+        return K.apply_along(self.T.operator(), x, x.ndim - 1, "f64")
 
     @staticmethod
     def _combine(a, data, b=None):

@@ -54,7 +54,9 @@ class TPMatrix:
 
     def _matmul_array(self, w, precision: str = "f64") -> torch.Tensor:
         """A0 along axis 0, A1 along axis 1, ... (la/tpmatrix.py:152-156)."""
-        w = _as_dev(w, K.io_dtype(precision))
+        from .matrix import _coefficients
+
+        w = _as_dev(_coefficients(w), K.io_dtype(precision))
         nb = w.ndim - self.dims
         if self.dims == 2 and not w.is_complex() and \
                 K.plane_ok(self.mats[0].operator(), self.mats[1].operator(), precision):
