@@ -132,6 +132,21 @@ int jfx_band_lu_factor_batched(const double* band, int nsys, int n, int p, int q
 int jfx_band_lu_solve_batched(const double* lu, int nsys, int n, int p, int q, double* X, long long P, long long Q,
                               int qdiv, hipStream_t stream);
 
+/* Complex (re/im interleaved) versions of the two calls above for
+ * per-wavenumber systems with complex weights (odd derivative orders on a
+ * Fourier axis): band (nsys, n, p+q+1) complex; X (P, n, Q) complex, line
+ * (pp, qq) using factor (pp * Q + qq) mod nsys (la/tpmatrix.py:590-1011). */
+int jfx_band_lu_factor_batched_z(const double* band, int nsys, int n, int p, int q, double* lu, int* status_dev,
+                                 hipStream_t stream);
+int jfx_band_lu_solve_batched_z(const double* lu, int nsys, int n, int p, int q, double* X, long long P, long long Q,
+                                hipStream_t stream);
+
+/* Y = X * D, complex (re/im interleaved), D (size complex values) broadcast
+ * over the leading batch: the diagonal solve of TPMatrix / TPMatrices with
+ * complex diagonal factors (la/tpmatrix.py:178-213). */
+int jfx_cmul_bcast(const double* X, const double* D, double* Y, long long total, long long size,
+                   hipStream_t stream);
+
 /* ---- FD eigenvalue division: Y = X / D with D (size) broadcast over the leading batch: the
  * eigenvalue division of TPMatricesLUFactors.solve (la/tpmatrix.py:565-579). */
 int jfx_div_bcast_f64(const double* X, const double* D, double* Y, long long total, long long size,

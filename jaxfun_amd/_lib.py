@@ -46,6 +46,9 @@ SIGNATURES: dict[str, tuple] = {
     "jfx_cplx_combine": (c_int, [c_vp, c_vp, c_ll, c_ll, c_ll, c_int, c_vp]),
     "jfx_fourier_basis": (c_int, [c_vp, c_int, c_int, c_int, c_dbl, c_int, c_int, c_dbl, c_vp, c_vp]),
     "jfx_band_lu_factor_batched": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp]),
+    "jfx_band_lu_factor_batched_z": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp]),
+    "jfx_band_lu_solve_batched_z": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_vp, c_ll, c_ll, c_vp]),
+    "jfx_cmul_bcast": (c_int, [c_vp, c_vp, c_vp, c_ll, c_ll, c_vp]),
     "jfx_band_lu_solve_batched": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_vp, c_ll, c_ll, c_int, c_vp]),
     "jfx_lincomb": (c_int, [c_int, c_int, c_vp, c_vp, c_vp, c_ll, c_int, c_vp]),
     "jfx_cast": (c_int, [c_int, c_vp, c_vp, c_ll, c_vp]),

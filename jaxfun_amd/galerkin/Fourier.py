@@ -153,21 +153,21 @@ class Fourier(OrthogonalSpace):
 
     # ------------------------------------------------------------ matrices
     def _matrices(self, i, trial, q=0):
-        """diag((i k)^j (-i k)^i 2 pi) (galerkin/Fourier.py:237-264); the
-        real (even i + j) cases."""
+        """diag((i k)^j (-i k)^i 2 pi) (galerkin/Fourier.py:237-264): real for
+        even i + j, complex otherwise."""
         from ..la import DiaMatrix
 
         u, j = trial
         if not isinstance(u, Fourier) or q != 0:
             return None
-        if (i + j) % 2:
-            raise NotImplementedError("odd total derivative order on a Fourier axis (complex matrix)")
-        k = self.wavenumbers().astype(float)
-        ku = u.wavenumbers().astype(float)
+        k = self.wavenumbers().astype(np.complex128)
+        ku = u.wavenumbers().astype(np.complex128)
         n = min(self.N, u.N)
-        sign = (-1.0) ** ((i + j) // 2 + i)
-        A = np.zeros((self.N, u.N))
-        A[np.arange(n), np.arange(n)] = sign * k[:n] ** i * ku[:n] ** j * TWO_PI
+        d = (1j * ku[:n]) ** j * (-1j * k[:n]) ** i * TWO_PI
+        if (i + j) % 2 == 0:
+            d = d.real
+        A = np.zeros((self.N, u.N), dtype=d.dtype)
+        A[np.arange(n), np.arange(n)] = d
         return DiaMatrix.from_dense(A)
 
     def stencil_matrix(self):

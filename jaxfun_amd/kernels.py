@@ -164,6 +164,17 @@ def add_bcast(X: torch.Tensor, D: torch.Tensor, alpha: float = 1.0,
     return out
 
 
+def cmul_bcast(X: torch.Tensor, D: torch.Tensor) -> torch.Tensor:
+    """``Y[b, ...] = X[b, ...] * D[...]`` (complex128) with D broadcast over leading dims."""
+    if not X.is_complex():
+        X = to_complex(X.contiguous())
+    _req(X, torch.complex128, "X")
+    _req(D, torch.complex128, "D")
+    Y = torch.empty_like(X)
+    _lib.call("jfx_cmul_bcast", _p(X), _p(D), _p(Y), X.numel(), D.numel(), _s())
+    return Y
+
+
 def to_complex(X: torch.Tensor) -> torch.Tensor:
     """Real FP64 array as complex128 (imaginary part zero) through the
     complex-combine kernel's real-input form."""
@@ -197,12 +208,17 @@ def band_lu_factor(band: torch.Tensor, p: int, q: int) -> torch.Tensor:
 
 
 def band_lu_factor_batched(band: torch.Tensor, p: int, q: int) -> torch.Tensor:
-    """Factor nsys band matrices (nsys, n, p+q+1) without pivoting, one lane each."""
-    _req(band, torch.float64, "band")
+    """Factor nsys band matrices (nsys, n, p+q+1) without pivoting, one lane
+    each (real FP64 or complex128)."""
     nsys, n = band.shape[0], band.shape[1]
     lu = torch.empty_like(band)
     st = torch.zeros(1, dtype=torch.int32, device=band.device)
-    _lib.call("jfx_band_lu_factor_batched", _p(band), nsys, n, p, q, _p(lu), _p(st), _s())
+    if band.is_complex():
+        _req(band, torch.complex128, "band")
+        _lib.call("jfx_band_lu_factor_batched_z", _p(band), nsys, n, p, q, _p(lu), _p(st), _s())
+    else:
+        _req(band, torch.float64, "band")
+        _lib.call("jfx_band_lu_factor_batched", _p(band), nsys, n, p, q, _p(lu), _p(st), _s())
     s = int(st.item())
     if s != 0:
         raise ValueError(f"Zero or non-finite pivot at row {s - 1} in a batched banded LU factorization")
@@ -218,6 +234,14 @@ def band_lu_solve_batched(lu: torch.Tensor, p: int, q: int, X: torch.Tensor, axi
         raise ValueError(f"band solve: axis {axis} has length {X.shape[axis]}, matrices are {n}")
     if (X.numel() // n) % nsys:
         raise ValueError(f"band solve: {X.numel() // n} lines is not a multiple of {nsys} systems")
+    if lu.is_complex():
+        if not X.is_complex():
+            raise TypeError("complex factors need a complex right-hand side")
+        _req(X, torch.complex128, "X")
+        P = math.prod(X.shape[:axis])
+        Q = math.prod(X.shape[axis + 1:])
+        _lib.call("jfx_band_lu_solve_batched_z", _p(lu), nsys, n, p, q, _p(X), P, Q, _s())
+        return X
     qdiv = 1
     Xr = X
     if X.is_complex():

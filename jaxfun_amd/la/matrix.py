@@ -20,9 +20,12 @@ class SolverNotApplicable(Exception):
 
 
 def _host(a) -> np.ndarray:
+    """Host FP64 copy; complex data (Fourier matrices with odd derivative
+    orders) stays complex128."""
     if isinstance(a, torch.Tensor):
-        return a.detach().to("cpu", torch.float64).numpy()
-    return np.asarray(a, dtype=np.float64)
+        a = a.detach().cpu().numpy()
+    a = np.asarray(a)
+    return a.astype(np.complex128 if np.iscomplexobj(a) else np.float64)
 
 
 def _coefficients(x):
@@ -49,8 +52,12 @@ class BaseMatrix:
         return self._host_data.shape
 
     @property
+    def is_complex(self) -> bool:
+        return np.iscomplexobj(self._host_data)
+
+    @property
     def dtype(self):
-        return torch.float64
+        return torch.complex128 if self.is_complex else torch.float64
 
     @property
     def size(self) -> int:
@@ -61,12 +68,12 @@ class BaseMatrix:
 
     def todense(self) -> torch.Tensor:
         if self._dev is None:
-            self._dev = torch.tensor(self._host_data, dtype=torch.float64, device=K.device())
+            self._dev = torch.tensor(self._host_data, dtype=self.dtype, device=K.device())
         return self._dev
 
-    def operator(self) -> K.AxisOperator:
+    def operator(self):
         if self._op is None:
-            self._op = K.AxisOperator(self.todense())
+            self._op = K.ComplexAxisOperator(self.todense()) if self.is_complex else K.AxisOperator(self.todense())
         return self._op
 
     def diagonal_or_none(self) -> np.ndarray | None:
@@ -145,8 +152,10 @@ class Matrix(BaseMatrix):
 
         b = as_device(b)
         LU, piv = self.lu_factor()
-        if b.is_complex():
+        if b.is_complex() and not LU.is_complex():
             LU = LU.to(torch.complex128)
+        elif LU.is_complex() and not b.is_complex():
+            b = K.to_complex(b)
         x = b.movedim(axis, 0).reshape(b.shape[axis], -1)
         y = torch.linalg.lu_solve(LU, piv, x)
         return y.reshape(b.movedim(axis, 0).shape).movedim(0, axis).contiguous()
@@ -188,7 +197,10 @@ class DiaMatrix(BaseMatrix):
         return out
 
     def lu_factor(self):
-        """No-pivot banded LU on the device (la/diamatrix.py:394-504)."""
+        """No-pivot banded LU on the device (la/diamatrix.py:394-504); a complex
+        matrix is factored densely (rocSOLVER)."""
+        if self.is_complex:
+            return Matrix.lu_factor(self)
         if self._lu is None:
             if self.shape[0] != self.shape[1]:
                 raise ValueError("lu_factor needs a square matrix")
@@ -201,6 +213,8 @@ class DiaMatrix(BaseMatrix):
         """Banded solve along an axis (la/diamatrix.py:592-770, 1880-1925)."""
         from ..galerkin.orthogonal import as_device
 
+        if self.is_complex:
+            return Matrix.solve(self, b, axis)
         lu = self.lu_factor()
         p, q = self.bandwidths
         x = as_device(b).clone()

@@ -118,6 +118,9 @@ class TPMatrix:
 
 def _solve_diagonal(d: np.ndarray, rhs) -> torch.Tensor:
     x = _as_dev(rhs)
+    if np.iscomplexobj(d):
+        Dinv = torch.tensor(np.ascontiguousarray(1.0 / np.asarray(d)), dtype=torch.complex128, device=x.device)
+        return K.cmul_bcast(x, Dinv)
     if x.is_complex():
         # the real view divides re and im by the same entry
         D2 = np.repeat(np.asarray(d, dtype=np.float64)[..., None], 2, axis=-1)
@@ -227,6 +230,8 @@ class TPMatricesWavenumberSolver:
 
     def solve(self, rhs, precision: str = "f64") -> torch.Tensor:
         x = _as_dev(rhs).clone()
+        if self.lu.is_complex() and not x.is_complex():
+            x = K.to_complex(x)
         nb = x.ndim - len(self.shape)
         if tuple(x.shape[nb:]) != self.shape:
             raise ValueError(f"rhs shape {tuple(x.shape)} does not end with {self.shape}")
@@ -237,7 +242,7 @@ def tpmats_wavenumber_factor(A) -> TPMatricesWavenumberSolver:
     """Factor a Fourier x polynomial sum of Kronecker products per wavenumber
     (la/tpmatrix.py:1236-1354).  Raises SolverNotApplicable unless exactly one
     axis is non-diagonal; complex wavenumber weights (odd derivative orders
-    on a Fourier axis) are not built and raise SolverNotApplicable too."""
+    on a Fourier axis) give complex bands and the complex kernels."""
     if isinstance(A, TPMatrices):
         tpmats = list(A.tpmats)
     elif isinstance(A, list):
@@ -264,9 +269,8 @@ def tpmats_wavenumber_factor(A) -> TPMatricesWavenumberSolver:
             w = np.outer(w, tp.mats[a].diagonal_or_none()).ravel()
         W.append(w)
     W = np.array(W)
-    if np.abs(W.imag).max() > 0:
-        raise SolverNotApplicable("complex wavenumber weights")
-    W = W.real
+    if np.abs(W.imag).max() == 0:
+        W = W.real
     dense = [tp.mats[pa].todense_host() for tp in tpmats]
     nz = np.zeros((n, n), dtype=bool)
     for P in dense:
@@ -275,7 +279,7 @@ def tpmats_wavenumber_factor(A) -> TPMatricesWavenumberSolver:
     p = int(max(0, (rows - cols).max())) if rows.size else 0
     q = int(max(0, (cols - rows).max())) if rows.size else 0
     Wd = p + q + 1
-    bands = np.zeros((len(tpmats), n, Wd))
+    bands = np.zeros((len(tpmats), n, Wd), dtype=np.result_type(*dense))
     for t, P in enumerate(dense):
         for o in range(-p, q + 1):
             i = np.arange(max(0, -o), min(n, n - o))

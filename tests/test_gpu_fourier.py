@@ -246,3 +246,72 @@ def test_fourier_lifted_legendre_poisson(dev):
     assert np.abs(uj - uej).max() < 1e-12
     # forward of the values returns the homogeneous coefficients
     assert O.rel_l2(_np(T.forward(T.backward(uh))), _np(uh)) < 1e-12
+
+
+def test_complex_band_and_cmul_kernels(dev):
+    import torch
+
+    from jaxfun_amd import kernels as K
+
+    rng = np.random.default_rng(4)
+    nsys, n, p, q = 5, 9, 1, 2
+    A = np.zeros((nsys, n, n), dtype=complex)
+    band = np.zeros((nsys, n, p + q + 1), dtype=complex)
+    for s in range(nsys):
+        for o in range(-p, q + 1):
+            i = np.arange(max(0, -o), min(n, n - o))
+            v = rng.standard_normal(i.size) + 1j * rng.standard_normal(i.size) + (6.0 if o == 0 else 0.0)
+            A[s, i, i + o] = v
+            band[s, i, o + p] = v
+    lu = K.band_lu_factor_batched(torch.tensor(band, device=dev), p, q)
+    X = rng.standard_normal((2, nsys, n)) + 1j * rng.standard_normal((2, nsys, n))
+    Y = _np(K.band_lu_solve_batched(lu, p, q, torch.tensor(X, device=dev), 2))
+    ref = np.stack([[np.linalg.solve(A[s], X[b, s]) for s in range(nsys)] for b in range(2)])
+    assert np.abs(Y - ref).max() < 1e-12
+    D = rng.standard_normal((3, 4)) + 1j * rng.standard_normal((3, 4))
+    Z = rng.standard_normal((2, 3, 4)) + 1j * rng.standard_normal((2, 3, 4))
+    got = _np(K.cmul_bcast(torch.tensor(Z, device=dev), torch.tensor(D, device=dev)))
+    assert np.allclose(got, Z * D[None], atol=1e-15)
+
+
+def test_fourier_first_derivative_1d(dev):
+    # u' + u = f on a periodic interval: complex diagonal matrices
+    from jaxfun_amd import Fourier, TestFunction, TrialFunction, inner
+
+    F = Fourier(16, domain=(0.0, 3.0))
+    x = F.system.x
+    u, v = TrialFunction(F), TestFunction(F)
+    s = 2 * sp.pi / 3
+    ue = sp.sin(s * x) + sp.cos(3 * s * x)
+    A, b = inner(v * (sp.diff(u, x) + u) - v * (sp.diff(ue, x) + ue), kind="system")
+    assert A.is_complex
+    uh = _np(A.solve(b))
+    Fo = O.Fourier1D(16, (0.0, 3.0))
+    ref = Fo.forward(sp.lambdify(x, ue)(Fo.mesh()))
+    assert np.abs(uh - ref).max() < 1e-13
+
+
+@pytest.mark.parametrize("poly", ["legendre", "chebyshev"])
+def test_fourier_poly_advection_diffusion(dev, poly):
+    # -Lap u + u_x = f: complex per-wavenumber weights (i k) on the Fourier axis
+    from jaxfun_amd import Chebyshev, Div, Fourier, FunctionSpace, Grad, Legendre, TensorProduct, TestFunction, \
+        TrialFunction, inner
+    from jaxfun_amd.la import TPMatricesWavenumberSolver, tpmats_to_kron
+
+    space = Legendre if poly == "legendre" else Chebyshev
+    N = 16
+    T = TensorProduct(FunctionSpace(N, Fourier), FunctionSpace(N, space, {"left": {"D": 0}, "right": {"D": 0}}))
+    v, u = TestFunction(T), TrialFunction(T)
+    x, y = T.system.base_scalars()
+    ue = (sp.cos(2 * x) + sp.sin(x) * y) * (1 - y**2)
+    A, b = inner(-v * Div(Grad(u)) + v * sp.diff(u, x) - v * (-Div(Grad(ue)) + sp.diff(ue, x)), sparse=True,
+                 kind="system")
+    wn = A.lu_factor()
+    assert isinstance(wn, TPMatricesWavenumberSolver) and wn.lu.is_complex()
+    uh = _np(A.solve(b))
+    bh = _np(b)
+    ref = np.linalg.solve(tpmats_to_kron(A.tpmats), bh.ravel()).reshape(bh.shape)
+    assert np.abs(uh - ref).max() < 1e-12 * max(1.0, np.abs(ref).max())
+    uj = _np(T.backward(uh, N=(24, 20)))
+    xj = [_np(m) for m in T.mesh(N=(24, 20))]
+    assert np.abs(uj - sp.lambdify((x, y), ue)(*xj)).max() < 1e-12
