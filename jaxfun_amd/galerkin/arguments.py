@@ -21,9 +21,11 @@ class _Data:
         self.value = value
 
 
-# JAXFunction instances by their Dummy argument, so that SymPy can rebuild
-# one from its arguments (``func(*args)``) without losing its coefficients
-_JAXF_DATA: dict = {}
+class _DataDummy(sp.Dummy):
+    """The extra argument of a JAXFunction: a Dummy that carries the
+    function's (array, space, name), so that SymPy can rebuild the function
+    from its arguments (``func(*args)``) without losing its coefficients,
+    and the data lives exactly as long as expressions that use it."""
 
 
 class JAXFunction(sp.Function):
@@ -45,7 +47,7 @@ class JAXFunction(sp.Function):
             # SymPy rebuilding the object from its arguments
             args = (array, V, name) + rest if V is not None else (array,)
             args = tuple(a for a in args if a is not None)
-            data = _JAXF_DATA[args[-1]]
+            data = args[-1].jaxf_data
             obj = sp.Function.__new__(cls, *args)
             obj.array, obj.functionspace, obj.name = data
             return obj
@@ -54,10 +56,10 @@ class JAXFunction(sp.Function):
 
         if isinstance(array, sp.Basic):
             array = project(array, V)
-        d = sp.Dummy()
+        d = _DataDummy()
         name = name if name is not None else "JAXFunction"
         arr = as_device(array)
-        _JAXF_DATA[d] = (arr, V, name)
+        d.jaxf_data = (arr, V, name)
         obj = sp.Function.__new__(cls, *(tuple(V.system.base_scalars()) + (d,)))
         obj.array, obj.functionspace, obj.name = arr, V, name
         return obj

@@ -275,12 +275,17 @@ class TensorMatrix:
 
     __radd__ = __add__
 
-    def todense(self) -> torch.Tensor:
-        """The (n0 n1 x n0 n1) matrix, column j = the operator applied to e_j."""
+    def todense(self, chunk: int = 4096) -> torch.Tensor:
+        """The (n0 n1 x n0 n1) matrix, column j = the operator applied to e_j
+        (unit vectors applied ``chunk`` at a time as one batch)."""
         n = self.shape[0]
-        E = torch.eye(n, dtype=torch.float64, device=K.device()).reshape((n,) + self.shape_dofs)
-        Y = self(E)
-        return Y.reshape(n, n).T.contiguous()
+        cols = []
+        for j0 in range(0, n, chunk):
+            m = min(chunk, n - j0)
+            E = torch.zeros((m, n), dtype=torch.float64, device=K.device())
+            E[torch.arange(m, device=E.device), torch.arange(j0, j0 + m, device=E.device)] = 1.0
+            cols.append(self(E.reshape((m,) + self.shape_dofs)).reshape(m, n))
+        return torch.cat(cols, 0).T.contiguous()
 
     def todense_host(self) -> np.ndarray:
         return self.todense().cpu().numpy()
