@@ -203,7 +203,7 @@ def _as_number(v):
     e = sp.sympify(v)
     if e.free_symbols:
         raise ValueError(f"boundary value {v} depends on {e.free_symbols}; a 1D space needs numbers")
-    return float(e)
+    return complex(e) if e.has(sp.I) else float(e)
 
 
 class BCGeneric(Composite):
@@ -244,7 +244,8 @@ class BCGeneric(Composite):
 
     def bnd_vals(self) -> np.ndarray:
         """Ordered boundary values as numbers (galerkin/composite.py:459-466)."""
-        return np.array([_as_number(v) for v in self.bcs.orderedvals()], dtype=np.float64)
+        vals = [_as_number(v) for v in self.bcs.orderedvals()]
+        return np.array(vals, dtype=np.complex128 if any(isinstance(v, complex) for v in vals) else np.float64)
 
     def get_homogeneous(self):
         raise NotImplementedError("BCGeneric does not support get_homogeneous()")
@@ -373,9 +374,9 @@ class DirectSum:
         (galerkin/composite.py:588-593)."""
         if "lift" not in self._ops:
             cb = self.bnd_vals() @ self[1].S
-            full = np.zeros(self.N)
+            full = np.zeros(self.N, dtype=cb.dtype)
             full[: cb.shape[0]] = cb
-            self._ops["lift"] = torch.tensor(full, dtype=torch.float64, device=K.device())
+            self._ops["lift"] = torch.tensor(full, device=K.device())
         return self._ops["lift"]
 
     def lift_values(self, N: int | None = None, k: int = 0, kind: str = "quadrature") -> torch.Tensor:

@@ -323,7 +323,7 @@ def _jaxfunction_parts(jf):
 
     V = jf.functionspace
     if isinstance(V, DirectSum):
-        return [(V[0], jf.array), (V[1], torch.tensor(V[1].bnd_vals(), dtype=torch.float64, device=K.device()))]
+        return [(V[0], jf.array), (V[1], torch.tensor(V[1].bnd_vals(), device=K.device()))]
     if isinstance(V, DirectSumTPS):
         out = [(V.get_homogeneous(), jf.array)]
         out += [(V.tpspaces[f], arr) for f, arr in V.bndvals.items()]
@@ -335,7 +335,7 @@ def _lift_of(space):
     """Fixed coefficients of a lifting trial space (1D BCGeneric: its boundary
     values; a tensor product with a lifting factor: its ``bndvals``), else None."""
     if isinstance(space, BCGeneric):
-        return torch.tensor(space.bnd_vals(), dtype=torch.float64, device=K.device())
+        return torch.tensor(space.bnd_vals(), device=K.device())
     return getattr(space, "bnd_coefficients", None)
 
 

@@ -200,9 +200,11 @@ def _project_1d(expr, V) -> torch.Tensor:
     if e.free_symbols:
         vals = lambdify(V.system.base_scalars(), e)(pts)
     else:
-        vals = float(e)
-    vals = np.broadcast_to(np.asarray(vals, dtype=np.float64), pts.shape).copy()
-    return V.forward(torch.tensor(vals, dtype=torch.float64, device=K.device()))
+        vals = complex(e) if e.has(sp.I) else float(e)
+    vals = np.asarray(vals)
+    dt = np.complex128 if np.iscomplexobj(vals) else np.float64
+    vals = np.broadcast_to(vals.astype(dt), pts.shape).copy()
+    return V.forward(torch.tensor(vals, device=K.device()))
 
 
 def _lifting_bcs(bcspace: BCGeneric, values) -> BCGeneric:
@@ -217,21 +219,25 @@ def _lifting_bcs(bcspace: BCGeneric, values) -> BCGeneric:
 
 
 class DirectSumTPS(TensorProductSpace):
-    """2D tensor product with one or two DirectSum factors
-    (galerkin/tensorproductspace.py:560-851).
+    """Tensor product with DirectSum factors (galerkin/tensorproductspace.py:560-851):
+    2D with one or two, 3D with one or two (and a plain or homogeneous third axis).
 
     The trial space splits into the tensor products of every combination of
-    homogeneous and lifting summands (``tpspaces``).  Each product with a
-    lifting factor carries a fixed coefficient array (``bndvals``): the
-    boundary functions of the other coordinate projected onto the other
-    axis's space; with two DirectSums the boundary functions are themselves
-    lifted by their corner values and the product of both lifting spaces
-    holds the four corner values.  Their sum, padded into the orthogonal
-    basis, is a constant lift (``lift_coefficients``) that ``to_orthogonal``
-    adds and ``from_orthogonal`` removes; ``backward``/``evaluate_mesh``/
-    ``backward_primitive`` add its values on the requested mesh (cached).
-    Coefficient arrays hold the homogeneous part only and may carry leading
-    batch dimensions.
+    homogeneous and lifting summands (``tpspaces``).  Each product with
+    lifting factors on the axes L carries a fixed coefficient array
+    (``bndvals``): for every choice of boundary (left/right) on the axes in L,
+    the boundary function of the first axis in L, evaluated at the chosen ends
+    of the other axes in L, projected onto the remaining axes' spaces — where a
+    remaining DirectSum axis is itself lifted by that function's values at its
+    ends (for one remaining axis a 1D DirectSum, for two a 2D DirectSumTPS).
+    In 2D this is: boundary functions of the other coordinate projected onto
+    the other axis (lifted by their corner values when both axes are lifted),
+    and the four corner values for the product of both lifting spaces.
+    Their sum, padded into the orthogonal basis, is a constant lift
+    (``lift_coefficients``) that ``to_orthogonal`` adds and ``from_orthogonal``
+    removes; ``backward``/``evaluate_mesh``/``backward_primitive`` add its
+    values on the requested mesh (cached).  Coefficient arrays hold the
+    homogeneous part only and may carry leading batch dimensions.
     """
 
     def __init__(self, basespaces, system: CartCoordSys | None = None, name: str = "DSTPS",
@@ -239,52 +245,71 @@ class DirectSumTPS(TensorProductSpace):
         super().__init__(basespaces, system, name, precision)
         if precision != "f64":
             raise ValueError("inhomogeneous (lifted) tensor-product spaces transform in f64 only")
-        if len(self.basespaces) != 2:
-            raise NotImplementedError("DirectSumTPS is built for 2D tensor products")
+        d = len(self.basespaces)
+        nds = sum(isinstance(s, DirectSum) for s in self.basespaces)
+        if d > 3 or nds > 2:
+            raise NotImplementedError("DirectSumTPS is built for 2D and 3D products with at most two DirectSums")
         self._ops: dict = {}
-        ds = [i for i, s in enumerate(self.basespaces) if isinstance(s, DirectSum)]
-        two = len(ds) == 2
-        # boundary functions of each DirectSum evaluated at the other axis's ends
-        projected: dict[int, list] = {}
-        if two:
-            for ax in (0, 1):
-                other = self.basespaces[1 - ax]
-                s_other = other.system.base_scalars()[0]
-                ends = (other.domain.lower, other.domain.upper)
-                rows = []
-                for g in self.basespaces[ax][1].bcs.orderedvals():
-                    g = sp.sympify(g)
-                    rows.append([_num_or_expr(g.subs(s_other, z)) for z in ends])
-                projected[ax] = rows
         self.tpspaces: dict[tuple, TensorProductSpace] = {}
         choices = [list(s.basespaces) if isinstance(s, DirectSum) else [s] for s in self.basespaces]
         for i, combo in enumerate(itertools.product(*choices)):
             self.tpspaces[combo] = TensorProductSpace(combo, self.system, f"{name}{i}", precision)
         self.bndvals: dict[tuple, torch.Tensor] = {}
         for combo, tps in self.tpspaces.items():
-            bc_axes = [a for a, s in enumerate(combo) if isinstance(s, BCGeneric)]
-            if not bc_axes:
+            L = [a for a, s in enumerate(combo) if isinstance(s, BCGeneric)]
+            if not L:
                 continue
-            if len(bc_axes) == 2:
-                # corner values u(x_j, y_k) = g_j(y_k)
-                arr = torch.tensor(np.array(projected[0], dtype=np.float64), device=K.device())
-            else:
-                a = bc_axes[0]
-                o = 1 - a
-                other = combo[o]
-                rows = []
-                for j, g in enumerate(combo[a].bcs.orderedvals()):
-                    V = other
-                    if two:
-                        V = DirectSum(other, _lifting_bcs(self.basespaces[o][1], projected[a][j]))
-                    rows.append(_project_1d(g, V))
-                arr = torch.stack(rows)
-                if a == 1:
-                    arr = arr.T
-            arr = arr.contiguous()
+            arr = self._boundary_coefficients(combo, L).contiguous()
             self.bndvals[combo] = arr
             tps.bnd_coefficients = arr
         self.orthogonal = self.get_orthogonal()
+
+    def _boundary_coefficients(self, combo, L) -> torch.Tensor:
+        d = len(self.basespaces)
+        R = [a for a in range(d) if a not in L]
+        syms = [s.system.base_scalars()[0] for s in self.basespaces]
+        ends = [(s.domain.lower, s.domain.upper) for s in self.basespaces]
+        nb = [combo[a].dim for a in L]
+        blocks = {}
+        for js in itertools.product(*[range(n) for n in nb]):
+            g = sp.sympify(combo[L[0]].bcs.orderedvals()[js[0]])
+            for a, j in zip(L[1:], js[1:]):
+                g = g.subs(syms[a], ends[a][j])
+            blocks[js] = self._project_remaining(g, combo, R, syms, ends)
+        cplx = any(isinstance(b, complex) or (isinstance(b, torch.Tensor) and b.is_complex())
+                   for b in blocks.values())
+        shape = [2 if a in L else combo[a].dim for a in range(d)]
+        out = torch.zeros(shape, dtype=torch.complex128 if cplx else torch.float64, device=K.device())
+        for js, b in blocks.items():
+            idx = tuple(js[L.index(a)] if a in L else slice(None) for a in range(d))
+            out[idx] = b if isinstance(b, torch.Tensor) else torch.tensor(b, dtype=out.dtype, device=out.device)
+        return out
+
+    def _project_remaining(self, g, combo, R, syms, ends):
+        """Coefficients of g (a function of the coordinates of R) in the
+        product of the R axes' spaces, each remaining DirectSum axis lifted by
+        g's values at its ends."""
+        if not R:
+            return _num_or_expr(g)
+        spaces = []
+        for r in R:
+            if isinstance(self.basespaces[r], DirectSum):
+                vals = [_num_or_expr(g.subs(syms[r], z)) for z in ends[r]]
+                spaces.append(DirectSum(combo[r], _lifting_bcs(self.basespaces[r][1], vals)))
+            else:
+                spaces.append(combo[r])
+        if len(spaces) == 1:
+            return _project_1d(g, spaces[0])
+        sub = CartCoordSys(self.system.name + "_r", tuple(syms[r] for r in R))
+        V = DirectSumTPS(spaces, sub) if any(isinstance(v, DirectSum) for v in spaces) else \
+            TensorProductSpace(spaces, sub)
+        grids = [v.mesh().cpu().numpy() for v in spaces]
+        mesh = np.meshgrid(*grids, indexing="ij")
+        e = sp.sympify(g)
+        vals = lambdify(tuple(syms[r] for r in R), e)(*mesh) if e.free_symbols else complex(e)
+        vals = np.broadcast_to(np.asarray(vals), mesh[0].shape)
+        dt = torch.complex128 if np.iscomplexobj(vals) else torch.float64
+        return V.forward(torch.tensor(np.ascontiguousarray(vals), dtype=dt, device=K.device()))
 
     # ------------------------------------------------------------ parts
     def get_homogeneous(self) -> TensorProductSpace:
@@ -314,14 +339,15 @@ class DirectSumTPS(TensorProductSpace):
             tot = torch.zeros(shape, dtype=torch.complex128 if cplx else torch.float64, device=K.device())
             for combo, arr in self.bndvals.items():
                 a = self.tpspaces[combo].to_orthogonal(arr)
-                tot[: a.shape[0], : a.shape[1]] += a
+                tot[tuple(slice(0, n) for n in a.shape)] += a
             self._ops["lift"] = tot.contiguous()
         return self._ops["lift"]
 
-    def lift_values(self, N=None, k=(0, 0), kind: str = MeshKind.QUADRATURE) -> torch.Tensor:
+    def lift_values(self, N=None, k=None, kind: str = MeshKind.QUADRATURE) -> torch.Tensor:
         N = tuple(s.num_quad_points if N is None or N[i] is None else N[i]
                   for i, s in enumerate(self.basespaces))
-        key = ("liftv", N, tuple(k), kind)
+        k = (0,) * len(self.basespaces) if k is None else tuple(k)
+        key = ("liftv", N, k, kind)
         if key not in self._ops:
             ops = [s.orthogonal.backward_operator(N[i], k[i], kind) for i, s in enumerate(self.basespaces)]
             X = self.lift_coefficients()
@@ -348,16 +374,16 @@ class DirectSumTPS(TensorProductSpace):
         as the homogeneous backward plus the lift's values on the mesh."""
         self._f64(precision)
         u = self.get_homogeneous().backward(c, N).contiguous()
-        return K.add_bcast(u, self.lift_values(tuple(u.shape[-2:])))
+        return K.add_bcast(u, self.lift_values(tuple(u.shape[u.ndim - len(self):])))
 
     def backward_primitive(self, c, k, N=None, precision: str | None = None):
         self._f64(precision)
         u = self.get_homogeneous().backward_primitive(c, k, N).contiguous()
-        return K.add_bcast(u, self.lift_values(tuple(u.shape[-2:]), k))
+        return K.add_bcast(u, self.lift_values(tuple(u.shape[u.ndim - len(self):]), k))
 
     def evaluate_mesh(self, c, kind: str = MeshKind.QUADRATURE, N=None):
         u = self.get_homogeneous().evaluate_mesh(c, kind, N).contiguous()
-        return K.add_bcast(u, self.lift_values(tuple(u.shape[-2:]), (0, 0), kind))
+        return K.add_bcast(u, self.lift_values(tuple(u.shape[u.ndim - len(self):]), None, kind))
 
     def forward(self, u, precision: str | None = None):
         """from_orthogonal(orthogonal.forward(u)) (tensorproductspace.py:778-780)."""

@@ -131,6 +131,25 @@ class BaseMatrix:
     def to_matrix(self) -> "Matrix":
         return Matrix(self._host_data)
 
+    def pin(self, constraints) -> "PinnedMatrix":
+        """Fix degrees of freedom (la/diamatrix.py:1410-1500): each (row, value)
+        replaces its row by the identity row and the right-hand side entry by
+        the value at solve time (negative rows count from the end)."""
+        items = sorted(dict(constraints).items()) if isinstance(constraints, dict) else list(constraints)
+        n = self.shape[0]
+        A = self._host_data.copy()
+        norm = []
+        for i, val in items:
+            i = int(i)
+            if i < 0:
+                i += n
+            if not 0 <= i < n:
+                raise IndexError(f"pinned row {i} out of range for a matrix of size {n}")
+            A[i, :] = 0.0
+            A[i, i] = 1.0
+            norm.append((i, val))
+        return PinnedMatrix(type(self)._combine(self, A), tuple(norm))
+
     def tosparse(self, tol: int = 100) -> "DiaMatrix":
         return DiaMatrix.from_dense(self._host_data, tol=tol)
 
@@ -221,3 +240,28 @@ class DiaMatrix(BaseMatrix):
         return K.band_lu_solve(lu, p, q, x, axis % x.ndim)
 
     lu_solve = solve
+
+
+class PinnedMatrix:
+    """A matrix with pinned degrees of freedom (la/pinned.py): ``solve``
+    writes the pinned values into a copy of the right-hand side and solves."""
+
+    def __init__(self, matrix: BaseMatrix, constraints):
+        self.matrix = matrix
+        self.constraints = tuple(constraints)
+
+    @property
+    def shape(self):
+        return self.matrix.shape
+
+    def solve(self, b, axis: int = 0):
+        from ..galerkin.orthogonal import as_device
+
+        x = as_device(_coefficients(b)).clone()
+        xm = x.movedim(axis, 0)
+        for i, val in self.constraints:
+            xm[i] = val
+        return self.matrix.solve(x, axis)
+
+    def __matmul__(self, other):
+        return self.matrix @ other

@@ -195,3 +195,23 @@ def test_fused_and_unfused_transforms_agree(dev):
     T.fused = False
     u2 = _np(T.backward(c))
     assert O.rel_l2(u1, u2) < 1e-14
+
+
+def test_poisson3d_example(dev):
+    # examples/poisson3D.py: Legendre-Dirichlet M = 20, scaling n + 1, method="lu"
+    from jaxfun_amd import Div, FunctionSpace, Grad, Legendre, TensorProduct, TestFunction, TrialFunction, inner, \
+        n, ulp
+
+    M = 20
+    D = FunctionSpace(M, Legendre, {"left": {"D": 0}, "right": {"D": 0}}, scaling=n + 1, name="D", fun_str="psi")
+    T = TensorProduct(D, D, D, name="T")
+    v, u = TestFunction(T), TrialFunction(T)
+    x, y, z = T.system.base_scalars()
+    ue = (1 - x**2) * (1 - y**2) * (1 - z**2)
+    A, b = inner(v * Div(Grad(u)) - v * Div(Grad(ue)), sparse=True, kind="system")
+    uh = A.solve(b, method="lu")
+    uj = _np(T.evaluate_mesh(uh, kind="uniform", N=(20, 20, 20)))
+    xj = [_np(m) for m in T.mesh(kind="uniform", N=(20, 20, 20))]
+    uej = sp.lambdify((x, y, z), ue)(*xj)
+    error = np.linalg.norm(uj - uej) / np.sqrt(T.dim)
+    assert error < ulp(1000), error

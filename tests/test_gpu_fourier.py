@@ -315,3 +315,19 @@ def test_fourier_poly_advection_diffusion(dev, poly):
     uj = _np(T.backward(uh, N=(24, 20)))
     xj = [_np(m) for m in T.mesh(N=(24, 20))]
     assert np.abs(uj - sp.lambdify((x, y), ue)(*xj)).max() < 1e-12
+
+
+def test_poisson1d_periodic_example(dev):
+    # examples/poisson1D_periodic.py: complex manufactured solution, the k = 0
+    # mode pinned
+    from jaxfun_amd import Div, Fourier, FunctionSpace, Grad, TestFunction, TrialFunction, inner, ulp
+
+    D = FunctionSpace(10, Fourier, name="D", fun_str="psi", domain=(-2 * np.pi, 2 * np.pi))
+    v, u = TestFunction(D), TrialFunction(D)
+    x = D.system.x
+    ue = sp.cos(2 * x) + sp.I * sp.sin(1 * x)
+    A, b = inner(v * Div(Grad(u)) - v * Div(Grad(ue)), sparse=True, kind="system")
+    uh = A.pin({0: 0.0}).solve(b)
+    uj = _np(D.backward(uh))
+    uej = sp.lambdify(x, ue)(_np(D.mesh()))
+    assert np.linalg.norm(uj - uej) < ulp(1000)

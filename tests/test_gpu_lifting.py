@@ -328,3 +328,61 @@ def test_multivar_coefficient_with_lift(dev):
     # ue lies in the lifted space: its homogeneous coefficients satisfy A u_h = b
     uh = Lo.forward(uq)
     assert O.rel_l2(_np(A(uh)), _np(b)) < 1e-12
+
+
+@pytest.mark.parametrize("lifted", [(2,), (1, 2)])
+def test_directsumtps_3d_polynomial(dev, lifted):
+    # 3D products with one or two lifted axes: the boundary data of the lifted
+    # axes (functions of the other two coordinates) are projected onto the
+    # remaining axes, recursively lifted; a polynomial solution is reproduced
+    from jaxfun_amd import Div, FunctionSpace, Grad, Legendre, TensorProduct, TestFunction, TrialFunction, inner
+    from jaxfun_amd.coordinates import x, y, z
+    from jaxfun_amd.galerkin import DirectSumTPS
+
+    # y^2 z^2 is not reproduced by the lift, so the homogeneous part is not zero
+    ue = (1 - x**2) * (1 + y + x * y**2 + z**2 * y**2 + x * z**3)
+    if 1 not in lifted:
+        ue = ue * (1 - y**2)
+    syms = (x, y, z)
+    spaces = []
+    for ax in range(3):
+        if ax in lifted:
+            s_ = syms[ax]
+            bcs = {"left": {"D": ue.subs(s_, -1)}, "right": {"D": ue.subs(s_, 1)}}
+        else:
+            bcs = {"left": {"D": 0}, "right": {"D": 0}}
+        spaces.append(FunctionSpace(10, Legendre, bcs))
+    T = TensorProduct(*spaces)
+    assert isinstance(T, DirectSumTPS) and len(T.tpspaces) == 2 ** len(lifted)
+    v, u = TestFunction(T), TrialFunction(T)
+    A, b = inner(v * Div(Grad(u)) - v * Div(Grad(ue)), sparse=True, kind="system")
+    uh = A.solve(b)
+    uj = _np(T.backward(uh))
+    xj = [_np(m) for m in T.mesh()]
+    assert np.abs(uj - sp.lambdify((x, y, z), ue)(*xj)).max() < 1e-11
+    assert np.abs(_np(uh)).max() > 1e-3
+    assert np.abs(_np(T.forward(T.backward(uh))) - _np(uh)).max() < 1e-12
+
+
+def test_directsumtps_fourier_3d(dev):
+    # tests/galerkin/test_tensorproductspace_extra.py:42-71: Fourier x lifted
+    # Chebyshev x lifted Chebyshev, complex boundary data
+    from jaxfun_amd import Chebyshev, Div, Fourier, FunctionSpace, Grad, TensorProduct, TestFunction, \
+        TrialFunction, inner, ulp
+    from jaxfun_amd.coordinates import x, y, z
+
+    N = 20
+    ue = sp.sin(2 * x) * sp.exp(2 * y + z * sp.I)
+    bcsy = {"left": {"D": ue.subs(y, -1)}, "right": {"D": ue.subs(y, 1)}}
+    bcsz = {"left": {"D": ue.subs(z, -1)}, "right": {"D": ue.subs(z, 1)}}
+    F = Fourier(N)
+    Dy = FunctionSpace(N, Chebyshev, bcs=bcsy)
+    Dz = FunctionSpace(N + 2, Chebyshev, bcs=bcsz)
+    T = TensorProduct(F, Dy, Dz)
+    v, u = TestFunction(T), TrialFunction(T)
+    A, b = inner(Div(Grad(u)) * v - Div(Grad(ue)) * v, kind="system")
+    uh = A.solve(b, method="lu")
+    uj = _np(T.backward(uh))
+    uej = sp.lambdify((x, y, z), ue)(*[_np(m) for m in T.mesh()])
+    assert np.linalg.norm(uj - uej) < np.sqrt(ulp(1))
+    assert np.linalg.norm(_np(T.forward(T.backward(uh))) - _np(uh)) < ulp(1000) * max(1.0, np.linalg.norm(_np(uh)))
