@@ -306,7 +306,7 @@ class DirectSumTPS(TensorProductSpace):
         grids = [v.mesh().cpu().numpy() for v in spaces]
         mesh = np.meshgrid(*grids, indexing="ij")
         e = sp.sympify(g)
-        vals = lambdify(tuple(syms[r] for r in R), e)(*mesh) if e.free_symbols else complex(e)
+        vals = lambdify(tuple(syms[r] for r in R), e)(*mesh) if e.free_symbols else _num_or_expr(e)
         vals = np.broadcast_to(np.asarray(vals), mesh[0].shape)
         dt = torch.complex128 if np.iscomplexobj(vals) else torch.float64
         return V.forward(torch.tensor(np.ascontiguousarray(vals), dtype=dt, device=K.device()))
@@ -396,7 +396,9 @@ class DirectSumTPS(TensorProductSpace):
 
 def _num_or_expr(e):
     e = sp.sympify(e)
-    return float(e) if not e.free_symbols else e
+    if e.free_symbols:
+        return e
+    return complex(e) if e.has(sp.I) else float(e)
 
 
 __all__ = ["TensorProductSpace", "TensorProduct", "DirectSumTPS", "torch"]

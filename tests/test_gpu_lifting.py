@@ -354,6 +354,7 @@ def test_directsumtps_3d_polynomial(dev, lifted):
         spaces.append(FunctionSpace(10, Legendre, bcs))
     T = TensorProduct(*spaces)
     assert isinstance(T, DirectSumTPS) and len(T.tpspaces) == 2 ** len(lifted)
+    assert not T.lift_coefficients().is_complex()
     v, u = TestFunction(T), TrialFunction(T)
     A, b = inner(v * Div(Grad(u)) - v * Div(Grad(ue)), sparse=True, kind="system")
     uh = A.solve(b)
