@@ -142,6 +142,43 @@ class TensorProductSpace:
         return self._run(c, [s.backward_operator(N[i], 0, kind) for i, s in enumerate(self.basespaces)],
                          "f64")
 
+    def _point_basis(self, ax: int, X: torch.Tensor) -> torch.Tensor:
+        """(P, dim) basis of axis ax at true-domain points X."""
+        s = self.basespaces[ax]
+        Xr = s.map_reference_domain(X).contiguous()
+        if getattr(s, "is_fourier", False):
+            return s._basis(Xr.shape[0], 0, False, 1.0, Xr)
+        return s._apply_stencil_cols(K.vandermonde(s.family, Xr, s.N, 0)).contiguous()
+
+    def evaluate(self, x, c):
+        """Series values at scattered points (tensorproductspace.py:263-321).
+
+        ``x`` holds one coordinate array per axis, all of one shape; ``c`` may
+        carry leading batch dimensions.  The last axis is a contraction with
+        its basis at the points; every other axis is a pointwise product with
+        its basis at the points followed by a sum over its modes.
+        """
+        from ..varcoeff import mul_bcast
+
+        shp = tuple(x[0].shape) if isinstance(x[0], torch.Tensor) else np.shape(x[0])
+        xs = [as_device(xi).reshape(-1) for xi in x]
+        d = len(self)
+        B = [self._point_basis(ax, xs[ax]) for ax in range(d)]
+        c = as_device(c)
+        op = K.ComplexAxisOperator(B[-1]) if B[-1].is_complex() else K.AxisOperator(B[-1])
+        X = K.apply_along(op, c, c.ndim - 1)
+        for ax in range(d - 2, -1, -1):
+            Bt = B[ax].T.contiguous()
+            X = X.contiguous()
+            if X.is_complex() or Bt.is_complex():
+                X = K.cmul_bcast(X, Bt if Bt.is_complex() else K.to_complex(Bt))
+            else:
+                X = mul_bcast(X, Bt)
+            ones = K.AxisOperator(torch.ones((1, Bt.shape[0]), dtype=torch.float64, device=X.device))
+            X = K.apply_along(ones, X, X.ndim - 2)
+            X = X.reshape(X.shape[:-2] + X.shape[-1:])
+        return X.reshape(X.shape[:-1] + tuple(shp))
+
     def to_orthogonal(self, c):
         c = as_device(c)
         nb = c.ndim - len(self)
@@ -384,6 +421,10 @@ class DirectSumTPS(TensorProductSpace):
     def evaluate_mesh(self, c, kind: str = MeshKind.QUADRATURE, N=None):
         u = self.get_homogeneous().evaluate_mesh(c, kind, N).contiguous()
         return K.add_bcast(u, self.lift_values(tuple(u.shape[u.ndim - len(self):]), None, kind))
+
+    def evaluate(self, x, c):
+        """orthogonal.evaluate(x, to_orthogonal(c)) (tensorproductspace.py:786-787)."""
+        return self.orthogonal.evaluate(x, self.to_orthogonal(c))
 
     def forward(self, u, precision: str | None = None):
         """from_orthogonal(orthogonal.forward(u)) (tensorproductspace.py:778-780)."""

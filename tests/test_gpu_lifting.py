@@ -387,3 +387,32 @@ def test_directsumtps_fourier_3d(dev):
     uej = sp.lambdify((x, y, z), ue)(*[_np(m) for m in T.mesh()])
     assert np.linalg.norm(uj - uej) < np.sqrt(ulp(1))
     assert np.linalg.norm(_np(T.forward(T.backward(uh))) - _np(uh)) < ulp(1000) * max(1.0, np.linalg.norm(_np(uh)))
+
+
+def test_tensor_evaluate_scattered(dev):
+    # TensorProductSpace.evaluate at scattered points (tensorproductspace.py:263-321)
+    # equals the backward transform at the same (mesh) points; DirectSumTPS.evaluate
+    # equals the sum over its split products (test_tensorproductspace_extra.py:253-258)
+    from jaxfun_amd import Fourier, FunctionSpace, Legendre, TensorProduct
+
+    rng = np.random.default_rng(7)
+    T = TensorProduct(FunctionSpace(10, Legendre, {"left": {"D": 0}, "right": {"D": 0}}), Legendre(9))
+    c = rng.standard_normal((2, 8, 9))
+    X, Y = [_np(m) for m in T.mesh()]
+    XX, YY = np.broadcast_arrays(X, Y)
+    got = _np(T.evaluate([XX.ravel(), YY.ravel()], c)).reshape(2, *XX.shape)
+    assert O.rel_l2(got, _np(T.backward(c))) < 1e-13
+    T3 = TensorProduct(Fourier(8), Legendre(6), Legendre(5))
+    c3 = rng.standard_normal((8, 6, 5)) + 1j * rng.standard_normal((8, 6, 5))
+    M = [_np(m) for m in T3.mesh()]
+    B = np.broadcast_arrays(*M)
+    got = _np(T3.evaluate([b.ravel() for b in B], c3)).reshape(B[0].shape)
+    assert O.rel_l2(got, _np(T3.backward(c3))) < 1e-13
+    N = (5, 6)
+    F1 = FunctionSpace(N[0], Legendre, bcs={"left": {"D": 1}, "right": {"D": 2}}, domain=(-2, 2))
+    F2 = FunctionSpace(N[1], Legendre, bcs={"left": {"D": 3}, "right": {"D": 4}}, domain=(-2, 2))
+    TD = TensorProduct(F1, F2)
+    cd = rng.standard_normal(TD.num_dofs)
+    pts = [rng.uniform(-2, 2, 13), rng.uniform(-2, 2, 13)]
+    z0 = sum(_np(v.evaluate(pts, TD.bndvals.get(f, cd))) for f, v in TD.tpspaces.items())
+    assert np.linalg.norm(z0 - _np(TD.evaluate(pts, cd))) < 1e-12
