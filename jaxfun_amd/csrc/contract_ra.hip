@@ -4,10 +4,12 @@
 // Q % 64 == 0 (a 64-column tile is 64 consecutive q of one p) and Q == 1
 // (a tile is 64 consecutive rows p, each K contiguous values).
 //
-// One persistent workgroup of 8 waves per CU.  Wave w keeps rows
-// 16w..16w+15 of the operator in registers for the whole launch (FP64/FP32:
-// one element per MFMA k-step; bf16x6: three bf16 slices per 32-k step), so
-// the only LDS operand traffic is the X tile.  Each 64-column tile of X (all
+// One persistent workgroup of 8 waves per CU (bf16x6: 16 waves in two column
+// groups, each computing half of the tile's 64 columns, so that more waves
+// per SIMD hide the staging).  Wave w keeps rows 16w..16w+15 of the operator
+// in registers for the whole launch (FP64/FP32: one element per MFMA k-step;
+// bf16x6: three bf16 slices per 32-k step), so the only LDS operand traffic
+// is the X tile.  Each 64-column tile of X (all
 // K rows) is staged through LDS once -- already split into three bf16 terms
 // for bf16x6 -- while the next tile is loaded into registers, so one barrier
 // pair covers a whole tile (up to 128 MFMAs per wave).  With Q == 1 the output
@@ -56,23 +58,26 @@ struct RMap {
 };
 
 // NS: k-steps run unconditionally (4 deep for FP64/FP32, 32 deep for bf16x6);
-// the operator registers and the staged rows past K are zero.
-template <int POL, int NS>
-__global__ __launch_bounds__(RNT) void contract_ra_kernel(const void* __restrict__ Aptr,
+// the operator registers and the staged rows past K are zero.  CG: column
+// groups of 8 waves (1 or 2).
+template <int POL, int NS, int CG>
+__global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __restrict__ Aptr,
                                                           const typename RT<POL>::TX* __restrict__ X,
                                                           typename RT<POL>::TX* __restrict__ Y, RMap rm) {
   typedef typename RT<POL>::TX TX;
   typedef typename RT<POL>::V V;
   constexpr int BLD = RLds<POL>::BLD;
   constexpr int EPC = 16 / sizeof(TX);            // elements per 16-byte chunk
-  constexpr int NCH = RMAX * RTN / EPC / RNT;     // plain: chunks per thread per tile
+  constexpr int NT = RNT * CG, JB = 4 / CG;        // threads; 16-column blocks per wave
+  constexpr int NCH = RMAX * RTN / EPC / NT;      // plain: chunks per thread per tile
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   TX* Bs0 = reinterpret_cast<TX*>(smem);  // buffer b at Bs0 + b * BSZ
   unsigned short* Bsl0 = reinterpret_cast<unsigned short*>(smem);
   constexpr int BSZ = (int)(RLds<POL>::B / sizeof(TX)), BSL = (int)(RLds<POL>::B / 2);
 
   const int tid = threadIdx.x, lane = tid & 63;
-  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int w = wv & 7, cg = wv >> 3;  // row block, column group
   const int M = rm.M, K = rm.K;
   const long long Q = rm.Q;
   const bool wact = 16 * w < M;  // this wave owns output rows
@@ -106,7 +111,7 @@ __global__ __launch_bounds__(RNT) void contract_ra_kernel(const void* __restrict
   // ---- X tile loads (registers) and LDS staging
   // plain: NCH 16-byte chunks per thread; bf16x6: 2 groups of 8 consecutive k of one column
   typedef typename std::conditional<POL == 3, float, TX>::type RV;
-  constexpr int NREG = (POL == 3) ? 16 : NCH * EPC;
+  constexpr int NREG = (POL == 3) ? 16 / CG : NCH * EPC;
   RV xr[NREG];
   auto tile_cols = [&](long long t, long long& xbase, long long& c0, int& ncol) {
     if (rm.qone) {
@@ -127,8 +132,8 @@ __global__ __launch_bounds__(RNT) void contract_ra_kernel(const void* __restrict
     tile_cols(t, xbase, c0, ncol);
     if constexpr (POL == 3) {
 #pragma unroll
-      for (int g = 0; g < 2; ++g) {
-        const int gi = tid + RNT * g;
+      for (int g = 0; g < 2 / CG; ++g) {
+        const int gi = tid + NT * g;
         if (rm.qone) {
           const int c = gi >> 4, k8 = (gi & 15) * 8;
           const bool ok = c < ncol && k8 < K;
@@ -146,7 +151,7 @@ __global__ __launch_bounds__(RNT) void contract_ra_kernel(const void* __restrict
     } else {
 #pragma unroll
       for (int i = 0; i < NCH; ++i) {
-        const int j = tid + RNT * i;
+        const int j = tid + NT * i;
         TX v[EPC];
         if (rm.qone) {
           const int e0 = j * EPC, c = e0 / K;
@@ -193,8 +198,8 @@ __global__ __launch_bounds__(RNT) void contract_ra_kernel(const void* __restrict
     unsigned short* Bsl = Bsl0 + buf * BSL;
     if constexpr (POL == 3) {
 #pragma unroll
-      for (int g = 0; g < 2; ++g) {
-        const int gi = tid + RNT * g;
+      for (int g = 0; g < 2 / CG; ++g) {
+        const int gi = tid + NT * g;
         const int c = rm.qone ? (gi >> 4) : (gi & 63), k8 = rm.qone ? (gi & 15) * 8 : (gi >> 6) * 8;
         float xv[8];
 #pragma unroll
@@ -208,7 +213,7 @@ __global__ __launch_bounds__(RNT) void contract_ra_kernel(const void* __restrict
     } else {
 #pragma unroll
       for (int i = 0; i < NCH; ++i) {
-        const int j = tid + RNT * i;
+        const int j = tid + NT * i;
         if (rm.qone) {
           const int e0 = j * EPC, c = e0 / K, k = e0 - c * K;
           if (c < RTN) {
@@ -227,36 +232,42 @@ __global__ __launch_bounds__(RNT) void contract_ra_kernel(const void* __restrict
 
   if constexpr (POL != 3) {
     // with Q == 1 only rows < K are staged: the rest of the k-steps reads zeros
-    for (int e = tid; e < 2 * BSZ; e += RNT) Bs0[e] = TX(0);
+    for (int e = tid; e < 2 * BSZ; e += NT) Bs0[e] = TX(0);
   }
-  typedef V AccT[4];
+  typedef V AccT[JB];
   auto compute = [&](AccT& acc, int buf) {
     const TX* Bs = Bs0 + buf * BSZ;
     const unsigned short* Bsl = Bsl0 + buf * BSL;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[j] = V{0, 0, 0, 0};
+    for (int j = 0; j < JB; ++j) acc[j] = V{0, 0, 0, 0};
       if (wact) {
         // B fragments of step s + 1 are read from LDS before step s's MFMAs issue
         if constexpr (POL == 3) {
-          v8u16 bf[2][4][3];
-          auto bload = [&](int s, v8u16(&dst)[4][3]) {
+          constexpr int NBB = CG == 2 ? 1 : 2;  // B fragments double-buffered
+          v8u16 bf[NBB][JB][3];
+          auto bload = [&](int s, v8u16(&dst)[JB][3]) {
   #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-              const int off = (16 * j + (lane & 15)) * BLD + 32 * s + 8 * (lane >> 4);
+            for (int j = 0; j < JB; ++j) {
+              const int off = (16 * (cg * JB + j) + (lane & 15)) * BLD + 32 * s + 8 * (lane >> 4);
               dst[j][0] = *(const v8u16*)&Bsl[off];
               dst[j][1] = *(const v8u16*)&Bsl[RTN * BLD + off];
               dst[j][2] = *(const v8u16*)&Bsl[2 * RTN * BLD + off];
             }
           };
-          bload(0, bf[0]);
+          if constexpr (NBB == 2) bload(0, bf[0]);
   #pragma unroll
           for (int s = 0; s < NS; ++s) {
-            if (s + 1 < NS) bload(s + 1, bf[(s + 1) & 1]);
+            if constexpr (NBB == 2) {
+              if (s + 1 < NS) bload(s + 1, bf[(s + 1) & 1]);
+            } else {
+              bload(s, bf[0]);
+            }
             __builtin_amdgcn_sched_barrier(0);  // the next step's reads issue before this step's MFMAs
             const v8u16 ah = ar[3 * s], am = ar[3 * s + 1], al = ar[3 * s + 2];
   #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-              const v8u16 bh = bf[s & 1][j][0], bm = bf[s & 1][j][1], bl = bf[s & 1][j][2];
+            for (int j = 0; j < JB; ++j) {
+              const int sb = NBB == 2 ? (s & 1) : 0;
+              const v8u16 bh = bf[sb][j][0], bm = bf[sb][j][1], bl = bf[sb][j][2];
               // products ordered small to large
               acc[j] = mfma_bf16(al, bh, acc[j]);
               acc[j] = mfma_bf16(am, bm, acc[j]);
@@ -269,22 +280,22 @@ __global__ __launch_bounds__(RNT) void contract_ra_kernel(const void* __restrict
         } else {
           // B fragments are read T k-steps at a time, so one LDS round trip
           // covers 4T MFMAs; a running offset keeps one address register
-          constexpr int T = 4;
+          constexpr int T = CG == 2 ? 2 : 4;
           int bo = (lane >> 4) * BLD + (lane & 15);
   #pragma unroll
           for (int s0 = 0; s0 < NS; s0 += T) {
-            TX bv[T][4];
+            TX bv[T][JB];
   #pragma unroll
             for (int u = 0; u < T; ++u)
   #pragma unroll
-              for (int j = 0; j < 4; ++j) bv[u][j] = Bs[bo + u * 4 * BLD + 16 * j];
+              for (int j = 0; j < JB; ++j) bv[u][j] = Bs[bo + u * 4 * BLD + 16 * (cg * JB + j)];
             __builtin_amdgcn_sched_barrier(0);  // all T steps' reads issue before their MFMAs
             bo += T * 4 * BLD;
             asm volatile("" : "+v"(bo));
   #pragma unroll
             for (int u = 0; u < T; ++u)
   #pragma unroll
-              for (int j = 0; j < 4; ++j) {
+              for (int j = 0; j < JB; ++j) {
                 if constexpr (POL == 0)
                   acc[j] = mfma_f64(ar[s0 + u], bv[u][j], acc[j]);
                 else
@@ -303,18 +314,18 @@ __global__ __launch_bounds__(RNT) void contract_ra_kernel(const void* __restrict
     if (!rm.qone) {
       TX* dst = Y + (xbase / ((long long)K * Q)) * (long long)M * Q + c0 + (lane & 15);
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
+      for (int j = 0; j < JB; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int m = orow(r);
-          if (m < M) dst[(long long)m * Q + 16 * j] = (TX)acc[j][r];
+          if (m < M) dst[(long long)m * Q + 16 * (cg * JB + j)] = (TX)acc[j][r];
         }
     } else {
       // output column c is the contiguous M-vector Y[(c0 + c) M ...]; with FP32
       // accumulators a lane holds 4 consecutive rows of it (one 16-byte store)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int c = 16 * j + (lane & 15);
+      for (int j = 0; j < JB; ++j) {
+        const int c = 16 * (cg * JB + j) + (lane & 15);
         if (c < ncol) {
           TX* dst = Y + (c0 + c) * M;
           if constexpr (POL == 0) {
@@ -398,13 +409,19 @@ int launch_ra(const void* A, int M, int K, const void* X, void* Y, long long P, 
     return 0;
   }
   typedef void (*KernT)(const void*, const TX*, TX*, RMap);
+  // bf16x6: 16 waves in two column groups (more waves per SIMD to hide the
+  // staging; 128 VGPRs fit); FP64/FP32 keep 8 waves (their operator rows and
+  // B fragments need more registers than 16 waves allow)
+  constexpr int RA_CG = (POL == 3) ? 2 : 1;
   KernT kern;
   if constexpr (POL == 3) {
     const int kst = (K + 31) / 32;  // 32-deep k-steps needed
-    kern = kst <= 1 ? contract_ra_kernel<POL, 1> : kst <= 2 ? contract_ra_kernel<POL, 2> : contract_ra_kernel<POL, 4>;
+    kern = kst <= 1 ? contract_ra_kernel<POL, 1, RA_CG> : kst <= 2 ? contract_ra_kernel<POL, 2, RA_CG>
+                                                           : contract_ra_kernel<POL, 4, RA_CG>;
   } else {
     const int kst = (K + 3) / 4;
-    kern = kst <= 8 ? contract_ra_kernel<POL, 8> : kst <= 16 ? contract_ra_kernel<POL, 16> : contract_ra_kernel<POL, 32>;
+    kern = kst <= 8 ? contract_ra_kernel<POL, 8, RA_CG> : kst <= 16 ? contract_ra_kernel<POL, 16, RA_CG>
+                                                            : contract_ra_kernel<POL, 32, RA_CG>;
   }
   const size_t shm = RLds<POL>::TOTAL;
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
@@ -415,7 +432,7 @@ int launch_ra(const void* A, int M, int K, const void* X, void* Y, long long P, 
   }
   long long grid = ra_grid();
   if (grid > rm.ntiles) grid = rm.ntiles;
-  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(RNT), shm, st, A, (const TX*)X, (TX*)Y, rm);
+  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(RNT * RA_CG), shm, st, A, (const TX*)X, (TX*)Y, rm);
   e = hipGetLastError();
   if (e != hipSuccess) {
     set_error("contract_ra launch: %s", hipGetErrorString(e));
