@@ -422,10 +422,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--with-burgers", action="store_true", help="also run the 2D Burgers IMEX workload (1000 ARS443 steps)")
+    ap.add_argument("--skip-burgers", action="store_true", help="skip the 2D Burgers IMEX workload (1000 ARS443 steps)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy 3x (known traffic, for PMC scaling)")
-    ap.add_argument("--with-pcg", action="store_true", help="also run the variable-coefficient PCG workload (train + evaluate)")
+    ap.add_argument("--skip-pcg", action="store_true", help="skip the variable-coefficient PCG workload (train + evaluate)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -485,9 +485,9 @@ def main():
         line["poisson_chebyshev_N64"] = bench_poisson_chebyshev()
         line["lifted_helmholtz_legendre_M50"] = bench_lifted_helmholtz()
         line["periodic_poisson_fourier20_legendre80"] = bench_periodic_poisson()
-    if args.with_burgers:
+    if not (args.skip_burgers or args.no_extras):
         line["burgers_imex_128x128"] = bench_burgers()
-    if args.with_pcg:
+    if not (args.skip_pcg or args.no_extras):
         line["learned_pcg_varcoeff_64"] = bench_learned_pcg()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(data)
