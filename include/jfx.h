@@ -41,6 +41,16 @@ int jfx_gauss_nodes(int family, int N, double* x, double* w, hipStream_t stream)
 int jfx_vandermonde_f64(int family, const double* x, int Q, int N, int k, double scale, double* V,
                         hipStream_t stream);
 
+/* Vandermonde of a basis given by its three-term recurrence, V (Q, N),
+ * V[q, n] = scale * d^k/dx^k Q_n(x[q]), k <= 4, with Q_0 = 1 and
+ * Q_n = r0[n] ((x - r1[n]) Q_{n-1} - r2[n] Q_{n-2}), rec = [r0 | r1 | r2]
+ * (3 N doubles, device).  Replaces Jacobi.eval_basis_functions
+ * (galerkin/Jacobi.py:165-202) and its nested-jacfwd derivatives
+ * (galerkin/orthogonal.py:168-178) for Jacobi, ChebyshevU and Ultraspherical
+ * (galerkin/ChebyshevU.py:134-157, galerkin/Ultraspherical.py:12-101). */
+int jfx_vandermonde_rec_f64(const double* x, int Q, int N, int k, const double* rec, double scale, double* V,
+                            hipStream_t stream);
+
 /* Fourier basis (complex, re/im interleaved): element (q, j) of
  * (i k_j s)^d exp(i k_j x_q) * post for the N-mode space (FFT wavenumber
  * order; Nyquist mode zeroed for odd d and even N) at x_q = 2 pi q / Q

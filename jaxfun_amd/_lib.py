@@ -32,6 +32,7 @@ SIGNATURES: dict[str, tuple] = {
     "jfx_device_sync": (c_int, []),
     "jfx_gauss_nodes": (c_int, [c_int, c_int, c_vp, c_vp, c_vp]),
     "jfx_vandermonde_f64": (c_int, [c_int, c_vp, c_int, c_int, c_int, c_dbl, c_vp, c_vp]),
+    "jfx_vandermonde_rec_f64": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_dbl, c_vp, c_vp]),
     "jfx_contract_f64": (c_int, [c_vp, c_int, c_int, c_vp, c_vp, c_ll, c_ll, c_vp]),
     "jfx_contract_f32": (c_int, [c_vp, c_int, c_int, c_vp, c_vp, c_ll, c_ll, c_vp]),
     "jfx_split_matrix": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp]),

@@ -1,5 +1,7 @@
 // Basis evaluation in FP64 on the GPU: Gauss nodes/weights and the
-// (derivative) pseudo-Vandermonde matrices of Legendre / Chebyshev bases.
+// (derivative) pseudo-Vandermonde matrices of Legendre / Chebyshev bases, of
+// any basis defined by a three-term recurrence (the Jacobi family) and of the
+// Fourier exponentials.
 #include "jfx_common.h"
 
 namespace jfx {
@@ -80,6 +82,42 @@ __global__ void vandermonde_kernel(int family, const double* __restrict__ x, int
       cur[d] = nxt[d];
     }
     row[n + 1] = cur[k] * scale;
+  }
+}
+
+// V[q, n] = scale * d^k/dx^k Q_n(x_q) for any basis given by its three-term
+// recurrence (Jacobi P^(a,b)_n scaled by g_n: Jacobi, ChebyshevU,
+// Ultraspherical): Q_0 = 1 and, for n >= 1,
+//     Q_n = r0[n] ((x - r1[n]) Q_{n-1} - r2[n] Q_{n-2})      (r2[1] unused)
+// with rec = [r0 | r1 | r2], each N long.  Differentiating d times:
+//     Q^(d)_n = r0[n] ((x - r1[n]) Q^(d)_{n-1} + d Q^(d-1)_{n-1} - r2[n] Q^(d)_{n-2}).
+// One thread per point, k <= 4.
+__global__ void recurrence_vandermonde_kernel(const double* __restrict__ x, int Q, int N, int k,
+                                              const double* __restrict__ rec, double scale,
+                                              double* __restrict__ V) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= Q) return;
+  const double X = x[q];
+  const double* r0 = rec;
+  const double* r1 = rec + N;
+  const double* r2 = rec + 2 * N;
+  double prev[5], cur[5];
+  for (int d = 0; d <= 4; ++d) {
+    prev[d] = 0.0;
+    cur[d] = (d == 0) ? 1.0 : 0.0;  // Q_0
+  }
+  double* row = V + (long long)q * N;
+  row[0] = cur[k] * scale;
+  for (int n = 1; n < N; ++n) {
+    const double a = r0[n], b = X - r1[n], c = (n >= 2) ? r2[n] : 0.0;
+    double nxt[5];
+    nxt[0] = a * (b * cur[0] - c * prev[0]);
+    for (int d = 1; d <= k; ++d) nxt[d] = a * (b * cur[d] + d * cur[d - 1] - c * prev[d]);
+    for (int d = 0; d <= k; ++d) {
+      prev[d] = cur[d];
+      cur[d] = nxt[d];
+    }
+    row[n] = cur[k] * scale;
   }
 }
 
@@ -173,6 +211,16 @@ int jfx_vandermonde_f64(int family, const double* x, int Q, int N, int k, double
     return set_error("vandermonde: bad arguments"), JFX_EARG;
   hipLaunchKernelGGL(vandermonde_kernel, dim3((Q + 127) / 128), dim3(128), 0, stream, family, x, Q, N,
                      k, scale, V);
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
+}
+
+int jfx_vandermonde_rec_f64(const double* x, int Q, int N, int k, const double* rec, double scale, double* V,
+                            hipStream_t stream) {
+  if (!x || !rec || !V || Q <= 0 || N <= 0 || k < 0 || k > 4)
+    return set_error("vandermonde_rec: bad arguments"), JFX_EARG;
+  hipLaunchKernelGGL(recurrence_vandermonde_kernel, dim3((Q + 127) / 128), dim3(128), 0, stream, x, Q, N, k,
+                     rec, scale, V);
   JFX_LAUNCH_CHECK();
   return JFX_OK;
 }

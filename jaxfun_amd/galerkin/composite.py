@@ -1,8 +1,10 @@
 """Boundary-constrained composite bases (reference: galerkin/composite.py).
 
-Covers Dirichlet conditions on Legendre / Chebyshev bases: the homogeneous
-stencil ``phi_k = (psi_k - psi_{k+2}) / s_k`` (galerkin/composite.py:786-789)
-with its transforms, and inhomogeneous values through the lifting space
+Covers Dirichlet conditions on the Legendre, Chebyshev and Jacobi-family
+bases: the homogeneous stencil ``phi_k = (psi_k - psi_{k+2}) / s_k``
+(galerkin/composite.py:786-789), or for the Jacobi family
+``phi_k = psi_k + d1_k psi_{k+1} + d2_k psi_{k+2}`` solved from the boundary
+values (galerkin/composite.py:765-832), with its transforms, and inhomogeneous values through the lifting space
 ``BCGeneric`` and the direct sum ``DirectSum = Composite (+) BCGeneric``
 (galerkin/composite.py:411-638).  The lift is a constant coefficient vector
 added on the device (``kernels.add_bcast``) to every transform.
@@ -45,6 +47,11 @@ class BoundaryConditions(dict):
         return BoundaryConditions({k: {s: 0 for s in v} for k, v in self.items()})
 
 
+def family_kwargs(kw: dict) -> dict:
+    """Pop the parameters of a Jacobi-family basis (alpha, beta, lambda_) from kw."""
+    return {k: kw.pop(k) for k in ("alpha", "beta", "lambda_") if k in kw}
+
+
 class Composite(OrthogonalSpace):
     """Basis phi_i = sum_j S_ij psi_j enforcing homogeneous Dirichlet BCs
     (galerkin/composite.py:121-408)."""
@@ -53,8 +60,9 @@ class Composite(OrthogonalSpace):
 
     def __init__(self, N: int, orthogonal: type, bcs, domain=None, name: str = "Composite",
                  fun_str: str = "phi", system=None, stencil=None, scaling=None, **kw):
+        fam = family_kwargs(kw)
         super().__init__(N, domain=domain, system=system, name=name, fun_str=fun_str, **kw)
-        self.orthogonal = orthogonal(N, domain=domain, system=system)
+        self.orthogonal = orthogonal(N, domain=domain, system=system, **fam)
         self.family = self.orthogonal.family
         self.bcs = BoundaryConditions(bcs)
         if "".join(self.bcs.orderednames()) != "LDRD" or not self.bcs.is_homogeneous():
@@ -62,7 +70,13 @@ class Composite(OrthogonalSpace):
                 "only homogeneous Dirichlet composite bases are built on the GPU path "
                 f"(got {dict(self.bcs)})")
         self.scaling = sp.S.One if scaling is None else sp.sympify(scaling)
-        self.stencil = {0: 1 / self.scaling, 2: -1 / self.scaling}
+        self._jacobi_stencil = getattr(self.orthogonal, "dirichlet_stencil", None)
+        if self._jacobi_stencil is not None:
+            S = self._jacobi_stencil()
+            self.stencil = {0: 1 / self.scaling, 1: S[:, 1:-1].diagonal() / self.scaling,
+                            2: S[:, 2:].diagonal() / self.scaling}
+        else:
+            self.stencil = {0: 1 / self.scaling, 2: -1 / self.scaling}
         self._mass_lu = None
 
     @property
@@ -82,6 +96,10 @@ class Composite(OrthogonalSpace):
         """S (dim, N) (galerkin/composite.py:255-269)."""
         m = self.dim
         s = self.scaling_values()
+        if self._jacobi_stencil is not None:
+            if "S_host" not in self._ops:
+                self._ops["S_host"] = self._jacobi_stencil()
+            return self._ops["S_host"] / s[:, None]
         S = np.zeros((m, self.N))
         S[np.arange(m), np.arange(m)] = 1.0 / s
         S[np.arange(m), np.arange(m) + 2] = -1.0 / s
@@ -171,15 +189,6 @@ class Composite(OrthogonalSpace):
 
 
 # ----------------------------------------------------------------------------- lifting
-def _boundary_value(family: str, j: int, side: str) -> float:
-    """psi_j at the reference-domain end: 1 at x = +1, (-1)^j at x = -1 for
-    Legendre and Chebyshev (the ``bnd_values`` of galerkin/Jacobi.py and
-    galerkin/Chebyshev.py for k = 0)."""
-    if family not in ("legendre", "chebyshev"):
-        raise NotImplementedError(f"boundary values of the {family!r} family")
-    return 1.0 if side == "R" else float((-1) ** j)
-
-
 def get_bc_basis(bcs, orthogonal) -> np.ndarray:
     """Lifting basis (num_bcs, num_bcs) (galerkin/composite.py:835-895).
 
@@ -188,14 +197,16 @@ def get_bc_basis(bcs, orthogonal) -> np.ndarray:
     matrix ``A[r, j] = psi_j(x_r)`` over the first num_bcs orthogonal
     functions, the rows are those of ``(A^-1)^T``.  Dirichlet conditions
     only (the conditions of the composite bases built here); for
-    ``{left: D, right: D}`` the rows are (psi_0 -+ psi_1) / 2.
+    ``{left: D, right: D}`` on Legendre/Chebyshev the rows are
+    (psi_0 -+ psi_1) / 2.  Boundary values come from the orthogonal space
+    (``bnd_value``), so the Jacobi family's scalings are honoured.
     """
     bcs = bcs if isinstance(bcs, BoundaryConditions) else BoundaryConditions(bcs)
     names = bcs.orderednames()
     if any(nm[1:] != "D" for nm in names):
         raise NotImplementedError(f"lifting of {names}: only Dirichlet conditions are built")
     nb = len(names)
-    A = np.array([[_boundary_value(orthogonal.family, j, nm[0]) for j in range(nb)] for nm in names])
+    A = np.array([[orthogonal.bnd_value(j, nm[0]) for j in range(nb)] for nm in names])
     return np.linalg.inv(A).T.copy()
 
 
@@ -224,7 +235,7 @@ class BCGeneric(Composite):
         OrthogonalSpace.__init__(self, nb, domain=domain, system=system, name=name, fun_str=fun_str,
                                  num_quad_points=nq)
         self.bcs = BoundaryConditions(bcs)
-        self.orthogonal = orthogonal(nb, domain=domain, system=system, num_quad_points=nq)
+        self.orthogonal = orthogonal(nb, domain=domain, system=system, num_quad_points=nq, **family_kwargs(kw))
         self.family = self.orthogonal.family
         self.S = get_bc_basis(self.bcs, self.orthogonal)
         self.scaling = sp.S.One

@@ -112,6 +112,8 @@ class OrthogonalSpace:
     def quad_points_and_weights(self, N: int | None = None) -> tuple[torch.Tensor, torch.Tensor]:
         """Reference-domain Gauss nodes (ascending) and weights, on the device."""
         N = self.num_quad_points if N is None else int(N)
+        if self.orthogonal is not self:
+            return self.orthogonal.quad_points_and_weights(N)
         key = ("quad", N)
         if key not in self._ops:
             self._ops[key] = K.gauss_nodes(self.family, N)
@@ -131,9 +133,21 @@ class OrthogonalSpace:
             return self.quad_points_and_weights(N)[0]
         return self.map_reference_domain(self.mesh(kind, N)).contiguous()
 
+    def _vander(self, X: torch.Tensor, k: int = 0, scale: float = 1.0) -> torch.Tensor:
+        """(Q, N) scale * d^k/dx^k psi_n(X) of this orthogonal basis on the device."""
+        return K.vandermonde(self.family, X.contiguous(), self.N, k, scale)
+
     def vandermonde(self, X: torch.Tensor, k: int = 0) -> torch.Tensor:
         """Orthogonal-basis (derivative) Vandermonde at reference points (orthogonal.py:131-178)."""
-        return K.vandermonde(self.family, X.contiguous(), self.N, k)
+        return self._vander(X, k)
+
+    def bnd_value(self, j: int, side: str) -> float:
+        """psi_j at the reference-domain end ("L": x = -1, "R": x = +1): 1 and
+        (-1)^j for Legendre and Chebyshev (the k = 0 ``bnd_values`` of
+        galerkin/Jacobi.py:257-288 and galerkin/Chebyshev.py)."""
+        if self.family not in ("legendre", "chebyshev"):
+            raise NotImplementedError(f"boundary values of the {self.family!r} family")
+        return 1.0 if side == "R" else float((-1) ** j)
 
     def evaluate_basis_derivative(self, X: torch.Tensor, k: int = 0) -> torch.Tensor:
         V = self.orthogonal.vandermonde(X, k)
@@ -164,14 +178,14 @@ class OrthogonalSpace:
         key = ("bwd", N, k, kind)
         if key not in self._ops:
             X = self._points(kind, N)
-            V = K.vandermonde(self.family, X, self.N, k, float(self.domain_factor) ** k)
+            V = self.orthogonal._vander(X, k, float(self.domain_factor) ** k)
             self._ops[key] = K.AxisOperator(self._apply_stencil_cols(V))
         return self._ops[key]
 
     def _weighted_vt(self, N: int) -> torch.Tensor:
         """(N_modes, Q): P^T diag(w) / df (galerkin/orthogonal.py:264-277, sg = 1)."""
         x, w = self.quad_points_and_weights(N)
-        V = K.vandermonde(self.family, x, self.N, 0)
+        V = self.orthogonal._vander(x, 0)
         return K.scale_transpose(V, w, None, 1.0 / float(self.domain_factor))
 
     def scalar_product_operator(self, N: int | None = None) -> K.AxisOperator:
@@ -190,7 +204,7 @@ class OrthogonalSpace:
         key = ("fwd", N)
         if key not in self._ops:
             x, w = self.quad_points_and_weights(N)
-            V = K.vandermonde(self.family, x, self.N, 0)
+            V = self.orthogonal._vander(x, 0)
             invh = torch.tensor(1.0 / self.norm_squared(), dtype=torch.float64, device=x.device)
             self._ops[key] = K.AxisOperator(K.scale_transpose(V, w, invh, 1.0))
         return self._ops[key]
@@ -213,10 +227,15 @@ class OrthogonalSpace:
 
     def evaluate(self, x, c):
         """Series values at arbitrary true-domain points x (galerkin/orthogonal.py:102-129):
-        the basis evaluated at the points, contracted with the last axis of c."""
+        the basis evaluated at the points, contracted with the last axis of c.
+        On an orthogonal space a shorter c is the series of its first len(c)
+        modes (the reference's recurrence runs over len(c))."""
         shp = tuple(x.shape) if isinstance(x, torch.Tensor) else np.shape(x)
         X = self.map_reference_domain(as_device(x).reshape(-1)).contiguous()
-        V = self._apply_stencil_cols(K.vandermonde(self.family, X, self.N, 0))
+        V = self._apply_stencil_cols(self.orthogonal._vander(X, 0))
+        nc = int(np.shape(c)[-1])
+        if self.is_orthogonal and nc < V.shape[1]:
+            V = V[:, :nc]
         out = _apply_1d(K.AxisOperator(V.contiguous()), c, "f64")
         return out.reshape(tuple(out.shape[:-1]) + tuple(shp))
 

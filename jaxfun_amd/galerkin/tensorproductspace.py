@@ -148,7 +148,7 @@ class TensorProductSpace:
         Xr = s.map_reference_domain(X).contiguous()
         if getattr(s, "is_fourier", False):
             return s._basis(Xr.shape[0], 0, False, 1.0, Xr)
-        return s._apply_stencil_cols(K.vandermonde(s.family, Xr, s.N, 0)).contiguous()
+        return s._apply_stencil_cols(s.orthogonal._vander(Xr, 0)).contiguous()
 
     def evaluate(self, x, c):
         """Series values at scattered points (tensorproductspace.py:263-321).

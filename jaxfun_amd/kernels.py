@@ -59,6 +59,17 @@ def vandermonde(family: str, x: torch.Tensor, N: int, k: int = 0, scale: float =
     return V
 
 
+def vandermonde_rec(rec: torch.Tensor, x: torch.Tensor, N: int, k: int = 0, scale: float = 1.0) -> torch.Tensor:
+    """(Q, N) Vandermonde of the basis with recurrence table ``rec`` (3, N)."""
+    _req(x, torch.float64, "x")
+    _req(rec, torch.float64, "rec")
+    if rec.numel() != 3 * N:
+        raise ValueError(f"vandermonde_rec: recurrence table has {rec.numel()} entries, need {3 * N}")
+    V = torch.empty((x.shape[0], N), dtype=torch.float64, device=x.device)
+    _lib.call("jfx_vandermonde_rec_f64", _p(x), x.shape[0], N, k, _p(rec), float(scale), _p(V), _s())
+    return V
+
+
 def scale_transpose(V: torch.Tensor, rowscale: torch.Tensor | None = None,
                     colscale: torch.Tensor | None = None, s: float = 1.0) -> torch.Tensor:
     """``out[c, r] = V[r, c] * rowscale[r] * colscale[c] * s``."""

@@ -10,6 +10,11 @@ pinned by checks that do not retype the formula (all in ``tests/test_oracle.py``
 * Vandermonde + derivatives: ``numpy.polynomial.{legendre,chebyshev}``
   ``legvander/chebvander`` + ``legder/chebder`` (what the reference's own
   ``tests/galerkin/test_approximations.py:48-87`` compares against).
+* Jacobi family: ``scipy.special.eval_jacobi`` / ``eval_chebyu`` /
+  ``eval_gegenbauer``; derivatives by exact power-basis differentiation;
+  norms by Gauss-Jacobi orthogonality; nodes as zeros of P_N with weights
+  summing to the weight's integral; the Dirichlet stencil vanishing at both
+  ends and its closed forms (Legendre for (0, 0), -(k+1)/(k+3) for U).
 * Gauss nodes: ``numpy.polynomial.legendre.leggauss``; exactness of the
   rule on monomials up to degree 2N-1; Chebyshev closed form.
 * Galerkin matrices: the literal shenfun golden diagonals of the
@@ -95,8 +100,112 @@ def chebgauss(N: int) -> tuple[np.ndarray, np.ndarray]:
     return np.cos(np.pi + (2 * k + 1) * np.pi / (2 * N)), np.full(N, np.pi / N)
 
 
-def quad(family: str, N: int) -> tuple[np.ndarray, np.ndarray]:
+def quad(family, N: int) -> tuple[np.ndarray, np.ndarray]:
+    fam = jacobi_family(family)
+    if fam is not None:
+        return fam.quad(N)
     return leggauss(N) if family == "legendre" else chebgauss(N)
+
+
+# --------------------------------------------------------------------------
+# Jacobi family: Jacobi, ChebyshevU, Ultraspherical
+# --------------------------------------------------------------------------
+
+
+class JacobiFamily:
+    """Q_n = g_n P_n^(a,b) on [-1, 1] under the weight (1-x)^a (1+x)^b.
+
+    Restates galerkin/Jacobi.py:25-357 (g_n = 1), galerkin/ChebyshevU.py
+    (a = b = 1/2, g_n = (n+1) / P_n(1)) and galerkin/Ultraspherical.py
+    (a = b = lambda - 1/2, g_n = 1 / P_n(1)).  The polynomials come from the
+    textbook recurrence (Abramowitz & Stegun 22.7.1)
+        2n(n+a+b)(2n+a+b-2) P_n = (2n+a+b-1)((2n+a+b)(2n+a+b-2) x + a^2-b^2) P_{n-1}
+                                  - 2(n+a-1)(n+b-1)(2n+a+b) P_{n-2},
+    P_0 = 1, P_1 = (a+1) + (a+b+2)(x-1)/2, and derivatives from
+    d^k P_n^(a,b) = Gamma(n+a+b+1+k) / (2^k Gamma(n+a+b+1)) P_{n-k}^(a+k,b+k)
+    (galerkin/Jacobi.py:290-304).  Nodes and weights are scipy's
+    ``roots_jacobi``, the routine galerkin/Jacobi.py:112-124 calls.
+    """
+
+    def __init__(self, alpha: float = 0.0, beta: float = 0.0, kind: str = "jacobi"):
+        self.a, self.b, self.kind = float(alpha), float(beta), kind
+
+    def gn(self, n) -> np.ndarray:
+        n = np.asarray(n, dtype=np.float64)
+        if self.kind == "jacobi":
+            return np.ones_like(n)
+        lg = np.vectorize(math.lgamma)
+        p1 = np.exp(lg(n + self.a + 1) - math.lgamma(self.a + 1) - lg(n + 1))  # P_n^(a,a)(1)
+        return (n + 1) / p1 if self.kind == "chebyshevu" else 1.0 / p1
+
+    @staticmethod
+    def raw(x, N: int, a: float, b: float) -> np.ndarray:
+        """(Q, N) unnormalised P_n^(a,b)(x)."""
+        x = np.asarray(x, dtype=np.float64)
+        P = np.zeros((x.shape[0], N))
+        if N > 0:
+            P[:, 0] = 1.0
+        if N > 1:
+            P[:, 1] = (a + 1) + (a + b + 2) * (x - 1) / 2
+        for n in range(2, N):
+            c = 2 * n + a + b
+            P[:, n] = ((c - 1) * (c * (c - 2) * x + a * a - b * b) * P[:, n - 1]
+                       - 2 * (n + a - 1) * (n + b - 1) * c * P[:, n - 2]) / (2 * n * (n + a + b) * (c - 2))
+        return P
+
+    def vandermonde(self, x, N: int, k: int = 0) -> np.ndarray:
+        x = np.asarray(x, dtype=np.float64)
+        V = np.zeros((x.shape[0], N))
+        if N > k:
+            Pk = self.raw(x, N - k, self.a + k, self.b + k)
+            for n in range(k, N):
+                f = math.exp(math.lgamma(n + self.a + self.b + 1 + k) - math.lgamma(n + self.a + self.b + 1)
+                             - k * math.log(2.0)) if k else 1.0
+                V[:, n] = f * Pk[:, n - k]
+        return V * self.gn(np.arange(N))[None, :]
+
+    def norm_squared(self, N: int) -> np.ndarray:
+        a, b = self.a, self.b
+        h = np.zeros(N)
+        for n in range(N):
+            if n == 0:
+                h[n] = 2 ** (a + b + 1) * math.exp(math.lgamma(a + 1) + math.lgamma(b + 1) - math.lgamma(a + b + 2))
+            else:
+                h[n] = (2 ** (a + b + 1) / (2 * n + a + b + 1)
+                        * math.exp(math.lgamma(n + a + 1) + math.lgamma(n + b + 1)
+                                   - math.lgamma(n + 1) - math.lgamma(n + a + b + 1)))
+        return h * self.gn(np.arange(N)) ** 2
+
+    def quad(self, N: int) -> tuple[np.ndarray, np.ndarray]:
+        from scipy.special import roots_jacobi
+
+        x, w = roots_jacobi(N, self.a, self.b)
+        return np.asarray(x), np.asarray(w)
+
+    def dirichlet_stencil(self, N: int) -> np.ndarray:
+        """phi_k = Q_k + d1 Q_{k+1} + d2 Q_{k+2} with phi_k(+-1) = 0
+        (get_stencil_matrix, galerkin/composite.py:765-832); the end values are
+        the evaluated polynomials, solved as a 2 x 2 system per k."""
+        E = self.vandermonde(np.array([-1.0, 1.0]), N)
+        S = np.zeros((N - 2, N))
+        for k in range(N - 2):
+            d = np.linalg.solve(E[:, k + 1:k + 3], -E[:, k])
+            S[k, k], S[k, k + 1], S[k, k + 2] = 1.0, d[0], d[1]
+        return S
+
+
+def jacobi_family(family):
+    """A JacobiFamily for ``family`` ("jacobi" = (0, 0), "chebyshevu",
+    "ultraspherical" = lambda 1, or an instance), None for Legendre/Chebyshev."""
+    if isinstance(family, JacobiFamily):
+        return family
+    if family == "jacobi":
+        return JacobiFamily(0.0, 0.0, "jacobi")
+    if family == "chebyshevu":
+        return JacobiFamily(0.5, 0.5, "chebyshevu")
+    if family == "ultraspherical":
+        return JacobiFamily(0.5, 0.5, "ultraspherical")
+    return None
 
 
 # --------------------------------------------------------------------------
@@ -114,6 +223,9 @@ def vandermonde(family: str, x, N: int, k: int = 0) -> np.ndarray:
     ``P^(d)_{n+1} = P^(d)_{n-1} + (2n+1) P^(d-1)_n`` (Legendre) and
     ``T^(d)_{n+1} = 2x T^(d)_n + 2d T^(d-1)_n - T^(d)_{n-1}`` (Chebyshev).
     """
+    fam = jacobi_family(family)
+    if fam is not None:
+        return fam.vandermonde(x, N, k)
     x = np.asarray(x, dtype=np.float64)
     Q = x.shape[0]
     V = np.zeros((k + 1, Q, N))
@@ -171,6 +283,9 @@ def norm_squared(family: str, N: int) -> np.ndarray:
     Legendre ``2/(2n+1)`` (galerkin/Jacobi.py:249-251 with alpha=beta=0);
     Chebyshev ``pi, pi/2, ...`` (galerkin/Chebyshev.py:331-338).
     """
+    fam = jacobi_family(family)
+    if fam is not None:
+        return fam.norm_squared(N)
     n = np.arange(N)
     if family == "legendre":
         return 2.0 / (2 * n + 1)
@@ -184,13 +299,18 @@ def norm_squared(family: str, N: int) -> np.ndarray:
 # --------------------------------------------------------------------------
 
 
-def dirichlet_stencil(N: int, scaling=None) -> np.ndarray:
+def dirichlet_stencil(N: int, scaling=None, family=None) -> np.ndarray:
     """Stencil ``S`` (N-2, N) of phi_k = (P_k - P_{k+2}) / s_k.
 
     Restates ``get_stencil_matrix`` LDRD fast case (galerkin/composite.py:786-789)
     and ``stencil_to_diamatrix`` (galerkin/composite.py:255-269); ``scaling``
-    divides each stencil row (galerkin/composite.py:178-179).
+    divides each stencil row (galerkin/composite.py:178-179).  A Jacobi-family
+    ``family`` takes its general stencil (``JacobiFamily.dirichlet_stencil``).
     """
+    fam = jacobi_family(family)
+    if fam is not None:
+        S = fam.dirichlet_stencil(N)
+        return S if scaling is None else S / np.asarray(scaling, dtype=np.float64)[:, None]
     n = N - 2
     S = np.zeros((n, N))
     s = np.ones(n) if scaling is None else np.asarray(scaling, dtype=np.float64)
@@ -316,7 +436,7 @@ class Space1D:
 
     def __init__(self, family: str, N: int, dirichlet: bool = False, scaling=None):
         self.family, self.N, self.dirichlet = family, N, dirichlet
-        self.S = dirichlet_stencil(N, scaling) if dirichlet else np.eye(N)
+        self.S = dirichlet_stencil(N, scaling, family) if dirichlet else np.eye(N)
         self.dim = self.S.shape[0]
         self.h = norm_squared(family, N)
 

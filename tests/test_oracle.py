@@ -489,3 +489,124 @@ def test_fourier_legendre_poisson_manufactured():
     uj = F.evaluate(xs, (U @ L.S @ O.vandermonde("legendre", ys, N).T))
     ue = np.cos(2 * xs)[:, None] * (1 - ys**2)[None, :]
     assert np.linalg.norm(uj - ue) / 40 < np.sqrt(np.spacing(10.0))
+
+
+# ----------------------------------------------------------------- Jacobi family
+JFAMS = [(0.0, 0.0, "jacobi"), (1.0, 2.0, "jacobi"), (-0.5, -0.5, "jacobi"), (0.3, -0.7, "jacobi"),
+         (0.5, 0.5, "chebyshevu"), (0.5, 0.5, "ultraspherical"), (1.5, 1.5, "ultraspherical")]
+
+
+def _scipy_basis(a, b, kind, n, x):
+    from scipy.special import eval_chebyu, eval_gegenbauer, eval_jacobi
+
+    if kind == "jacobi":
+        return eval_jacobi(n, a, b, x)
+    if kind == "chebyshevu":
+        return eval_chebyu(n, x)
+    lam = a + 0.5
+    return eval_gegenbauer(n, lam, x) / eval_gegenbauer(n, lam, 1.0)
+
+
+@pytest.mark.parametrize("a,b,kind", JFAMS)
+def test_jacobi_vandermonde_vs_scipy(a, b, kind):
+    # galerkin/Jacobi.py:165-202, ChebyshevU.py, Ultraspherical.py against scipy.special
+    F = O.JacobiFamily(a, b, kind)
+    x = np.linspace(-1, 1, 23)
+    N = 30
+    V = F.vandermonde(x, N)
+    R = np.stack([_scipy_basis(a, b, kind, n, x) for n in range(N)], 1)
+    assert np.abs(V - R).max() < 1e-13 * max(1.0, np.abs(R).max())
+    if kind == "jacobi" and a == b == 0.0:
+        assert np.allclose(V, L.legvander(x, N - 1), atol=1e-14)
+
+
+@pytest.mark.parametrize("a,b,kind", JFAMS)
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_jacobi_derivatives_vs_power_basis(a, b, kind, k):
+    # each Q_n converted to the power basis from exact samples (degree n, n+1
+    # Chebyshev points), differentiated there and evaluated
+    F = O.JacobiFamily(a, b, kind)
+    N = 9
+    x = np.linspace(-0.9, 0.95, 11)
+    V = F.vandermonde(x, N, k)
+    for n in range(N):
+        xs = np.cos(np.pi * (np.arange(n + 1) + 0.5) / (n + 1))
+        p = np.polynomial.Polynomial.fit(xs, _scipy_basis(a, b, kind, n, xs), n, domain=[-1, 1], window=[-1, 1])
+        ref = p.deriv(k)(x) if n >= k else np.zeros_like(x)
+        assert np.abs(V[:, n] - ref).max() < 1e-10 * max(1.0, np.abs(ref).max()), n
+
+
+@pytest.mark.parametrize("a,b,kind", JFAMS)
+def test_jacobi_quadrature_and_norms(a, b, kind):
+    from scipy.special import beta as B
+
+    F = O.JacobiFamily(a, b, kind)
+    N = 20
+    x, w = F.quad(N)
+    assert np.all(np.diff(x) > 0)
+    # nodes are the zeros of P_N; weights integrate the weight function exactly
+    assert np.abs(O.JacobiFamily(a, b).raw(x, N + 1, a, b)[:, N]).max() < 1e-12 * np.abs(
+        O.JacobiFamily(a, b).raw(np.array([1.0]), N + 1, a, b)).max()
+    assert abs(w.sum() - 2 ** (a + b + 1) * B(a + 1, b + 1)) < 1e-13
+    # orthogonality under a rule of N + 2 points gives the norms
+    xq, wq = F.quad(N + 2)
+    V = F.vandermonde(xq, N)
+    G = (V.T * wq) @ V
+    h = F.norm_squared(N)
+    assert np.abs(G - np.diag(h)).max() < 1e-12 * h.max()  # roots_jacobi weights near a (1+x)^-0.7 end
+    if kind == "chebyshevu":
+        assert np.allclose(h, np.pi / 2)
+        # galerkin/ChebyshevU.py:84-105 closed-form nodes
+        t = (np.arange(N) + 1) * np.pi / (N + 1)
+        assert np.allclose(x, np.cos(t + np.pi), atol=1e-14)
+        assert np.allclose(w, np.pi / (N + 1) * (1 - x**2), atol=1e-14)
+
+
+@pytest.mark.parametrize("a,b,kind", JFAMS)
+def test_jacobi_dirichlet_stencil(a, b, kind):
+    F = O.JacobiFamily(a, b, kind)
+    N = 16
+    S = F.dirichlet_stencil(N)
+    E = F.vandermonde(np.array([-1.0, 1.0]), N)
+    assert np.abs(E @ S.T).max() < 1e-13
+    k = np.arange(N - 2)
+    if a == b:
+        assert np.abs(np.diag(S, 1)).max() < 1e-14
+    if kind == "jacobi" and a == b == 0.0:
+        assert np.allclose(S, O.dirichlet_stencil(N))
+    if kind == "chebyshevu":
+        assert np.allclose(np.diag(S, 2), -(k + 1) / (k + 3))
+    if kind == "ultraspherical":
+        assert np.allclose(np.diag(S, 2), -1.0)
+
+
+@pytest.mark.parametrize("a,b,kind", JFAMS)
+def test_jacobi_host_tables_vs_oracle(a, b, kind):
+    # the space's host set-up (recurrence table, norms, end values, stencil)
+    # in exact arithmetic against the oracle; the recurrence is run here in
+    # numpy exactly as the device kernel runs it
+    from jaxfun_amd.galerkin import ChebyshevU, Jacobi, Ultraspherical
+
+    N = 30
+    if kind == "jacobi":
+        sp_ = Jacobi(N, alpha=sp.nsimplify(a), beta=sp.nsimplify(b))
+    elif kind == "chebyshevu":
+        sp_ = ChebyshevU(N)
+    else:
+        sp_ = Ultraspherical(N, lambda_=sp.nsimplify(a + 0.5))
+    F = O.JacobiFamily(a, b, kind)
+    r0, r1, r2 = sp_.recurrence_table()
+    x = np.linspace(-1, 1, 13)
+    prev, cur = np.zeros_like(x), np.ones_like(x)
+    V = np.zeros((x.size, N))
+    V[:, 0] = 1.0
+    for n in range(1, N):
+        prev, cur = cur, r0[n] * ((x - r1[n]) * cur - (r2[n] if n >= 2 else 0.0) * prev)
+        V[:, n] = cur
+    R = F.vandermonde(x, N)
+    assert np.abs(V - R).max() < 1e-13 * max(1.0, np.abs(R).max())
+    assert np.allclose(sp_.norm_squared(), F.norm_squared(N), rtol=1e-13)
+    E = F.vandermonde(np.array([-1.0, 1.0]), N)
+    assert np.allclose([sp_.bnd_value(j, "L") for j in range(N)], E[0], rtol=1e-13)
+    assert np.allclose([sp_.bnd_value(j, "R") for j in range(N)], E[1], rtol=1e-13)
+    assert np.allclose(sp_.dirichlet_stencil(), F.dirichlet_stencil(N), atol=1e-13)
