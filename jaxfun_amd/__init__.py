@@ -8,12 +8,15 @@ from . import kernels  # noqa: F401
 from .coordinates import x, y, z  # noqa: F401
 from .galerkin import (  # noqa: F401
     Chebyshev,
+    ChebyshevU,
     Fourier,
     FunctionSpace,
+    Jacobi,
     Legendre,
     TensorProduct,
     TestFunction,
     TrialFunction,
+    Ultraspherical,
     inner,
     project,
 )
