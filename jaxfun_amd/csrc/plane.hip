@@ -5,8 +5,10 @@
 // from HBM once and written once: a 2D TensorProductSpace transform, the two
 // eigen-transforms + eigenvalue division of a 2D fast-diagonalisation solve,
 // or the last two axes of a 3D transform, in one pass.
-// One persistent workgroup of 8 waves per CU walks the slabs; wave w owns
-// output row block w (16 rows) of each phase and all of its column blocks.
+// Persistent workgroups of 8 waves walk the slabs: two per CU when the LDS
+// buffer (K0 rows rounded up to 32) fits in half the CU's LDS and a wave holds
+// at most 2 column blocks.  The work split inside a slab is described at the
+// kernel.
 //   POL 0: FP64 MFMA (v_mfma_f64_16x16x4f64), FP64 I/O
 //   POL 3: bf16 3-term float split (6 products), FP32 accumulation, FP32 I/O
 #include <type_traits>
@@ -43,54 +45,80 @@ __device__ __forceinline__ void split3q(float x, unsigned short& h, unsigned sho
 template <int POL>
 struct QAcc {
   typedef typename std::conditional<POL == 0, v4d, v4f>::type V;
-  V v[8];
 };
 
-template <int POL>
-__global__ __launch_bounds__(QNT) void plane_kernel(const double* __restrict__ A0d, const double* __restrict__ A1d,
-                                                    const unsigned short* __restrict__ A0s,
-                                                    const unsigned short* __restrict__ A1s,
-                                                    const typename QT<POL>::TX* __restrict__ X,
-                                                    typename QT<POL>::TX* __restrict__ Y,
-                                                    const typename QT<POL>::TX* __restrict__ E, int epi,
-                                                    long long B, int M0, int K0, int M1, int K1) {
+// CBW: 16-column blocks per wave (1, 2, 4 or 8).  Each 16-row block of a
+// phase is shared by wpr waves (8 / row blocks, at most 8), which take
+// consecutive groups of CBW column blocks; every wave runs CBW MFMA chains per
+// k-step unconditionally, so the loop has no divergent branches.  Operands
+// are loaded from clamped, always-valid addresses: rows and columns past the
+// array edges then hold copies of edge values, which reach only output rows
+// and columns that are never stored, and the contraction index is cut off by
+// zeros (X times a mask factor for k >= K1; T's zeroed rows >= K0; the bf16
+// slices are zero-padded to Kp).  FP64 operands of the next k-step are loaded
+// while the current step's MFMAs run.
+template <int POL, int CBW>
+__global__ __launch_bounds__(QNT, CBW <= 2 ? 4 : 2) void plane_kernel(
+    const double* __restrict__ A0d, const double* __restrict__ A1d, const unsigned short* __restrict__ A0s,
+    const unsigned short* __restrict__ A1s, const typename QT<POL>::TX* __restrict__ X,
+    typename QT<POL>::TX* __restrict__ Y, const typename QT<POL>::TX* __restrict__ E, int epi, long long B,
+    int M0, int K0, int M1, int K1, int wpr) {
   typedef typename QT<POL>::TX TX;
+  typedef typename QAcc<POL>::V V;
   constexpr int LDT = QT<POL>::LDT;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-  TX* T = reinterpret_cast<TX*>(smem);  // T[i][m], i < K0, m < M1
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  TX* T = reinterpret_cast<TX*>(smem);  // T[i][m], i < K0p (rows >= K0 zero), m < LDT
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int K0p = (K0 + 31) & ~31, K1p = (K1 + 31) & ~31;  // slice row strides
   const int ncb1 = (M1 + 15) >> 4;
+  const int rb = w / wpr, c0 = (w % wpr) * CBW;
+  const bool actA = 16 * rb < K0 && c0 < ncb1, actB = 16 * rb < M0 && c0 < ncb1;
+  const int l16 = lane & 15, kl = lane >> 4;
+  // T has K0p rows; rows >= K0 and columns >= M1 are zeroed once and never
+  // written, so phase B reads T without masks
+  for (int e = tid; e < K0p * LDT; e += QNT) T[e] = (TX)0;
+  __syncthreads();
 
   for (long long b = blockIdx.x; b < B; b += gridDim.x) {
     const TX* Xb = X + b * (long long)K0 * K1;
     TX* Yb = Y + b * (long long)M0 * M1;
-    // ---------------- phase A: T (K0 x M1) = X_b A1^T ; wave w -> rows 16w..16w+15 of T
-    if (16 * w < K0) {
-      typename QAcc<POL>::V acc[8];
+    // ---------------- phase A: T (K0 x M1) = X_b A1^T
+    if (actA) {
+      V acc[CBW];
 #pragma unroll
-      for (int c = 0; c < 8; ++c) acc[c] = typename QAcc<POL>::V{0, 0, 0, 0};
-      const int i = 16 * w + (lane & 15);
+      for (int t = 0; t < CBW; ++t) acc[t] = V{0, 0, 0, 0};
+      const int i = 16 * rb + l16;
+      const int ic = i < K0 ? i : K0 - 1;
       if constexpr (POL == 0) {
-        for (int k0 = 0; k0 < K1; k0 += 4) {
-          const int k = k0 + (lane >> 4);
-          const double a = (i < K0 && k < K1) ? Xb[(long long)i * K1 + k] : 0.0;
+        double a0, a1, b0[CBW], b1[CBW];
+        // a mask factor (not a select) zeroes k >= K1, so that no load is
+        // predicated and the loads of a k-step issue back to back
+        auto load = [&](int k0, double& a, double(&bv)[CBW]) {
+          const int k = k0 + kl, kc = k < K1 ? k : K1 - 1;
+          a = Xb[(long long)ic * K1 + kc] * (k < K1 ? 1.0 : 0.0);
 #pragma unroll
-          for (int c = 0; c < 8; ++c) {
-            if (c < ncb1) {
-              const int m = 16 * c + (lane & 15);
-              const double bv = (m < M1 && k < K1) ? A1d[(long long)m * K1 + k] : 0.0;
-              acc[c] = mfma_f64(a, bv, acc[c]);
-            }
+          for (int t = 0; t < CBW; ++t) {
+            const int m = 16 * (c0 + t) + l16, mc = m < M1 ? m : M1 - 1;
+            bv[t] = A1d[(long long)mc * K1 + kc];
           }
+        };
+        load(0, a0, b0);
+        for (int k0 = 0; k0 < K1; k0 += 8) {
+          load(k0 + 4, a1, b1);
+#pragma unroll
+          for (int t = 0; t < CBW; ++t) acc[t] = mfma_f64(a0, b0[t], acc[t]);
+          load(k0 + 8, a0, b0);
+#pragma unroll
+          for (int t = 0; t < CBW; ++t) acc[t] = mfma_f64(a1, b1[t], acc[t]);
         }
       } else {
         for (int k0 = 0; k0 < K1; k0 += 32) {
           v8u16 ah, am, al;
 #pragma unroll
           for (int e = 0; e < 8; ++e) {
-            const int k = k0 + 8 * (lane >> 4) + e;
-            float x = (i < K0 && k < K1) ? (float)Xb[(long long)i * K1 + k] : 0.f;
+            const int k = k0 + 8 * kl + e, kc = k < K1 ? k : K1 - 1;
+            const float x = (float)Xb[(long long)ic * K1 + kc] * (k < K1 ? 1.f : 0.f);
             unsigned short h, m_, l;
             split3q(x, h, m_, l);
             ah[e] = h;
@@ -98,104 +126,86 @@ __global__ __launch_bounds__(QNT) void plane_kernel(const double* __restrict__ A
             al[e] = l;
           }
 #pragma unroll
-          for (int c = 0; c < 8; ++c) {
-            if (c < ncb1) {
-              const int m = 16 * c + (lane & 15);
-              v8u16 bh = {0, 0, 0, 0, 0, 0, 0, 0}, bm = bh, bl = bh;
-              if (m < M1) {
-                const long long o = (long long)m * K1p + k0 + 8 * (lane >> 4);
-                const long long sl = (long long)M1 * K1p;
-                bh = *(const v8u16*)(A1s + o);
-                bm = *(const v8u16*)(A1s + o + sl);
-                bl = *(const v8u16*)(A1s + o + 2 * sl);
-              }
-              acc[c] = mfma_bf16(al, bh, acc[c]);
-              acc[c] = mfma_bf16(am, bm, acc[c]);
-              acc[c] = mfma_bf16(ah, bl, acc[c]);
-              acc[c] = mfma_bf16(am, bh, acc[c]);
-              acc[c] = mfma_bf16(ah, bm, acc[c]);
-              acc[c] = mfma_bf16(ah, bh, acc[c]);
-            }
+          for (int t = 0; t < CBW; ++t) {
+            const int m = 16 * (c0 + t) + l16, mc = m < M1 ? m : M1 - 1;
+            const long long o = (long long)mc * K1p + k0 + 8 * kl;
+            const long long sl = (long long)M1 * K1p;
+            const v8u16 bh = *(const v8u16*)(A1s + o), bm = *(const v8u16*)(A1s + o + sl),
+                        bl = *(const v8u16*)(A1s + o + 2 * sl);
+            acc[t] = mfma_bf16(al, bh, acc[t]);
+            acc[t] = mfma_bf16(am, bm, acc[t]);
+            acc[t] = mfma_bf16(ah, bl, acc[t]);
+            acc[t] = mfma_bf16(am, bh, acc[t]);
+            acc[t] = mfma_bf16(ah, bm, acc[t]);
+            acc[t] = mfma_bf16(ah, bh, acc[t]);
           }
         }
       }
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        if (c >= ncb1) continue;
-        const int m = 16 * c + (lane & 15);
+      for (int t = 0; t < CBW; ++t) {
+        const int m = 16 * (c0 + t) + l16;
         if (m >= M1) continue;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int ii = 16 * w + ((POL == 0) ? mfma16_row_f64(lane, r) : mfma16_row_f32(lane, r));
-          if (ii < K0) T[ii * LDT + m] = (TX)acc[c][r];
+          const int ii = 16 * rb + ((POL == 0) ? mfma16_row_f64(lane, r) : mfma16_row_f32(lane, r));
+          if (ii < K0) T[ii * LDT + m] = (TX)acc[t][r];
         }
       }
     }
     __syncthreads();
-    // ---------------- phase B: Y (M0 x M1) = A0 T ; wave w -> rows 16w..16w+15 of Y
-    if (16 * w < M0) {
-      typename QAcc<POL>::V acc[8];
+    // ---------------- phase B: Y (M0 x M1) = A0 T
+    if (actB) {
+      V acc[CBW];
 #pragma unroll
-      for (int c = 0; c < 8; ++c) acc[c] = typename QAcc<POL>::V{0, 0, 0, 0};
-      const int mrow = 16 * w + (lane & 15);
+      for (int t = 0; t < CBW; ++t) acc[t] = V{0, 0, 0, 0};
+      const int mrow = 16 * rb + l16, mrc = mrow < M0 ? mrow : M0 - 1;
       if constexpr (POL == 0) {
+        auto lda = [&](int k) { return A0d[(long long)mrc * K0 + (k < K0 ? k : K0 - 1)]; };
+        double an = lda(kl);
         for (int k0 = 0; k0 < K0; k0 += 4) {
-          const int k = k0 + (lane >> 4);
-          const double a = (mrow < M0 && k < K0) ? A0d[(long long)mrow * K0 + k] : 0.0;
+          const double a = an;
+          const int k = k0 + kl;
+          an = lda(k + 4);
 #pragma unroll
-          for (int c = 0; c < 8; ++c) {
-            if (c < ncb1) {
-              const int j = 16 * c + (lane & 15);
-              const double bv = (j < M1 && k < K0) ? T[k * LDT + j] : 0.0;
-              acc[c] = mfma_f64(a, bv, acc[c]);
-            }
-          }
+          for (int t = 0; t < CBW; ++t) acc[t] = mfma_f64(a, T[k * LDT + 16 * (c0 + t) + l16], acc[t]);
         }
       } else {
         for (int k0 = 0; k0 < K0; k0 += 32) {
-          v8u16 ah = {0, 0, 0, 0, 0, 0, 0, 0}, am = ah, al = ah;
-          if (mrow < M0) {
-            const long long o = (long long)mrow * K0p + k0 + 8 * (lane >> 4);
-            const long long sl = (long long)M0 * K0p;
-            ah = *(const v8u16*)(A0s + o);
-            am = *(const v8u16*)(A0s + o + sl);
-            al = *(const v8u16*)(A0s + o + 2 * sl);
-          }
+          const long long o = (long long)mrc * K0p + k0 + 8 * kl;
+          const long long sl = (long long)M0 * K0p;
+          const v8u16 ah = *(const v8u16*)(A0s + o), am = *(const v8u16*)(A0s + o + sl),
+                      al = *(const v8u16*)(A0s + o + 2 * sl);
 #pragma unroll
-          for (int c = 0; c < 8; ++c) {
-            if (c < ncb1) {
-              const int j = 16 * c + (lane & 15);
-              v8u16 bh, bm, bl;
+          for (int t = 0; t < CBW; ++t) {
+            const int j = 16 * (c0 + t) + l16;
+            v8u16 bh, bm, bl;
 #pragma unroll
-              for (int e = 0; e < 8; ++e) {
-                const int k = k0 + 8 * (lane >> 4) + e;
-                float x = (j < M1 && k < K0) ? (float)T[k * LDT + j] : 0.f;
-                unsigned short h, m_, l;
-                split3q(x, h, m_, l);
-                bh[e] = h;
-                bm[e] = m_;
-                bl[e] = l;
-              }
-              acc[c] = mfma_bf16(al, bh, acc[c]);
-              acc[c] = mfma_bf16(am, bm, acc[c]);
-              acc[c] = mfma_bf16(ah, bl, acc[c]);
-              acc[c] = mfma_bf16(am, bh, acc[c]);
-              acc[c] = mfma_bf16(ah, bm, acc[c]);
-              acc[c] = mfma_bf16(ah, bh, acc[c]);
+            for (int e = 0; e < 8; ++e) {
+              const float x = (float)T[(k0 + 8 * kl + e) * LDT + j];
+              unsigned short h, m_, l;
+              split3q(x, h, m_, l);
+              bh[e] = h;
+              bm[e] = m_;
+              bl[e] = l;
             }
+            acc[t] = mfma_bf16(al, bh, acc[t]);
+            acc[t] = mfma_bf16(am, bm, acc[t]);
+            acc[t] = mfma_bf16(ah, bl, acc[t]);
+            acc[t] = mfma_bf16(am, bh, acc[t]);
+            acc[t] = mfma_bf16(ah, bm, acc[t]);
+            acc[t] = mfma_bf16(ah, bh, acc[t]);
           }
         }
       }
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        if (c >= ncb1) continue;
-        const int j = 16 * c + (lane & 15);
+      for (int t = 0; t < CBW; ++t) {
+        const int j = 16 * (c0 + t) + l16;
         if (j >= M1) continue;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int m = 16 * w + ((POL == 0) ? mfma16_row_f64(lane, r) : mfma16_row_f32(lane, r));
+          const int m = 16 * rb + ((POL == 0) ? mfma16_row_f64(lane, r) : mfma16_row_f32(lane, r));
           if (m >= M0) continue;
-          TX v = (TX)acc[c][r];
+          TX v = (TX)acc[t][r];
           if (epi == 1) v = v / E[(long long)m * M1 + j];
           if (epi == 2) v = v * E[(long long)m * M1 + j];
           Yb[(long long)m * M1 + j] = v;
@@ -227,23 +237,36 @@ int jfx_plane_contract(int prec, const double* A0d, const double* A1d, const uns
   if (B == 0) return JFX_OK;
   int dev = 0, ncu = 256;
   if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  long long grid = ncu > 0 ? ncu : 256;
-  if (prec == 1) grid *= 2;
+  if (ncu <= 0) ncu = 256;
+  // waves per 16-row block and column blocks per wave
+  const int nrb = ((K0 > M0 ? K0 : M0) + 15) >> 4, ncb = (M1 + 15) >> 4;
+  const int wpr = nrb == 1 ? 8 : nrb == 2 ? 4 : nrb <= 4 ? 2 : 1;
+  const int nb = (ncb + wpr - 1) / wpr, cbw = nb <= 1 ? 1 : nb <= 2 ? 2 : nb <= 4 ? 4 : 8;
+  // T holds K0p rows; two workgroups share a CU when the LDS and registers allow
+  const int K0p = (K0 + 31) & ~31;
+  const size_t shm = (size_t)K0p * (prec == 0 ? QT<0>::LDT * 8 : QT<3>::LDT * 4);
+  long long grid = (long long)ncu * ((shm <= 80 * 1024 && cbw <= 2) ? 2 : 1);
   if (grid > B) grid = B;
+  typedef void (*KF64)(const double*, const double*, const unsigned short*, const unsigned short*, const double*,
+                       double*, const double*, int, long long, int, int, int, int, int);
+  typedef void (*KF32)(const double*, const double*, const unsigned short*, const unsigned short*, const float*,
+                       float*, const float*, int, long long, int, int, int, int, int);
   if (prec == 0) {
     if (!A0d || !A1d) return set_error("plane_contract: FP64 operators missing"), JFX_EARG;
-    const size_t shm = (size_t)QMAX * QT<0>::LDT * 8;
-    hipError_t e = hipFuncSetAttribute((const void*)plane_kernel<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    KF64 kern = cbw == 1 ? plane_kernel<0, 1> : cbw == 2 ? plane_kernel<0, 2> : cbw == 4 ? plane_kernel<0, 4>
+                                                                                         : plane_kernel<0, 8>;
+    hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     if (e != hipSuccess) return set_error("plane: %s", hipGetErrorString(e)), JFX_EHIP;
-    hipLaunchKernelGGL(plane_kernel<0>, dim3((unsigned)grid), dim3(QNT), shm, stream, A0d, A1d, A0s, A1s,
-                       (const double*)X, (double*)Y, (const double*)E, epi, B, M0, K0, M1, K1);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(QNT), shm, stream, A0d, A1d, A0s, A1s,
+                       (const double*)X, (double*)Y, (const double*)E, epi, B, M0, K0, M1, K1, wpr);
   } else if (prec == 1) {
     if (!A0s || !A1s) return set_error("plane_contract: bf16 slices missing"), JFX_EARG;
-    const size_t shm = (size_t)QMAX * QT<3>::LDT * 4;
-    hipError_t e = hipFuncSetAttribute((const void*)plane_kernel<3>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    KF32 kern = cbw == 1 ? plane_kernel<3, 1> : cbw == 2 ? plane_kernel<3, 2> : cbw == 4 ? plane_kernel<3, 4>
+                                                                                         : plane_kernel<3, 8>;
+    hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     if (e != hipSuccess) return set_error("plane: %s", hipGetErrorString(e)), JFX_EHIP;
-    hipLaunchKernelGGL(plane_kernel<3>, dim3((unsigned)grid), dim3(QNT), shm, stream, A0d, A1d, A0s, A1s,
-                       (const float*)X, (float*)Y, (const float*)E, epi, B, M0, K0, M1, K1);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(QNT), shm, stream, A0d, A1d, A0s, A1s,
+                       (const float*)X, (float*)Y, (const float*)E, epi, B, M0, K0, M1, K1, wpr);
   } else {
     return set_error("plane_contract: bad prec"), JFX_EARG;
   }

@@ -140,7 +140,8 @@ def test_elementwise(dev):
     assert f.dtype == torch.float32 and np.allclose(f.cpu().numpy(), c.astype(np.float32))
 
 
-@pytest.mark.parametrize("dims", [(128, 128, 128, 128), (62, 64, 62, 64), (96, 30, 17, 128), (1, 5, 7, 3)])
+@pytest.mark.parametrize("dims", [(128, 128, 128, 128), (62, 64, 62, 64), (96, 30, 17, 128), (1, 5, 7, 3),
+                                  (80, 72, 64, 50), (20, 40, 100, 33)])
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-14), ("bf16x6", 2e-6)])
 @pytest.mark.parametrize("epi", [None, "div", "mul"])
 def test_plane_contract(dev, dims, prec, tol, epi):
@@ -163,4 +164,20 @@ def test_plane_contract(dev, dims, prec, tol, epi):
         Yr = Yr / E
     elif epi == "mul":
         Yr = Yr * E
+    assert O.rel_l2(Y.to(torch.float64).cpu().numpy(), Yr) < tol
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-14), ("bf16x6", 2e-6)])
+def test_plane_contract_many_slabs(dev, prec, tol):
+    # more slabs than resident workgroups: every workgroup walks several slabs
+    import torch
+
+    from jaxfun_amd import kernels as K
+
+    rng = np.random.default_rng(8)
+    A0, A1 = rng.standard_normal((64, 62)), rng.standard_normal((64, 62))
+    X = rng.standard_normal((700, 62, 62))
+    Xd = _t(X, dev, K.io_dtype(prec))
+    Y = K.plane(K.AxisOperator(_t(A0, dev)), K.AxisOperator(_t(A1, dev)), Xd, prec)
+    Yr = np.einsum("mk,bkl,nl->bmn", A0, Xd.to(torch.float64).cpu().numpy(), A1)
     assert O.rel_l2(Y.to(torch.float64).cpu().numpy(), Yr) < tol
