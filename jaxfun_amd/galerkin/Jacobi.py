@@ -28,6 +28,19 @@ def _mpf(v) -> mp.mpf:
     return mp.mpf(str(sp.sympify(v).evalf(_DPS)))
 
 
+def jacobi_end_derivative(a, b, g, j: int, side: str, k: int = 0) -> mp.mpf:
+    """d^k/dx^k of g P_j^(a,b) at x = +1 ("R") or x = -1 ("L"), from
+    d^k P_j = psi P_{j-k}^(a+k,b+k), psi = (j+a+b+1)_k / 2^k, and
+    P_n^(a,b)(1) = C(n+a, n), P_n^(a,b)(-1) = (-1)^n C(n+b, n)
+    (galerkin/Jacobi.py:257-304)."""
+    if j < k:
+        return mp.mpf(0)
+    psi = mp.rf(j + a + b + 1, k) / mp.mpf(2) ** k
+    if side == "R":
+        return g * psi * mp.binomial(j + a, j - k)
+    return g * (-1) ** (j + k) * psi * mp.binomial(j + b, j - k)
+
+
 class Jacobi(OrthogonalSpace):
     """P_n^(alpha,beta), n < N (galerkin/Jacobi.py:25-63)."""
 
@@ -106,14 +119,13 @@ class Jacobi(OrthogonalSpace):
                 h.append(float(self.gn(n) ** 2 * h0))
         return np.array(h)
 
-    def bnd_value(self, j: int, side: str) -> float:
-        """Q_j at x = -1 ("L") or x = +1 ("R"): g_j (-1)^j C(j + beta, j) and
-        g_j C(j + alpha, j) (galerkin/Jacobi.py:257-288, k = 0)."""
+    def bnd_value(self, j: int, side: str, k: int = 0, exact: bool = False):
+        """d^k Q_j at x = -1 ("L") or x = +1 ("R") on the reference domain
+        (galerkin/Jacobi.py:257-288); an mpmath number when ``exact``."""
         with mp.workdps(_DPS):
             a, b = self._params()
-            if side == "R":
-                return float(self.gn(j) * mp.binomial(j + a, j))
-            return float(self.gn(j) * (-1) ** j * mp.binomial(j + b, j))
+            v = jacobi_end_derivative(a, b, self.gn(j), j, side, k)
+            return +v if exact else float(v)
 
     def dirichlet_stencil(self) -> np.ndarray:
         """(N-2, N) S with phi_k = Q_k + d1_k Q_{k+1} + d2_k Q_{k+2} vanishing

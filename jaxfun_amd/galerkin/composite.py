@@ -23,13 +23,35 @@ from ..utils.common import lambdify, n
 from .orthogonal import OrthogonalSpace
 
 
+BC_ORDER = {"D": 0, "N": 1, "N2": 2, "N3": 3, "N4": 4}
+
+
+def bc_order(key: str) -> int:
+    """Derivative order of a condition code: D 0, N 1, N2..N4 2..4."""
+    if key not in BC_ORDER:
+        raise NotImplementedError(f"boundary condition {key!r}: D, N, N2, N3 and N4 are built")
+    return BC_ORDER[key]
+
+
 class BoundaryConditions(dict):
-    """Left/right boundary conditions (galerkin/composite.py:28-118)."""
+    """Left/right boundary conditions (galerkin/composite.py:28-118).
+
+    Built from a plain dict with a domain, the values of derivative
+    conditions Nk are divided by df^k (df = 2 / (b - a)), so that they are
+    given in physical units and stored for the reference domain; a
+    BoundaryConditions argument is taken as already normalised."""
 
     def __init__(self, bc: dict, domain=None) -> None:
         bcs = {"left": {}, "right": {}}
         bcs.update(copy.deepcopy(dict(bc)))
         dict.__init__(self, bcs)
+        if domain is not None and not isinstance(bc, BoundaryConditions):
+            df = 2.0 / (float(domain[1]) - float(domain[0]))
+            for side in self.values():
+                for k, v in side.items():
+                    if k[0] == "N":
+                        nd = int(k[1:]) if len(k) > 1 else 1
+                        side[k] = v / df**nd
 
     def orderednames(self) -> list[str]:
         return ["L" + b for b in sorted(self["left"])] + ["R" + b for b in sorted(self["right"])]
@@ -39,6 +61,14 @@ class BoundaryConditions(dict):
 
     def num_bcs(self) -> int:
         return len(self.orderedvals())
+
+    def num_derivatives(self) -> int:
+        """Sum of the derivative orders of all conditions (galerkin/composite.py:94-101)."""
+        return sum(bc_order(k) for side in self.values() for k in side)
+
+    def conditions(self) -> list[tuple[str, int]]:
+        """(side "L"/"R", derivative order) of each condition, in ordered form."""
+        return [(nm[0], bc_order(nm[1:])) for nm in self.orderednames()]
 
     def is_homogeneous(self) -> bool:
         return all(v == 0 for side in self.values() for v in side.values())
@@ -65,23 +95,50 @@ class Composite(OrthogonalSpace):
         self.orthogonal = orthogonal(N, domain=domain, system=system, **fam)
         self.family = self.orthogonal.family
         self.bcs = BoundaryConditions(bcs)
-        if "".join(self.bcs.orderednames()) != "LDRD" or not self.bcs.is_homogeneous():
-            raise NotImplementedError(
-                "only homogeneous Dirichlet composite bases are built on the GPU path "
-                f"(got {dict(self.bcs)})")
+        if not self.bcs.is_homogeneous():
+            raise ValueError(f"a composite basis takes homogeneous conditions (got {dict(self.bcs)})")
+        self.nbc = self.bcs.num_bcs()
+        self._conds = self.bcs.conditions()
+        if self.nbc == 0 or N <= self.nbc:
+            raise ValueError(f"need 0 < number of conditions < N (got {self.nbc} for N={N})")
+        self._dirichlet = "".join(self.bcs.orderednames()) == "LDRD"
         self.scaling = sp.S.One if scaling is None else sp.sympify(scaling)
-        self._jacobi_stencil = getattr(self.orthogonal, "dirichlet_stencil", None)
-        if self._jacobi_stencil is not None:
-            S = self._jacobi_stencil()
-            self.stencil = {0: 1 / self.scaling, 1: S[:, 1:-1].diagonal() / self.scaling,
-                            2: S[:, 2:].diagonal() / self.scaling}
-        else:
+        if self._dirichlet and self.family in ("legendre", "chebyshev"):
             self.stencil = {0: 1 / self.scaling, 2: -1 / self.scaling}
+        else:
+            S = self._general_stencil()
+            self.stencil = {o: S.diagonal(o) / self.scaling for o in range(self.nbc + 1)
+                            if np.any(S.diagonal(o) != 0)}
         self._mass_lu = None
 
     @property
     def dim(self) -> int:
-        return self.N - 2
+        return self.N - self.nbc
+
+    def _general_stencil(self) -> np.ndarray:
+        """(N - nb, N) stencil of phi_k = psi_k + sum_{i=1..nb} d_i psi_{k+i}
+        with every condition of ``bcs`` homogeneous (get_stencil_matrix,
+        galerkin/composite.py:765-832): per k the nb x nb system of the
+        conditions' end values, solved in 40-digit arithmetic; entries below
+        1e-30 (the ones that vanish by symmetry) are set to zero."""
+        if "S_general" in self._ops:
+            return self._ops["S_general"]
+        import mpmath as mp
+
+        N, nb = self.N, self.nbc
+        S = np.zeros((N - nb, N))
+        with mp.workdps(40):
+            f = [[mp.mpf(self.orthogonal.bnd_value(j, side, k, exact=True)) for j in range(N)]
+                 for side, k in self._conds]
+            for row in range(N - nb):
+                A = mp.matrix([[f[c][row + i] for i in range(1, nb + 1)] for c in range(nb)])
+                b = mp.matrix([-f[c][row] for c in range(nb)])
+                d = mp.lu_solve(A, b)
+                S[row, row] = 1.0
+                for i in range(nb):
+                    S[row, row + 1 + i] = 0.0 if abs(d[i]) < mp.mpf(10) ** -30 else float(d[i])
+        self._ops["S_general"] = S
+        return S
 
     def norm_squared(self) -> np.ndarray:
         return self.orthogonal.norm_squared()
@@ -96,43 +153,50 @@ class Composite(OrthogonalSpace):
         """S (dim, N) (galerkin/composite.py:255-269)."""
         m = self.dim
         s = self.scaling_values()
-        if self._jacobi_stencil is not None:
-            if "S_host" not in self._ops:
-                self._ops["S_host"] = self._jacobi_stencil()
-            return self._ops["S_host"] / s[:, None]
+        if not (self._dirichlet and self.family in ("legendre", "chebyshev")):
+            return self._general_stencil() / s[:, None]
         S = np.zeros((m, self.N))
         S[np.arange(m), np.arange(m)] = 1.0 / s
         S[np.arange(m), np.arange(m) + 2] = -1.0 / s
         return S
 
+    @property
+    def bandwidth(self) -> int:
+        """Half bandwidth w of S S^T and of the mass matrix: the stencil reaches nb modes ahead."""
+        return self.nbc
+
+    @staticmethod
+    def _to_band(M: np.ndarray, w: int) -> np.ndarray:
+        m = M.shape[0]
+        band = np.zeros((m, 2 * w + 1))
+        for i in range(m):
+            for o in range(-w, w + 1):
+                if 0 <= i + o < m:
+                    band[i, o + w] = M[i, i + o]
+        return band
+
     def mass_band(self) -> np.ndarray:
-        """S diag(h/df) S^T as a (dim, 5) by-row band, offsets -2..2 (composite.py:330-337)."""
+        """S diag(h/df) S^T as a (dim, 2w+1) by-row band, offsets -w..w (composite.py:330-337)."""
         S = self.stencil_matrix()
         M = S @ np.diag(self.orthogonal.norm_squared() / float(self.domain_factor)) @ S.T
-        m = self.dim
-        band = np.zeros((m, 5))
-        for i in range(m):
-            for o in range(-2, 3):
-                if 0 <= i + o < m:
-                    band[i, o + 2] = M[i, i + o]
-        return band
+        return self._to_band(M, self.bandwidth)
 
     def mass_lu(self) -> torch.Tensor:
         if self._mass_lu is None:
             band = torch.tensor(self.mass_band(), dtype=torch.float64, device=K.device())
-            self._mass_lu = K.band_lu_factor(band, 2, 2)
+            self._mass_lu = K.band_lu_factor(band, self.bandwidth, self.bandwidth)
         return self._mass_lu
 
     def mass_matrix(self):
         from ..la import DiaMatrix
 
-        band = self.mass_band()
+        band, w = self.mass_band(), self.bandwidth
         m = self.dim
         M = np.zeros((m, m))
         for i in range(m):
-            for o in range(-2, 3):
+            for o in range(-w, w + 1):
                 if 0 <= i + o < m:
-                    M[i, i + o] = band[i, o + 2]
+                    M[i, i + o] = band[i, o + w]
         return DiaMatrix.from_dense(M)
 
     def forward_operator(self, N: int | None = None) -> K.AxisOperator:
@@ -143,7 +207,7 @@ class Composite(OrthogonalSpace):
         key = ("fwd", N)
         if key not in self._ops:
             L = self.scalar_product_operator(N).A64.clone()
-            K.band_lu_solve(self.mass_lu(), 2, 2, L, 0)
+            K.band_lu_solve(self.mass_lu(), self.bandwidth, self.bandwidth, L, 0)
             self._ops[key] = K.AxisOperator(L)
         return self._ops[key]
 
@@ -167,15 +231,10 @@ class Composite(OrthogonalSpace):
         key = "P_lu"
         if key not in self._ops:
             S = self.stencil_matrix()
-            P = S @ S.T
-            m = self.dim
-            band = np.zeros((m, 5))
-            for i in range(m):
-                for o in range(-2, 3):
-                    if 0 <= i + o < m:
-                        band[i, o + 2] = P[i, i + o]
-            self._ops[key] = K.band_lu_factor(torch.tensor(band, device=K.device()), 2, 2)
-        return K.band_lu_solve(self._ops[key], 2, 2, b, b.ndim - 1)
+            w = self.bandwidth
+            band = self._to_band(S @ S.T, w)
+            self._ops[key] = K.band_lu_factor(torch.tensor(band, device=K.device()), w, w)
+        return K.band_lu_solve(self._ops[key], self.bandwidth, self.bandwidth, b, b.ndim - 1)
 
     def get_homogeneous(self):
         return self
@@ -190,24 +249,40 @@ class Composite(OrthogonalSpace):
 
 # ----------------------------------------------------------------------------- lifting
 def get_bc_basis(bcs, orthogonal) -> np.ndarray:
-    """Lifting basis (num_bcs, num_bcs) (galerkin/composite.py:835-895).
+    """Lifting basis (num_bcs, first + num_bcs) (galerkin/composite.py:835-895).
 
-    Row r holds the orthogonal coefficients of the function that takes the
-    value 1 in boundary condition r and 0 in the others: with the boundary
-    matrix ``A[r, j] = psi_j(x_r)`` over the first num_bcs orthogonal
-    functions, the rows are those of ``(A^-1)^T``.  Dirichlet conditions
-    only (the conditions of the composite bases built here); for
-    ``{left: D, right: D}`` on Legendre/Chebyshev the rows are
-    (psi_0 -+ psi_1) / 2.  Boundary values come from the orthogonal space
-    (``bnd_value``), so the Jacobi family's scalings are honoured.
+    Row r holds the orthogonal coefficients of the function whose condition r
+    equals 1 and whose other conditions vanish: with the boundary matrix
+    ``A[r, j] = d^{k_r} psi_{first+j}(x_r)`` (x_r the end of condition r,
+    k_r its derivative order) the rows are those of ``(A^-1)^T``, placed in
+    columns first..first+nb-1.  ``first`` is the smallest offset (up to the
+    number of derivatives) whose A is invertible; for ``{left: D, right: D}``
+    on Legendre/Chebyshev first = 0 and the rows are (psi_0 -+ psi_1) / 2.
+    End values come from the orthogonal space (``bnd_value``), so the Jacobi
+    family's scalings are honoured.
     """
+    import mpmath as mp
+
     bcs = bcs if isinstance(bcs, BoundaryConditions) else BoundaryConditions(bcs)
-    names = bcs.orderednames()
-    if any(nm[1:] != "D" for nm in names):
-        raise NotImplementedError(f"lifting of {names}: only Dirichlet conditions are built")
-    nb = len(names)
-    A = np.array([[orthogonal.bnd_value(j, nm[0]) for j in range(nb)] for nm in names])
-    return np.linalg.inv(A).T.copy()
+    conds = bcs.conditions()
+    nb = len(conds)
+    with mp.workdps(40):
+        for first in range(bcs.num_derivatives() + 1):
+            A = mp.matrix([[mp.mpf(orthogonal.bnd_value(first + j, side, k, exact=True)) for j in range(nb)]
+                           for side, k in conds])
+            try:
+                singular = abs(mp.det(A)) < mp.mpf(10) ** -20
+            except (ZeroDivisionError, TypeError, ValueError):  # mpmath's LU finds no pivot
+                singular = True
+            if singular:
+                continue
+            Ainv = A ** -1
+            S = np.zeros((nb, first + nb))
+            for r in range(nb):
+                for j in range(nb):
+                    S[r, first + j] = float(Ainv[j, r])
+            return S
+    raise ValueError(f"no lifting basis for {bcs.orderednames()}")
 
 
 def _as_number(v):
@@ -232,12 +307,16 @@ class BCGeneric(Composite):
         self.bcs = BoundaryConditions(bcs)
         nb = self.bcs.num_bcs()
         nq = int(num_quad_points) if num_quad_points else nb
-        OrthogonalSpace.__init__(self, nb, domain=domain, system=system, name=name, fun_str=fun_str,
+        fam = family_kwargs(kw)
+        # the orthogonal space is sized once the lifting basis is known
+        probe = orthogonal(nb + self.bcs.num_derivatives(), domain=domain, system=system, **fam)
+        S = get_bc_basis(self.bcs, probe)
+        OrthogonalSpace.__init__(self, S.shape[1], domain=domain, system=system, name=name, fun_str=fun_str,
                                  num_quad_points=nq)
         self.bcs = BoundaryConditions(bcs)
-        self.orthogonal = orthogonal(nb, domain=domain, system=system, num_quad_points=nq, **family_kwargs(kw))
+        self.orthogonal = orthogonal(S.shape[1], domain=domain, system=system, num_quad_points=nq, **fam)
         self.family = self.orthogonal.family
-        self.S = get_bc_basis(self.bcs, self.orthogonal)
+        self.S = S
         self.scaling = sp.S.One
         self.stencil = None
         self._mass_lu = None

@@ -141,13 +141,25 @@ class OrthogonalSpace:
         """Orthogonal-basis (derivative) Vandermonde at reference points (orthogonal.py:131-178)."""
         return self._vander(X, k)
 
-    def bnd_value(self, j: int, side: str) -> float:
-        """psi_j at the reference-domain end ("L": x = -1, "R": x = +1): 1 and
-        (-1)^j for Legendre and Chebyshev (the k = 0 ``bnd_values`` of
-        galerkin/Jacobi.py:257-288 and galerkin/Chebyshev.py)."""
+    def bnd_value(self, j: int, side: str, k: int = 0, exact: bool = False):
+        """d^k psi_j at the reference-domain end ("L": x = -1, "R": x = +1)
+        (the ``bnd_values`` of galerkin/Jacobi.py:257-288 and
+        galerkin/Chebyshev.py): Legendre is P^(0,0); Chebyshev is
+        P^(-1/2,-1/2) scaled by 1 / C(j - 1/2, j), so T_j(1) = 1 and
+        T_j'(1) = j^2.  An mpmath number when ``exact``."""
+        import mpmath as mp
+
+        from .Jacobi import jacobi_end_derivative
+
         if self.family not in ("legendre", "chebyshev"):
             raise NotImplementedError(f"boundary values of the {self.family!r} family")
-        return 1.0 if side == "R" else float((-1) ** j)
+        with mp.workdps(40):
+            if self.family == "legendre":
+                v = jacobi_end_derivative(mp.mpf(0), mp.mpf(0), mp.mpf(1), j, side, k)
+            else:
+                h = mp.mpf(-1) / 2
+                v = jacobi_end_derivative(h, h, 1 / mp.binomial(j + h, j), j, side, k)
+            return +v if exact else float(v)
 
     def evaluate_basis_derivative(self, X: torch.Tensor, k: int = 0) -> torch.Tensor:
         V = self.orthogonal.vandermonde(X, k)

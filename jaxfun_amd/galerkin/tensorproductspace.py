@@ -19,7 +19,7 @@ import torch
 from .. import kernels as K
 from ..coordinates import CartCoordSys, default_system
 from ..utils.common import lambdify
-from .composite import BCGeneric, BoundaryConditions, DirectSum
+from .composite import BCGeneric, BoundaryConditions, DirectSum, bc_order
 from .orthogonal import MeshKind, OrthogonalSpace, as_device
 
 
@@ -245,7 +245,8 @@ def _project_1d(expr, V) -> torch.Tensor:
 
 
 def _lifting_bcs(bcspace: BCGeneric, values) -> BCGeneric:
-    """Copy of a lifting space with new ordered Dirichlet values."""
+    """Copy of a lifting space with new ordered values (already in
+    reference-domain units)."""
     b = copy.copy(bcspace)
     names = bcspace.bcs.orderednames()
     bcs = {"left": {}, "right": {}}
@@ -262,11 +263,13 @@ class DirectSumTPS(TensorProductSpace):
     The trial space splits into the tensor products of every combination of
     homogeneous and lifting summands (``tpspaces``).  Each product with
     lifting factors on the axes L carries a fixed coefficient array
-    (``bndvals``): for every choice of boundary (left/right) on the axes in L,
-    the boundary function of the first axis in L, evaluated at the chosen ends
-    of the other axes in L, projected onto the remaining axes' spaces — where a
-    remaining DirectSum axis is itself lifted by that function's values at its
-    ends (for one remaining axis a 1D DirectSum, for two a 2D DirectSumTPS).
+    (``bndvals``): for every choice of boundary condition on the axes in L,
+    the boundary function of the first axis in L, with the chosen conditions
+    of the other axes in L applied to it (its value, or its k-th derivative
+    for an Nk condition, at that end), projected onto the remaining axes'
+    spaces — where a remaining DirectSum axis is itself lifted by that
+    function's own boundary data (for one remaining axis a 1D DirectSum, for
+    two a 2D DirectSumTPS).
     In 2D this is: boundary functions of the other coordinate projected onto
     the other axis (lifted by their corner values when both axes are lifted),
     and the four corner values for the product of both lifting spaces.
@@ -311,27 +314,41 @@ class DirectSumTPS(TensorProductSpace):
         for js in itertools.product(*[range(n) for n in nb]):
             g = sp.sympify(combo[L[0]].bcs.orderedvals()[js[0]])
             for a, j in zip(L[1:], js[1:]):
-                g = g.subs(syms[a], ends[a][j])
+                g = self._apply_condition(g, a, combo[a].bcs.orderednames()[j], syms, ends)
             blocks[js] = self._project_remaining(g, combo, R, syms, ends)
         cplx = any(isinstance(b, complex) or (isinstance(b, torch.Tensor) and b.is_complex())
                    for b in blocks.values())
-        shape = [2 if a in L else combo[a].dim for a in range(d)]
+        shape = [combo[a].dim for a in range(d)]
         out = torch.zeros(shape, dtype=torch.complex128 if cplx else torch.float64, device=K.device())
         for js, b in blocks.items():
             idx = tuple(js[L.index(a)] if a in L else slice(None) for a in range(d))
             out[idx] = b if isinstance(b, torch.Tensor) else torch.tensor(b, dtype=out.dtype, device=out.device)
         return out
 
+    def _apply_condition(self, g, a, name, syms, ends):
+        """Condition ``name`` ("LD", "RN2", ...) of axis a applied to g: the
+        value (D) or the k-th derivative (Nk) at that end, in reference-domain
+        units (divided by df^k), as galerkin/tensorproductspace.py:636-662
+        builds the projected conditions."""
+        k = bc_order(name[1:])
+        z = ends[a][0 if name[0] == "L" else 1]
+        e = sp.sympify(g)
+        if k:
+            df = 2.0 / (ends[a][1] - ends[a][0])
+            e = sp.diff(e, syms[a], k) / df**k
+        return e.subs(syms[a], z)
+
     def _project_remaining(self, g, combo, R, syms, ends):
         """Coefficients of g (a function of the coordinates of R) in the
         product of the R axes' spaces, each remaining DirectSum axis lifted by
-        g's values at its ends."""
+        g's own boundary data on it (values, or derivatives for Nk conditions)."""
         if not R:
             return _num_or_expr(g)
         spaces = []
         for r in R:
             if isinstance(self.basespaces[r], DirectSum):
-                vals = [_num_or_expr(g.subs(syms[r], z)) for z in ends[r]]
+                names = self.basespaces[r][1].bcs.orderednames()
+                vals = [_num_or_expr(self._apply_condition(g, r, nm, syms, ends)) for nm in names]
                 spaces.append(DirectSum(combo[r], _lifting_bcs(self.basespaces[r][1], vals)))
             else:
                 spaces.append(combo[r])

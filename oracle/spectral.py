@@ -10,6 +10,11 @@ pinned by checks that do not retype the formula (all in ``tests/test_oracle.py``
 * Vandermonde + derivatives: ``numpy.polynomial.{legendre,chebyshev}``
   ``legvander/chebvander`` + ``legder/chebder`` (what the reference's own
   ``tests/galerkin/test_approximations.py:48-87`` compares against).
+* General boundary conditions: the reference's closed-form stencils
+  (galerkin/composite.py:786-800, the Neumann example of its docstring),
+  basis functions meeting every condition under ``numpy.polynomial``
+  differentiation, lifting rows meeting theirs, exact recovery of
+  polynomial biharmonic and mixed Neumann solutions.
 * Jacobi family: ``scipy.special.eval_jacobi`` / ``eval_chebyu`` /
   ``eval_gegenbauer``; derivatives by exact power-basis differentiation;
   norms by Gauss-Jacobi orthogonality; nodes as zeros of P_N with weights
@@ -206,6 +211,53 @@ def jacobi_family(family):
     if family == "ultraspherical":
         return JacobiFamily(0.5, 0.5, "ultraspherical")
     return None
+
+
+# --------------------------------------------------------------------------
+# General boundary conditions (D, N, N2, N3, N4)
+# --------------------------------------------------------------------------
+
+BC_ORDER = {"D": 0, "N": 1, "N2": 2, "N3": 3, "N4": 4}
+
+
+def bc_conditions(bcs: dict) -> list[tuple[str, int]]:
+    """(side, derivative order) per condition, in the reference's order
+    (left before right, codes sorted; galerkin/composite.py:73-90)."""
+    return [("L", BC_ORDER[k]) for k in sorted(bcs.get("left", {}))] + \
+        [("R", BC_ORDER[k]) for k in sorted(bcs.get("right", {}))]
+
+
+def end_values(family, N: int, side: str, k: int) -> np.ndarray:
+    """d^k psi_j at x = -1 ("L") or +1 ("R"), j < N, from the oracle's
+    derivative Vandermonde."""
+    return vandermonde(family, np.array([-1.0 if side == "L" else 1.0]), N, k)[0]
+
+
+def general_stencil(family, N: int, conds) -> np.ndarray:
+    """(N - nb, N) stencil of phi_n = psi_n + sum_i d_i psi_{n+i} satisfying
+    every condition homogeneously: per n the nb x nb system of end values
+    (get_stencil_matrix, galerkin/composite.py:765-832)."""
+    nb = len(conds)
+    E = np.array([end_values(family, N, side, k) for side, k in conds])
+    S = np.zeros((N - nb, N))
+    for row in range(N - nb):
+        d = np.linalg.solve(E[:, row + 1:row + 1 + nb], -E[:, row])
+        S[row, row] = 1.0
+        S[row, row + 1:row + 1 + nb] = d
+    return S
+
+
+def general_lift_basis(family, conds, num_derivatives: int) -> np.ndarray:
+    """(nb, first + nb) lifting basis (get_bc_basis, galerkin/composite.py:835-895):
+    rows of (A^-1)^T for the first offset whose boundary matrix is regular."""
+    nb = len(conds)
+    for first in range(num_derivatives + 1):
+        A = np.array([end_values(family, first + nb, side, k)[first:] for side, k in conds])
+        if abs(np.linalg.det(A)) > 1e-12:
+            S = np.zeros((nb, first + nb))
+            S[:, first:] = np.linalg.inv(A).T
+            return S
+    raise ValueError("no lifting basis")
 
 
 # --------------------------------------------------------------------------

@@ -610,3 +610,52 @@ def test_jacobi_host_tables_vs_oracle(a, b, kind):
     assert np.allclose([sp_.bnd_value(j, "L") for j in range(N)], E[0], rtol=1e-13)
     assert np.allclose([sp_.bnd_value(j, "R") for j in range(N)], E[1], rtol=1e-13)
     assert np.allclose(sp_.dirichlet_stencil(), F.dirichlet_stencil(N), atol=1e-13)
+
+
+# ----------------------------------------------------------------- general boundary conditions
+BCSETS = {
+    "LDLNRDRN": {"left": {"D": 0, "N": 0}, "right": {"D": 0, "N": 0}},
+    "LNRN": {"left": {"N": 0}, "right": {"N": 0}},
+    "LDRN": {"left": {"D": 0}, "right": {"N": 0}},
+    "LDLN2RD": {"left": {"D": 0, "N2": 0}, "right": {"D": 0}},
+}
+
+
+@pytest.mark.parametrize("family", ["legendre", "chebyshev"])
+def test_general_stencil_closed_forms(family):
+    # galerkin/composite.py:786-800 (LDLNRDRN) and the Chebyshev Neumann
+    # stencil T_n - n^2/(n+2)^2 T_{n+2} of get_stencil_matrix's docstring
+    N = 16
+    S = O.general_stencil(family, N, O.bc_conditions(BCSETS["LDLNRDRN"]))
+    n = np.arange(N - 4)
+    if family == "legendre":
+        d2, d4 = 2 * (-2 * n - 5) / (2 * n + 7), (2 * n + 3) / (2 * n + 7)
+    else:
+        d2, d4 = 2 * (-n - 2) / (n + 3), (n + 1) / (n + 3)
+    assert np.allclose(np.diag(S, 2), d2, atol=1e-13) and np.allclose(np.diag(S, 4), d4, atol=1e-13)
+    assert np.abs(np.diag(S, 1)).max() < 1e-13 and np.abs(np.diag(S, 3)).max() < 1e-13
+    dd = O.bc_conditions({"left": {"D": 0}, "right": {"D": 0}})
+    assert np.allclose(O.general_stencil(family, N, dd), O.dirichlet_stencil(N), atol=1e-14)
+    if family == "chebyshev":
+        S = O.general_stencil(family, N, O.bc_conditions(BCSETS["LNRN"]))
+        k = np.arange(N - 2)
+        assert np.allclose(np.diag(S, 2), -k**2 / (k + 2) ** 2, atol=1e-14)
+
+
+@pytest.mark.parametrize("family", ["legendre", "chebyshev"])
+@pytest.mark.parametrize("bcs", list(BCSETS))
+def test_general_stencil_meets_conditions(family, bcs):
+    # every basis function meets every condition, differentiated by numpy.polynomial
+    val, der = FNS[family]
+    N = 14
+    conds = O.bc_conditions(BCSETS[bcs])
+    S = O.general_stencil(family, N, conds)
+    for row in S:
+        for side, k in conds:
+            c = der(row, k) if k else row
+            assert abs(val(-1.0 if side == "L" else 1.0, c)) < 1e-13 * max(1.0, np.abs(c).sum())
+    B = O.general_lift_basis(family, conds, sum(k for _, k in conds))
+    for r, row in enumerate(B):
+        for c_i, (side, k) in enumerate(conds):
+            c = der(row, k) if k else row
+            assert abs(val(-1.0 if side == "L" else 1.0, c) - (r == c_i)) < 1e-12
