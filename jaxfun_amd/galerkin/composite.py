@@ -94,9 +94,7 @@ class Composite(OrthogonalSpace):
         super().__init__(N, domain=domain, system=system, name=name, fun_str=fun_str, **kw)
         self.orthogonal = orthogonal(N, domain=domain, system=system, **fam)
         self.family = self.orthogonal.family
-        self.bcs = BoundaryConditions(bcs)
-        if not self.bcs.is_homogeneous():
-            raise ValueError(f"a composite basis takes homogeneous conditions (got {dict(self.bcs)})")
+        self.bcs = BoundaryConditions(bcs)  # the stencil depends on the condition codes only
         self.nbc = self.bcs.num_bcs()
         self._conds = self.bcs.conditions()
         if self.nbc == 0 or N <= self.nbc:
@@ -237,7 +235,12 @@ class Composite(OrthogonalSpace):
         return K.band_lu_solve(self._ops[key], self.bandwidth, self.bandwidth, b, b.ndim - 1)
 
     def get_homogeneous(self):
-        return self
+        """This basis with every boundary value set to zero (galerkin/composite.py)."""
+        if self.bcs.is_homogeneous():
+            return self
+        h = copy.copy(self)
+        h.bcs = self.bcs.get_homogeneous()
+        return h
 
     def get_orthogonal(self):
         return self.orthogonal

@@ -1,0 +1,144 @@
+"""The reference's own approximation and composite tests, run on the GPU path
+(tests/galerkin/test_approximations.py:90-200, tests/galerkin/test_composite.py:44-110)
+with its families, domains and tolerances (FP64: sqrt(ulp(10)) etc.)."""
+
+import numpy as np
+import pytest
+import sympy as sp
+
+from oracle import spectral as O
+
+pytestmark = pytest.mark.gpu
+
+EPS = np.finfo(float).eps
+JSPACES = ["Legendre", "Chebyshev", "ChebyshevU", "Jacobi"]
+DOMAINS = [(-1.0, 1.0), (-2.0, 2.0)]  # the reference's domain fixture (test_approximations.py:33-37)
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _cls(name):
+    import jaxfun_amd.galerkin as G
+
+    return getattr(G, name)
+
+
+@pytest.mark.parametrize("space", JSPACES)
+@pytest.mark.parametrize("domain", DOMAINS)
+def test_backward_primitive(dev, space, domain):
+    # test_approximations.py:90-104
+    from jaxfun_amd.galerkin.arguments import JAXFunction
+
+    D = _cls(space)(16, domain=domain)
+    x = D.system.x
+    f = sp.sin(x)
+    uf = JAXFunction(f, D)
+    for k, tol in ((1, np.sqrt(10 * EPS)), (2, np.sqrt(100 * EPS))):
+        du = JAXFunction(sp.diff(f, x, k), D)
+        err = np.linalg.norm(_np(D.backward_primitive(uf.array, k)) - _np(du.backward()))
+        assert err < tol, (k, err)
+
+
+@pytest.mark.parametrize("space", JSPACES)
+@pytest.mark.parametrize("domain", DOMAINS)
+def test_backward_primitive_composite(dev, space, domain):
+    # test_approximations.py:107-122
+    from jaxfun_amd import FunctionSpace
+    from jaxfun_amd.galerkin.arguments import JAXFunction
+
+    D = FunctionSpace(24, _cls(space), bcs={"left": {"D": 0}, "right": {"D": 0}}, domain=domain)
+    x = D.system.x
+    a, b = domain
+    f = sp.sin((x - a) / (b - a) * sp.pi)  # vanishes at both ends of the domain
+    uf = JAXFunction(f, D)
+    for k, tol in ((1, np.sqrt(10 * EPS)), (2, np.sqrt(100 * EPS))):
+        du = JAXFunction(sp.diff(f, x, k), D.orthogonal)
+        err = np.linalg.norm(_np(D.backward_primitive(uf.array, k)) - _np(du.backward()))
+        assert err < tol * max(1.0, (np.pi / (b - a)) ** k), (k, err)
+
+
+@pytest.mark.parametrize("space", JSPACES)
+@pytest.mark.parametrize("domain", DOMAINS)
+def test_backward_primitive_directsum(dev, space, domain):
+    # test_approximations.py:125-147
+    from jaxfun_amd import FunctionSpace
+    from jaxfun_amd.coordinates import x as X
+    from jaxfun_amd.galerkin.arguments import JAXFunction
+
+    f = sp.cos(X * sp.pi)
+    bcs = {"left": {"D": f.subs(X, domain[0])}, "right": {"D": f.subs(X, domain[1])}}
+    D = FunctionSpace(24, _cls(space), bcs=bcs, domain=domain)
+    (x,) = D.system.base_scalars()
+    f = f.subs(X, x)
+    uf = JAXFunction(f, D)
+    for k in (1, 2):
+        du = JAXFunction(sp.diff(f, x, k), D.orthogonal)
+        err = np.linalg.norm(_np(D.backward_primitive(uf.array, k)) - _np(du.backward()))
+        assert err < np.sqrt(100 * EPS) * np.pi**k, (k, err)
+
+
+@pytest.mark.parametrize("space", ["Legendre", "Chebyshev", "ChebyshevU", "Ultraspherical", "Fourier"])
+def test_backward_primitive_2d(dev, space):
+    # test_approximations.py:150-163
+    from jaxfun_amd import TensorProduct
+    from jaxfun_amd.galerkin.arguments import JAXFunction
+
+    D = _cls(space)(16)
+    T = TensorProduct(D, D)
+    x, y = T.system.base_scalars()
+    f = sp.sin(x) * sp.cos(y)
+    uf = JAXFunction(f, T)
+    du = JAXFunction(sp.diff(f, x, y), T)
+    err = np.linalg.norm(_np(T.backward_primitive(uf.get_array(), (1, 1))) - _np(du.backward()))
+    assert err < np.sqrt(100 * EPS), err
+
+
+@pytest.mark.parametrize("space", JSPACES)
+@pytest.mark.parametrize("domain", DOMAINS)
+def test_backward_primitive_directsum_2d(dev, space, domain):
+    # test_approximations.py:166-200
+    from jaxfun_amd import FunctionSpace, TensorProduct
+    from jaxfun_amd.coordinates import x as X, y as Y
+    from jaxfun_amd.galerkin.arguments import JAXFunction
+
+    f = sp.cos(X * sp.pi) * sp.cos(Y * sp.pi)
+    bcsx = {"left": {"D": f.subs(X, domain[0])}, "right": {"D": f.subs(X, domain[1])}}
+    bcsy = {"left": {"D": f.subs(Y, domain[0])}, "right": {"D": f.subs(Y, domain[1])}}
+    Dx = FunctionSpace(24, _cls(space), bcs=bcsx, domain=domain)
+    Dy = FunctionSpace(24, _cls(space), bcs=bcsy, domain=domain)
+    T = TensorProduct(Dx, Dy)
+    x, y = T.system.base_scalars()
+    f = f.subs({X: x, Y: y})
+    uf = JAXFunction(f, T)
+    du = JAXFunction(sp.diff(f, x, y), T.get_orthogonal())
+    err = np.linalg.norm(_np(T.backward_primitive(uf.get_array(), (1, 1))) - _np(du.backward()))
+    assert err < np.sqrt(1000 * EPS) * np.pi**2, err
+
+
+@pytest.mark.parametrize("space", ["Legendre", "Chebyshev", "ChebyshevU"])
+def test_composite_cases(dev, space):
+    # test_composite.py:44-110
+    from jaxfun_amd import FunctionSpace
+    from jaxfun_amd.galerkin import BCGeneric, BoundaryConditions, Composite, DirectSum, get_bc_basis
+    from jaxfun_amd.galerkin.arguments import JAXFunction
+
+    cls = _cls(space)
+    C = Composite(10, cls, {"left": {"D": 0}, "right": {"D": 0}})
+    M = C.mass_matrix().todense_host()
+    assert M.shape == (C.dim, C.dim) and np.all(np.linalg.eigvalsh(0.5 * (M + M.T)) > 0)
+    B = BCGeneric(3, cls, {"left": {"D": 0, "N": 1}, "right": {"D": 2}})
+    assert B.dim == 3
+    assert _np(B.quad_points_and_weights()[0]).shape[0] == B.orthogonal.num_quad_points
+    FS = FunctionSpace(6, cls, bcs={"left": {"D": 1}, "right": {"D": 2}})
+    assert isinstance(FS, DirectSum)
+    C0 = FS[0]
+    assert np.isfinite(float(_np(FS.evaluate(np.array([0.1]), np.ones(C0.dim)))[0]))
+    assert _np(FS.backward(np.ones(C0.dim))).shape[0] == C0.num_quad_points
+    u = JAXFunction(np.zeros(4), FS)
+    assert abs(float(u(1)) - 2.0) < 1e-14 and abs(float(u(-1)) - 1.0) < 1e-14
+    Bb = get_bc_basis(BoundaryConditions({"left": {"D": 0}, "right": {"N": 0}}), cls(6))
+    assert Bb.shape[0] == 2
+    H = Composite(8, cls, {"left": {"D": 1}, "right": {"N": 1}}).get_homogeneous()
+    assert H.bcs.is_homogeneous()
