@@ -2,6 +2,7 @@
 
 from .matrix import BaseMatrix, DiaMatrix, Matrix, SolverNotApplicable
 from .tpmatrix import (
+    TPLUFactors,
     TPMatrices,
     TPMatricesDenseLUFactors,
     TPMatricesLUFactors,
@@ -18,6 +19,7 @@ __all__ = [
     "DiaMatrix",
     "Matrix",
     "SolverNotApplicable",
+    "TPLUFactors",
     "TPMatrices",
     "TPMatricesDenseLUFactors",
     "TPMatricesLUFactors",

@@ -99,6 +99,10 @@ class TPMatrix:
             x = m.solve(x, axis=nb + i)
         return K.lincomb([(1.0 / float(self.coefficient), x)])
 
+    def lu_factor(self) -> "TPLUFactors":
+        """Per-factor factorisations of this Kronecker product (la/tpmatrix.py:205-220)."""
+        return TPLUFactors(self)
+
     def todense_host(self) -> np.ndarray:
         return tpmats_to_kron([self])
 
@@ -114,6 +118,20 @@ class TPMatrix:
         if isinstance(other, TPMatrices):
             return TPMatrices([self, *other.tpmats])
         return NotImplemented
+
+
+class TPLUFactors:
+    """Solve of ``scale * A0 (x) A1 (x) ...`` by one factor solve per axis
+    (la/tpmatrix.py:223-255); the factors keep their own (banded or dense)
+    factorisations."""
+
+    def __init__(self, tp: TPMatrix) -> None:
+        self.tp = tp
+        self.scale = tp.coefficient
+        self.shape = tuple(m.shape[0] for m in tp.mats)
+
+    def solve(self, rhs) -> torch.Tensor:
+        return self.tp.solve(rhs)
 
 
 def _solve_diagonal(d: np.ndarray, rhs) -> torch.Tensor:
