@@ -134,10 +134,12 @@ class IMEXRungeKutta:
         return self.mass(U, self.precision)
 
     def apply_linear(self, U):
-        return self.linear(U, self.precision)
+        return self.linear(U, self.precision) if self.linear is not None else None
 
     def nonlinear_rhs_scalar_product(self, U):
         """(v, N(u)) with N evaluated on the padded mesh (integrators/base.py:238-248)."""
+        if not self.nonlinear:
+            return None
         T = self.testspace
         prec = self.precision
         nb = U.ndim - 2
@@ -158,7 +160,9 @@ class IMEXRungeKutta:
     # -------------------------------------------------------------- stepping
     def step(self, U, dt: float):
         """One step (integrators/imex_rk.py:96-159); stage 0 of a tableau with
-        a_00 = 0 is U itself (M^-1 M U)."""
+        a_00 = 0 is U itself (M^-1 M U).  A globally stiffly accurate tableau
+        returns its last stage; otherwise the update is
+        M^-1 (M U + dt sum_i (bE_i N_i + bI_i L_i))."""
         tab = self.tableau
         aE, aI = tab.explicit.A, tab.implicit.A
         mu = self.apply_mass(U)
@@ -167,9 +171,9 @@ class IMEXRungeKutta:
         for i in range(tab.stages):
             terms = [(1.0, mu)]
             for j in range(i):
-                if aE[i][j] != 0.0:
+                if aE[i][j] != 0.0 and Ns[j] is not None:
                     terms.append((dt * aE[i][j], Ns[j]))
-                if aI[i][j] != 0.0:
+                if aI[i][j] != 0.0 and Ls[j] is not None:
                     terms.append((dt * aI[i][j], Ls[j]))
             if aI[i][i] == 0.0:
                 if i != 0:
@@ -183,7 +187,19 @@ class IMEXRungeKutta:
                 Ls.append(self.apply_linear(stage))
         if tab.is_stiffly_accurate:
             return stage
-        raise NotImplementedError("only globally stiffly accurate tableaux are built")
+        terms = [(1.0, mu)]
+        for i in range(tab.stages):
+            if tab.explicit.b[i] != 0.0 and Ns[i] is not None:
+                terms.append((dt * tab.explicit.b[i], Ns[i]))
+            if tab.implicit.b[i] != 0.0 and Ls[i] is not None:
+                terms.append((dt * tab.implicit.b[i], Ls[i]))
+        return self.mass_solver().solve(K.lincomb(terms), self.precision)
+
+    def mass_solver(self):
+        """M^-1 for the final combination of a tableau that is not stiffly accurate."""
+        if "mass" not in self._stage_ops:
+            self._stage_ops["mass"] = self.mass.fd_factor()
+        return self._stage_ops["mass"]
 
     def solve(self, dt: float, steps: int, U0=None):
         U = self.initial_coefficients() if U0 is None else as_device(U0, self.dt_io)

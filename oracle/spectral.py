@@ -712,6 +712,35 @@ def imex_ars443_step(u, dt, mass_apply, mass_solve, linear_apply, stage_solve,
     return U
 
 
+def imex_step(u, dt, aE, aI, bE, bI, mass_apply, mass_solve, linear_apply, stage_solve, nonlinear_sp):
+    """One step of any IMEX tableau whose implicit diagonal is 0 or one value
+    (``stage_solve`` solves (M - dt a L) U = r for that value).
+
+    Restates ``IMEXRungeKutta.step`` (integrators/imex_rk.py:96-159): stages
+    as in ``imex_ars443_step``; a globally stiffly accurate tableau returns
+    its last stage, any other takes M^-1 (M u + dt sum_i (bE_i N_i + bI_i L_i)).
+    """
+    aE, aI, bE, bI = (np.asarray(a, dtype=float) for a in (aE, aI, bE, bI))
+    s = len(bI)
+    gsa = np.allclose(aE[-1], bE) and np.allclose(aI[-1], bI)
+    mu = mass_apply(u)
+    Ns, Ls = [], []
+    U = None
+    for i in range(s):
+        rhs = mu.copy()
+        for j in range(i):
+            rhs = rhs + dt * aE[i, j] * Ns[j] + dt * aI[i, j] * Ls[j]
+        U = mass_solve(rhs) if aI[i, i] == 0 else stage_solve(rhs)
+        Ns.append(nonlinear_sp(U))
+        Ls.append(linear_apply(U))
+    if gsa:
+        return U
+    rhs = mu.copy()
+    for i in range(s):
+        rhs = rhs + dt * bE[i] * Ns[i] + dt * bI[i] * Ls[i]
+    return mass_solve(rhs)
+
+
 # --------------------------------------------------------------------------
 # Burgers pieces (Chebyshev Dirichlet, 3/2 padding)
 # --------------------------------------------------------------------------

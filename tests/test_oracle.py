@@ -659,3 +659,26 @@ def test_general_stencil_meets_conditions(family, bcs):
         for c_i, (side, k) in enumerate(conds):
             c = der(row, k) if k else row
             assert abs(val(-1.0 if side == "L" else 1.0, c) - (r == c_i)) < 1e-12
+
+
+def test_imex_step_general_stability_function():
+    # the tableau-generic IMEX step on y' = lam y (implicit part only, M = 1)
+    # equals the stability function R(z) = 1 + z b^T (I - z A)^-1 1 of the
+    # implicit tableau, for a stiffly accurate and a weighted-update tableau
+    from jaxfun_amd.integrators import ARS222, ARS443, IMEX_SSP2_222
+
+    lam, dt = -3.0, 0.1
+    z = lam * dt
+    for tab in (ARS222, ARS443, IMEX_SSP2_222):
+        A, b = np.array(tab.implicit.A), np.array(tab.implicit.b)
+        R = 1 + z * b @ np.linalg.solve(np.eye(len(b)) - z * A, np.ones(len(b)))
+        diag = [a for a in np.diag(A) if a != 0]
+        y = O.imex_step(np.array([1.0]), dt, tab.explicit.A, tab.implicit.A, tab.explicit.b, tab.implicit.b,
+                        lambda u: u, lambda r: r, lambda u: lam * u, lambda r: r / (1 - dt * diag[0] * lam),
+                        lambda u: 0 * u)
+        assert abs(y[0] - R) < 1e-14, (tab, y, R)
+    # and for ARS443 it is the dedicated restatement
+    u = np.array([0.7, -0.2])
+    args = (lambda v: v, lambda r: r, lambda v: -2.0 * v, lambda r: r / (1 + 2.0 * 0.5 * 0.05), lambda v: v**2)
+    a = O.imex_step(u, 0.05, ARS443.explicit.A, ARS443.implicit.A, ARS443.explicit.b, ARS443.implicit.b, *args)
+    assert np.allclose(a, O.imex_ars443_step(u, 0.05, *args), atol=1e-15)
