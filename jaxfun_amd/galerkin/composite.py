@@ -104,9 +104,9 @@ class Composite(OrthogonalSpace):
         if self._dirichlet and self.family in ("legendre", "chebyshev"):
             self.stencil = {0: 1 / self.scaling, 2: -1 / self.scaling}
         else:
+            # per-row coefficients of each shift, before the row scaling
             S = self._general_stencil()
-            self.stencil = {o: S.diagonal(o) / self.scaling for o in range(self.nbc + 1)
-                            if np.any(S.diagonal(o) != 0)}
+            self.stencil = {o: S.diagonal(o).copy() for o in range(self.nbc + 1) if np.any(S.diagonal(o) != 0)}
         self._mass_lu = None
 
     @property
