@@ -251,6 +251,31 @@ class OrthogonalSpace:
         out = _apply_1d(K.AxisOperator(V.contiguous()), c, "f64")
         return out.reshape(tuple(out.shape[:-1]) + tuple(shp))
 
+    def derivative_coeffs(self, c, k: int = 0):
+        """Coefficients of d^k/dX^k (reference coordinate) of the series c, of
+        the same length (Legendre.py:186-216, Chebyshev.py:281-315,
+        Jacobi.py:204-247).  The reference runs a backward recurrence per
+        series; here the exact map D_k = F V_k (the projection at N Gauss
+        points, exact since the products have degree < 2N, of the k-th
+        derivative Vandermonde) is built once on the device and applied along
+        the last axis, so batches of series cost one contraction."""
+        c = as_device(c)
+        if k == 0:
+            return c
+        if not self.is_orthogonal:
+            raise NotImplementedError("derivative_coeffs of an orthogonal basis (use to_orthogonal first)")
+        key = ("dcoef", k)
+        if key not in self._ops:
+            x, _ = self.quad_points_and_weights(self.N)
+            Vk = self._vander(x, k).contiguous()                                   # (N, N)
+            self._ops[key] = K.AxisOperator(
+                K.apply_along(self.forward_operator(self.N), Vk, 0, "f64").contiguous())
+        D = self._ops[key]
+        n = int(c.shape[-1])
+        if n < self.N:  # a shorter series: its derivative stays in the first n modes
+            D = K.AxisOperator(D.A64[:n, :n].contiguous())
+        return K.apply_along(D, c, c.ndim - 1, "f64")
+
     def mass_matrix(self):
         from ..la import DiaMatrix
 

@@ -219,3 +219,29 @@ def test_tensorproduct_get_homogeneous(dev):
     H = T.get_homogeneous()
     assert isinstance(H, type(next(iter(T.tpspaces.values()))))
     assert isinstance(inner(TestFunction(H) * TrialFunction(H)), TPMatrix)
+
+
+@pytest.mark.parametrize("space", ["Legendre", "Chebyshev"])
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_derivative_coeffs_vs_oracle(dev, space, k):
+    # Legendre.py:186-216 / Chebyshev.py:281-315 against the oracle's backward
+    # recurrences (pinned by numpy legder/chebder)
+    D = _cls(space)(18)
+    c = np.random.default_rng(k).standard_normal((3, 18))
+    got = _np(D.derivative_coeffs(c, k))
+    ref = np.stack([O.derivative_coeffs(space.lower(), ci, k) for ci in c])
+    assert np.abs(got - ref).max() < 1e-11 * max(1.0, np.abs(ref).max())
+    short = _np(D.derivative_coeffs(c[0, :7], k))
+    assert np.abs(short - O.derivative_coeffs(space.lower(), c[0, :7], k)).max() < 1e-11 * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.parametrize("space", FAM5)
+@pytest.mark.parametrize("domain", DOMAINS)
+def test_derivative_coeffs_backward_primitive(dev, space, domain):
+    # orthogonal.py:229-246: backward_primitive(c, k) = df^k backward(derivative_coeffs(c, k))
+    D = _cls(space)(16, domain=domain)
+    c = np.random.default_rng(3).standard_normal(16) / (1 + np.arange(16)) ** 2
+    for k in (1, 2):
+        lhs = _np(D.backward_primitive(c, k))
+        rhs = float(D.domain_factor) ** k * _np(D.backward(D.derivative_coeffs(c, k)))
+        assert np.abs(lhs - rhs).max() < 1e-11 * max(1.0, np.abs(lhs).max())
