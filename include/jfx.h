@@ -115,6 +115,16 @@ int jfx_plane_contract(int prec, const double* A0d, const double* A1d, const uns
                        const unsigned short* A1s, const void* X, void* Y, const void* E, int epi, long long B,
                        int M0, int K0, int M1, int K1, hipStream_t stream);
 
+/* 2D fast-diagonalisation solve of n x n slabs in one launch:
+ *     Y[b] = V0 ((W0 X[b] W1^T) / E) V1^T        (X, Y: B x n x n, E: n x n),
+ * the intermediate kept in LDS.  prec 0: FP64 operators, n <= 64; prec 1:
+ * 3-term bf16 slices ([3][n][Kp]) on FP32 data, n <= 128.  Replaces
+ * TPMatricesLUFactors.solve for two axes (la/tpmatrix.py:545-587). */
+int jfx_plane_fd(int prec, const double* W0d, const double* W1d, const double* V0d, const double* V1d,
+                 const unsigned short* W0s, const unsigned short* W1s, const unsigned short* V0s,
+                 const unsigned short* V1s, const void* X, void* Y, const void* E, long long B, int n,
+                 hipStream_t stream);
+
 /* ---- banded LU without pivoting.  band is (n, p+q+1) "by row":
  * band[i*(p+q+1) + o + p] = A[i, i+o].  status_dev (device int) receives 0,
  * or 1 + the row of the first zero/non-finite pivot.

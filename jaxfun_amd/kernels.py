@@ -454,6 +454,39 @@ def plane_ok(op0: AxisOperator, op1: AxisOperator, prec: str) -> bool:
     return prec in PLANE_PRECISIONS and max(op0.M, op0.K, op1.M, op1.K) <= 128
 
 
+def plane_fd_ok(ops, prec: str) -> bool:
+    """The fused 2D solve applies: four real n x n operators, n <= 64 (FP64) or 128 (bf16x6)."""
+    if prec not in ("f64", "bf16x6"):
+        return False
+    if any(type(o) is not AxisOperator for o in ops):
+        return False
+    n = ops[0].M
+    if any(o.M != n or o.K != n for o in ops):
+        return False
+    return n <= (64 if prec == "f64" else 128)
+
+
+def plane_fd(W0: AxisOperator, W1: AxisOperator, V0: AxisOperator, V1: AxisOperator, X: torch.Tensor,
+             E: torch.Tensor, prec: str = "f64") -> torch.Tensor:
+    """``Y[b] = V0 ((W0 X[b] W1^T) / E) V1^T`` over the last two axes in one kernel."""
+    dt = io_dtype(prec)
+    _req(X, dt, "X")
+    _req(E, dt, "E")
+    n = W0.M
+    if not plane_fd_ok([W0, W1, V0, V1], prec) or tuple(X.shape[-2:]) != (n, n) or tuple(E.shape) != (n, n):
+        raise ValueError("plane_fd: operators, array or divisor do not fit the fused solve")
+    B = math.prod(X.shape[:-2])
+    Y = torch.empty_like(X)
+    if prec == "f64":
+        _lib.call("jfx_plane_fd", 0, _p(W0.A64), _p(W1.A64), _p(V0.A64), _p(V1.A64), _p(None), _p(None), _p(None),
+                  _p(None), _p(X), _p(Y), _p(E), B, n, _s())
+    else:
+        s = [o.prepared("bf16x6")[0] for o in (W0, W1, V0, V1)]
+        _lib.call("jfx_plane_fd", 1, _p(None), _p(None), _p(None), _p(None), _p(s[0]), _p(s[1]), _p(s[2]), _p(s[3]),
+                  _p(X), _p(Y), _p(E), B, n, _s())
+    return Y
+
+
 def plane(op0: AxisOperator, op1: AxisOperator, X: torch.Tensor, prec: str = "f64",
           epi: tuple | None = None) -> torch.Tensor:
     """``Y[b] = A0 X[b] A1^T`` over the last two axes of X in one kernel

@@ -196,6 +196,8 @@ class TPMatricesLUFactors:
             for i, V in enumerate(self.V):
                 U = K.apply_along(V, U, nb + i, "f64")
             return U
+        if len(self.shape) == 2 and K.plane_fd_ok([*self.W, *self.V], precision):
+            return K.plane_fd(self.W[0], self.W[1], self.V[0], self.V[1], F.contiguous(), self.D(dt), precision)
         if len(self.shape) == 2 and K.plane_ok(self.W[0], self.W[1], precision):
             F = F.contiguous()
             Ft = K.plane(self.W[0], self.W[1], F, precision, ("div", self.D(dt)))

@@ -167,6 +167,32 @@ def test_plane_contract(dev, dims, prec, tol, epi):
     assert O.rel_l2(Y.to(torch.float64).cpu().numpy(), Yr) < tol
 
 
+@pytest.mark.parametrize("prec,n", [("f64", 5), ("f64", 17), ("f64", 62), ("f64", 64), ("bf16x6", 30),
+                                    ("bf16x6", 62), ("bf16x6", 126), ("bf16x6", 128)])
+@pytest.mark.parametrize("B", [1, 3, 300])
+def test_plane_fd_fused(dev, prec, n, B):
+    # Y = V0 ((W0 X W1^T) / E) V1^T in one launch against the two-launch path and einsum
+    import torch
+
+    from jaxfun_amd import kernels as K
+
+    rng = np.random.default_rng(n + B)
+    W0, W1, V0, V1 = (rng.standard_normal((n, n)) for _ in range(4))
+    E = rng.uniform(1.0, 2.0, size=(n, n))
+    X = rng.standard_normal((B, n, n))
+    dt = K.io_dtype(prec)
+    ops = [K.AxisOperator(_t(a, dev)) for a in (W0, W1, V0, V1)]
+    Xd, Ed = _t(X, dev, dt), _t(E, dev, dt)
+    assert K.plane_fd_ok(ops, prec)
+    Y = K.plane_fd(*ops, Xd, Ed, prec)
+    Y2 = K.plane(ops[2], ops[3], K.plane(ops[0], ops[1], Xd, prec, ("div", Ed)), prec)
+    Xr = Xd.to(torch.float64).cpu().numpy()
+    Yr = V0 @ ((W0 @ Xr @ W1.T) / E) @ V1.T
+    tol = 1e-13 if prec == "f64" else 4e-6
+    assert O.rel_l2(Y.to(torch.float64).cpu().numpy(), Yr) < tol
+    assert O.rel_l2(Y.to(torch.float64).cpu().numpy(), Y2.to(torch.float64).cpu().numpy()) < tol
+
+
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-14), ("bf16x6", 2e-6)])
 def test_plane_contract_many_slabs(dev, prec, tol):
     # more slabs than resident workgroups: every workgroup walks several slabs
@@ -179,5 +205,5 @@ def test_plane_contract_many_slabs(dev, prec, tol):
     X = rng.standard_normal((700, 62, 62))
     Xd = _t(X, dev, K.io_dtype(prec))
     Y = K.plane(K.AxisOperator(_t(A0, dev)), K.AxisOperator(_t(A1, dev)), Xd, prec)
-    Yr = np.einsum("mk,bkl,nl->bmn", A0, Xd.to(torch.float64).cpu().numpy(), A1)
+    Yr = A0 @ Xd.to(torch.float64).cpu().numpy() @ A1.T
     assert O.rel_l2(Y.to(torch.float64).cpu().numpy(), Yr) < tol
