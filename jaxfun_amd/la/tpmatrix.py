@@ -196,7 +196,10 @@ class TPMatricesLUFactors:
             for i, V in enumerate(self.V):
                 U = K.apply_along(V, U, nb + i, "f64")
             return U
-        if len(self.shape) == 2 and K.plane_fd_ok([*self.W, *self.V], precision):
+        nslab = int(np.prod(F.shape[:nb])) if nb else 1
+        if len(self.shape) == 2 and nslab <= K.num_cus() and K.plane_fd_ok([*self.W, *self.V], precision):
+            # a few slabs: one launch, latency-bound; many slabs: the two-launch
+            # path runs two workgroups per CU and streams better
             return K.plane_fd(self.W[0], self.W[1], self.V[0], self.V[1], F.contiguous(), self.D(dt), precision)
         if len(self.shape) == 2 and K.plane_ok(self.W[0], self.W[1], precision):
             F = F.contiguous()
