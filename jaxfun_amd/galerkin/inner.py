@@ -497,3 +497,11 @@ def project(ue, V):
     spaces = V.basespaces if isinstance(V, TensorProductSpace) else [V]
     vals = _mesh_values(spaces, V.system.base_scalars(), sp.sympify(ue), tuple(s.num_quad_points for s in spaces))
     return V.forward(vals)
+
+
+def project1D(ue, V):
+    """Project a SymPy expression of one coordinate onto a 1D space
+    (galerkin/inner.py:1027-1055): the forward transform of its mesh values."""
+    if isinstance(V, TensorProductSpace):
+        raise ValueError("project1D takes a 1D space")
+    return project(ue, V)

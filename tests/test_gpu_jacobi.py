@@ -95,11 +95,11 @@ def test_jacobi_general_parameters(dev):
 @pytest.mark.parametrize("domain", [None, (-2.0, 3.0)])
 def test_forward_backward(dev, case, domain):
     # tests/galerkin/test_forward_backward.py:20-47: forward(backward(project(sin x))) == project(sin x)
-    from jaxfun_amd import project
+    from jaxfun_amd.galerkin import project1D
 
     D, fam = _space(case, 12, domain=domain)
     x = D.system.x
-    ue = _np(project(sp.sin(x), D))
+    ue = _np(project1D(sp.sin(x), D))
     uh = _np(D.forward(D.backward(ue)))
     assert np.abs(uh - ue).max() < 5e-13
     # the projection itself against the oracle (orthogonal projection L / h)
