@@ -1,6 +1,6 @@
 """Linear algebra on the device (reference: la/)."""
 
-from .matrix import BaseMatrix, DiaMatrix, Matrix, SolverNotApplicable
+from .matrix import BaseMatrix, DiaMatrix, Matrix, SolverNotApplicable, diags
 from .tpmatrix import (
     TPLUFactors,
     TPMatrices,
@@ -19,6 +19,7 @@ __all__ = [
     "DiaMatrix",
     "Matrix",
     "SolverNotApplicable",
+    "diags",
     "TPLUFactors",
     "TPMatrices",
     "TPMatricesDenseLUFactors",

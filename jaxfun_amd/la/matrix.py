@@ -39,6 +39,7 @@ class BaseMatrix:
     """Common interface (la/matrixprotocol.py:68-242)."""
 
     is_zero = False
+    __array_ufunc__ = None  # numpy defers ``x @ A`` to ``A.__rmatmul__``
 
     def __init__(self, data):
         self._host_data = _host(data).copy()
@@ -153,6 +154,28 @@ class BaseMatrix:
     def tosparse(self, tol: int = 100) -> "DiaMatrix":
         return DiaMatrix.from_dense(self._host_data, tol=tol)
 
+    # ---------------------------------------------------------------- inspection
+    @property
+    def ndim(self) -> int:
+        return 2
+
+    @property
+    def nnz(self) -> int:
+        """Stored nonzero entries (la/diamatrix.py)."""
+        return int(np.count_nonzero(self._host_data))
+
+    def diagonal(self, k: int = 0) -> np.ndarray:
+        """The k-th diagonal (zeros for an unstored one; empty outside the shape)."""
+        return np.diagonal(self._host_data, k).copy()
+
+    def __getitem__(self, idx):
+        """Entries as of the dense matrix (basic and advanced indexing)."""
+        return self._host_data[idx]
+
+    def rmatvec(self, x, axis: int = 0, precision: str = "f64") -> torch.Tensor:
+        """A^T applied along ``axis`` (la/diamatrix.py:227-349)."""
+        return self.T.matvec(x, axis, precision)
+
     def __repr__(self) -> str:
         return f"{type(self).__name__}(shape={self.shape})"
 
@@ -240,6 +263,36 @@ class DiaMatrix(BaseMatrix):
         return K.band_lu_solve(lu, p, q, x, axis % x.ndim)
 
     lu_solve = solve
+
+
+def diags(diagonals, offsets=(0,), shape=None) -> "DiaMatrix":
+    """DiaMatrix from its diagonals (la/diamatrix.py diags, scipy.sparse.diags
+    semantics): ``diagonals[i]`` fills offset ``offsets[i]`` from its start; a
+    scalar is broadcast along the diagonal; ``shape`` defaults to the square
+    matrix that the first diagonal and its offset imply."""
+    offsets = [int(o) for o in (offsets if isinstance(offsets, (list, tuple, np.ndarray)) else [offsets])]
+    if not isinstance(diagonals, (list, tuple)):
+        diagonals = [diagonals]  # one array or scalar
+    elif len(offsets) == 1 and len(diagonals) != 1:
+        diagonals = [np.asarray(diagonals)]  # one diagonal given as a list of numbers
+    diagonals = list(diagonals)
+    if len(diagonals) != len(offsets):
+        raise ValueError("one diagonal per offset")
+    if shape is None:
+        d0 = np.atleast_1d(np.asarray(diagonals[0]))
+        if d0.size == 1 and np.ndim(diagonals[0]) == 0:
+            raise ValueError("shape is needed for scalar diagonals")
+        n = d0.size + abs(offsets[0])
+        shape = (n, n)
+    m, n = shape
+    cplx = any(np.iscomplexobj(d) for d in diagonals)
+    A = np.zeros((m, n), dtype=np.complex128 if cplx else np.float64)
+    for d, o in zip(diagonals, offsets):
+        L = len(np.diagonal(A, o))
+        vals = np.broadcast_to(np.asarray(d), (L,)) if np.ndim(d) == 0 else np.asarray(d)[:L]
+        r0, c0 = (0, o) if o >= 0 else (-o, 0)
+        A[r0 + np.arange(len(vals)), c0 + np.arange(len(vals))] = vals
+    return DiaMatrix(A, offsets=tuple(offsets))
 
 
 class PinnedMatrix:
