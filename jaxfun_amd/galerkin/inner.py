@@ -290,6 +290,11 @@ class TensorMatrix:
     def todense_host(self) -> np.ndarray:
         return self.todense().cpu().numpy()
 
+    @property
+    def data(self) -> np.ndarray:
+        """Dense (test_0, test_1, trial_0, trial_1, ...) array (la/tensormatrix.py)."""
+        return self.todense_host().reshape(self.shape_dofs + self.shape_dofs)
+
     def solve(self, rhs, axis: int = 0):
         """Dense LU solve (la/tensormatrix.py:76-80); rocSOLVER factorisation."""
         from .orthogonal import as_device

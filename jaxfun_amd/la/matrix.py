@@ -160,6 +160,11 @@ class BaseMatrix:
         return 2
 
     @property
+    def data(self) -> np.ndarray:
+        """The dense entries (host copy kept by the matrix)."""
+        return self._host_data
+
+    @property
     def nnz(self) -> int:
         """Stored nonzero entries (la/diamatrix.py)."""
         return int(np.count_nonzero(self._host_data))

@@ -107,3 +107,106 @@ def test_inner_padding(dev, space, dims):
         if not resolved:
             n2 = 16 if dims == 1 else (16, 16)
             assert np.linalg.norm(y3 - _np(T.forward(uf.backward(N=n2) ** 2))) < tol
+
+
+@pytest.mark.parametrize("dims", [1, 2])
+def test_inner_padding_fourier(dev, dims):
+    # test_inner.py:418-453: exp(8ix) aliases to wavenumber 2 without padding
+    from jaxfun_amd import Fourier, TensorProduct, TestFunction, inner
+    from jaxfun_amd.galerkin.arguments import JAXFunction
+
+    F = Fourier(6)
+    T = F if dims == 1 else TensorProduct(F, F)
+    xs = T.system.base_scalars()
+    f = sp.exp(4j * xs[0]) if dims == 1 else sp.exp(4j * xs[0]) * sp.exp(4j * xs[1])
+    uf = JAXFunction(f, T)
+    v = TestFunction(T)
+    nq = 9 if dims == 1 else (9, 9)
+    tol = np.sqrt(100 * EPS)
+    y0 = _np(inner(v * uf**2, num_quad_points=None, kind="linear"))
+    y1 = _np(inner(v * uf**2, num_quad_points=nq, kind="linear"))
+    idx = 2 if dims == 1 else (2, 2)
+    assert np.linalg.norm(y1) < tol and abs(y0[idx] - (2 * np.pi) ** dims) < tol
+    y2 = _np(T.forward(uf.backward() ** 2))
+    y3 = _np(T.forward(uf.backward(N=nq) ** 2))
+    assert np.linalg.norm(y3) < tol and abs(y2[idx] - 1) < tol
+
+
+@pytest.mark.parametrize("space", SPACES)
+@pytest.mark.parametrize("kind", ["composite", "directsum", "directsumtps"])
+def test_inner_padding_lifted_resolved(dev, space, kind):
+    # test_inner.py:456-516
+    from jaxfun_amd import FunctionSpace, TensorProduct, TestFunction, inner
+    from jaxfun_amd.coordinates import x as X, y as Y
+    from jaxfun_amd.galerkin.arguments import JAXFunction
+
+    if kind == "directsumtps":
+        f = sp.cos(X * 2 * sp.pi) * sp.cos(Y * 2 * sp.pi)
+        Dx = FunctionSpace(36, _cls(space), bcs={"left": {"D": f.subs(X, -1)}, "right": {"D": f.subs(X, 1)}})
+        Dy = FunctionSpace(36, _cls(space), bcs={"left": {"D": f.subs(Y, -1)}, "right": {"D": f.subs(Y, 1)}})
+        T = TensorProduct(Dx, Dy)
+        x, y = T.system.base_scalars()
+        f = f.subs({X: x, Y: y})
+        nq = (48, 48)
+    else:
+        bcs = {"left": {"D": 0}, "right": {"D": 0}} if kind == "composite" else {"left": {"D": 1}, "right": {"D": 1}}
+        T = FunctionSpace(36, _cls(space), bcs=bcs)
+        x = T.system.x
+        f = sp.sin(x * 2 * sp.pi) if kind == "composite" else sp.cos(x * 2 * sp.pi)
+        nq = 48
+    uf = JAXFunction(f, T)
+    v = TestFunction(T)
+    tol = np.sqrt(100 * EPS)
+    y0 = _np(inner(v * uf**2, num_quad_points=None, kind="linear"))
+    y1 = _np(inner(v * uf**2, num_quad_points=nq, kind="linear"))
+    assert np.linalg.norm(y0 - y1) < tol
+    assert np.linalg.norm(_np(T.forward(uf.backward() ** 2)) - _np(T.forward(uf.backward(N=nq) ** 2))) < tol
+
+
+def test_inner_exact_poly(dev):
+    # test_inner.py:519-577: x^2 P_5 P_5 needs 7 Gauss points
+    from jaxfun_amd import Legendre, TensorProduct, TestFunction, TrialFunction, inner
+
+    N = 6
+    L = Legendre(N, name="L")
+    x = L.system.x
+    t = float(sp.integrate(x**2 * sp.legendre(N - 1, x) ** 2, (x, -1, 1)).n())
+    tol = np.sqrt(100 * EPS)
+    u, v = TrialFunction(L), TestFunction(L)
+    A = inner(x**2 * v * u, use_precomputed_matrices=False, kind="bilinear")
+    assert not abs(A[N - 1, N - 1] - t) < tol
+    A = inner(x**2 * v * u, num_quad_points=7, use_precomputed_matrices=False, kind="bilinear")
+    assert abs(A[N - 1, N - 1] - t) < tol
+    T = TensorProduct(L, L)
+    x, y = T.system.base_scalars()
+    u, v = TrialFunction(T), TestFunction(T)
+    B = inner((x**2 + y**2) * v * u, use_precomputed_matrices=False, kind="bilinear")
+    assert not abs(B.tpmats[0].mats[0].data[N - 1, N - 1] - t) < tol
+    B = inner((x**2 + y**2) * v * u, num_quad_points=(7, 7), use_precomputed_matrices=False, kind="bilinear")
+    assert abs(B.tpmats[0].mats[0].data[N - 1, N - 1] - t) < tol
+    assert abs(B.tpmats[1].mats[1].data[N - 1, N - 1] - t) < tol
+
+
+def test_inner_exact_nonseparable_2d(dev):
+    # test_inner.py:580-655: a JAXFunction coefficient and sqrt(x^2 + y^2)
+    from scipy.integrate import dblquad
+
+    from jaxfun_amd import Legendre, TensorProduct, TestFunction, TrialFunction, inner
+    from jaxfun_amd.galerkin.arguments import JAXFunction
+
+    N = 6
+    T = TensorProduct(Legendre(N, name="L"), Legendre(N, name="L"))
+    x, y = T.system.base_scalars()
+    u, v = TrialFunction(T), TestFunction(T)
+    tol = np.sqrt(100 * EPS)
+    t = float(sp.integrate((x**2 + y**2) * sp.legendre(N - 1, x) ** 2 * sp.legendre(N - 1, y) ** 2,
+                           (x, -1, 1), (y, -1, 1)).n())
+    uf = JAXFunction(x**2 + y**2, T)
+    assert not abs(inner(uf * v * u, kind="bilinear").data[-1, -1, -1, -1] - t) < tol
+    assert abs(inner(uf * v * u, num_quad_points=(7, 7), kind="bilinear").data[-1, -1, -1, -1] - t) < tol
+    xs, ys = sp.symbols("x,y", real=True)
+    t2, _ = dblquad(sp.lambdify((xs, ys), sp.sqrt(xs**2 + ys**2) * sp.legendre(N - 1, xs) ** 2
+                                * sp.legendre(N - 1, ys) ** 2), -1, 1, -1, 1, epsabs=100 * EPS)
+    assert not abs(inner(sp.sqrt(x**2 + y**2) * v * u, kind="bilinear").data[-1, -1, -1, -1] - t2) < tol
+    B = inner(sp.sqrt(x**2 + y**2) * v * u, num_quad_points=(32, 32), kind="bilinear")
+    assert abs(B.data[-1, -1, -1, -1] - t2) < tol
