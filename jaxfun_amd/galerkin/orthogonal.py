@@ -75,6 +75,26 @@ class OrthogonalSpace:
             return X
         return self.domain.lower + (X - self.reference_domain.lower) / self.domain_factor
 
+    def map_expr_reference_domain(self, u):
+        """u(x) rewritten with the reference coordinate X = c + (x - a) df
+        (galerkin/orthogonal.py:357-373)."""
+        u = sp.sympify(u)
+        if not u.free_symbols:
+            return u
+        x = self.system.base_scalars()[0]
+        a, c = self.domain.lower, self.reference_domain.lower
+        return u.xreplace({x: c + (x - a) * float(self.domain_factor)})
+
+    def map_expr_true_domain(self, u):
+        """u(X) rewritten with the true coordinate x = a + (X - c) / df
+        (galerkin/orthogonal.py:376-392)."""
+        u = sp.sympify(u)
+        if not u.free_symbols:
+            return u
+        x = self.system.base_scalars()[0]
+        a, c = self.domain.lower, self.reference_domain.lower
+        return u.xreplace({x: a + (x - c) / float(self.domain_factor)})
+
     @property
     def num_quad_points(self) -> int:
         return self._num_quad_points

@@ -142,6 +142,20 @@ class TensorProductSpace:
         return self._run(c, [s.backward_operator(N[i], 0, kind) for i, s in enumerate(self.basespaces)],
                          "f64")
 
+    def map_expr_true_domain(self, u):
+        """Every axis' reference coordinate mapped to its true coordinate
+        (galerkin/tensorproductspace.py:210-214)."""
+        for s in self.basespaces:
+            u = s.map_expr_true_domain(u)
+        return u
+
+    def map_expr_reference_domain(self, u):
+        """Every axis' true coordinate mapped to its reference coordinate
+        (galerkin/tensorproductspace.py:216-220)."""
+        for s in self.basespaces:
+            u = s.map_expr_reference_domain(u)
+        return u
+
     def _point_basis(self, ax: int, X: torch.Tensor) -> torch.Tensor:
         """(P, dim) basis of axis ax at true-domain points X."""
         s = self.basespaces[ax]

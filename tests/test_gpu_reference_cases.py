@@ -245,3 +245,53 @@ def test_derivative_coeffs_backward_primitive(dev, space, domain):
         lhs = _np(D.backward_primitive(c, k))
         rhs = float(D.domain_factor) ** k * _np(D.backward(D.derivative_coeffs(c, k)))
         assert np.abs(lhs - rhs).max() < 1e-11 * max(1.0, np.abs(lhs).max())
+
+
+def test_orthogonal_mappings_and_uniform_mesh(dev):
+    # tests/galerkin/test_orthogonal_extras.py and test_additional_coverage.py:117-129
+    from jaxfun_amd.galerkin import Chebyshev, Legendre, MeshKind
+
+    L = Legendre(5, domain=(-2, 3))
+    x = L.system.x
+    expr = x**3 + 2 * x + 1
+    tru = L.map_expr_true_domain(L.map_expr_reference_domain(expr))
+    for p in np.linspace(-2.0, 3.0, 5):
+        assert abs(float(tru.subs(x, p)) - (p**3 + 2 * p + 1)) < 100 * EPS * 30
+    pts = np.linspace(-2.0, 3.0, 6)
+    assert np.allclose([L.map_true_domain(L.map_reference_domain(p)) for p in pts], pts)
+    assert np.allclose(_np(L.mesh(kind=MeshKind.UNIFORM, N=7)), np.linspace(-2.0, 3.0, 7))
+    C = Chebyshev(6, domain=(-3, 1))
+    assert np.allclose(_np(C.mesh(kind=MeshKind.UNIFORM, N=5)), np.linspace(-3.0, 1.0, 5))
+    L2 = Legendre(5, domain=(-2, 2))
+    x2 = L2.system.x
+    assert sp.simplify(L2.map_expr_true_domain(L2.map_expr_reference_domain(x2**2 + 1)) - (x2**2 + 1)) == 0
+
+
+def test_functionspace_variants_and_projection(dev):
+    # test_additional_coverage.py:101-115, 132-137, 139-152, 175-184
+    from jaxfun_amd import Chebyshev, Fourier, FunctionSpace, Legendre, TensorProduct, project
+    from jaxfun_amd.galerkin import DirectSum
+
+    assert not hasattr(FunctionSpace(6, Legendre), "basespaces")
+    assert FunctionSpace(6, Legendre, bcs={"left": {"D": 0}, "right": {"D": 0}}).bcs.is_homogeneous()
+    D = FunctionSpace(6, Legendre, bcs={"left": {"D": 1}, "right": {"D": 2}})
+    assert isinstance(D, DirectSum)
+    assert abs(float(_np(D.evaluate(np.array([1.0]), np.zeros(4)))[0]) - 2.0) < 1e-14
+    assert abs(float(_np(D.evaluate(np.array([-1.0]), np.zeros(4)))[0]) - 1.0) < 1e-14
+    F = Fourier(8)
+    c = np.random.default_rng(2).standard_normal(8) + 0j
+    assert _np(F.backward(c, N=12)).shape[0] == 12
+    C = Chebyshev(6)
+    assert C.matrices(0, (C, 0)) is not None and C.matrices(3, (C, 3)) is None
+    T = TensorProduct(Chebyshev(4), Legendre(4))
+    x, y = T.system.base_scalars()
+    uh = _np(project(sp.chebyshevt(1, x) * sp.legendre(2, y), T))
+    assert abs(uh[1, 2] - 1.0) < 100 * EPS and uh.shape == tuple(T.num_dofs)
+    C3 = Chebyshev(3)
+    T3 = TensorProduct(C3, C3, C3)
+    u = np.random.default_rng(3).standard_normal(tuple(T3.num_dofs))
+    cc = T3.forward(u)
+    assert tuple(T3.backward(cc).shape) == u.shape
+    assert tuple(np.squeeze(_np(T3.evaluate_mesh(cc))).shape) == u.shape
+    xs = T3.system.base_scalars()
+    assert T3.map_expr_true_domain(xs[0] + xs[1] + xs[2]).free_symbols == set(xs)
