@@ -295,3 +295,54 @@ def test_functionspace_variants_and_projection(dev):
     assert tuple(np.squeeze(_np(T3.evaluate_mesh(cc))).shape) == u.shape
     xs = T3.system.base_scalars()
     assert T3.map_expr_true_domain(xs[0] + xs[1] + xs[2]).free_symbols == set(xs)
+
+
+def test_tensorproductspace_cases(dev):
+    # tests/galerkin/test_tensorproductspace.py:29-211
+    from jaxfun_amd import Chebyshev, Fourier, FunctionSpace, Legendre, TensorProduct, TestFunction, \
+        TrialFunction, inner
+    from jaxfun_amd.galerkin import DirectSum, TensorMatrix
+    from jaxfun_amd.la import TPMatrices, TPMatrix
+
+    # Fourier x Fourier projection system, then a padded backward
+    F = Fourier(8)
+    T = TensorProduct(F, F)
+    x, y = T.system.base_scalars()
+    u, v = TrialFunction(T), TestFunction(T)
+    M, b = inner(v * (u - sp.sin(x) * sp.sin(y)), kind="system")
+    uh = M.solve(b)
+    assert tuple(T.backward(uh, N=(12, 8)).shape) == (12, 8)
+    X, Y = [m.cpu().numpy() for m in T.mesh()]
+    assert np.abs(_np(T.backward(uh)).real - np.sin(X) * np.sin(Y)).max() < 1e-13
+    # one lifted axis: the stiffness form and a backward of zero coefficients
+    C = FunctionSpace(6, Legendre, bcs={"left": {"D": 1}, "right": {"D": 2}})
+    L = Legendre(6)
+    T = TensorProduct(C, L)
+    assert hasattr(T, "tpspaces") and isinstance(C, DirectSum)
+    x, y = T.system.base_scalars()
+    u, v = TrialFunction(T), TestFunction(T)
+    A, b = inner(sp.diff(u, x) * sp.diff(v, x) + sp.diff(u, y) * sp.diff(v, y), kind="system")
+    assert isinstance(A, TPMatrices)
+    u0 = _np(T.backward(np.zeros((C[0].dim, L.dim))))
+    Xq = T.mesh()[0].cpu().numpy()
+    assert np.abs(u0 - (1.5 + 0.5 * Xq)).max() < 1e-13  # the lift alone: linear in x
+    # a mass TPMatrix applied to an array equals A0 X A1^T
+    T = TensorProduct(Chebyshev(6), Legendre(6))
+    u, v = TrialFunction(T), TestFunction(T)
+    A = inner(v * u, kind="bilinear")
+    assert isinstance(A, TPMatrix)
+    Xr = np.random.default_rng(0).standard_normal(tuple(T.num_dofs))
+    Y1 = _np(A @ Xr)
+    Y2 = A.mats[0].todense_host() @ Xr @ A.mats[1].todense_host().T
+    assert np.linalg.norm(Y1 - Y2) < 100 * EPS * max(1.0, np.linalg.norm(Y2))
+    # a non-separable coefficient gives a TensorMatrix
+    T = TensorProduct(Chebyshev(5), Legendre(5))
+    x, y = T.system.base_scalars()
+    assert isinstance(inner(sp.sqrt(x + y + 3) * TrialFunction(T) * TestFunction(T), kind="bilinear"), TensorMatrix)
+    # linear form only, and a lifted 1D system
+    C6 = Chebyshev(6)
+    assert _np(inner(sp.sin(C6.system.x) * TestFunction(C6), kind="linear")).shape[0] == C6.N
+    V = FunctionSpace(6, Legendre, bcs={"left": {"D": 1}, "right": {"D": 2}})
+    x = V.system.x
+    A, b = inner(TestFunction(V) * TrialFunction(V) + x * TestFunction(V) * TrialFunction(V), kind="system")
+    assert A.shape[0] == A.shape[1] == _np(b).shape[0]
