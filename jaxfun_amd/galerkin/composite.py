@@ -23,14 +23,19 @@ from ..utils.common import lambdify, n
 from .orthogonal import OrthogonalSpace
 
 
-BC_ORDER = {"D": 0, "N": 1, "N2": 2, "N3": 3, "N4": 4}
+BC_ORDER = {"D": 0, "R": 0, "N": 1, "N2": 2, "N3": 3, "N4": 4}
 
 
 def bc_order(key: str) -> int:
-    """Derivative order of a condition code: D 0, N 1, N2..N4 2..4."""
+    """Derivative order of a condition code: D and R (Robin) 0, N 1, N2..N4 2..4."""
     if key not in BC_ORDER:
-        raise NotImplementedError(f"boundary condition {key!r}: D, N, N2, N3 and N4 are built")
+        raise NotImplementedError(f"boundary condition {key!r}: D, R, N, N2, N3 and N4 are built")
     return BC_ORDER[key]
+
+
+def _bc_value(v):
+    """The value of a condition; a Robin condition is the pair (alpha, value)."""
+    return v[1] if isinstance(v, (tuple, list)) else v
 
 
 class BoundaryConditions(dict):
@@ -38,8 +43,9 @@ class BoundaryConditions(dict):
 
     Built from a plain dict with a domain, the values of derivative
     conditions Nk are divided by df^k (df = 2 / (b - a)), so that they are
-    given in physical units and stored for the reference domain; a
-    BoundaryConditions argument is taken as already normalised."""
+    given in physical units and stored for the reference domain; the Robin
+    condition R: (alpha, g), u + alpha u' = g, stores alpha * df for the same
+    reason.  A BoundaryConditions argument is taken as already normalised."""
 
     def __init__(self, bc: dict, domain=None) -> None:
         bcs = {"left": {}, "right": {}}
@@ -52,12 +58,14 @@ class BoundaryConditions(dict):
                     if k[0] == "N":
                         nd = int(k[1:]) if len(k) > 1 else 1
                         side[k] = v / df**nd
+                    elif k == "R":
+                        side[k] = (v[0] * df, v[1])
 
     def orderednames(self) -> list[str]:
         return ["L" + b for b in sorted(self["left"])] + ["R" + b for b in sorted(self["right"])]
 
     def orderedvals(self) -> list:
-        return [self[s][k] for s in ("left", "right") for k in sorted(self[s])]
+        return [_bc_value(self[s][k]) for s in ("left", "right") for k in sorted(self[s])]
 
     def num_bcs(self) -> int:
         return len(self.orderedvals())
@@ -66,15 +74,22 @@ class BoundaryConditions(dict):
         """Sum of the derivative orders of all conditions (galerkin/composite.py:94-101)."""
         return sum(bc_order(k) for side in self.values() for k in side)
 
-    def conditions(self) -> list[tuple[str, int]]:
-        """(side "L"/"R", derivative order) of each condition, in ordered form."""
-        return [(nm[0], bc_order(nm[1:])) for nm in self.orderednames()]
+    def conditions(self) -> list[tuple[str, int, float]]:
+        """(side "L"/"R", derivative order, Robin alpha or 0) of each
+        condition, in ordered form."""
+        out = []
+        for nm in self.orderednames():
+            v = self["left" if nm[0] == "L" else "right"][nm[1:]]
+            out.append((nm[0], bc_order(nm[1:]), float(v[0]) if nm[1:] == "R" else 0.0))
+        return out
 
     def is_homogeneous(self) -> bool:
-        return all(v == 0 for side in self.values() for v in side.values())
+        return all(_bc_value(v) == 0 for side in self.values() for v in side.values())
 
     def get_homogeneous(self) -> "BoundaryConditions":
-        return BoundaryConditions({k: {s: 0 for s in v} for k, v in self.items()})
+        """Every value set to zero; a Robin condition keeps its alpha."""
+        return BoundaryConditions({k: {s: ((v[0], 0) if s == "R" else 0) for s, v in vals.items()}
+                                   for k, vals in self.items()})
 
 
 def family_kwargs(kw: dict) -> dict:
@@ -126,8 +141,8 @@ class Composite(OrthogonalSpace):
         N, nb = self.N, self.nbc
         S = np.zeros((N - nb, N))
         with mp.workdps(40):
-            f = [[mp.mpf(self.orthogonal.bnd_value(j, side, k, exact=True)) for j in range(N)]
-                 for side, k in self._conds]
+            f = [[_functional(self.orthogonal, j, side, k, al) for j in range(N)]
+                 for side, k, al in self._conds]
             for row in range(N - nb):
                 A = mp.matrix([[f[c][row + i] for i in range(1, nb + 1)] for c in range(nb)])
                 b = mp.matrix([-f[c][row] for c in range(nb)])
@@ -251,6 +266,17 @@ class Composite(OrthogonalSpace):
 
 
 # ----------------------------------------------------------------------------- lifting
+def _functional(orthogonal, j: int, side: str, k: int, alpha: float):
+    """A condition applied to psi_j: d^k psi_j at the end, plus alpha psi_j'
+    there for a Robin condition (get_stencil_matrix, composite.py:806-822)."""
+    import mpmath as mp
+
+    v = mp.mpf(orthogonal.bnd_value(j, side, k, exact=True))
+    if alpha:
+        v += mp.mpf(alpha) * mp.mpf(orthogonal.bnd_value(j, side, k + 1, exact=True))
+    return v
+
+
 def get_bc_basis(bcs, orthogonal) -> np.ndarray:
     """Lifting basis (num_bcs, first + num_bcs) (galerkin/composite.py:835-895).
 
@@ -259,7 +285,7 @@ def get_bc_basis(bcs, orthogonal) -> np.ndarray:
     ``A[r, j] = d^{k_r} psi_{first+j}(x_r)`` (x_r the end of condition r,
     k_r its derivative order) the rows are those of ``(A^-1)^T``, placed in
     columns first..first+nb-1.  ``first`` is the smallest offset (up to the
-    number of derivatives) whose A is invertible; for ``{left: D, right: D}``
+    number of derivatives, plus one) whose A is invertible; for ``{left: D, right: D}``
     on Legendre/Chebyshev first = 0 and the rows are (psi_0 -+ psi_1) / 2.
     End values come from the orthogonal space (``bnd_value``), so the Jacobi
     family's scalings are honoured.
@@ -270,9 +296,11 @@ def get_bc_basis(bcs, orthogonal) -> np.ndarray:
     conds = bcs.conditions()
     nb = len(conds)
     with mp.workdps(40):
-        for first in range(bcs.num_derivatives() + 1):
-            A = mp.matrix([[mp.mpf(orthogonal.bnd_value(first + j, side, k, exact=True)) for j in range(nb)]
-                           for side, k in conds])
+        # one offset beyond the reference's search, for Robin conditions whose
+        # first functionals coincide (u + 2u' at x = -1 on P_0, P_1)
+        for first in range(bcs.num_derivatives() + 2):
+            A = mp.matrix([[_functional(orthogonal, first + j, side, k, al) for j in range(nb)]
+                           for side, k, al in conds])
             try:
                 singular = abs(mp.det(A)) < mp.mpf(10) ** -20
             except (ZeroDivisionError, TypeError, ValueError):  # mpmath's LU finds no pivot

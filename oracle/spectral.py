@@ -217,14 +217,28 @@ def jacobi_family(family):
 # General boundary conditions (D, N, N2, N3, N4)
 # --------------------------------------------------------------------------
 
-BC_ORDER = {"D": 0, "N": 1, "N2": 2, "N3": 3, "N4": 4}
+BC_ORDER = {"D": 0, "R": 0, "N": 1, "N2": 2, "N3": 3, "N4": 4}
 
 
-def bc_conditions(bcs: dict) -> list[tuple[str, int]]:
-    """(side, derivative order) per condition, in the reference's order
-    (left before right, codes sorted; galerkin/composite.py:73-90)."""
-    return [("L", BC_ORDER[k]) for k in sorted(bcs.get("left", {}))] + \
-        [("R", BC_ORDER[k]) for k in sorted(bcs.get("right", {}))]
+def bc_conditions(bcs: dict, df: float = 1.0) -> list[tuple]:
+    """(side, derivative order[, Robin alpha]) per condition, in the
+    reference's order (left before right, codes sorted;
+    galerkin/composite.py:73-90).  A Robin condition R: (alpha, g) means
+    u + alpha u' = g in physical units: alpha * df on the reference domain."""
+    out = []
+    for side, key in (("L", "left"), ("R", "right")):
+        for k in sorted(bcs.get(key, {})):
+            v = bcs[key][k]
+            out.append((side, BC_ORDER[k], float(v[0]) * df) if k == "R" else (side, BC_ORDER[k]))
+    return out
+
+
+def _cond_row(family, N, cond):
+    side, k = cond[0], cond[1]
+    row = end_values(family, N, side, k)
+    if len(cond) > 2:
+        row = row + cond[2] * end_values(family, N, side, k + 1)
+    return row
 
 
 def end_values(family, N: int, side: str, k: int) -> np.ndarray:
@@ -238,7 +252,7 @@ def general_stencil(family, N: int, conds) -> np.ndarray:
     every condition homogeneously: per n the nb x nb system of end values
     (get_stencil_matrix, galerkin/composite.py:765-832)."""
     nb = len(conds)
-    E = np.array([end_values(family, N, side, k) for side, k in conds])
+    E = np.array([_cond_row(family, N, c) for c in conds])
     S = np.zeros((N - nb, N))
     for row in range(N - nb):
         d = np.linalg.solve(E[:, row + 1:row + 1 + nb], -E[:, row])
@@ -251,8 +265,8 @@ def general_lift_basis(family, conds, num_derivatives: int) -> np.ndarray:
     """(nb, first + nb) lifting basis (get_bc_basis, galerkin/composite.py:835-895):
     rows of (A^-1)^T for the first offset whose boundary matrix is regular."""
     nb = len(conds)
-    for first in range(num_derivatives + 1):
-        A = np.array([end_values(family, first + nb, side, k)[first:] for side, k in conds])
+    for first in range(num_derivatives + 2):
+        A = np.array([_cond_row(family, first + nb, c)[first:] for c in conds])
         if abs(np.linalg.det(A)) > 1e-12:
             S = np.zeros((nb, first + nb))
             S[:, first:] = np.linalg.inv(A).T

@@ -682,3 +682,20 @@ def test_imex_step_general_stability_function():
     args = (lambda v: v, lambda r: r, lambda v: -2.0 * v, lambda r: r / (1 + 2.0 * 0.5 * 0.05), lambda v: v**2)
     a = O.imex_step(u, 0.05, ARS443.explicit.A, ARS443.implicit.A, ARS443.explicit.b, ARS443.implicit.b, *args)
     assert np.allclose(a, O.imex_ars443_step(u, 0.05, *args), atol=1e-15)
+
+
+@pytest.mark.parametrize("family", ["legendre", "chebyshev"])
+def test_robin_stencil_and_lift_meet_conditions(family):
+    # u + a u' = 0 at x = -1 and u = 0 at x = 1, checked with numpy.polynomial
+    val, der = FNS[family]
+    a = 0.7
+    conds = O.bc_conditions({"left": {"R": (a, 0)}, "right": {"D": 0}})
+    S = O.general_stencil(family, 12, conds)
+    for row in S:
+        assert abs(val(-1.0, row) + a * val(-1.0, der(row, 1))) < 1e-13 * max(1.0, np.abs(row).sum())
+        assert abs(val(1.0, row)) < 1e-13 * max(1.0, np.abs(row).sum())
+    B = O.general_lift_basis(family, O.bc_conditions({"left": {"R": (2.0, 1)}, "right": {"D": 3}}), 0)
+    assert B.shape == (2, 3)  # u + 2u' at -1 coincides on P_0, P_1 for Legendre: offset 1
+    for r, row in enumerate(B):
+        assert abs(val(-1.0, row) + 2.0 * val(-1.0, der(row, 1)) - (r == 0)) < 1e-13
+        assert abs(val(1.0, row) - (r == 1)) < 1e-13
