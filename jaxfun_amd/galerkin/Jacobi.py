@@ -127,24 +127,6 @@ class Jacobi(OrthogonalSpace):
             v = jacobi_end_derivative(a, b, self.gn(j), j, side, k)
             return +v if exact else float(v)
 
-    def dirichlet_stencil(self) -> np.ndarray:
-        """(N-2, N) S with phi_k = Q_k + d1_k Q_{k+1} + d2_k Q_{k+2} vanishing
-        at both ends (get_stencil_matrix, galerkin/composite.py:765-832): the
-        2 x 2 boundary system solved by Cramer's rule, so d1 is exactly 0
-        when alpha = beta."""
-        N = self.N
-        S = np.zeros((N - 2, N))
-        with mp.workdps(_DPS):
-            a, b = self._params()
-            L = [self.gn(j) * (-1) ** j * mp.binomial(j + b, j) for j in range(N)]
-            R = [self.gn(j) * mp.binomial(j + a, j) for j in range(N)]
-            for k in range(N - 2):
-                det = L[k + 1] * R[k + 2] - L[k + 2] * R[k + 1]
-                S[k, k] = 1.0
-                S[k, k + 1] = float((L[k + 2] * R[k] - L[k] * R[k + 2]) / det)
-                S[k, k + 2] = float((L[k] * R[k + 1] - L[k + 1] * R[k]) / det)
-        return S
-
     # ------------------------------------------------------------ quadrature
     def quad_points_and_weights(self, N: int | None = None):
         """Gauss-Jacobi nodes (ascending) and weights (galerkin/Jacobi.py:112-124):

@@ -609,7 +609,12 @@ def test_jacobi_host_tables_vs_oracle(a, b, kind):
     E = F.vandermonde(np.array([-1.0, 1.0]), N)
     assert np.allclose([sp_.bnd_value(j, "L") for j in range(N)], E[0], rtol=1e-13)
     assert np.allclose([sp_.bnd_value(j, "R") for j in range(N)], E[1], rtol=1e-13)
-    assert np.allclose(sp_.dirichlet_stencil(), F.dirichlet_stencil(N), atol=1e-13)
+    from jaxfun_amd.galerkin import FunctionSpace
+
+    kw = {"alpha": sp_.alpha, "beta": sp_.beta} if kind == "jacobi" else \
+        ({"lambda_": sp_.lambda_} if kind == "ultraspherical" else {})
+    C = FunctionSpace(N, type(sp_), {"left": {"D": 0}, "right": {"D": 0}}, **kw)
+    assert np.allclose(C.stencil_matrix(), F.dirichlet_stencil(N), atol=1e-13)
 
 
 # ----------------------------------------------------------------- general boundary conditions
