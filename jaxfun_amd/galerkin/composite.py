@@ -23,13 +23,14 @@ from ..utils.common import lambdify, n
 from .orthogonal import OrthogonalSpace
 
 
-BC_ORDER = {"D": 0, "R": 0, "N": 1, "N2": 2, "N3": 3, "N4": 4}
+BC_ORDER = {"D": 0, "R": 0, "N": 1, "W": 1, "N2": 2, "N3": 3, "N4": 4}
 
 
 def bc_order(key: str) -> int:
-    """Derivative order of a condition code: D and R (Robin) 0, N 1, N2..N4 2..4."""
+    """Derivative order of a condition code: D and R (Robin) 0, N and W
+    (weighted Neumann) 1, N2..N4 2..4."""
     if key not in BC_ORDER:
-        raise NotImplementedError(f"boundary condition {key!r}: D, R, N, N2, N3 and N4 are built")
+        raise NotImplementedError(f"boundary condition {key!r}: D, R, N, W, N2, N3 and N4 are built")
     return BC_ORDER[key]
 
 
@@ -45,7 +46,9 @@ class BoundaryConditions(dict):
     conditions Nk are divided by df^k (df = 2 / (b - a)), so that they are
     given in physical units and stored for the reference domain; the Robin
     condition R: (alpha, g), u + alpha u' = g, stores alpha * df for the same
-    reason.  A BoundaryConditions argument is taken as already normalised."""
+    reason, and the weighted condition W: (alpha, g), u' + alpha u'' = g,
+    stores (alpha * df, g / df).  A BoundaryConditions argument is taken as
+    already normalised."""
 
     def __init__(self, bc: dict, domain=None) -> None:
         bcs = {"left": {}, "right": {}}
@@ -60,6 +63,8 @@ class BoundaryConditions(dict):
                         side[k] = v / df**nd
                     elif k == "R":
                         side[k] = (v[0] * df, v[1])
+                    elif k == "W":
+                        side[k] = (v[0] * df, v[1] / df)
 
     def orderednames(self) -> list[str]:
         return ["L" + b for b in sorted(self["left"])] + ["R" + b for b in sorted(self["right"])]
@@ -80,15 +85,15 @@ class BoundaryConditions(dict):
         out = []
         for nm in self.orderednames():
             v = self["left" if nm[0] == "L" else "right"][nm[1:]]
-            out.append((nm[0], bc_order(nm[1:]), float(v[0]) if nm[1:] == "R" else 0.0))
+            out.append((nm[0], bc_order(nm[1:]), float(v[0]) if nm[1:] in ("R", "W") else 0.0))
         return out
 
     def is_homogeneous(self) -> bool:
         return all(_bc_value(v) == 0 for side in self.values() for v in side.values())
 
     def get_homogeneous(self) -> "BoundaryConditions":
-        """Every value set to zero; a Robin condition keeps its alpha."""
-        return BoundaryConditions({k: {s: ((v[0], 0) if s == "R" else 0) for s, v in vals.items()}
+        """Every value set to zero; Robin and weighted conditions keep their alpha."""
+        return BoundaryConditions({k: {s: ((v[0], 0) if s in ("R", "W") else 0) for s, v in vals.items()}
                                    for k, vals in self.items()})
 
 
@@ -146,7 +151,11 @@ class Composite(OrthogonalSpace):
             for row in range(N - nb):
                 A = mp.matrix([[f[c][row + i] for i in range(1, nb + 1)] for c in range(nb)])
                 b = mp.matrix([-f[c][row] for c in range(nb)])
-                d = mp.lu_solve(A, b)
+                try:
+                    d = mp.lu_solve(A, b)
+                except ZeroDivisionError:
+                    raise ValueError(f"the conditions {self.bcs.orderednames()} give a singular stencil row "
+                                     f"{row} for this basis") from None
                 S[row, row] = 1.0
                 for i in range(nb):
                     S[row, row + 1 + i] = 0.0 if abs(d[i]) < mp.mpf(10) ** -30 else float(d[i])
@@ -267,8 +276,9 @@ class Composite(OrthogonalSpace):
 
 # ----------------------------------------------------------------------------- lifting
 def _functional(orthogonal, j: int, side: str, k: int, alpha: float):
-    """A condition applied to psi_j: d^k psi_j at the end, plus alpha psi_j'
-    there for a Robin condition (get_stencil_matrix, composite.py:806-822)."""
+    """A condition applied to psi_j: d^k psi_j at the end, plus alpha
+    d^(k+1) psi_j there for a Robin (k = 0) or weighted (k = 1) condition
+    (get_stencil_matrix, composite.py:806-822)."""
     import mpmath as mp
 
     v = mp.mpf(orthogonal.bnd_value(j, side, k, exact=True))

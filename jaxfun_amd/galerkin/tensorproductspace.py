@@ -344,8 +344,8 @@ class DirectSumTPS(TensorProductSpace):
         value (D) or the k-th derivative (Nk) at that end, in reference-domain
         units (divided by df^k), as galerkin/tensorproductspace.py:636-662
         builds the projected conditions."""
-        if name[1:] == "R":
-            raise NotImplementedError("Robin conditions on a lifted axis of a tensor product")
+        if name[1:] in ("R", "W"):
+            raise NotImplementedError("Robin/weighted conditions on a lifted axis of a tensor product")
         k = bc_order(name[1:])
         z = ends[a][0 if name[0] == "L" else 1]
         e = sp.sympify(g)

@@ -217,7 +217,7 @@ def jacobi_family(family):
 # General boundary conditions (D, N, N2, N3, N4)
 # --------------------------------------------------------------------------
 
-BC_ORDER = {"D": 0, "R": 0, "N": 1, "N2": 2, "N3": 3, "N4": 4}
+BC_ORDER = {"D": 0, "R": 0, "N": 1, "W": 1, "N2": 2, "N3": 3, "N4": 4}
 
 
 def bc_conditions(bcs: dict, df: float = 1.0) -> list[tuple]:
@@ -229,7 +229,7 @@ def bc_conditions(bcs: dict, df: float = 1.0) -> list[tuple]:
     for side, key in (("L", "left"), ("R", "right")):
         for k in sorted(bcs.get(key, {})):
             v = bcs[key][k]
-            out.append((side, BC_ORDER[k], float(v[0]) * df) if k == "R" else (side, BC_ORDER[k]))
+            out.append((side, BC_ORDER[k], float(v[0]) * df) if k in ("R", "W") else (side, BC_ORDER[k]))
     return out
 
 

@@ -179,3 +179,27 @@ def test_robin_lifted_1d(dev, family):
     X = torch.tensor([-1.0], dtype=torch.float64, device=K.device())
     v0 = _np(D[0].evaluate_basis_derivative(X, 0))[0] + 2.0 * 2.0 * _np(D[0].evaluate_basis_derivative(X, 1))[0]
     assert np.abs(v0).max() < 1e-11
+
+
+@pytest.mark.parametrize("family", ["legendre", "chebyshev"])
+def test_weighted_neumann_lifted_1d(dev, family):
+    # u'' = f on (0, 2) with u'(0) + 0.3 u''(0) = g (W, physical units) and u(2) = ue(2)
+    # (alpha = 0.5 makes the Legendre functionals of P_2 and P_3 coincide, so
+    # that stencil row is singular in the reference's construction as well)
+    from jaxfun_amd import Chebyshev, FunctionSpace, Legendre, TestFunction, TrialFunction, inner
+
+    N = 22
+    space = Legendre if family == "legendre" else Chebyshev
+    s_ = sp.Symbol("x")
+    ue0 = sp.cos(s_) * sp.exp(s_ / 2) + s_**2
+    g = float((sp.diff(ue0, s_) + 0.3 * sp.diff(ue0, s_, 2)).subs(s_, 0))
+    D = FunctionSpace(N, space, {"left": {"W": (0.3, g)}, "right": {"D": float(ue0.subs(s_, 2))}}, domain=(0, 2))
+    x = D.system.x
+    ue = ue0.subs(s_, x)
+    conds = O.bc_conditions({"left": {"W": (0.3, 0)}, "right": {"D": 0}}, df=1.0)
+    assert np.allclose(D[0].stencil_matrix(), O.general_stencil(family, N, conds), atol=1e-12)
+    v, u = TestFunction(D), TrialFunction(D)
+    A, b = inner(v * sp.diff(u, x, 2) - v * sp.diff(ue, x, 2), kind="system")
+    uh = A.solve(b)
+    xs = np.linspace(0, 2, 9)
+    assert np.abs(_np(D.evaluate(xs, uh)) - sp.lambdify(x, ue)(xs)).max() < 1e-10
