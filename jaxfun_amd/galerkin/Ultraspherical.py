@@ -1,8 +1,10 @@
 """Ultraspherical (Gegenbauer) polynomials (reference: galerkin/Ultraspherical.py).
 
 C_n^(lambda) = g_n P_n^(a,a), a = lambda - 1/2, scaled so that
-C_n(+-1) = (+-1)^n: g_n = 1 / P_n^(a,a)(1).  lambda = 0 is the Chebyshev
-basis, which this build provides as ``Chebyshev``.
+C_n(+-1) = (+-1)^n: g_n = 1 / P_n^(a,a)(1).  lambda = 0 (a = -1/2) is the
+Chebyshev basis T_n; the recurrence table's cancelled first entries and the
+Jacobi norms cover it without the reference's special cases
+(galerkin/Ultraspherical.py:65-101).
 """
 
 from __future__ import annotations
@@ -23,8 +25,6 @@ class Ultraspherical(Jacobi):
         kw.pop("alpha", None)
         kw.pop("beta", None)
         lam = sp.sympify(lambda_)
-        if lam == 0:
-            raise NotImplementedError("Ultraspherical with lambda_ = 0 is the Chebyshev basis: use Chebyshev")
         super().__init__(N, domain=domain, system=system, name=name, fun_str=fun_str,
                          alpha=lam - sp.S.Half, beta=lam - sp.S.Half, **kw)
 

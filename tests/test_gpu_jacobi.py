@@ -25,6 +25,7 @@ CASES = {
     "chebyshevu": ("ChebyshevU", {}, O.jacobi_family("chebyshevu")),
     "ultraspherical": ("Ultraspherical", {}, O.jacobi_family("ultraspherical")),
     "ultraspherical2": ("Ultraspherical", {"lambda_": 2}, O.JacobiFamily(1.5, 1.5, "ultraspherical")),
+    "ultraspherical0": ("Ultraspherical", {"lambda_": 0}, O.JacobiFamily(-0.5, -0.5, "ultraspherical")),
 }
 BASIC = ["jacobi00", "chebyshevu", "ultraspherical"]
 
@@ -229,3 +230,22 @@ def test_jaxfunction_forms(dev, case):
     M1 = inner(u * v, kind="bilinear", use_precomputed_matrices=False).todense_host()
     assert np.allclose(M0, np.diag(fam.norm_squared(N)), atol=1e-14)
     assert np.allclose(M0, M1, atol=1e-13)
+
+
+def test_ultraspherical_lambda0_is_chebyshev(dev):
+    # galerkin/Ultraspherical.py:65-101: lambda = 0 is the Chebyshev basis
+    import torch
+
+    from jaxfun_amd import kernels as K
+    from jaxfun_amd.galerkin import Ultraspherical
+
+    U = Ultraspherical(24, lambda_=0)
+    xs = np.linspace(-1, 1, 31)
+    X = torch.tensor(xs, dtype=torch.float64, device=K.device())
+    for k in range(3):
+        assert np.abs(_np(U.evaluate_basis_derivative(X, k)) - O.vandermonde("chebyshev", xs, 24, k)).max() < 1e-11
+    x, w = U.quad_points_and_weights()
+    xc, wc = O.quad("chebyshev", 24)
+    assert np.allclose(_np(x), xc, atol=1e-14) and np.allclose(_np(w), wc, atol=1e-14)
+    assert np.allclose(U.norm_squared(), O.norm_squared("chebyshev", 24))
+
