@@ -380,7 +380,7 @@ int launch_persist(const void* A, const double* Arow, int M, int K, const void* 
                    long long Q, hipStream_t st, int* rc);
 template <int POL>
 int launch_ra(const void* A, int M, int K, const void* X, void* Y, long long P, long long Q, hipStream_t st,
-              int* rc);
+              int* rc, int Mtot = 0, int moff = 0);
 
 template <typename T>
 static int launch_plain(const T* A, int M, int K, const T* X, T* Y, long long P, long long Q,
@@ -388,6 +388,18 @@ static int launch_plain(const T* A, int M, int K, const T* X, T* Y, long long P,
   {
     int rc = 0;
     if (launch_ra<(sizeof(T) == 8) ? 0 : 1>(A, M, K, X, Y, P, Q, st, &rc)) return rc;
+    // 128 < M <= 256 (the padded backward transforms): two register-resident
+    // launches over row halves of A, each writing its rows of Y
+    if (M > 128 && M <= 256 && M % 4 == 0 && K <= 128) {
+      const int M0 = ((M + 1) / 2 + 3) & ~3, M1 = M - M0;
+      int rc1 = 0;
+      if (M0 <= 128 && M1 > 0 && launch_ra<(sizeof(T) == 8) ? 0 : 1>(A, M0, K, X, Y, P, Q, st, &rc, M, 0)) {
+        if (rc) return rc;
+        if (launch_ra<(sizeof(T) == 8) ? 0 : 1>(A + (long long)M0 * K, M1, K, X, Y, P, Q, st, &rc1, M, M0))
+          return rc1;
+        return set_error("contract: second row half rejected"), JFX_ESHAPE;
+      }
+    }
     if (launch_persist<(sizeof(T) == 8) ? 0 : 1>(A, nullptr, M, K, X, Y, P, Q, st, &rc)) return rc;
   }
   ColMap cm;

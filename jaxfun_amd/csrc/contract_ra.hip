@@ -55,6 +55,7 @@ struct RLds {
 struct RMap {
   long long P, Q, ncols, ntiles, nqt;  // nqt: 64-column tiles per p (Q % 64 == 0)
   int M, K, qone;
+  int Mtot, moff;  // the output has Mtot rows along the axis; this launch writes rows moff..moff+M-1
 };
 
 // NS: k-steps run unconditionally (4 deep for FP64/FP32, 32 deep for bf16x6);
@@ -312,7 +313,7 @@ __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __res
     tile_cols(tt, xbase, c0, ncol);
     if (!wact) return;
     if (!rm.qone) {
-      TX* dst = Y + (xbase / ((long long)K * Q)) * (long long)M * Q + c0 + (lane & 15);
+      TX* dst = Y + (xbase / ((long long)K * Q)) * (long long)rm.Mtot * Q + (long long)rm.moff * Q + c0 + (lane & 15);
 #pragma unroll
       for (int j = 0; j < JB; ++j)
 #pragma unroll
@@ -327,7 +328,7 @@ __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __res
       for (int j = 0; j < JB; ++j) {
         const int c = 16 * (cg * JB + j) + (lane & 15);
         if (c < ncol) {
-          TX* dst = Y + (c0 + c) * M;
+          TX* dst = Y + (c0 + c) * rm.Mtot + rm.moff;
           if constexpr (POL == 0) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -382,7 +383,7 @@ static long long ra_grid() {
 // the call, 0 if the shape or alignment needs the general kernels.
 template <int POL>
 int launch_ra(const void* A, int M, int K, const void* X, void* Y, long long P, long long Q, hipStream_t st,
-              int* rc) {
+              int* rc, int Mtot, int moff) {
   typedef typename RT<POL>::TX TX;
   constexpr int EPC = 16 / sizeof(TX);
   if (M <= 0 || K <= 0 || M > RMAX || K > RMAX || P <= 0 || Q <= 0) return 0;
@@ -392,12 +393,15 @@ int launch_ra(const void* A, int M, int K, const void* X, void* Y, long long P, 
   rm.Q = Q;
   rm.ncols = P * Q;
   rm.M = M;
+  rm.Mtot = Mtot > 0 ? Mtot : M;
+  rm.moff = moff;
+  if (rm.moff < 0 || rm.moff + M > rm.Mtot) return 0;
   rm.K = K;
   if (Q == 1) {
     // 16-byte chunks never straddle rows (K, M multiples of the chunk) and the
     // staged rows cover whole MFMA k-steps (K % 4 == 0; bf16: K % 8 == 0 with
     // the rest of the 32-step zero-filled)
-    if (K % (POL == 3 ? 8 : 4) || M % EPC) return 0;
+    if (K % (POL == 3 ? 8 : 4) || M % EPC || rm.Mtot % EPC || rm.moff % EPC) return 0;
     rm.qone = 1;
     rm.nqt = 0;
     rm.ntiles = (P + RTN - 1) / RTN;
@@ -443,8 +447,11 @@ int launch_ra(const void* A, int M, int K, const void* X, void* Y, long long P, 
   return 1;
 }
 
-template int launch_ra<0>(const void*, int, int, const void*, void*, long long, long long, hipStream_t, int*);
-template int launch_ra<1>(const void*, int, int, const void*, void*, long long, long long, hipStream_t, int*);
-template int launch_ra<3>(const void*, int, int, const void*, void*, long long, long long, hipStream_t, int*);
+template int launch_ra<0>(const void*, int, int, const void*, void*, long long, long long, hipStream_t, int*, int,
+                          int);
+template int launch_ra<1>(const void*, int, int, const void*, void*, long long, long long, hipStream_t, int*, int,
+                          int);
+template int launch_ra<3>(const void*, int, int, const void*, void*, long long, long long, hipStream_t, int*, int,
+                          int);
 
 }  // namespace jfx

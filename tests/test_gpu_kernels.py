@@ -207,3 +207,24 @@ def test_plane_contract_many_slabs(dev, prec, tol):
     Y = K.plane(K.AxisOperator(_t(A0, dev)), K.AxisOperator(_t(A1, dev)), Xd, prec)
     Yr = A0 @ Xd.to(torch.float64).cpu().numpy() @ A1.T
     assert O.rel_l2(Y.to(torch.float64).cpu().numpy(), Yr) < tol
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-14), ("f32", 2e-6)])
+@pytest.mark.parametrize("M", [132, 192, 256])
+@pytest.mark.parametrize("shape,axis", [((3, 126, 64), 1), ((5, 7, 126), 2), ((126, 128), 0)])
+def test_contract_row_split(dev, prec, tol, M, shape, axis):
+    # operators with 128 < M <= 256 rows (the 3/2-padded backward transforms)
+    # run as two register-resident launches over row halves
+    import torch
+
+    from jaxfun_amd import kernels as K
+
+    rng = np.random.default_rng(M + axis)
+    A = rng.standard_normal((M, 126))
+    X = rng.standard_normal(shape)
+    dt = K.io_dtype(prec)
+    Y = K.apply_along(K.AxisOperator(_t(A, dev)), _t(X, dev, dt), axis, prec)
+    Xr = _t(X, dev, dt).to(torch.float64).cpu().numpy()
+    ref = np.moveaxis(np.tensordot(A, Xr, axes=([1], [axis])), 0, axis)
+    assert Y.shape == ref.shape
+    assert O.rel_l2(Y.to(torch.float64).cpu().numpy(), ref) < tol
