@@ -398,10 +398,12 @@ int launch_ra(const void* A, int M, int K, const void* X, void* Y, long long P, 
   if (rm.moff < 0 || rm.moff + M > rm.Mtot) return 0;
   rm.K = K;
   if (Q == 1) {
-    // 16-byte chunks never straddle rows (K, M multiples of the chunk) and the
-    // staged rows cover whole MFMA k-steps (K % 4 == 0; bf16: K % 8 == 0 with
-    // the rest of the 32-step zero-filled)
-    if (K % (POL == 3 ? 8 : 4) || M % EPC || rm.Mtot % EPC || rm.moff % EPC) return 0;
+    // 16-byte chunks never straddle rows (K, M multiples of the chunk; bf16:
+    // K % 8 == 0 for its 8-wide k groups).  Staged rows K..4*NS-1 of a k-step
+    // keep the zero fill (stores write rows < K only) and the operator
+    // registers past K are zero, so FP64 takes any even K (126: a 128-point
+    // Dirichlet composite)
+    if (K % (POL == 3 ? 8 : EPC) || M % EPC || rm.Mtot % EPC || rm.moff % EPC) return 0;
     rm.qone = 1;
     rm.nqt = 0;
     rm.ntiles = (P + RTN - 1) / RTN;

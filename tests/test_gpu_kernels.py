@@ -210,6 +210,26 @@ def test_plane_contract_many_slabs(dev, prec, tol):
 
 
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-14), ("f32", 2e-6)])
+@pytest.mark.parametrize("M", [16, 96, 128, 192])
+@pytest.mark.parametrize("Kc", [10, 62, 126])
+def test_contract_last_axis_even_k(dev, prec, tol, M, Kc):
+    # last-axis contractions (Q == 1) with K even but not a multiple of 4: the
+    # register-resident FP64 kernel's k-steps past K read zeros
+    import torch
+
+    from jaxfun_amd import kernels as K
+
+    rng = np.random.default_rng(M * 7 + Kc)
+    A = rng.standard_normal((M, Kc))
+    X = rng.standard_normal((5, 33, Kc))
+    dt = K.io_dtype(prec)
+    Y = K.apply_along(K.AxisOperator(_t(A, dev)), _t(X, dev, dt), 2, prec)
+    ref = _t(X, dev, dt).to(torch.float64).cpu().numpy() @ A.T
+    assert Y.shape == ref.shape
+    assert O.rel_l2(Y.to(torch.float64).cpu().numpy(), ref) < tol
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-14), ("f32", 2e-6)])
 @pytest.mark.parametrize("M", [132, 192, 256])
 @pytest.mark.parametrize("shape,axis", [((3, 126, 64), 1), ((5, 7, 126), 2), ((126, 128), 0)])
 def test_contract_row_split(dev, prec, tol, M, shape, axis):
