@@ -380,7 +380,7 @@ int launch_persist(const void* A, const double* Arow, int M, int K, const void* 
                    long long Q, hipStream_t st, int* rc);
 template <int POL>
 int launch_ra(const void* A, int M, int K, const void* X, void* Y, long long P, long long Q, hipStream_t st,
-              int* rc, int Mtot = 0, int moff = 0);
+              int* rc, int Mtot = 0, int moff = 0, int lda = 0, long long xps = 0, int accum = 0);
 
 template <typename T>
 static int launch_plain(const T* A, int M, int K, const T* X, T* Y, long long P, long long Q,
@@ -399,6 +399,27 @@ static int launch_plain(const T* A, int M, int K, const T* X, T* Y, long long P,
           return rc1;
         return set_error("contract: second row half rejected"), JFX_ESHAPE;
       }
+    }
+    // 128 < K <= 256 with Q % 64 == 0 (the padded forward transforms): the
+    // columns of A in two halves, each half's rows in one launch (M <= 128)
+    // or two; the second column half adds its partial sums to Y
+    if (K > 128 && K <= 256 && Q % 64 == 0 && M <= 256 && (M <= 128 || M % 4 == 0)) {
+      const int K0 = ((K + 1) / 2 + 3) & ~3;
+      const int Mh = M <= 128 ? M : (((M + 1) / 2 + 3) & ~3);
+      int n = 0;
+      for (int kh = 0; kh < 2; ++kh)
+        for (int m0 = 0; m0 < M; m0 += Mh) {
+          const int k0 = kh ? K0 : 0, kn = kh ? K - K0 : K0, mr = M - m0 < Mh ? M - m0 : Mh;
+          if (!launch_ra<(sizeof(T) == 8) ? 0 : 1>(A + (long long)m0 * K + k0, mr, kn, X + (long long)k0 * Q, Y, P,
+                                                   Q, st, &rc, M, m0, K, (long long)K * Q, kh)) {
+            if (n) return set_error("contract: K-split launch rejected"), JFX_ESHAPE;
+            kh = 2;
+            break;
+          }
+          if (rc) return rc;
+          ++n;
+        }
+      if (n) return JFX_OK;
     }
     if (launch_persist<(sizeof(T) == 8) ? 0 : 1>(A, nullptr, M, K, X, Y, P, Q, st, &rc)) return rc;
   }

@@ -56,6 +56,10 @@ struct RMap {
   long long P, Q, ncols, ntiles, nqt;  // nqt: 64-column tiles per p (Q % 64 == 0)
   int M, K, qone;
   int Mtot, moff;  // the output has Mtot rows along the axis; this launch writes rows moff..moff+M-1
+  // Q % 64 == 0 only: A row stride lda, X stride xps between consecutive p,
+  // and accum = 1 adds this launch's sums to Y (a split of K over launches)
+  int lda, accum;
+  long long xps;
 };
 
 // NS: k-steps run unconditionally (4 deep for FP64/FP32, 32 deep for bf16x6);
@@ -104,7 +108,7 @@ __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __res
 #pragma unroll
       for (int s = 0; s < NS; ++s) {
         const int k = 4 * s + (lane >> 4);
-        ar[s] = (row < M && k < K) ? A[(long long)row * K + k] : TX(0);
+        ar[s] = (row < M && k < K) ? A[(long long)row * rm.lda + k] : TX(0);
       }
     }
   }
@@ -123,7 +127,7 @@ __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __res
     } else {
       const long long p = t / rm.nqt, q0 = (t - p * rm.nqt) * RTN;
       c0 = q0;
-      xbase = p * K * Q + q0;
+      xbase = p * rm.xps + q0;
       ncol = RTN;
     }
   };
@@ -313,13 +317,16 @@ __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __res
     tile_cols(tt, xbase, c0, ncol);
     if (!wact) return;
     if (!rm.qone) {
-      TX* dst = Y + (xbase / ((long long)K * Q)) * (long long)rm.Mtot * Q + (long long)rm.moff * Q + c0 + (lane & 15);
+      TX* dst = Y + (tt / rm.nqt) * (long long)rm.Mtot * Q + (long long)rm.moff * Q + c0 + (lane & 15);
 #pragma unroll
       for (int j = 0; j < JB; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int m = orow(r);
-          if (m < M) dst[(long long)m * Q + 16 * (cg * JB + j)] = (TX)acc[j][r];
+          if (m < M) {
+            TX& d = dst[(long long)m * Q + 16 * (cg * JB + j)];
+            d = rm.accum ? d + (TX)acc[j][r] : (TX)acc[j][r];
+          }
         }
     } else {
       // output column c is the contiguous M-vector Y[(c0 + c) M ...]; with FP32
@@ -383,7 +390,7 @@ static long long ra_grid() {
 // the call, 0 if the shape or alignment needs the general kernels.
 template <int POL>
 int launch_ra(const void* A, int M, int K, const void* X, void* Y, long long P, long long Q, hipStream_t st,
-              int* rc, int Mtot, int moff) {
+              int* rc, int Mtot, int moff, int lda, long long xps, int accum) {
   typedef typename RT<POL>::TX TX;
   constexpr int EPC = 16 / sizeof(TX);
   if (M <= 0 || K <= 0 || M > RMAX || K > RMAX || P <= 0 || Q <= 0) return 0;
@@ -397,6 +404,11 @@ int launch_ra(const void* A, int M, int K, const void* X, void* Y, long long P, 
   rm.moff = moff;
   if (rm.moff < 0 || rm.moff + M > rm.Mtot) return 0;
   rm.K = K;
+  rm.lda = lda > 0 ? lda : K;
+  rm.xps = xps > 0 ? xps : (long long)K * Q;
+  rm.accum = accum;
+  const bool ksplit = rm.lda != K || rm.xps != (long long)K * Q || accum;
+  if (ksplit && (POL == 3 || Q % RTN || rm.lda < K || rm.xps < (long long)K * Q)) return 0;
   if (Q == 1) {
     // 16-byte chunks never straddle rows (K, M multiples of the chunk; bf16:
     // K % 8 == 0 for its 8-wide k groups).  Staged rows K..4*NS-1 of a k-step
@@ -450,10 +462,10 @@ int launch_ra(const void* A, int M, int K, const void* X, void* Y, long long P, 
 }
 
 template int launch_ra<0>(const void*, int, int, const void*, void*, long long, long long, hipStream_t, int*, int,
-                          int);
+                          int, int, long long, int);
 template int launch_ra<1>(const void*, int, int, const void*, void*, long long, long long, hipStream_t, int*, int,
-                          int);
+                          int, int, long long, int);
 template int launch_ra<3>(const void*, int, int, const void*, void*, long long, long long, hipStream_t, int*, int,
-                          int);
+                          int, int, long long, int);
 
 }  // namespace jfx

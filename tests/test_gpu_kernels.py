@@ -210,6 +210,29 @@ def test_plane_contract_many_slabs(dev, prec, tol):
 
 
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-14), ("f32", 2e-6)])
+@pytest.mark.parametrize("M", [16, 126, 192, 256])
+@pytest.mark.parametrize("Kc", [132, 192, 256])
+@pytest.mark.parametrize("shape,axis", [((3, None, 64), 1), ((2, None, 192), 1), ((None, 128), 0)])
+def test_contract_k_split(dev, prec, tol, M, Kc, shape, axis):
+    # 128 < K <= 256 with Q % 64 == 0 (the padded forward transforms): two
+    # column halves of the operator, the second accumulating into the output
+    import torch
+
+    from jaxfun_amd import kernels as K
+
+    shape = tuple(Kc if d is None else d for d in shape)
+    rng = np.random.default_rng(M + Kc + axis)
+    A = rng.standard_normal((M, Kc))
+    X = rng.standard_normal(shape)
+    dt = K.io_dtype(prec)
+    Y = K.apply_along(K.AxisOperator(_t(A, dev)), _t(X, dev, dt), axis, prec)
+    Xr = _t(X, dev, dt).to(torch.float64).cpu().numpy()
+    ref = np.moveaxis(np.tensordot(A, Xr, axes=([1], [axis])), 0, axis)
+    assert Y.shape == ref.shape
+    assert O.rel_l2(Y.to(torch.float64).cpu().numpy(), ref) < tol
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-14), ("f32", 2e-6)])
 @pytest.mark.parametrize("M", [16, 96, 128, 192])
 @pytest.mark.parametrize("Kc", [10, 62, 126])
 def test_contract_last_axis_even_k(dev, prec, tol, M, Kc):
