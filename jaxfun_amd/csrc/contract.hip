@@ -400,10 +400,12 @@ static int launch_plain(const T* A, int M, int K, const T* X, T* Y, long long P,
         return set_error("contract: second row half rejected"), JFX_ESHAPE;
       }
     }
-    // 128 < K <= 256 with Q % 64 == 0 (the padded forward transforms): the
-    // columns of A in two halves, each half's rows in one launch (M <= 128)
-    // or two; the second column half adds its partial sums to Y
-    if (K > 128 && K <= 256 && Q % 64 == 0 && M <= 256 && (M <= 128 || M % 4 == 0)) {
+    // 128 < K <= 256 with Q % 64 == 0 or Q == 1 (the padded forward
+    // transforms): the columns of A in two halves, each half's rows in one
+    // launch (M <= 128) or two; the second column half adds its partial sums
+    // to Y.  With Q == 1 a half reads a strided slice of each row of X.
+    if (K > 128 && K <= 256 && (Q % 64 == 0 || (Q == 1 && K % (16 / (int)sizeof(T)) == 0)) && M <= 256 &&
+        (M <= 128 || M % 4 == 0)) {
       const int K0 = ((K + 1) / 2 + 3) & ~3;
       const int Mh = M <= 128 ? M : (((M + 1) / 2 + 3) & ~3);
       int n = 0;
@@ -446,8 +448,20 @@ static int launch_slices(const unsigned short* Asl, const double* Arow, int M, i
   if (S == 2 || S == 3) {
     constexpr int POL = (MODE == 1) ? 4 : (S == 2 ? 2 : 3);
     int rc = 0;
-    if constexpr (MODE == 0 && S == 3)
+    if constexpr (MODE == 0 && S == 3) {
       if (launch_ra<3>(Asl, M, K, X, Y, P, Q, st, &rc)) return rc;
+      // 128 < M <= 256: two launches over row halves of the slices (rows of
+      // Kp = K rounded up to 32 bf16 values)
+      if (M > 128 && M <= 256 && M % 4 == 0 && K <= 128) {
+        const int M0 = ((M + 1) / 2 + 3) & ~3, M1 = M - M0, Kp = (K + KTB - 1) / KTB * KTB;
+        int rc1 = 0;
+        if (M0 <= 128 && M1 > 0 && launch_ra<3>(Asl, M0, K, X, Y, P, Q, st, &rc, M, 0)) {
+          if (rc) return rc;
+          if (launch_ra<3>(Asl + (long long)M0 * Kp, M1, K, X, Y, P, Q, st, &rc1, M, M0)) return rc1;
+          return set_error("contract: second row half rejected"), JFX_ESHAPE;
+        }
+      }
+    }
     if ((MODE == 0 || S == 3) && launch_persist<POL>(Asl, Arow, M, K, X, Y, P, Q, st, &rc)) return rc;
   }
   ColMap cm;

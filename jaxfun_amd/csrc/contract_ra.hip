@@ -56,8 +56,9 @@ struct RMap {
   long long P, Q, ncols, ntiles, nqt;  // nqt: 64-column tiles per p (Q % 64 == 0)
   int M, K, qone;
   int Mtot, moff;  // the output has Mtot rows along the axis; this launch writes rows moff..moff+M-1
-  // Q % 64 == 0 only: A row stride lda, X stride xps between consecutive p,
-  // and accum = 1 adds this launch's sums to Y (a split of K over launches)
+  // A row stride lda, X stride xps between consecutive p (the row stride of X
+  // when Q == 1), and accum = 1 adds this launch's sums to Y (a split of K
+  // over launches); FP64/FP32 only
   int lda, accum;
   long long xps;
 };
@@ -93,6 +94,7 @@ __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __res
   {
     const int row = 16 * w + (lane & 15);
     if constexpr (POL == 3) {
+      // slices of Mtot rows each (a row split passes Aptr at its first row)
       const unsigned short* Asl = reinterpret_cast<const unsigned short*>(Aptr);
       const int Kp = (K + 31) & ~31;
 #pragma unroll
@@ -100,7 +102,7 @@ __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __res
 #pragma unroll
         for (int sl = 0; sl < 3; ++sl) {
           v8u16 v = {0, 0, 0, 0, 0, 0, 0, 0};
-          if (row < M && 32 * s < Kp) v = *(const v8u16*)(Asl + ((long long)sl * M + row) * Kp + 32 * s + 8 * (lane >> 4));
+          if (row < M && 32 * s < Kp) v = *(const v8u16*)(Asl + ((long long)sl * rm.Mtot + row) * Kp + 32 * s + 8 * (lane >> 4));
           ar[3 * s + sl] = v;
         }
     } else {
@@ -121,7 +123,7 @@ __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __res
   auto tile_cols = [&](long long t, long long& xbase, long long& c0, int& ncol) {
     if (rm.qone) {
       c0 = t * RTN;
-      xbase = c0 * K;
+      xbase = c0 * rm.xps;
       const long long left = rm.ncols - c0;
       ncol = left < RTN ? (int)left : RTN;
     } else {
@@ -162,7 +164,7 @@ __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __res
           const int e0 = j * EPC, c = e0 / K;
           const bool ok = c < ncol;
           if (ok) {
-            const TX* src = X + xbase + e0;
+            const TX* src = X + xbase + (long long)c * rm.xps + (e0 - c * K);
             if constexpr (EPC == 2) {
               const double2 d = *(const double2*)src;
               v[0] = d.x;
@@ -340,11 +342,18 @@ __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __res
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
               const int m = orow(r);
-              if (m < M) dst[m] = acc[j][r];
+              if (m < M) dst[m] = rm.accum ? dst[m] + acc[j][r] : acc[j][r];
             }
           } else {
             const int m = orow(0);  // rows m .. m+3, M % 4 == 0
-            if (m < M) *(float4*)(dst + m) = float4{acc[j][0], acc[j][1], acc[j][2], acc[j][3]};
+            if (m < M) {
+              float4 o = float4{acc[j][0], acc[j][1], acc[j][2], acc[j][3]};
+              if (rm.accum) {
+                const float4 d = *(const float4*)(dst + m);
+                o.x += d.x; o.y += d.y; o.z += d.z; o.w += d.w;
+              }
+              *(float4*)(dst + m) = o;
+            }
           }
         }
       }
@@ -408,7 +417,9 @@ int launch_ra(const void* A, int M, int K, const void* X, void* Y, long long P, 
   rm.xps = xps > 0 ? xps : (long long)K * Q;
   rm.accum = accum;
   const bool ksplit = rm.lda != K || rm.xps != (long long)K * Q || accum;
-  if (ksplit && (POL == 3 || Q % RTN || rm.lda < K || rm.xps < (long long)K * Q)) return 0;
+  if (ksplit && (POL == 3 || (Q != 1 && Q % RTN) || rm.lda < K || rm.xps < (long long)K * Q ||
+                 (Q == 1 && rm.xps % EPC)))
+    return 0;
   if (Q == 1) {
     // 16-byte chunks never straddle rows (K, M multiples of the chunk; bf16:
     // K % 8 == 0 for its 8-wide k groups).  Staged rows K..4*NS-1 of a k-step

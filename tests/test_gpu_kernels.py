@@ -211,11 +211,13 @@ def test_plane_contract_many_slabs(dev, prec, tol):
 
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-14), ("f32", 2e-6)])
 @pytest.mark.parametrize("M", [16, 126, 192, 256])
-@pytest.mark.parametrize("Kc", [132, 192, 256])
-@pytest.mark.parametrize("shape,axis", [((3, None, 64), 1), ((2, None, 192), 1), ((None, 128), 0)])
+@pytest.mark.parametrize("Kc", [130, 132, 192, 256])
+@pytest.mark.parametrize("shape,axis", [((3, None, 64), 1), ((2, None, 192), 1), ((None, 128), 0),
+                                        ((5, 33, None), 2), ((70, None), 1)])
 def test_contract_k_split(dev, prec, tol, M, Kc, shape, axis):
-    # 128 < K <= 256 with Q % 64 == 0 (the padded forward transforms): two
-    # column halves of the operator, the second accumulating into the output
+    # 128 < K <= 256 with Q % 64 == 0 or Q == 1 (the padded forward
+    # transforms): two column halves of the operator, the second accumulating
+    # into the output (f32 with K % 4 != 0 on the last axis: general kernel)
     import torch
 
     from jaxfun_amd import kernels as K
@@ -252,7 +254,7 @@ def test_contract_last_axis_even_k(dev, prec, tol, M, Kc):
     assert O.rel_l2(Y.to(torch.float64).cpu().numpy(), ref) < tol
 
 
-@pytest.mark.parametrize("prec,tol", [("f64", 1e-14), ("f32", 2e-6)])
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-14), ("f32", 2e-6), ("bf16x6", 2e-6)])
 @pytest.mark.parametrize("M", [132, 192, 256])
 @pytest.mark.parametrize("shape,axis", [((3, 126, 64), 1), ((5, 7, 126), 2), ((126, 128), 0)])
 def test_contract_row_split(dev, prec, tol, M, shape, axis):
