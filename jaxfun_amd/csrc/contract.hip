@@ -461,6 +461,28 @@ static int launch_slices(const unsigned short* Asl, const double* Arow, int M, i
           return set_error("contract: second row half rejected"), JFX_ESHAPE;
         }
       }
+      // 128 < K <= 256: two column halves split at a multiple of 32 (the
+      // first half's padded k-steps end where the second half starts), the
+      // second accumulating into Y; rows in one launch or two halves
+      if (K > 128 && K <= 256 && (Q % 64 == 0 || (Q == 1 && K % 8 == 0)) && M <= 256 &&
+          (M <= 128 || M % 4 == 0)) {
+        const int K0 = (K + 63) / 64 * 32, Kp = (K + KTB - 1) / KTB * KTB;
+        const int Mh = M <= 128 ? M : (((M + 1) / 2 + 3) & ~3);
+        int n = 0;
+        for (int kh = 0; kh < 2; ++kh)
+          for (int m0 = 0; m0 < M; m0 += Mh) {
+            const int k0 = kh ? K0 : 0, kn = kh ? K - K0 : K0, mr = M - m0 < Mh ? M - m0 : Mh;
+            if (!launch_ra<3>(Asl + (long long)m0 * Kp + k0, mr, kn, X + (long long)k0 * Q, Y, P, Q, st, &rc, M, m0,
+                              Kp, (long long)K * Q, kh)) {
+              if (n) return set_error("contract: K-split launch rejected"), JFX_ESHAPE;
+              kh = 2;
+              break;
+            }
+            if (rc) return rc;
+            ++n;
+          }
+        if (n) return JFX_OK;
+      }
     }
     if ((MODE == 0 || S == 3) && launch_persist<POL>(Asl, Arow, M, K, X, Y, P, Q, st, &rc)) return rc;
   }

@@ -56,9 +56,9 @@ struct RMap {
   long long P, Q, ncols, ntiles, nqt;  // nqt: 64-column tiles per p (Q % 64 == 0)
   int M, K, qone;
   int Mtot, moff;  // the output has Mtot rows along the axis; this launch writes rows moff..moff+M-1
-  // A row stride lda, X stride xps between consecutive p (the row stride of X
-  // when Q == 1), and accum = 1 adds this launch's sums to Y (a split of K
-  // over launches); FP64/FP32 only
+  // A row stride lda (bf16x6: of the slices), X stride xps between
+  // consecutive p (the row stride of X when Q == 1), and accum = 1 adds this
+  // launch's sums to Y (a split of K over launches)
   int lda, accum;
   long long xps;
 };
@@ -94,7 +94,8 @@ __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __res
   {
     const int row = 16 * w + (lane & 15);
     if constexpr (POL == 3) {
-      // slices of Mtot rows each (a row split passes Aptr at its first row)
+      // slices of Mtot rows of lda bf16 values each (a row split passes Aptr
+      // at its first row, a column split at its first column)
       const unsigned short* Asl = reinterpret_cast<const unsigned short*>(Aptr);
       const int Kp = (K + 31) & ~31;
 #pragma unroll
@@ -102,7 +103,7 @@ __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __res
 #pragma unroll
         for (int sl = 0; sl < 3; ++sl) {
           v8u16 v = {0, 0, 0, 0, 0, 0, 0, 0};
-          if (row < M && 32 * s < Kp) v = *(const v8u16*)(Asl + ((long long)sl * rm.Mtot + row) * Kp + 32 * s + 8 * (lane >> 4));
+          if (row < M && 32 * s < Kp) v = *(const v8u16*)(Asl + ((long long)sl * rm.Mtot + row) * rm.lda + 32 * s + 8 * (lane >> 4));
           ar[3 * s + sl] = v;
         }
     } else {
@@ -144,7 +145,7 @@ __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __res
         if (rm.qone) {
           const int c = gi >> 4, k8 = (gi & 15) * 8;
           const bool ok = c < ncol && k8 < K;
-          const float* src = X + xbase + (long long)c * K + k8;
+          const float* src = X + xbase + (long long)c * rm.xps + k8;
           float4 v0 = ok ? *(const float4*)src : float4{0, 0, 0, 0};
           float4 v1 = ok ? *(const float4*)(src + 4) : float4{0, 0, 0, 0};
           xr[8 * g + 0] = v0.x; xr[8 * g + 1] = v0.y; xr[8 * g + 2] = v0.z; xr[8 * g + 3] = v0.w;
@@ -413,12 +414,14 @@ int launch_ra(const void* A, int M, int K, const void* X, void* Y, long long P, 
   rm.moff = moff;
   if (rm.moff < 0 || rm.moff + M > rm.Mtot) return 0;
   rm.K = K;
-  rm.lda = lda > 0 ? lda : K;
+  // bf16x6 slice rows are zero-padded to a multiple of 32 values
+  const int lda0 = POL == 3 ? (K + 31) & ~31 : K;
+  rm.lda = lda > 0 ? lda : lda0;
   rm.xps = xps > 0 ? xps : (long long)K * Q;
   rm.accum = accum;
-  const bool ksplit = rm.lda != K || rm.xps != (long long)K * Q || accum;
-  if (ksplit && (POL == 3 || (Q != 1 && Q % RTN) || rm.lda < K || rm.xps < (long long)K * Q ||
-                 (Q == 1 && rm.xps % EPC)))
+  const bool ksplit = rm.lda != lda0 || rm.xps != (long long)K * Q || accum;
+  if (ksplit && ((Q != 1 && Q % RTN) || rm.lda < lda0 || rm.xps < (long long)K * Q ||
+                 (Q == 1 && rm.xps % (POL == 3 ? 8 : EPC))))
     return 0;
   if (Q == 1) {
     // 16-byte chunks never straddle rows (K, M multiples of the chunk; bf16:

@@ -209,7 +209,7 @@ def test_plane_contract_many_slabs(dev, prec, tol):
     assert O.rel_l2(Y.to(torch.float64).cpu().numpy(), Yr) < tol
 
 
-@pytest.mark.parametrize("prec,tol", [("f64", 1e-14), ("f32", 2e-6)])
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-14), ("f32", 2e-6), ("bf16x6", 2e-6)])
 @pytest.mark.parametrize("M", [16, 126, 192, 256])
 @pytest.mark.parametrize("Kc", [130, 132, 192, 256])
 @pytest.mark.parametrize("shape,axis", [((3, None, 64), 1), ((2, None, 192), 1), ((None, 128), 0),
