@@ -312,7 +312,7 @@ class TensorMatrix:
 
 
 def pointwise_mul(X: torch.Tensor, C: torch.Tensor) -> torch.Tensor:
-    from ..varcoeff import mul_bcast
+    from ..kernels import mul_bcast
 
     return mul_bcast(X, C)
 

@@ -172,7 +172,7 @@ class TensorProductSpace:
         its basis at the points; every other axis is a pointwise product with
         its basis at the points followed by a sum over its modes.
         """
-        from ..varcoeff import mul_bcast
+        from ..kernels import mul_bcast
 
         shp = tuple(x[0].shape) if isinstance(x[0], torch.Tensor) else np.shape(x[0])
         xs = [as_device(xi).reshape(-1) for xi in x]

@@ -211,18 +211,27 @@ class HelmholtzBatch:
                   K._p(self.VTd), K._p(self.Vs), K._p(self.VTs), K._p(self.lam), K._p(F), K._p(U),
                   K._p(self.scratch), self.nblocks, K._s())
 
+    def _check_io(self, F: torch.Tensor, U: torch.Tensor | None) -> None:
+        """F (and U) must be contiguous device tensors of the I/O dtype holding
+        exactly the packed batch: the kernel takes no sizes and trusts them."""
+        for name, t in (("F", F), ("U", U)):
+            if t is None:
+                continue
+            K._req(t, self.io_dtype, name)
+            if t.numel() != self.total:
+                raise ValueError(f"{name} has {t.numel()} entries, the batch needs {self.total}")
+        if U is not None and U.data_ptr() == F.data_ptr():
+            raise ValueError("U must not alias F (the kernel reads F while it writes U)")
+
     def solve(self, F: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
-        K._req(F, self.io_dtype, "F")
-        if F.numel() != self.total:
-            raise ValueError(f"F has {F.numel()} entries, the batch needs {self.total}")
+        self._check_io(F, out)
         U = torch.empty_like(F) if out is None else out
         self._launch(F, U)
         return U
 
     def capture(self, F: torch.Tensor, U: torch.Tensor) -> None:
         """Capture the whole batch (counter reset + persistent kernel) in a HIP graph."""
-        K._req(F, self.io_dtype, "F")
-        K._req(U, self.io_dtype, "U")
+        self._check_io(F, U)
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):

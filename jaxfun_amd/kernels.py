@@ -129,17 +129,35 @@ def lincomb(terms, out: torch.Tensor | None = None, accumulate: bool = False) ->
 def sum_products(terms, out: torch.Tensor | None = None) -> torch.Tensor:
     """``out = sum_t c_t * a_t * b_t`` pointwise for up to 4 (c, a, b) terms."""
     terms = [(float(c), a, b) for c, a, b in terms]
+    if not 1 <= len(terms) <= 4:
+        raise ValueError("sum_products takes 1 to 4 terms")
     ref = terms[0][1]
     out = torch.empty_like(ref) if out is None else out
+    _req(out, out.dtype, "out")
     for _, a, b in terms:
         _req(a, out.dtype, "a")
         _req(b, out.dtype, "b")
+        if a.shape != out.shape or b.shape != out.shape:
+            raise ValueError(f"sum_products: shapes {tuple(a.shape)}, {tuple(b.shape)} do not match "
+                             f"the output {tuple(out.shape)}")
     A = (ctypes.c_void_p * 4)(*[a.data_ptr() for _, a, _ in terms])
     B = (ctypes.c_void_p * 4)(*[b.data_ptr() for _, _, b in terms])
     C = (ctypes.c_double * 4)(*[c for c, _, _ in terms])
     dt = {torch.float64: 0, torch.float32: 1}[out.dtype]
     _lib.call("jfx_sum_products", dt, len(terms), A, B, C, _p(out), out.numel(), _s())
     return out
+
+
+def mul_bcast(X: torch.Tensor, C: torch.Tensor) -> torch.Tensor:
+    """``Y[b, ...] = X[b, ...] * C[...]`` (FP64, C broadcast over leading dims):
+    the pointwise coefficient product of a non-separable form on the mesh."""
+    _req(X, torch.float64, "X")
+    _req(C, torch.float64, "C")
+    if C.numel() == 0 or X.numel() % C.numel() != 0:
+        raise ValueError(f"mul_bcast: {X.numel()} entries are not a whole number of {C.numel()}-entry blocks")
+    Y = torch.empty_like(X)
+    _lib.call("jfx_mul_bcast_f64", _p(X), _p(C), _p(Y), X.numel(), C.numel(), _s())
+    return Y
 
 
 def div_bcast(X: torch.Tensor, D: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:

@@ -39,7 +39,7 @@ from . import kernels as K
 from .galerkin import FunctionSpace, Legendre, TensorProduct, TestFunction, TrialFunction, inner
 from .galerkin.inner import test_operator
 from .operators import Div, Grad
-from .varcoeff import mul_bcast
+from .kernels import mul_bcast
 
 
 def _f32(x):

@@ -33,9 +33,15 @@ pinned by checks that do not retype the formula (all in ``tests/test_oracle.py``
   Kronecker matrix; manufactured solution (1-x^2)(1-y^2).
 * IMEX ARS443: exact modal decay exp(-nu*lambda*t) of the linear heat
   equation (third-order convergence), and the tableau's row sums.
+* Burgers nonlinear term (``Burgers2D.nonlinear_sp``): exact
+  Chebyshev-weighted integrals of -phi_k phi_l u (u_x + u_y) for a
+  non-symmetric polynomial u (sympy moments), where the 3/2-padded rule is
+  exact; wrong sign, axis, dropped term and unpadded variants fail the check.
 * Variable-coefficient operator / PCG / MLP: dense assembled
-  operator vs the matrix-free one; PCG vs direct solve; MLP gradients vs
-  central finite differences.
+  operator vs the matrix-free one (and the constant-coefficient stiffness at
+  k = 1); load ``VarCoeffPoisson.rhs`` and field ``kfield`` vs
+  ``numpy.polynomial.legendre`` evaluation and quadrature (transposed data
+  fails); PCG vs direct solve; MLP gradients vs central finite differences.
 * Fourier transforms: ``numpy.fft`` is the transform itself (the reference
   calls ``jnp.fft``); pinned further by exactness on trigonometric
   polynomials, the reference's truncation/padding test

@@ -704,3 +704,99 @@ def test_robin_stencil_and_lift_meet_conditions(family):
     for r, row in enumerate(B):
         assert abs(val(-1.0, row) + 2.0 * val(-1.0, der(row, 1)) - (r == 0)) < 1e-13
         assert abs(val(1.0, row) - (r == 1)) < 1e-13
+
+
+# ---------------------------------------------------------------- Burgers nonlinear term, variable-coefficient data
+#
+# The padded nonlinear scalar product of the Burgers workload and the load /
+# coefficient field of the variable-coefficient workload are checked against
+# values computed without the oracle's matrices: exact Chebyshev-weighted
+# integrals of polynomials (sympy, moments pi (a-1)!!/a!!) and numpy's own
+# Legendre evaluation and quadrature.  Each check also runs on a deliberately
+# wrong variant (sign, axis, dropped term, no padding, transposed data) and
+# requires that variant to fail, so the check can tell the mistakes apart.
+
+
+def _cheb_weighted_moment(a: int):
+    """int_{-1}^{1} x^a (1 - x^2)^(-1/2) dx / pi (exact rational)."""
+    if a % 2:
+        return sp.Integer(0)
+    return sp.factorial2(a - 1) / sp.factorial2(a)
+
+
+def _burgers_nonlinear_exact(U, N):
+    """-int int phi_k(x) phi_l(y) u (u_x + u_y) w(x) w(y) dx dy for
+    u = sum U_ij phi_i(x) phi_j(y), phi_k = T_k - T_{k+2}, w = (1 - x^2)^(-1/2)."""
+    x, y = sp.symbols("x y")
+    n = N - 2
+    phi = [sp.Poly(sp.chebyshevt(k, x) - sp.chebyshevt(k + 2, x), x) for k in range(n)]
+    u = sum(U[i][j] * phi[i].as_expr() * phi[j].as_expr().subs(x, y) for i in range(n) for j in range(n))
+    G = sp.Poly(sp.expand(u * (sp.diff(u, x) + sp.diff(u, y))), x, y)
+    dx = G.degree(x) + N
+    # Mk[k][a] = int x^a phi_k(x) w dx / pi
+    Mk = [[sum(c * _cheb_weighted_moment(a + m[0]) for m, c in zip(phi[k].monoms(), phi[k].coeffs()))
+           for a in range(dx + 1)] for k in range(n)]
+    out = np.zeros((n, n))
+    terms = list(zip(G.monoms(), G.coeffs()))
+    for k in range(n):
+        for l in range(n):
+            s = sum(c * Mk[k][a] * Mk[l][b] for (a, b), c in terms)
+            out[k, l] = -float(s * sp.pi**2)
+    return out
+
+
+def test_burgers_nonlinear_sp_vs_exact_integral():
+    # integrators/base.py:238-248 + integrators/nonlinear.py:89-133: the padded
+    # (3/2-rule) scalar product of u (u_x + u_y); at N = 6 the 9-point
+    # Gauss-Chebyshev rule integrates the degree-15 integrand exactly
+    N = 6
+    rng = np.random.default_rng(31)
+    Ui = rng.integers(-9, 10, size=(N - 2, N - 2))
+    Ui[0, 1], Ui[1, 0] = 5, -3  # not symmetric: u(x, y) != u(y, x)
+    U = [[sp.Rational(int(v), 7) for v in row] for row in Ui]
+    Uf = Ui / 7.0
+    exact = _burgers_nonlinear_exact(U, N)
+    bg = O.Burgers2D(N, 0.1)
+    got = bg.nonlinear_sp(Uf)
+    assert O.rel_l2(got, exact) < 1e-13, O.rel_l2(got, exact)
+
+    # wrong variants must fail the same check
+    u = bg.B0 @ Uf @ bg.B0.T
+    ux = bg.B1 @ Uf @ bg.B0.T
+    uy = bg.B0 @ Uf @ bg.B1.T
+    SP = bg.SP
+    wrong = {
+        "sign": SP @ (u * (ux + uy)) @ SP.T,
+        "axis": -(SP @ (u * (uy + uy)) @ SP.T),
+        "dropped u_y": -(SP @ (u * ux) @ SP.T),
+        "transposed output": (-(SP @ (u * (ux + uy)) @ SP.T)).T,
+        "no padding": O.Burgers2D(N, 0.1, pad=1.0).nonlinear_sp(Uf),
+    }
+    for name, w in wrong.items():
+        assert O.rel_l2(w, exact) > 1e-3, name
+
+
+def test_varcoeff_rhs_and_kfield_vs_numpy_quadrature():
+    # b_kl = (phi_k phi_l, f) for f = sum fhat_ij P_i(x) P_j(y) (the linear
+    # form of inner(), galerkin/inner.py:914-984), and k = exp(g) at the
+    # Legendre-Gauss mesh, both from numpy.polynomial.legendre alone
+    N, m = 10, 4
+    vc = O.VarCoeffPoisson(N)
+    rng = np.random.default_rng(32)
+    fhat = rng.standard_normal((N, N))
+    xq, wq = L.leggauss(3 * N)  # exact for the degree <= 4N - 2 integrand
+    E = np.eye(N)
+    Phi = np.stack([L.legval(xq, E[k] - E[k + 2]) for k in range(N - 2)])  # (n, Q)
+    Vq = L.legvander(xq, N - 1)
+    fq = Vq @ fhat @ Vq.T  # f(x_i, y_j)
+    b = (Phi * wq) @ fq @ (Phi * wq).T
+    got = vc.rhs(fhat)
+    assert O.rel_l2(got, b) < 1e-13
+    assert O.rel_l2(vc.rhs(fhat.T), b) > 1e-2  # a transposed forcing is caught
+
+    ghat = rng.standard_normal((m, m)) * 0.5
+    xg, _ = L.leggauss(N)
+    Vg = L.legvander(xg, m - 1)
+    kq = np.exp(Vg @ ghat @ Vg.T)
+    assert O.rel_l2(vc.kfield(ghat), kq) < 1e-13
+    assert O.rel_l2(vc.kfield(ghat.T), kq) > 1e-2
