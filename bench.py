@@ -72,6 +72,9 @@ def bench_helmholtz(steps: int, warmup: int):
     return res, err, (Ns, alphas, loads)
 
 
+PRECISIONS_3D = ("f64", "f32", "bf16x6", "ozaki4")
+
+
 def bench_transform_3d(reps: int = 3):
     import torch
 
@@ -83,8 +86,11 @@ def bench_transform_3d(reps: int = 3):
     L = Legendre(N)
     T = TensorProduct(L, L, L)
     u64 = T.backward(c)
+    fx = np.load(os.path.join(ROOT, "tests", "golden", "c1_legendre_3d_samples.npz"))
+    idx = torch.tensor(fx["idx"].astype(np.int64), device="cuda")
+    uref = fx["u"]
     out = {}
-    for prec in ("f64", "f32", "bf16x6", "ozaki4"):
+    for prec in PRECISIONS_3D:
         u = T.backward(c, precision=prec)
         ch = T.forward(u, precision=prec)
         torch.cuda.synchronize()
@@ -96,12 +102,21 @@ def bench_transform_3d(reps: int = 3):
         e1.record()
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / reps
-        elt = 8 if prec in ("f64",) or prec.startswith("ozaki") else 4
+        elt = u.element_size()
         alg_bytes = 4.0 * N**3 * B * elt
         eb = float(torch.linalg.vector_norm(u.to(torch.float64) - u64) / torch.linalg.vector_norm(u64))
+        us = u[idx[:, 0], idx[:, 1], idx[:, 2], idx[:, 3]].to(torch.float64).cpu().numpy()
+        eo = float(np.linalg.norm(us - uref) / np.linalg.norm(uref))
+        # the exact round trip is the identity: its error is measured against the input itself
         er = float(torch.linalg.vector_norm(ch.to(torch.float64) - c) / torch.linalg.vector_norm(c))
-        out[prec] = dict(ms=ms, GBps=alg_bytes / ms / 1e6, TFLOPs=12.0 * N**4 * B / ms / 1e9,
-                         rel_err_backward=eb, rel_err_roundtrip=er)
+        out[prec] = dict(ms=ms, GBps=alg_bytes / ms / 1e6, TFLOPs=12.0 * N**4 * B / ms / 1e9, io_bytes_per_elt=elt,
+                         rel_err_backward_vs_fp64_gpu=eb, rel_err_backward_vs_oracle_fixture=eo,
+                         rel_err_roundtrip=er, **{"meets_1e-6": max(eo, er) <= 1e-6})
+    ok = {p: r for p, r in out.items() if r["meets_1e-6"]}
+    if ok:
+        best = max(ok, key=lambda p: ok[p]["GBps"])
+        out["best_at_1e-6"] = dict(precision=best, GBps=ok[best]["GBps"], ms=ok[best]["ms"],
+                                   speedup_vs_f64=out["f64"]["ms"] / ok[best]["ms"])
     return out
 
 
@@ -123,6 +138,9 @@ def bench_poisson_chebyshev(reps: int = 50):
         return T.backward(fd.solve(rhs))
 
     uj = step(b)
+    fx = np.load(os.path.join(ROOT, "tests", "golden", "c0_chebyshev_poisson_N64.npz"))
+    err_U = float(np.linalg.norm(fd.solve(b).cpu().numpy() - fx["U"]) / np.linalg.norm(fx["U"]))
+    err_u = float(np.linalg.norm(uj.cpu().numpy() - fx["u"]) / np.linalg.norm(fx["u"]))
     torch.cuda.synchronize()
     g = torch.cuda.CUDAGraph()
     s = torch.cuda.Stream()
@@ -151,7 +169,8 @@ def bench_poisson_chebyshev(reps: int = 50):
     torch.cuda.synchronize()
     bt = (time.perf_counter() - t0) / 5
     return dict(single_solve_us=lat * 1e6, single_solves_per_s=1.0 / lat, graph_matches_eager=same,
-                batched_4096_solves_per_s=4096 / bt)
+                batched_4096_solves_per_s=4096 / bt,
+                rel_err_coeffs_vs_oracle_kron_solve=err_U, rel_err_mesh_values_vs_oracle=err_u)
 
 
 def bench_lifted_helmholtz(reps: int = 50, batch: int = 4096):
@@ -394,25 +413,110 @@ def bench_learned_pcg(ntrain: int = 20000, neval: int = 1024, chunk: int = 1024,
                 iteration_ratio_no_network_over_learned=ita / itl, max_true_residual=res)
 
 
-def cpu_baseline(data, budget_s: float = 15.0):
-    """The oracle (numpy/scipy FP64 partial diagonalisation) on a bounded
-    sample of the same ragged batch, on the host cores."""
+def _host_info():
+    """CPU model, core count, BLAS vendor/threads and library versions of this host."""
+    import platform
+
+    import numpy
+    import scipy
+
+    model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    blas = []
+    try:
+        import threadpoolctl
+
+        blas = [dict(api=d.get("internal_api"), version=d.get("version"), threads=d.get("num_threads"))
+                for d in threadpoolctl.threadpool_info() if d.get("user_api") == "blas"]
+    except Exception:  # threadpoolctl is optional
+        pass
+    return dict(cpu_model=model, nproc=os.cpu_count(), sched_cores=len(os.sched_getaffinity(0)),
+                blas=blas, numpy=numpy.__version__, scipy=scipy.__version__)
+
+
+def cpu_baseline(data, nsample: int = 256, warmups: int = 3, reps: int = 10):
+    """CPU FP64 restatement of jaxfun's algorithm (oracle.spectral's
+    partial diagonalisation with BLAS transforms and LAPACK banded row
+    solves) on an N-stratified sample of the same ragged batch, timed on the
+    host cores in steady state: the per-N eigenpairs of every sampled N are
+    built before timing (the GPU timing likewise excludes table set-up), then
+    3 warm-up passes and the median of 10 timed passes over the sample."""
     from oracle import spectral as O
 
     Ns, alphas, loads = data
+    order = np.argsort(Ns, kind="stable")
+    pick = order[np.linspace(0, len(order) - 1, nsample).round().astype(int)]  # every (4096/256)-th by N
     t0 = time.perf_counter()
-    done = 0
-    for N, a, F in zip(Ns, alphas, loads):
-        O.helmholtz_partial_fd_lapack(int(N), float(a), F)
-        done += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=done / dt, unit="solves/s", cores=threads, kind="port",
-                sample=f"first {done} of the 4096 solves (seed 2), oracle.spectral.helmholtz_partial_fd_lapack "
-                       "(FP64 eigh per N + BLAS + LAPACK banded), "
-                       f"{dt:.1f} s, numpy/scipy BLAS threads={threads}")
+    for i in pick:  # table set-up, reported separately
+        O.helmholtz_partial_fd_lapack(int(Ns[i]), float(alphas[i]), loads[i])
+    setup_s = time.perf_counter() - t0
+
+    def one_pass():
+        for i in pick:
+            O.helmholtz_partial_fd_lapack(int(Ns[i]), float(alphas[i]), loads[i])
+
+    for _ in range(warmups):
+        one_pass()
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        one_pass()
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    info = _host_info()
+    threads = max([b["threads"] or 1 for b in info["blas"]] or [1])
+    return dict(value=nsample / med, unit="solves/s", cores=threads, kind="port",
+                label="CPU FP64 restatement of jaxfun's algorithm",
+                sample=f"{nsample} of the 4096 solves (seed 2) stratified by N (every 16th in N order, "
+                       f"N {int(Ns[pick].min())}..{int(Ns[pick].max())}), median of {reps} passes after "
+                       f"{warmups} warm-ups, per-N eigenpairs built beforehand",
+                pass_s_median=med, pass_s_min=float(min(times)), table_setup_s=setup_s, host=info)
+
+
+def _kernel_src_sha(name: str) -> str:
+    import hashlib
+
+    with open(os.path.join(ROOT, "jaxfun_amd", "csrc", name), "rb") as fh:
+        return hashlib.sha256(fh.read()).hexdigest()[:16]
+
+
+def helm_roofline(mixed, kname):
+    """Roofline of the ragged-batch kernel by the workload's algorithmic count:
+    F = sum(4 n^3 + 14 n^2) flop per batch, m = 6 bf16 products per logical
+    product, bytes = F in + U out once (FP32).  Roofline time is
+    max(F m / 2.5 PFLOP/s, bytes / 8 TB/s); the binding roof is reported."""
+    ks = mixed["kernel_ms"] * 1e-3
+    m = 6.0
+    t_mfma = mixed["alg_flops"] * m / (BF16_PEAK_TFS * 1e12)
+    t_hbm = mixed["io_bytes"] / (HBM_PEAK_GBS * 1e9)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "helm_ragged_4096_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tj = json.load(fh)
+        # only valid for the kernel source it was measured on
+        if tj.get("kernel") == kname and tj.get("src_sha") == _kernel_src_sha("helmholtz.hip"):
+            traffic = tj["fetch_bytes"] + tj["write_bytes"]
+    mfma_exec = mixed["mfma_flops"] / ks / 1e12
+    if t_mfma >= t_hbm:
+        achieved, peak, unit = mixed["alg_flops"] * m / ks / 1e12, BF16_PEAK_TFS, "TFLOP/s"
+    else:
+        achieved, peak, unit = mixed["io_bytes"] / ks / 1e9, HBM_PEAK_GBS, "GB/s"
+    return {"bound": "mfma" if t_mfma >= t_hbm else "hbm", "achieved": achieved, "peak": peak, "unit": unit,
+            "frac": achieved / peak, "traffic": traffic, "kernel": kname, "kernel_ms": mixed["kernel_ms"],
+            "roof_ms": {"mfma": t_mfma * 1e3, "hbm": t_hbm * 1e3},
+            "alg_flops_per_launch": mixed["alg_flops"], "products_per_flop": m,
+            "alg_bytes_per_launch": mixed["io_bytes"],
+            "hbm_frac": mixed["io_bytes"] / ks / 1e9 / HBM_PEAK_GBS,
+            "launch": {"nwaves": mixed["nwaves"], "nblocks": mixed["nblocks"], "queues": mixed["queues"]},
+            "mfma_executed_TFLOPs": mfma_exec, "mfma_executed_frac_of_bf16_peak": mfma_exec / BF16_PEAK_TFS}
 
 
 def main():
@@ -443,16 +547,7 @@ def main():
         del src, dst
     res, err, data = bench_helmholtz(args.steps, args.warmup)
     mixed, fp64 = res["mixed"], res["fp64"]
-    achieved = mixed["io_bytes"] / (mixed["kernel_ms"] * 1e-3) / 1e9
     kname = "helm_compact_kernel<float>" if mixed["nwaves"] == 4 else f"helm_ragged_kernel<1,{mixed['nwaves']},float>"
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "helm_ragged_4096_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            tj = json.load(fh)
-        if tj.get("kernel") == kname:
-            traffic = tj["fetch_bytes"] + tj["write_bytes"]
-    mfma_tfs = mixed["mfma_flops"] / (mixed["kernel_ms"] * 1e-3) / 1e12
     line = {
         "metric": "solves/s + 3D transform GB/s at <=1e-6 rel err vs plain FP64 GPU; PCG iters",
         "value": mixed["solves_per_s"],
@@ -464,21 +559,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16x6-split/fp32-acc (fp64 tridiagonal)",
+        "dtype": "bf16x6-split/fp32-acc (fp64 tridiagonal recurrences, fp32-stored sweep values)",
         "data": "synthetic (seeded generators, jaxfun_amd/workloads.py)",
         "config": {"workload": "ragged batch of 4096 2D Legendre-Dirichlet Helmholtz solves, "
                                "N~U{32..256}, alpha=10^U(-2,4), seed 2", "io_dtype": "float32",
                    "parallelism": "single GPU"},
         "rel_err_vs_fp64_gpu": err,
-        "fp64_path": {"solves_per_s": fp64["solves_per_s"], "ms_per_step": fp64["ms"]},
+        "meets_1e-6": err <= 1e-6,
+        "fp64_path": {"solves_per_s": fp64["solves_per_s"], "ms_per_step": fp64["ms"],
+                      "kernel_ms": fp64["kernel_ms"]},
         "speedup_vs_fp64_gpu": mixed["solves_per_s"] / fp64["solves_per_s"],
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": kname, "kernel_ms": mixed["kernel_ms"],
-                     "launch": {"nwaves": mixed["nwaves"], "nblocks": mixed["nblocks"], "queues": mixed["queues"]},
-                     "alg_bytes_per_launch": mixed["io_bytes"],
-                     "mfma_executed_TFLOPs": mfma_tfs, "mfma_frac_of_bf16_dense_peak": mfma_tfs / BF16_PEAK_TFS,
-                     "alg_TFLOPs_dense_4n3": mixed["alg_flops"] / (mixed["kernel_ms"] * 1e-3) / 1e12},
+        "roofline": helm_roofline(mixed, kname),
     }
     if not args.no_extras:
         line["transform_3d_16x128^3"] = bench_transform_3d()

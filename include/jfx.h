@@ -232,10 +232,12 @@ int jfx_cast(int direction, const void* x, void* y, long long n, hipStream_t str
  *   parity blocks of K_p V = M_p V diag(lam); Vs/VTs: the same as 3 bf16
  *   float-split slices each ([3][AP][AP] per table at 3*voff).  lam at loff.
  * mode 0: FP64 MFMA (fp64 I/O); mode 1: 3-term BF16 split MFMA with FP32
- *   accumulation/storage and FP64 tridiagonal solves with FP32-stored
- *   multipliers (io_f32 selects float
- *   or double F/U).  nwaves 4 or 8; scratch holds
- *   nblocks * jfx_helm_scratch_doubles_per_block() doubles.
+ *   accumulation/storage and FP64 tridiagonal recurrences whose sweep values
+ *   and multipliers are stored in FP32 (io_f32 selects float or double F/U).
+ *   nwaves 4 or 8 (mode 1: 4 = two 4-wave workgroups per CU, one column
+ *   parity at a time; 8 = one 8-wave workgroup per CU, both column parities
+ *   at once with whole-row F reads and U writes); scratch holds
+ *   nblocks * jfx_helm_scratch_doubles_per_block() doubles (mode 0 only).
  * Replaces TPMatrices.solve -> tpmats_lu_factor / TPMatricesLUFactors.solve
  * (la/tpmatrix.py:429-587, 1089-1233) applied once per solve, and the banded
  * DiaMatrix solve along an axis (la/diamatrix.py:592-770). */

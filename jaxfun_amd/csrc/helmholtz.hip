@@ -307,7 +307,7 @@ __global__ __launch_bounds__(NW * 64) void helm_ragged_kernel(
     const double* __restrict__ Vd, const double* __restrict__ VTd,
     const unsigned short* __restrict__ Vs, const unsigned short* __restrict__ VTs,
     const double* __restrict__ lam, const TIO* __restrict__ F, TIO* __restrict__ U,
-    double* __restrict__ scratch) {
+    double* __restrict__ mult_scratch) {
   typedef typename HelmTypes<MODE>::TS TS;
   constexpr int LDW = HelmLDS<MODE>::LDW, NT = NW * 64;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(NW * 64) void helm_ragged_kernel(
   double* Dg = H2 + HMAXA;                                                                 // 4c + 6
   __shared__ int s_nx[2];
   const int tid = threadIdx.x;
-  CPStore<MODE> cps{reinterpret_cast<float*>(u1), scratch + (long long)blockIdx.x * HMAXA * HMAXA};
+  CPStore<MODE> cps{reinterpret_cast<float*>(u1), mult_scratch + (long long)blockIdx.x * HMAXA * HMAXA};
 
   // the next item's index is claimed at the start of the current one, so the
   // counter round trip overlaps the work (two slots: slot k is rewritten only
@@ -909,6 +909,471 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   }
 }
 
+// ---------------------------------------------------------------------------
+// Pair variant (mixed precision): one 8-wave workgroup per CU runs both column
+// parities of a work item (solve, row parity p) at once -- waves 0-3 take
+// q = 0, waves 4-7 take q = 1 -- so the rows of F[p::2, :] are read whole and
+// once, both halves share every table slab, and the rows of U[p::2, :] are
+// assembled in LDS and written whole and once.
+//   Wk0, Wk1 : 128 x 130 FP32 parity blocks F_p0, F_p1 -> G_pq -> U~_pq; after
+//              the second GEMM they hold the interleaved output rows
+//              (128 x 257 FP32 staging, row m = row p + 2m of U)
+//   slab     : 3 bf16 slices x 128 table rows x 32 k (16-byte chunks swizzled)
+//   H        : per parity h_{q+2j} and h_{q+2j} + h_{q+2j+2}   (FP64)
+struct HelmPair {
+  static constexpr int LDW = 130, NT = 512, LDU = 257;
+  static constexpr int WK1 = HMAXA * LDW + 32;  // offset of Wk1 (floats): shifted by 32 banks from Wk0
+  static constexpr size_t WK = (size_t)(WK1 + HMAXA * LDW) * 4;
+  static constexpr size_t SLAB = 3 * HMAXA * 32 * 2;
+  static constexpr size_t HT = 2 * (HMAXA + 2) * 8;
+  static constexpr size_t BYTES = WK + SLAB + 2 * HT;
+};
+static_assert(HelmPair::BYTES + 64 <= 160 * 1024, "the pair workgroup's LDS must fit one CU");
+static_assert((size_t)HMAXA * HelmPair::LDU * 4 <= HelmPair::WK, "the output staging must fit the work blocks");
+
+__device__ __forceinline__ int pslab_off(int sl, int row, int ck) {  // in bf16 units
+  return ((sl * HMAXA + row) * 4 + (ck ^ ((row >> 2) & 3))) * 8;
+}
+
+struct PairSlab {
+  v8u16 v[3];
+};
+
+// Loads of k-slab k0 of a 3-slice (AP x AP) bf16 table: 3 x AP rows x 4 chunks of 8.
+__device__ __forceinline__ void pslab_fetch(PairSlab& P, const unsigned short* __restrict__ T, int AP, int k0,
+                                            int tid) {
+#pragma unroll
+  for (int u = 0; u < 3; ++u) {
+    const int e = tid + u * 512, sl = e >> 9, row = (e & 511) >> 2, ck = e & 3;
+    if (row < AP) P.v[u] = *(const v8u16*)(T + (long long)sl * AP * AP + (long long)row * AP + k0 + ck * 8);
+  }
+}
+
+__device__ __forceinline__ void pslab_store(const PairSlab& P, unsigned short* slab, int AP, int tid) {
+#pragma unroll
+  for (int u = 0; u < 3; ++u) {
+    const int e = tid + u * 512, sl = e >> 9, row = (e & 511) >> 2, ck = e & 3;
+    if (row < AP) *(v8u16*)&slab[pslab_off(sl, row, ck)] = P.v[u];
+  }
+}
+
+// Rows of F[p::2, :] in registers: wave w takes rows w + 8i for i in
+// [i0, i0 + 8), lane l columns l + 64c (c < 4; n <= 256).  FP32 I/O only.
+// Columns past the row's end are loaded from its last column and land in
+// parity-block columns >= b, which only feed output columns that are never
+// stored, so the loads need no per-lane branches.
+struct PairGather {
+  float v[8][4];
+};
+
+__device__ __forceinline__ void pair_gather_issue(PairGather& G, const float* __restrict__ Fb, int n, int p, int a,
+                                                  int i0, int lane) {
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  unsigned col[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) col[c] = (unsigned)min(lane + 64 * c, n - 1);
+  unsigned off = (unsigned)((p + 2 * (w + 8 * i0)) * n);  // < n^2 <= 65536
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    if (w + 8 * (i0 + i) < a) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) G.v[i][c] = Fb[off + col[c]];
+    }
+    off += 16u * (unsigned)n;
+  }
+}
+
+// Column j = l + 64c of row m goes to parity block j & 1 = l & 1 at column j >> 1 (<= 127).
+__device__ __forceinline__ void pair_gather_store(const PairGather& G, int a, int i0, float* Wk, int lane) {
+  constexpr int LDW = HelmPair::LDW;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  float* dst = Wk + (lane & 1) * HelmPair::WK1 + (w + 8 * i0) * LDW + (lane >> 1);
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    if (w + 8 * (i0 + i) < a) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) dst[32 * c] = G.v[i][c];
+    }
+    dst += 8 * LDW;
+    asm volatile("" : "+v"(dst));  // one running address, not 8 hoisted ones
+  }
+}
+
+// All rows of one item (FP32 I/O): rows w + 8i, i < 8, through G, then the rest.
+__device__ __forceinline__ void pair_gather_all(PairGather& G, const float* __restrict__ Fb, int n, int p, int a,
+                                                float* Wk, int lane) {
+  pair_gather_issue(G, Fb, n, p, a, 0, lane);
+  pair_gather_store(G, a, 0, Wk, lane);
+  if (a > 64) {
+    pair_gather_issue(G, Fb, n, p, a, 8, lane);
+    pair_gather_store(G, a, 8, Wk, lane);
+  }
+}
+
+// FP64 I/O: the same gather without the register prefetch (loads then stores, 4 rows at a time).
+__device__ __forceinline__ void pair_gather_direct(const double* __restrict__ Fb, int n, int p, int a, float* Wk) {
+  constexpr int LDW = HelmPair::LDW;
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int col[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) col[c] = min(lane + 64 * c, n - 1);
+  float* dst0 = Wk + (lane & 1) * HelmPair::WK1 + (lane >> 1);
+  for (int m0 = w; m0 < a; m0 += 32) {
+    double v[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int m = m0 + 8 * i;
+      if (m < a) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[i][c] = Fb[(p + 2 * m) * n + col[c]];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int m = m0 + 8 * i;
+      if (m < a) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) dst0[m * LDW + 32 * c] = (float)v[i][c];
+      }
+    }
+  }
+}
+
+// acc = T * Wkq over k < AP for this wave's two column panels (wq, wq + 4) of
+// parity block Wkq (a x b); T is the 3-slice (AP x AP) table whose slab k0 = 0
+// is already in flight in P.  Rows a..AP-1 of Wkq hold zeros.  On return P
+// holds nothing useful and the slab area is free.
+__device__ __forceinline__ void pair_gemm(int a, int AP, int b, const unsigned short* __restrict__ T,
+                                          const float* Wkq, unsigned short* slab, PairSlab& P, v4f (&acc)[2][8],
+                                          int wq, int lane, int tid) {
+  constexpr int LDW = HelmPair::LDW;
+  const int npanel = (b + 15) >> 4, nrb = (a + 15) >> 4;
+  const int npp = (wq + 4 < npanel) ? 2 : (wq < npanel ? 1 : 0);
+#pragma unroll
+  for (int pp = 0; pp < 2; ++pp)
+#pragma unroll
+    for (int rb = 0; rb < 8; ++rb) acc[pp][rb] = v4f{0, 0, 0, 0};
+  const int off0 = pslab_off(0, lane & 15, lane >> 4);
+  for (int k0 = 0; k0 < AP; k0 += 32) {
+    v8u16 bh[2], bm[2], bl[2];
+#pragma unroll
+    for (int pp = 0; pp < 2; ++pp) {
+      const float* src = Wkq + (k0 + 8 * (lane >> 4)) * LDW + (wq + 4 * pp) * 16 + (lane & 15);
+      float xv[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) xv[e] = src[e * LDW];
+      split3_pk8(xv, bh[pp], bm[pp], bl[pp]);
+    }
+    pslab_store(P, slab, AP, tid);
+    __syncthreads();
+    if (k0 + 32 < AP) pslab_fetch(P, T, AP, k0 + 32, tid);
+    v8u16 A[2][3];
+    auto aload = [&](v8u16(&dst)[3], int rb) {
+      const int off = off0 + rb * 16 * 4 * 8;  // 16 rows further; the swizzle depends on row bits 2-3 only
+      dst[0] = *(const v8u16*)&slab[off];
+      dst[1] = *(const v8u16*)&slab[off + HMAXA * 32];
+      dst[2] = *(const v8u16*)&slab[off + 2 * HMAXA * 32];
+    };
+    aload(A[0], 0);
+#pragma unroll
+    for (int rb = 0; rb < 8; ++rb) {
+      if (rb + 1 < 8 && rb + 1 < nrb) aload(A[(rb + 1) & 1], rb + 1);
+      if (rb < nrb) {
+        const v8u16 ah = A[rb & 1][0], am = A[rb & 1][1], al = A[rb & 1][2];
+#pragma unroll
+        for (int pp = 0; pp < 2; ++pp) {
+          if (pp < npp) {
+            // products ordered small to large
+            acc[pp][rb] = mfma_bf16(al, bh[pp], acc[pp][rb]);
+            acc[pp][rb] = mfma_bf16(am, bm[pp], acc[pp][rb]);
+            acc[pp][rb] = mfma_bf16(ah, bl[pp], acc[pp][rb]);
+            acc[pp][rb] = mfma_bf16(am, bh[pp], acc[pp][rb]);
+            acc[pp][rb] = mfma_bf16(ah, bm[pp], acc[pp][rb]);
+            acc[pp][rb] = mfma_bf16(ah, bh[pp], acc[pp][rb]);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// FP64 row solves of one parity block, two lanes per row (twisted
+// factorisation, as in helmc_pass): tq in [0, 256) is the thread's index
+// within its parity half.  Rows >= a hold the GEMMs' zero padding, so their
+// idle lanes work on `dummy` instead.
+__device__ __forceinline__ void pair_row_solves(const HelmItem& I, int q, int b, int tq, float* Wkq,
+                                                const double* __restrict__ lam, const double* H, float* dummy) {
+  constexpr int LDW = HelmPair::LDW;
+  const int a = I.a;
+  const double* Hs = H + HMAXA + 2;
+  int t2 = tq;
+  asm volatile("" : "+v"(t2));
+  const int r = t2 >> 1, side = t2 & 1;
+  const bool act = r < a;
+  const double s = act ? I.alpha - lam[I.loff + r] : 1.0;
+  float* row = act ? Wkq + r * LDW : dummy + side * LDW;
+  const int len = b >> 1;
+  const int j0 = side ? b - 1 : 0, dj = side ? -1 : 1;
+  const double* Ho = side ? H : H + 1;  // o towards the middle = -s Ho[j]
+  const double dg0 = fma(8.0, (double)j0, 4.0 * q + 6.0), ddg = 8.0 * dj;
+  float cm[64];
+  double P1 = 1.0, P2 = 0.0, yprev = 0.0, oprev = 0.0, cl = 0.0;
+  auto fwd_v = [&](int i, double g, double hs, double ho) {
+    const int j = j0 + dj * i;
+    const double d = fma(s, hs, fma((double)i, ddg, dg0));
+    const double on = -s * ho;
+    const double Pj = fma(d, P1, -(oprev * oprev) * P2);
+    const double rj = P1 * rcp64(Pj);
+    const double yj = fma(-oprev, yprev, g) * rj;
+    cl = on * rj;
+    cm[i] = (float)cl;
+    row[j] = (float)yj;
+    P2 = P1;
+    P1 = Pj;
+    yprev = yj;
+    oprev = on;
+  };
+  auto fwd = [&](int i) {
+    const int j = j0 + dj * i;
+    fwd_v(i, row[j], Hs[j], Ho[j]);
+  };
+#pragma unroll
+  for (int ch = 0; ch < 8; ++ch) {
+    if (ch * 8 + 8 <= len) {
+      float g[8];
+      double hs[8], ho[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int j = j0 + dj * (ch * 8 + u);
+        g[u] = row[j];
+        hs[u] = Hs[j];
+        ho[u] = Ho[j];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) fwd_v(ch * 8 + u, g[u], hs[u], ho[u]);
+      renorm(P1, P2);
+    } else if (ch * 8 < len) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (ch * 8 + u < len) fwd(ch * 8 + u);
+    }
+  }
+  const double co = __shfl_xor(cl, 1), yo = __shfl_xor(yprev, 1);  // the partner's last step
+  double own;
+  int start;
+  if (b & 1) {
+    const int m = len;
+    const double ct = side == 0 ? cl : co, yt = side == 0 ? yprev : yo;  // rows < m
+    const double cb = side == 0 ? co : cl, yb = side == 0 ? yo : yprev;  // rows > m
+    const double am = (m > 0) ? -s * H[m] : 0.0;
+    const double cmm = (m < b - 1) ? -s * H[m + 1] : 0.0;
+    const double dm = fma(s, Hs[m], fma(8.0, (double)m, 4.0 * q + 6.0));
+    own = ((double)row[m] - am * yt - cmm * yb) / (dm - am * ct - cmm * cb);
+    start = len - 1;
+    if (side == 0) row[m] = (float)own;  // both lanes have read row[m] above
+  } else {
+    const double ct = side == 0 ? cl : co, yt = side == 0 ? yprev : yo;
+    const double cb = side == 0 ? co : cl, yb = side == 0 ? yo : yprev;
+    const double ut = (yt - ct * yb) / (1.0 - ct * cb);
+    own = side == 0 ? ut : yb - cb * ut;
+    start = len - 2;
+    row[j0 + dj * (len - 1)] = (float)own;
+  }
+  double uprev = own;
+  auto bwd = [&](int i, float y) {
+    const double v = fma(-(double)cm[i], uprev, (double)y);
+    row[j0 + dj * i] = (float)v;
+    uprev = v;
+  };
+#pragma unroll
+  for (int ch = 7; ch >= 0; --ch) {
+    if (ch * 8 + 7 <= start) {
+      float y[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) y[u] = row[j0 + dj * (ch * 8 + u)];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int u = 7; u >= 0; --u) bwd(ch * 8 + u, y[u]);
+    } else if (ch * 8 <= start) {
+#pragma unroll
+      for (int u = 7; u >= 0; --u)
+        if (ch * 8 + u <= start) bwd(ch * 8 + u, row[j0 + dj * (ch * 8 + u)]);
+    }
+  }
+}
+
+template <typename TIO>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void helm_pair_kernel(
+    const HelmItem* __restrict__ items, int nitems, HelmQueues Q, const unsigned short* __restrict__ Vs,
+    const unsigned short* __restrict__ VTs, const double* __restrict__ lam, const TIO* __restrict__ F,
+    TIO* __restrict__ U) {
+  constexpr int LDW = HelmPair::LDW, LDU = HelmPair::LDU, NT = HelmPair::NT;
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  float* Wk = reinterpret_cast<float*>(smem_raw);
+  unsigned short* slab = reinterpret_cast<unsigned short*>(smem_raw + HelmPair::WK);
+  double* Htab = reinterpret_cast<double*>(smem_raw + HelmPair::WK + HelmPair::SLAB);  // [q][2][HMAXA + 2]
+  __shared__ int s_nx[2];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int q = w >> 2, wq = w & 3;  // this wave's column parity and its index within the half
+  float* Wkq = Wk + q * HelmPair::WK1;
+  double* Hq = Htab + q * 2 * (HMAXA + 2);
+
+  int qi = blockIdx.x % Q.nq;
+  if (tid < 64) {
+    const int first = Q.claim(qi);
+    if (tid == 0) s_nx[0] = first;
+  }
+  __syncthreads();
+  const int it0 = __builtin_amdgcn_readfirstlane(s_nx[0]);
+  if (it0 >= nitems) return;
+  HelmItem I = items[it0];
+  if constexpr (sizeof(TIO) == 4) {
+    PairGather G;
+    pair_gather_all(G, reinterpret_cast<const float*>(F) + I.foff, I.n, I.p, I.a, Wk, lane);
+  } else {
+    pair_gather_direct(reinterpret_cast<const double*>(F) + I.foff, I.n, I.p, I.a, Wk);
+  }
+  PairSlab P;
+  pslab_fetch(P, VTs + 3 * I.voff, I.AP, 0, tid);
+  int slot = 1;
+  for (;;) {
+    int nxt = 0;
+    if (tid < 64) nxt = Q.claim(qi);
+    // opaque per-item copies of the thread indices: every lane-dependent
+    // address is recomputed per item instead of being hoisted out of the
+    // item loop into registers
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int lane = tid & 63;
+    const int a = I.a, AP = I.AP, n = I.n;
+    const int b = (n - q + 1) >> 1;
+    // ---------------- phase 0: zero pad rows (both blocks), h tables of this wave's parity
+    for (int e = tid; e < (AP - a) * 128; e += NT) {
+      const int r = a + (e >> 7), c = e & 127;
+      Wk[r * LDW + c] = 0.f;
+      Wk[HelmPair::WK1 + r * LDW + c] = 0.f;
+    }
+    for (int j = tid & 255; j <= b; j += 256) {
+      const double hj = 2.0 / (2 * (q + 2 * j) + 1);
+      Hq[j] = hj;
+      Hq[HMAXA + 2 + j] = hj + 2.0 / (2 * (q + 2 * j) + 5);
+    }
+    __syncthreads();
+
+    // ---------------- phase 1: G_pq = V_p^T F_pq, in place
+    {
+      v4f acc[2][8];
+      pair_gemm(a, AP, b, VTs + 3 * I.voff, Wkq, slab, P, acc, wq, lane, tid);
+      pslab_fetch(P, Vs + 3 * I.voff, AP, 0, tid);  // phase 3's first slab flies during the row solves
+      int ln = lane;
+      asm volatile("" : "+v"(ln));  // keeps the lane-dependent addresses inside the item loop
+      const int npanel = (b + 15) >> 4, npp = (wq + 4 < npanel) ? 2 : (wq < npanel ? 1 : 0);
+      const int nrb = (a + 15) >> 4, lr = 4 * (ln >> 4), lim = a - lr;
+#pragma unroll
+      for (int pp = 0; pp < 2; ++pp) {
+        if (pp >= npp) continue;
+        const int col = (wq + 4 * pp) * 16 + (ln & 15);
+        float* dst = Wkq + lr * LDW + col;
+#pragma unroll
+        for (int rb = 0; rb < 8; ++rb) {
+          if (rb >= nrb) continue;
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (rb * 16 + r < lim) dst[r * LDW] = acc[pp][rb][r];
+          dst += 16 * LDW;
+          asm volatile("" : "+v"(dst));
+        }
+      }
+    }
+    __syncthreads();
+
+    // ---------------- phase 2: FP64 row solves of both parities
+    if (wq * 32 < a)
+      pair_row_solves(I, q, b, tid & 255, Wkq, lam, Hq, reinterpret_cast<float*>(slab) + q * 2 * LDW);
+    __syncthreads();
+
+    // ---------------- phase 3: U_pq = V_p U~_pq; next item's loads in flight around the epilogue
+    if (tid == 0) s_nx[slot] = nxt;
+    v4f acc[2][8];
+    pair_gemm(a, AP, b, Vs + 3 * I.voff, Wkq, slab, P, acc, wq, lane, tid);
+    const int itn = __builtin_amdgcn_readfirstlane(s_nx[slot]);
+    slot ^= 1;
+    const bool has = itn < nitems;
+    const HelmItem In = items[has ? itn : it0];
+    if (has) pslab_fetch(P, VTs + 3 * In.voff, In.AP, 0, tid);
+    // interleaved staging: row m, column 2c + q
+    {
+      int ln = lane;
+      asm volatile("" : "+v"(ln));
+      const int npanel = (b + 15) >> 4, npp = (wq + 4 < npanel) ? 2 : (wq < npanel ? 1 : 0);
+      const int nrb = (a + 15) >> 4, lr = 4 * (ln >> 4), lim = a - lr;
+#pragma unroll
+      for (int pp = 0; pp < 2; ++pp) {
+        if (pp >= npp) continue;
+        const int col = (wq + 4 * pp) * 16 + (ln & 15);
+        const bool cok = col < b;
+        float* dst = Wk + lr * LDU + 2 * col + q;
+#pragma unroll
+        for (int rb = 0; rb < 8; ++rb) {
+          if (rb >= nrb) continue;
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (cok && rb * 16 + r < lim) dst[r * LDU] = acc[pp][rb][r];
+          dst += 16 * LDU;
+          asm volatile("" : "+v"(dst));
+        }
+      }
+    }
+    // the next item's first rows fly while this item's rows are stored
+    PairGather G;
+    if constexpr (sizeof(TIO) == 4)
+      if (has) pair_gather_issue(G, reinterpret_cast<const float*>(F) + In.foff, In.n, In.p, In.a, 0, lane);
+    __syncthreads();
+    // whole rows of U[p::2, :]: wave w rows w, w + 8, ..., lane l columns l + 64c
+    {
+      TIO* Ub = U + I.foff;
+      int ln = lane;
+      asm volatile("" : "+v"(ln));
+      for (int m = w; m < a; m += 8) {
+        TIO* dst = Ub + (unsigned)((I.p + 2 * m) * n);
+        const float* src = Wk + m * LDU;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          if (ln + 64 * c < n) dst[(unsigned)(ln + 64 * c)] = (TIO)src[ln + 64 * c];
+      }
+    }
+    __syncthreads();
+    if (!has) break;
+    if constexpr (sizeof(TIO) == 4) {
+      pair_gather_store(G, In.a, 0, Wk, lane);
+      if (In.a > 64) {
+        pair_gather_issue(G, reinterpret_cast<const float*>(F) + In.foff, In.n, In.p, In.a, 8, lane);
+        pair_gather_store(G, In.a, 8, Wk, lane);
+      }
+    } else
+      pair_gather_direct(reinterpret_cast<const double*>(F) + In.foff, In.n, In.p, In.a, Wk);
+    I = In;
+    // (phase 0 of the next item writes only rows >= a and the h tables; its barrier publishes everything)
+  }
+}
+
+template <typename TIO>
+static int launch_helm_pair(const HelmItem* items, int nitems, HelmQueues Q, const unsigned short* Vs,
+                            const unsigned short* VTs, const double* lam, const TIO* F, TIO* U, int nblocks,
+                            hipStream_t st) {
+  const size_t shm = HelmPair::BYTES;
+  auto kern = helm_pair_kernel<TIO>;
+  hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+  if (e != hipSuccess) return set_error("helm: hipFuncSetAttribute: %s", hipGetErrorString(e)), JFX_EHIP;
+  hipLaunchKernelGGL(helm_reset, dim3(1), dim3(64), 0, st, Q.counters, Q.nq);
+  hipLaunchKernelGGL(kern, dim3(nblocks), dim3(HelmPair::NT), shm, st, items, nitems, Q, Vs, VTs, lam, F, U);
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
+}
+
 template <typename TIO>
 static int launch_helm_compact(const HelmItem* items, int nitems, HelmQueues Q, const unsigned short* Vs,
                                const unsigned short* VTs, const double* lam, const TIO* F, TIO* U, int nblocks,
@@ -927,7 +1392,7 @@ static int launch_helm_compact(const HelmItem* items, int nitems, HelmQueues Q, 
 template <int MODE, int NW, typename TIO>
 static int launch_helm(const HelmItem* items, int nitems, HelmQueues Q, const double* Vd,
                        const double* VTd, const unsigned short* Vs, const unsigned short* VTs,
-                       const double* lam, const TIO* F, TIO* U, double* scratch, int nblocks,
+                       const double* lam, const TIO* F, TIO* U, double* mult_scratch, int nblocks,
                        hipStream_t st) {
   size_t shm = HelmLDS<MODE>::WK + HelmLDS<MODE>::U1 + HTAB;
   auto kern = helm_ragged_kernel<MODE, NW, TIO>;
@@ -935,7 +1400,7 @@ static int launch_helm(const HelmItem* items, int nitems, HelmQueues Q, const do
   if (e != hipSuccess) return set_error("helm: hipFuncSetAttribute: %s", hipGetErrorString(e)), JFX_EHIP;
   hipLaunchKernelGGL(helm_reset, dim3(1), dim3(64), 0, st, Q.counters, Q.nq);
   hipLaunchKernelGGL(kern, dim3(nblocks), dim3(NW * 64), shm, st, items, nitems, Q, Vd, VTd, Vs, VTs,
-                     lam, F, U, scratch);
+                     lam, F, U, mult_scratch);
   JFX_LAUNCH_CHECK();
   return JFX_OK;
 }
@@ -953,12 +1418,12 @@ int jfx_helm_scratch_doubles_per_block(void) { return HMAXA * HMAXA; }
 // mode 0: FP64 (F, U double; Vd/VTd used)
 // mode 1: BF16 3-term split, FP32 storage (F, U float if io_f32 else double; Vs/VTs used)
 // nwaves: 4 or 8 waves per workgroup; nblocks: persistent grid size
-//   (scratch must hold nblocks * jfx_helm_scratch_doubles_per_block() doubles).
+//   (mult_scratch must hold nblocks * jfx_helm_scratch_doubles_per_block() doubles).
 int jfx_helm_ragged(int mode, int io_f32, int nwaves, const void* items, int nitems, int* counters,
                     const int* qbounds, int nqueues, const double* Vd, const double* VTd, const unsigned short* Vs,
                     const unsigned short* VTs, const double* lam, const void* F, void* U,
-                    double* scratch, int nblocks, hipStream_t stream) {
-  if (!items || !counters || !qbounds || !lam || !F || !U || !scratch || nitems < 0 || nblocks <= 0)
+                    double* mult_scratch, int nblocks, hipStream_t stream) {
+  if (!items || !counters || !qbounds || !lam || !F || !U || !mult_scratch || nitems < 0 || nblocks <= 0)
     return set_error("helm_ragged: bad arguments"), JFX_EARG;
   if (nqueues < 1 || nqueues > HMAXQ) return set_error("helm_ragged: nqueues must be in [1, 16]"), JFX_EARG;
   if (nitems == 0) return JFX_OK;
@@ -968,10 +1433,10 @@ int jfx_helm_ragged(int mode, int io_f32, int nwaves, const void* items, int nit
     if (!Vd || !VTd || io_f32) return set_error("helm_ragged: mode 0 needs fp64 tables and fp64 I/O"), JFX_EARG;
     if (nwaves == 4)
       return launch_helm<0, 4, double>(it, nitems, queues, Vd, VTd, Vs, VTs, lam, (const double*)F,
-                                       (double*)U, scratch, nblocks, stream);
+                                       (double*)U, mult_scratch, nblocks, stream);
     if (nwaves == 8)
       return launch_helm<0, 8, double>(it, nitems, queues, Vd, VTd, Vs, VTs, lam, (const double*)F,
-                                       (double*)U, scratch, nblocks, stream);
+                                       (double*)U, mult_scratch, nblocks, stream);
   } else if (mode == 1) {
     if (!Vs || !VTs) return set_error("helm_ragged: mode 1 needs bf16 slice tables"), JFX_EARG;
     if (io_f32) {
@@ -979,15 +1444,15 @@ int jfx_helm_ragged(int mode, int io_f32, int nwaves, const void* items, int nit
         return launch_helm_compact<float>(it, nitems, queues, Vs, VTs, lam, (const float*)F, (float*)U, nblocks,
                                           stream);
       if (nwaves == 8)
-        return launch_helm<1, 8, float>(it, nitems, queues, Vd, VTd, Vs, VTs, lam, (const float*)F,
-                                        (float*)U, scratch, nblocks, stream);
+        return launch_helm_pair<float>(it, nitems, queues, Vs, VTs, lam, (const float*)F, (float*)U, nblocks,
+                                       stream);
     } else {
       if (nwaves == 4)
         return launch_helm_compact<double>(it, nitems, queues, Vs, VTs, lam, (const double*)F, (double*)U,
                                            nblocks, stream);
       if (nwaves == 8)
-        return launch_helm<1, 8, double>(it, nitems, queues, Vd, VTd, Vs, VTs, lam, (const double*)F,
-                                         (double*)U, scratch, nblocks, stream);
+        return launch_helm_pair<double>(it, nitems, queues, Vs, VTs, lam, (const double*)F, (double*)U, nblocks,
+                                        stream);
     }
   }
   return set_error("helm_ragged: bad mode/nwaves"), JFX_EARG;

@@ -64,9 +64,13 @@ class HelmholtzBatch:
         Ns: per-solve number of Legendre modes N_b (unknowns (N_b-2)^2), 4 <= N_b <= 258.
         alphas: per-solve Helmholtz shifts alpha_b >= 0.
         mode: ``"fp64"`` (FP64 MFMA) or ``"mixed"`` (3-term bf16 split MFMA,
-            FP32 accumulation, FP64 tridiagonal solves).
+            FP32 accumulation, FP64 tridiagonal recurrences with FP32-stored
+            sweep values and multipliers).
         io_dtype: dtype of F/U (``mixed`` defaults to float32, ``fp64`` to float64).
-        nwaves, blocks_per_cu: launch shape (see :func:`autotune`).
+        nwaves, blocks_per_cu: launch shape (see :func:`autotune`); in mixed mode
+            nwaves 4 runs one column parity at a time (two workgroups fit a CU),
+            nwaves 8 both at once with whole-row reads of F and writes of U
+            (one workgroup per CU).
         queues: number of work queues (1..16); items sharing tables share a queue.
     """
 
@@ -275,10 +279,14 @@ class HelmholtzBatch:
         return float(2 * self.total * torch.tensor([], dtype=self.io_dtype).element_size())
 
 
-def autotune(batch: HelmholtzBatch, F: torch.Tensor,
-             candidates=((4, 1), (8, 1), (4, 2), (8, 2)), reps: int = 3) -> tuple[int, int]:
+def autotune(batch: HelmholtzBatch, F: torch.Tensor, candidates=None, reps: int = 3) -> tuple[int, int]:
     """Time each (waves per workgroup, workgroups per CU) launch shape on the
-    batch itself, keep the fastest on ``batch`` and return it."""
+    batch itself, keep the fastest on ``batch`` and return it.  In mixed mode
+    the shapes are the two kernels at their occupancy: (4, 2) two 4-wave
+    workgroups per CU (one column parity at a time) and (8, 1) one 8-wave
+    workgroup per CU (both column parities at once), plus (4, 1)."""
+    if candidates is None:
+        candidates = ((4, 2), (8, 1), (4, 1)) if batch.mode == "mixed" else ((4, 1), (8, 1), (4, 2), (8, 2))
     U = torch.empty_like(F)
     best, best_t = None, float("inf")
     for nw, bpc in candidates:

@@ -37,7 +37,8 @@ def _residual(N, alpha, U, F):
 
 # two FP64 implementations of the same solve differ by ~cond * eps (cond ~ N^4 / alpha);
 # the FP64 residual is bounded by 10x the oracle's own residual
-# nwaves selects the kernel: fp64 4/8 waves; mixed 4 = compact (two workgroups per CU), 8 = wide
+# nwaves selects the kernel: fp64 4/8 waves; mixed 4 = compact (two workgroups per CU, one column parity at
+# a time), 8 = pair (one workgroup per CU, both column parities at once, whole-row I/O)
 @pytest.mark.parametrize("mode,nwaves,tol,rtol", [("fp64", 4, 1e-8, None), ("fp64", 8, 1e-8, None),
                                                   ("mixed", 4, 1e-6, 1e-6), ("mixed", 8, 1e-6, 1e-6)])
 def test_ragged_vs_oracle(dev, mode, nwaves, tol, rtol):
@@ -105,3 +106,37 @@ def test_rejects_bad_sizes(dev):
         HelmholtzBatch([3], [1.0])
     with pytest.raises(ValueError):
         HelmholtzBatch([40], [-1.0])
+
+
+@pytest.mark.parametrize("nwaves", [4, 8])
+def test_mixed_fp64_io_vs_oracle(dev, nwaves):
+    import torch
+
+    from jaxfun_amd.helmholtz import HelmholtzBatch
+
+    Ns, alphas, loads = _case([4, 5, 34, 97, 129, 256, 258], seed=14)
+    hb = HelmholtzBatch(Ns, alphas, mode="mixed", io_dtype=torch.float64, nwaves=nwaves, blocks_per_cu=1)
+    U = hb.unpack(hb.solve(hb.pack(loads)))
+    for N, a, F, Ub in zip(Ns, alphas, loads, U):
+        err = O.rel_l2(Ub, _ref(int(N), a, F))
+        assert err < 1e-6, (nwaves, N, a, err)
+
+
+def test_io_checks(dev):
+    import torch
+
+    from jaxfun_amd.helmholtz import HelmholtzBatch
+
+    Ns, alphas, loads = _case([34, 40], seed=15)
+    hb = HelmholtzBatch(Ns, alphas, mode="mixed")
+    F = hb.pack(loads)
+    with pytest.raises(ValueError):
+        hb.solve(F, out=torch.empty(hb.total - 1, dtype=torch.float32, device=F.device))
+    with pytest.raises(TypeError):
+        hb.solve(F, out=torch.empty(hb.total, dtype=torch.float64, device=F.device))
+    with pytest.raises(ValueError):
+        hb.solve(F[:-1].contiguous())
+    with pytest.raises(ValueError):
+        hb.capture(F, torch.empty(hb.total + 1, dtype=torch.float32, device=F.device))
+    with pytest.raises(ValueError):
+        hb.solve(F, out=F)
