@@ -248,6 +248,18 @@ int jfx_helm_ragged(int mode, int io_f32, int nwaves, const void* items, int nit
                     const unsigned short* VTs, const double* lam, const void* F, void* U,
                     double* scratch, int nblocks, hipStream_t stream);
 
+/* Allocation-free form of jfx_helm_ragged in mode 1 (3-term bf16 split,
+ * nwaves 4 or 8).  No counterpart in the reference: it serves callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops).  The
+ * queue counters live in a static device array of
+ * jfx_helm_noalloc_max_queues() (= 16) ints per GPU, cleared by the launch
+ * itself; nqueues above that limit is rejected with -2.  No hipMalloc/hipFree
+ * is made.  Overlapping calls on one GPU share the counters and race. */
+int jfx_helm_noalloc_max_queues(void);
+int jfx_helm_ragged_noalloc(int io_f32, int nwaves, const void* items, int nitems, const int* qbounds, int nqueues,
+                            const unsigned short* Vs, const unsigned short* VTs, const double* lam, const void* F,
+                            void* U, int nblocks, hipStream_t stream);
+
 /* ---- learned preconditioner: no reference counterpart; the
  * nearest analogue is the Linear+activation MLP of pinns/module.py:335-380.
  * C (M x N) = alpha op(A) op(B) + beta C (+ bias[N]) (tanh if act = 1);

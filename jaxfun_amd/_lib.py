@@ -58,6 +58,9 @@ SIGNATURES: dict[str, tuple] = {
     "jfx_helm_scratch_doubles_per_block": (c_int, []),
     "jfx_helm_ragged": (c_int, [c_int, c_int, c_int, c_vp, c_int, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp,
                                 c_vp, c_vp, c_vp, c_vp, c_int, c_vp]),
+    "jfx_helm_noalloc_max_queues": (c_int, []),
+    "jfx_helm_ragged_noalloc": (c_int, [c_int, c_int, c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                        c_int, c_vp]),
     "jfx_gemm_bf16": (c_int, [c_int, c_int, c_int, c_int, c_int, ctypes.c_float, c_vp, c_int, c_vp, c_int,
                               ctypes.c_float, c_vp, c_int, c_vp, c_int, c_vp]),
     "jfx_tanh_backward": (c_int, [c_vp, c_vp, c_vp, c_ll, c_vp]),

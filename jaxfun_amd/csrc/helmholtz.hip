@@ -19,6 +19,7 @@
 //   MODE 0: FP64 MFMA, FP64 storage
 //   MODE 1: BF16 3-term float split (6 products) MFMA, FP32 accumulation and
 //           storage, FP64 tridiagonal solve.
+#include <mutex>
 #include <type_traits>
 
 #include "jfx_common.h"
@@ -1409,7 +1410,71 @@ static int launch_helm(const HelmItem* items, int nitems, HelmQueues Q, const do
 
 using namespace jfx;
 
+// ---------------------------------------------------------------------------
+// Allocation-free variant of the mixed-precision ragged batch.  Its only
+// scratch memory is the queue counters: HMAXQ ints in this file-scope device
+// array (one copy per GPU, held until the process ends).  Host code never
+// passes the array's name to a kernel: the device address of each GPU's copy
+// is looked up once with hipGetSymbolAddress and kept per device.
+static __device__ __align__(256) int scratch[HMAXQ];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current GPU's copy of `scratch` (looked up on the
+// first call on that device), or nullptr with the error set.
+int* noalloc_counters() {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return set_error("helm_ragged_noalloc: hipGetDevice: %s", hipGetErrorString(e)), nullptr;
+  if (dev < 0 || dev >= kMaxDevices) return set_error("helm_ragged_noalloc: device %d out of range", dev), nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return set_error("helm_ragged_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e)), nullptr;
+    g_scratch_ptr[dev] = static_cast<int*>(ptr);
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
+
 extern "C" {
+
+int jfx_helm_noalloc_max_queues(void) { return HMAXQ; }
+
+// Same work as jfx_helm_ragged(mode 1, nwaves): the queue counters live in
+// the static `scratch` array (cleared by the launch's own reset kernel), so
+// nothing is allocated.  Calls that overlap on one GPU share the counters and
+// race; calls on one stream are ordered.
+int jfx_helm_ragged_noalloc(int io_f32, int nwaves, const void* items, int nitems, const int* qbounds, int nqueues,
+                            const unsigned short* Vs, const unsigned short* VTs, const double* lam, const void* F,
+                            void* U, int nblocks, hipStream_t stream) {
+  if (!items || !qbounds || !lam || !F || !U || !Vs || !VTs || nitems < 0 || nblocks <= 0)
+    return set_error("helm_ragged_noalloc: bad arguments"), JFX_EARG;
+  if (nwaves != 4 && nwaves != 8) return set_error("helm_ragged_noalloc: nwaves must be 4 or 8"), JFX_EARG;
+  if (nqueues < 1 || nqueues > HMAXQ)
+    return set_error("helm_ragged_noalloc: nqueues %d exceeds the %d counters of the static scratch", nqueues, HMAXQ),
+           JFX_ESHAPE;
+  if (nitems == 0) return JFX_OK;
+  int* counters = noalloc_counters();
+  if (!counters) return JFX_EHIP;
+  const HelmItem* it = (const HelmItem*)items;
+  const HelmQueues queues{counters, qbounds, nqueues};
+  if (nwaves == 8) {
+    if (io_f32)
+      return launch_helm_pair<float>(it, nitems, queues, Vs, VTs, lam, (const float*)F, (float*)U, nblocks, stream);
+    return launch_helm_pair<double>(it, nitems, queues, Vs, VTs, lam, (const double*)F, (double*)U, nblocks,
+                                    stream);
+  }
+  if (io_f32)
+    return launch_helm_compact<float>(it, nitems, queues, Vs, VTs, lam, (const float*)F, (float*)U, nblocks, stream);
+  return launch_helm_compact<double>(it, nitems, queues, Vs, VTs, lam, (const double*)F, (double*)U, nblocks,
+                                     stream);
+}
 
 int jfx_helm_item_size(void) { return (int)sizeof(HelmItem); }
 
