@@ -72,7 +72,7 @@ def bench_helmholtz(steps: int, warmup: int):
     return res, err, (Ns, alphas, loads)
 
 
-PRECISIONS_3D = ("f64", "f32", "bf16x6", "ozaki4")
+PRECISIONS_3D = ("f64", "f32", "bf16x6", "ozaki4", "i8x5", "i8x6")
 
 
 def bench_transform_3d(reps: int = 3):

@@ -103,6 +103,24 @@ int jfx_contract_ozaki(const unsigned short* Aslices, const double* Arowscale, i
                        int K, const double* X, double* Y, long long P, long long Q,
                        hipStream_t stream);
 
+/* Split an FP64 (M, K) matrix into nslices (5, 6 or 7) balanced int8 digit
+ * slices [s][M][Kp] (Kp = 64 ceil(K / 64), zero padded), digits in [-64, 64]:
+ * A[m, k] = 2^rowexp[m] sum_s slices[s][m][k] 2^(-6-7s) up to 2^(-6-7(nslices-1))
+ * of the row maximum (|A[m, :]| < 2^rowexp[m]).  For jfx_contract_i8.
+ * Same reference interface as the contractions above. */
+int jfx_split_matrix_i8(const double* A, int M, int K, int nslices, signed char* slices, int* rowexp,
+                        hipStream_t stream);
+
+/* FP64 contraction on int8 MFMA (M, K <= 128; Q == 1 with K even, or
+ * Q % 64 == 0; X 16-byte aligned): X is cut per column into the same
+ * balanced digits, the nslices (nslices + 1) / 2 digit products of levels
+ * s + t < nslices accumulate exactly in int32 and are combined in FP64.
+ * Error ~ 2^(-6-7(nslices-1)) of the row and column maxima.  Same reference
+ * interface as the contractions above (galerkin/tensorproductspace.py:330-417
+ * for the 3D transforms). */
+int jfx_contract_i8(const signed char* Aslices, const int* Arowexp, int nslices, int M, int K, const double* X,
+                    double* Y, long long P, long long Q, hipStream_t stream);
+
 /* Fused two-axis contraction of 2D slabs: Y[b] = A0 X[b] A1^T (X: B x K0 x K1,
  * Y: B x M0 x M1, all dims <= 128), optionally followed by Y /= E (epi 1) or
  * Y *= E (epi 2) with E (M0 x M1).  prec 0: FP64 operators A0d, A1d; prec 1:

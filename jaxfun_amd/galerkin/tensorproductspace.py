@@ -4,7 +4,9 @@ Every transform is a sequence of per-axis contractions with the 1D spaces'
 device operators, run by the MFMA kernels of ``libjfx``.  Leading batch
 dimensions are allowed (the reference vmaps externally).  ``precision``
 selects the contraction arithmetic: ``f64`` (default), ``f32``, ``bf16x3``,
-``bf16x6`` (fp32 I/O) or ``ozaki3/4/5`` (fp64 I/O, fixed-point bf16 slices).
+``bf16x6`` (fp32 I/O), ``ozaki3/4/5`` (fp64 I/O, fixed-point bf16 slices) or
+``i8x5/6/7`` (fp64 I/O, balanced int8 digit slices on int8 MFMA with exact
+integer sums; i8x6 keeps a 16 x 128^3 Legendre round trip near 1e-8).
 """
 
 from __future__ import annotations

@@ -13,7 +13,7 @@ import torch
 
 from . import _lib
 
-PRECISIONS = ("f64", "f32", "bf16x3", "bf16x6", "ozaki3", "ozaki4", "ozaki5")
+PRECISIONS = ("f64", "f32", "bf16x3", "bf16x6", "ozaki3", "ozaki4", "ozaki5", "i8x5", "i8x6", "i8x7")
 FAMILY = {"legendre": 0, "chebyshev": 1}
 
 
@@ -309,7 +309,8 @@ class AxisOperator:
 
     ``f64`` uses the FP64 matrix; ``f32`` an FP32 copy; ``bf16x3``/``bf16x6``
     2/3 bf16 float slices; ``ozakiS`` S fixed-point digit slices with
-    power-of-two row scales.  Preparations are cached on the object, like the
+    power-of-two row scales; ``i8xS`` S balanced int8 digit slices with
+    power-of-two row scales (FP64 data, int8 MFMA, exact integer sums).  Preparations are cached on the object, like the
     reference caches factorizations on its matrix objects.
     """
 
@@ -350,6 +351,13 @@ class AxisOperator:
             rs = torch.empty(self.M, dtype=torch.float64, device=A.device)
             _lib.call("jfx_split_matrix", _p(Ap), self.M, Kp, 1, S, _p(sl), _p(rs), _s())
             r = (sl, rs, S)
+        elif prec.startswith("i8x") and prec in PRECISIONS:
+            S = int(prec[3:])
+            Kp = (self.K + 63) // 64 * 64
+            sl = torch.empty((S, self.M, Kp), dtype=torch.int8, device=A.device)
+            ex = torch.empty(self.M, dtype=torch.int32, device=A.device)
+            _lib.call("jfx_split_matrix_i8", _p(A), self.M, self.K, S, _p(sl), _p(ex), _s())
+            r = (sl, ex, S)
         else:
             raise ValueError(f"unknown precision {prec!r}; expected one of {PRECISIONS}")
         self._prep[prec] = r
@@ -446,6 +454,11 @@ def apply_along(op: AxisOperator, X: torch.Tensor, axis: int, prec: str = "f64",
         _lib.call("jfx_contract_f32", _p(prep[0]), op.M, K, _p(X), _p(Y), P, Q, _s())
     elif prec in ("bf16x3", "bf16x6"):
         _lib.call("jfx_contract_bf16x", _p(prep[0]), prep[1], op.M, K, _p(X), _p(Y), P, Q, _s())
+    elif prec.startswith("i8x"):
+        if op.M > 128 or K > 128 or not (Q == 1 or Q % 64 == 0) or (Q == 1 and K % 2):
+            raise ValueError(f"precision {prec}: needs M, K <= 128 and a trailing size of 1 (K even) or a "
+                             f"multiple of 64; got M={op.M}, K={K}, trailing size {Q}")
+        _lib.call("jfx_contract_i8", _p(prep[0]), _p(prep[1]), prep[2], op.M, K, _p(X), _p(Y), P, Q, _s())
     else:
         _lib.call("jfx_contract_ozaki", _p(prep[0]), _p(prep[1]), prep[2], op.M, K, _p(X), _p(Y), P, Q,
                   _s())
