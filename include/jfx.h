@@ -170,6 +170,30 @@ int jfx_band_lu_factor_batched(const double* band, int nsys, int n, int p, int q
 int jfx_band_lu_solve_batched(const double* lu, int nsys, int n, int p, int q, double* X, long long P, long long Q,
                               int qdiv, hipStream_t stream);
 
+/* One large band matrix (the Kronecker matrix of banded factors), same band
+ * storage: LU without pivoting by one 1024-thread workgroup (each
+ * elimination step updates its p x q window in parallel; p, q <= 1024);
+ * status_dev receives 0 or 1 + the row of a zero / non-finite pivot.
+ * Replaces DiaMatrix.lu_factor (la/diamatrix.py:394-504) as reached from
+ * TPMatrices.solve(kron_method="banded" | "rcm") (la/tpmatrix.py:429-509,
+ * la/diamatrix.py:592-770). */
+int jfx_band_lu_factor_wide(const double* band, int n, int p, int q, double* lu, int* status_dev,
+                            hipStream_t stream);
+/* In-place solves of nrhs right-hand sides X (nrhs, n), rows contiguous, one
+ * wave each (n <= 20480, the vector lives in LDS).  Replaces LUFactors.solve
+ * (la/diamatrix.py:1880-2195) on the assembled Kronecker matrix. */
+int jfx_band_lu_solve_wide(const double* lu, int n, int p, int q, double* X, long long nrhs, hipStream_t stream);
+
+/* out (r m x r m) = sum_t s[t] kron(X_t, Y_t), X (nt, r, r), Y (nt, m, m),
+ * s (nt) on the device: the Kronecker assembly of tpmats_to_kron
+ * (la/tpmatrix.py:1357-1402). */
+int jfx_kron_sum(const double* Xs, const double* Ys, const double* s, int nt, int r, int m, double* out,
+                 hipStream_t stream);
+
+/* Y[b, i] = X[b, perm[i]] for X, Y (nrhs, n), perm (n) int64: the RCM
+ * reordering of right-hand sides (la/diamatrix.py:687-700). */
+int jfx_gather_rows(const double* X, const long long* perm, int n, long long nrhs, double* Y, hipStream_t stream);
+
 /* Complex (re/im interleaved) versions of the two calls above for
  * per-wavenumber systems with complex weights (odd derivative orders on a
  * Fourier axis): band (nsys, n, p+q+1) complex; X (P, n, Q) complex, line

@@ -40,6 +40,10 @@ SIGNATURES: dict[str, tuple] = {
     "jfx_contract_ozaki": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_vp, c_vp, c_ll, c_ll, c_vp]),
     "jfx_split_matrix_i8": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_vp, c_vp]),
     "jfx_contract_i8": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_vp, c_vp, c_ll, c_ll, c_vp]),
+    "jfx_band_lu_factor_wide": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_vp, c_vp]),
+    "jfx_band_lu_solve_wide": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_ll, c_vp]),
+    "jfx_kron_sum": (c_int, [c_vp, c_vp, c_vp, c_int, c_int, c_int, c_vp, c_vp]),
+    "jfx_gather_rows": (c_int, [c_vp, c_vp, c_int, c_ll, c_vp, c_vp]),
     "jfx_band_lu_factor": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_vp, c_vp]),
     "jfx_band_lu_solve": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_ll, c_ll, c_vp]),
     "jfx_div_bcast_f64": (c_int, [c_vp, c_vp, c_vp, c_ll, c_ll, c_vp]),

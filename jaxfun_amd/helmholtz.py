@@ -279,6 +279,84 @@ class HelmholtzBatch:
         return float(2 * self.total * torch.tensor([], dtype=self.io_dtype).element_size())
 
 
+# ---------------------------------------------------------------------- operator route
+def legendre_dirichlet_pair(n: int) -> tuple[np.ndarray, np.ndarray]:
+    """Closed-form mass M = S diag(h) S^T and stiffness K = (phi_k, phi_l'') =
+    -diag(4k + 6) of phi_k = P_k - P_{k+2}, k < n (galerkin/Legendre.py:439-443,
+    galerkin/composite.py:330-337)."""
+    N = n + 2
+    h = 2.0 / (2 * np.arange(N) + 1)
+    M = np.diag(h[:n] + h[2:n + 2]) - np.diag(h[2:n], 2) - np.diag(h[2:n], -2)
+    Kd = np.diag(-(4.0 * np.arange(n) + 6.0))
+    return M, Kd
+
+
+def helmholtz_form(A, rtol: float = 1e-11) -> tuple[float, float, int]:
+    """Recognise a 2D ``TPMatrices`` / ``TPMatrix`` as
+    ``kappa * (alpha M (x) M - K (x) M - M (x) K)`` in the Legendre-Dirichlet
+    pair (M, K) of :func:`legendre_dirichlet_pair`: every 1D factor must be a
+    combination ``a M + b K`` (checked to ``rtol``), no ``K (x) K`` term may
+    remain, the two mixed terms must be equal and ``alpha >= 0``.  Returns
+    ``(alpha, kappa, n)``; raises ``SolverNotApplicable`` otherwise (the
+    operator of ``inner(v*(alpha*u - Div(Grad(u))), sparse=True)`` on a
+    Legendre-Dirichlet tensor space, la/tpmatrix.py:386-509)."""
+    from .la.matrix import SolverNotApplicable
+
+    tps = list(getattr(A, "tpmats", [A]))
+    if not tps or any(len(tp.mats) != 2 for tp in tps):
+        raise SolverNotApplicable("the ragged Helmholtz batch takes 2D tensor-product operators")
+    n = tps[0].mats[0].shape[0]
+    if any(m.shape != (n, n) for tp in tps for m in tp.mats):
+        raise SolverNotApplicable("every factor must be square and of one size")
+    M, Kd = legendre_dirichlet_pair(n)
+    G = np.stack([M.ravel(), Kd.ravel()], axis=1)
+    coef = np.zeros((2, 2))
+    for tp in tps:
+        ab = []
+        for m in tp.mats:
+            D = np.asarray(m.todense_host())
+            if np.iscomplexobj(D):
+                raise SolverNotApplicable("complex factor")
+            c, *_ = np.linalg.lstsq(G, D.ravel(), rcond=None)
+            if np.linalg.norm(G @ c - D.ravel()) > rtol * max(np.linalg.norm(D), 1e-300):
+                raise SolverNotApplicable("a factor is not a combination of the Legendre-Dirichlet mass and stiffness")
+            ab.append(c)
+        coef += complex(tp.coefficient).real * np.outer(ab[0], ab[1])
+    mu, k01, k10, kk = coef[0, 0], coef[0, 1], coef[1, 0], coef[1, 1]
+    big = np.abs(coef).max()
+    if big == 0 or abs(kk) > rtol * big or abs(k01 - k10) > rtol * big or k01 == 0:
+        raise SolverNotApplicable("not of the form alpha M(x)M - K(x)M - M(x)K")
+    kappa = -k01
+    alpha = 0.0 if abs(mu) <= rtol * big else mu / kappa  # a zero shift up to rounding is zero
+    if alpha < 0 or not np.isfinite(alpha):
+        raise SolverNotApplicable(f"shift alpha = {alpha} is negative")
+    return float(alpha), float(kappa), int(n)
+
+
+def solve_ragged(systems, rhs, mode: str = "mixed", **kw) -> list[torch.Tensor]:
+    """Solve ``A_b U_b = rhs_b`` for a list of 2D Legendre-Dirichlet Helmholtz
+    operators (e.g. built by ``inner(v*(alpha*u - Div(Grad(u))), sparse=True)``,
+    any sizes and shifts) in one ragged launch; returns FP64 (n_b, n_b) device
+    arrays, as ``TPMatrices.solve`` would one at a time.  Keyword arguments go
+    to :class:`HelmholtzBatch`."""
+    if len(systems) != len(rhs):
+        raise ValueError("one right-hand side per system")
+    forms = [helmholtz_form(A) for A in systems]
+    alphas = [f[0] for f in forms]
+    Ns = [f[2] + 2 for f in forms]
+    hb = HelmholtzBatch(Ns, alphas, mode=mode, **kw)
+    F64 = torch.empty(hb.total, dtype=torch.float64, device=hb.dev)
+    for i, ((_, kappa, n), r) in enumerate(zip(forms, rhs)):
+        r = K.cast(torch.as_tensor(r, device=hb.dev).to(torch.float64).contiguous(), torch.float64)
+        if tuple(r.shape) != (n, n):
+            raise ValueError(f"right-hand side {i} has shape {tuple(r.shape)}, the system needs {(n, n)}")
+        seg = F64[int(hb.offsets[i]):int(hb.offsets[i + 1])].view(n, n)
+        K.lincomb([(1.0 / kappa, r)], out=seg)  # A = kappa (...): the batch solves (...) U = rhs / kappa
+    U = hb.solve(K.cast(F64, hb.io_dtype))
+    U = K.cast(U, torch.float64)
+    return [U[int(hb.offsets[i]):int(hb.offsets[i + 1])].view(f[2], f[2]) for i, f in enumerate(forms)]
+
+
 def autotune(batch: HelmholtzBatch, F: torch.Tensor, candidates=None, reps: int = 3) -> tuple[int, int]:
     """Time each (waves per workgroup, workgroups per CU) launch shape on the
     batch itself, keep the fastest on ``batch`` and return it.  In mixed mode

@@ -14,8 +14,11 @@ broadcast division.
 
 from __future__ import annotations
 
+import warnings
+
 import numpy as np
 import scipy.linalg as sla
+import scipy.sparse as ssp
 import torch
 
 from .. import kernels as K
@@ -216,9 +219,10 @@ class TPMatricesLUFactors:
 class TPMatricesDenseLUFactors:
     """Dense Kronecker LU (la/tpmatrix.py:1016-1049), factored by rocSOLVER."""
 
-    def __init__(self, A: np.ndarray, shape):
+    def __init__(self, A, shape):
         self.shape = tuple(shape)
-        self.LU, self.piv = torch.linalg.lu_factor(torch.tensor(A, device=K.device()))
+        A = A if isinstance(A, torch.Tensor) else torch.tensor(A, device=K.device())
+        self.LU, self.piv = torch.linalg.lu_factor(A)
 
     def solve(self, rhs, precision: str = "f64") -> torch.Tensor:
         b = _as_dev(rhs)
@@ -229,6 +233,92 @@ class TPMatricesDenseLUFactors:
             LU = self.LU.to(torch.complex128)
         x = torch.linalg.lu_solve(LU, self.piv, flat.reshape(-1, flat.shape[-1]).T.contiguous())
         return x.T.reshape(b.shape).contiguous()
+
+
+class TPMatricesBandedLUFactors:
+    """LU without pivoting of the assembled Kronecker matrix in band storage
+    (the ``"banded"`` and ``"rcm"`` methods of ``DiaMatrix.lu_solve``,
+    la/diamatrix.py:592-770): factored by one workgroup per elimination
+    step window (``jfx_band_lu_factor_wide``), solved one wave per
+    right-hand side.  With ``perm`` the matrix is the reordered
+    ``A[perm][:, perm]`` and right-hand sides are permuted on the way."""
+
+    def __init__(self, band: np.ndarray, p: int, q: int, shape, perm: np.ndarray | None = None):
+        self.shape = tuple(shape)
+        self.n = int(np.prod(self.shape))
+        self.p, self.q = int(p), int(q)
+        dev = K.device()
+        b = torch.tensor(np.ascontiguousarray(band), dtype=torch.float64, device=dev)
+        self.lu = torch.empty_like(b)
+        st = torch.zeros(1, dtype=torch.int32, device=dev)
+        from .. import _lib
+
+        _lib.call("jfx_band_lu_factor_wide", K._p(b), self.n, self.p, self.q, K._p(self.lu), K._p(st), K._s())
+        k = int(st.item())
+        if k != 0:
+            raise ValueError(f"Zero or non-finite pivot at row {k - 1} in banded LU factorization")
+        self.perm = self.inv = None
+        if perm is not None:
+            self.perm = torch.tensor(np.asarray(perm, dtype=np.int64), device=dev)
+            inv = np.empty_like(np.asarray(perm, dtype=np.int64))
+            inv[np.asarray(perm)] = np.arange(len(perm))
+            self.inv = torch.tensor(inv, device=dev)
+
+    def _solve_real(self, X: torch.Tensor) -> torch.Tensor:
+        from .. import _lib
+
+        nrhs = X.shape[0]
+        if self.perm is not None:
+            Y = torch.empty_like(X)
+            _lib.call("jfx_gather_rows", K._p(X), K._p(self.perm), self.n, nrhs, K._p(Y), K._s())
+            X = Y
+        _lib.call("jfx_band_lu_solve_wide", K._p(self.lu), self.n, self.p, self.q, K._p(X), nrhs, K._s())
+        if self.inv is not None:
+            Y = torch.empty_like(X)
+            _lib.call("jfx_gather_rows", K._p(X), K._p(self.inv), self.n, nrhs, K._p(Y), K._s())
+            X = Y
+        return X
+
+    def solve(self, rhs, precision: str = "f64") -> torch.Tensor:
+        b = _as_dev(rhs)
+        if b.is_complex():
+            bc = b.contiguous()
+            re = self.solve(torch.view_as_real(bc)[..., 0].contiguous())
+            im = self.solve(torch.view_as_real(bc)[..., 1].contiguous())
+            return torch.complex(re, im)
+        X = b.reshape(-1, self.n).clone()  # solved in place
+        return self._solve_real(X).reshape(b.shape)
+
+
+def _sparse_kron(tpmats) -> ssp.csr_matrix:
+    """Sum of Kronecker products of the (banded) factors as a sparse matrix
+    (the DiaMatrix that tpmats_to_kron forms for sparse factors,
+    la/tpmatrix.py:1357-1402)."""
+    out = None
+    for tp in tpmats:
+        Kd = ssp.csr_matrix(tp.mats[0].todense_host())
+        for m in tp.mats[1:]:
+            Kd = ssp.kron(Kd, ssp.csr_matrix(m.todense_host()), format="csr")
+        Kd = complex(tp.coefficient).real * Kd if not np.iscomplexobj(Kd.data) else tp.coefficient * Kd
+        out = Kd if out is None else out + Kd
+    out = ssp.csr_matrix(out)
+    out.eliminate_zeros()
+    return out
+
+
+def _bandwidths(A: ssp.spmatrix) -> tuple[int, int]:
+    C = A.tocoo()
+    if C.nnz == 0:
+        return 0, 0
+    d = C.col.astype(np.int64) - C.row.astype(np.int64)
+    return int(max(0, -d.min())), int(max(0, d.max()))
+
+
+def _band_storage(A: ssp.spmatrix, p: int, q: int) -> np.ndarray:
+    C = A.tocoo()
+    band = np.zeros((A.shape[0], p + q + 1))
+    band[C.row, C.col - C.row + p] = C.data
+    return band
 
 
 class TPMatricesWavenumberSolver:
@@ -332,7 +422,37 @@ def tpmats_dense_lu_factor(A) -> TPMatricesDenseLUFactors:
                 raise TypeError(f"tpmats_dense_lu_factor requires all factor matrices to be Matrix (dense); "
                                 f"got {type(m).__name__}.")
     shape = tuple(int(m.shape[0]) for m in tpmats[0].mats)
-    return TPMatricesDenseLUFactors(tpmats_to_kron(tpmats), shape)
+    return TPMatricesDenseLUFactors(kron_device(tpmats), shape)
+
+
+def kron_device(tpmats):
+    """The dense Kronecker sum assembled on the device (``jfx_kron_sum``) for
+    real factors; complex factors fall back to the host assembly."""
+    from .. import _lib
+
+    if any(m.is_complex for tp in tpmats for m in tp.mats) or any(np.iscomplexobj(tp.coefficient) for tp in tpmats):
+        return tpmats_to_kron(tpmats)
+    dev = K.device()
+    d = len(tpmats[0].mats)
+    parts = [[torch.as_tensor(m.todense_host(), dtype=torch.float64, device=dev) for m in tp.mats] for tp in tpmats]
+    s = torch.tensor([float(tp.coefficient) for tp in tpmats], dtype=torch.float64, device=dev)
+    ones = torch.ones(1, dtype=torch.float64, device=dev)
+    # leading factors combined term by term, the last one summed over terms
+    lead = []
+    for mats in parts:
+        Z = mats[0].contiguous()
+        for m in mats[1:-1]:
+            r, mm = Z.shape[0], m.shape[0]
+            out = torch.empty((r * mm, r * mm), dtype=torch.float64, device=dev)
+            _lib.call("jfx_kron_sum", K._p(Z), K._p(m.contiguous()), K._p(ones), 1, r, mm, K._p(out), K._s())
+            Z = out
+        lead.append(Z)
+    X = torch.stack(lead).contiguous()
+    Y = torch.stack([mats[-1] for mats in parts]).contiguous()
+    r, mm = X.shape[1], Y.shape[1]
+    out = torch.empty((r * mm, r * mm), dtype=torch.float64, device=dev)
+    _lib.call("jfx_kron_sum", K._p(X), K._p(Y), K._p(s), len(tpmats), r, mm, K._p(out), K._s())
+    return out
 
 
 def _is_spd(A: np.ndarray) -> bool:
@@ -513,22 +633,80 @@ class TPMatrices:
             raise ValueError(f"'{method}' is not a valid TPSolveMethod")
         if method == "fd":
             return self.fd_factor().solve(rhs, precision)
+        if kron_method not in ("auto", "banded", "rcm", "dense"):
+            raise ValueError(f"method must be one of ['auto', 'banded', 'dense', 'rcm'], got {kron_method!r}")
         if method == "kron":
-            return self._kron_solve(rhs)
+            return self._kron_solve(rhs, kron_method, auto_threshold)
         if method == "lu":
             return self.lu_factor().solve(rhs, precision)
         try:
             return self.lu_factor().solve(rhs, precision)
         except SolverNotApplicable:
-            return self._kron_solve(rhs)
+            return self._kron_solve(rhs, kron_method, auto_threshold)
 
-    def _kron_solve(self, rhs):
+    def kron_factor(self, kron_method: str = "auto", auto_threshold: int = 100):
+        """Factor the assembled Kronecker matrix by ``kron_method`` (cached per
+        method).  All-sparse factors follow ``DiaMatrix.lu_solve``'s choice
+        (la/diamatrix.py:592-770): ``banded`` (warns when p (q + 1) exceeds
+        ``auto_threshold``), ``rcm`` (reverse Cuthill-McKee reordering, then
+        banded), ``dense``, or ``auto`` (banded when p (q + 1) <=
+        auto_threshold, else rcm when that brings it under the threshold,
+        else dense, with the reference's warnings).  Any dense factor gives
+        the dense Kronecker LU whatever ``kron_method`` says."""
         if self._kron_cache is None:
-            shape = tuple(int(m.shape[0]) for m in self.tpmats[0].mats)
-            self._kron_cache = TPMatricesDenseLUFactors(self.todense_host(), shape)
-        return self._kron_cache.solve(rhs)
+            self._kron_cache = {}
+        shape = tuple(int(m.shape[0]) for m in self.tpmats[0].mats)
+        sparse = all(isinstance(m, DiaMatrix) for tp in self.tpmats for m in tp.mats)
+        method = kron_method if sparse else "dense"
+        if method in self._kron_cache:
+            return self._kron_cache[method]
+        if method == "dense":
+            f = TPMatricesDenseLUFactors(kron_device(self.tpmats), shape)
+            self._kron_cache["dense"] = f
+            return f
+        A = _sparse_kron(self.tpmats)
+        p, q = _bandwidths(A)
+        if method == "banded":
+            if p * (q + 1) > auto_threshold:
+                warnings.warn(f"DiaMatrix.lu_solve(method='banded'): bandwidth metric p*(q+1)={p * (q + 1)} "
+                              f"(p={p}, q={q}) exceeds {auto_threshold}.", stacklevel=3)
+            f = TPMatricesBandedLUFactors(_band_storage(A, p, q), p, q, shape)
+        elif method in ("rcm", "auto"):
+            if method == "auto" and p * (q + 1) <= auto_threshold:
+                f = TPMatricesBandedLUFactors(_band_storage(A, p, q), p, q, shape)
+            else:
+                from scipy.sparse.csgraph import reverse_cuthill_mckee
+
+                perm = np.asarray(reverse_cuthill_mckee(A, symmetric_mode=False), dtype=np.int64)
+                Ap = A[perm][:, perm]
+                pp, qp = _bandwidths(Ap)
+                if method == "rcm":
+                    f = TPMatricesBandedLUFactors(_band_storage(Ap, pp, qp), pp, qp, shape, perm)
+                elif pp * (qp + 1) > p * (q + 1):
+                    warnings.warn(f"DiaMatrix.lu_solve(method='auto'): RCM reordering increased bandwidth from "
+                                  f"p*(q+1)={p * (q + 1)} to p'*(q'+1)={pp * (qp + 1)}. ", stacklevel=3)
+                    f = None
+                elif pp * (qp + 1) <= auto_threshold:
+                    f = TPMatricesBandedLUFactors(_band_storage(Ap, pp, qp), pp, qp, shape, perm)
+                else:
+                    warnings.warn(f"DiaMatrix.lu_solve(method='auto'): bandwidth p*(q+1)={pp * (qp + 1)} exceeds "
+                                  f"threshold {auto_threshold} even after RCM reordering. ", stacklevel=3)
+                    f = None
+                if f is None:
+                    warnings.warn("Falling back to dense solver. Consider using a larger auto_threshold or "
+                                  "method='dense' to suppress this warning.", stacklevel=3)
+                    f = self._kron_cache.get("dense") or TPMatricesDenseLUFactors(kron_device(self.tpmats), shape)
+                    self._kron_cache["dense"] = f
+        else:
+            raise ValueError(f"method must be one of ['auto', 'banded', 'dense', 'rcm'], got {kron_method!r}")
+        self._kron_cache[method] = f
+        return f
+
+    def _kron_solve(self, rhs, kron_method: str = "auto", auto_threshold: int = 100):
+        return self.kron_factor(kron_method, auto_threshold).solve(rhs)
 
 
-__all__ = ["TPMatrix", "TPMatrices", "TPMatricesLUFactors", "TPMatricesDenseLUFactors",
+__all__ = ["TPMatrix", "TPMatrices", "TPMatricesLUFactors", "TPMatricesDenseLUFactors", "TPMatricesBandedLUFactors",
+           "kron_device",
            "TPMatricesWavenumberSolver", "tpmats_wavenumber_factor", "DiaMatrix",
            "BaseMatrix", "tpmats_lu_factor", "tpmats_dense_lu_factor", "tpmats_to_kron", "axis_eig"]

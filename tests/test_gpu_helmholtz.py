@@ -140,3 +140,27 @@ def test_io_checks(dev):
         hb.capture(F, torch.empty(hb.total + 1, dtype=torch.float32, device=F.device))
     with pytest.raises(ValueError):
         hb.solve(F, out=F)
+
+
+@pytest.mark.parametrize("mode,tol", [("fp64", 1e-10), ("mixed", 1e-6)])
+def test_solve_ragged_from_inner_systems(dev, mode, tol):
+    # systems assembled through the reference's operator API, solved in one
+    # ragged launch, against each system's own TPMatrices.solve
+    from jaxfun_amd import Div, FunctionSpace, Grad, Legendre, TensorProduct, TestFunction, TrialFunction, inner
+    from jaxfun_amd.helmholtz import solve_ragged
+
+    rng = np.random.default_rng(16)
+    systems, rhs, refs = [], [], []
+    for N, alpha in [(34, 0.5), (63, 1e4), (100, 0.02), (129, 7.0), (200, 300.0), (256, 0.0)]:
+        D = FunctionSpace(N, Legendre, {"left": {"D": 0}, "right": {"D": 0}})
+        T = TensorProduct(D, D)
+        v, u = TestFunction(T), TrialFunction(T)
+        A = inner(v * (alpha * u - Div(Grad(u))), sparse=True)
+        b = rng.standard_normal((N - 2, N - 2))
+        systems.append(A)
+        rhs.append(b)
+        refs.append(A.solve(b).cpu().numpy())
+    U = solve_ragged(systems, rhs, mode=mode)
+    for Ub, Ur in zip(U, refs):
+        assert Ub.dtype.is_floating_point and Ub.shape == Ur.shape
+        assert O.rel_l2(Ub.cpu().numpy(), Ur) < tol
