@@ -80,6 +80,14 @@ struct HelmQueues {
   }
 };
 
+// FP64 reciprocal refined by one Newton step from the hardware estimate:
+// ~2^-46 relative, ample where the results are stored in FP32 (the mixed
+// kernels' multipliers and sweep values).
+__device__ __forceinline__ double rcp64_1(double x) {
+  const double r = __builtin_amdgcn_rcp(x);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+
 // FP64 reciprocal: hardware estimate refined by two Newton steps (x > 0, normal).
 __device__ __forceinline__ double rcp64(double x) {
   double r = __builtin_amdgcn_rcp(x);
@@ -748,7 +756,7 @@ __device__ __forceinline__ void helmc_pass(const HelmItem& I, int q, const HelmI
         const double d = fma(s, hs, fma((double)i, ddg, dg0));
         const double on = -s * ho;
         const double Pj = fma(d, P1, -(oprev * oprev) * P2);
-        const double rj = P1 * rcp64(Pj);
+        const double rj = P1 * rcp64_1(Pj);
         const double yj = fma(-oprev, yprev, g) * rj;
         cl = on * rj;
         cm[i] = (float)cl;
@@ -1125,7 +1133,7 @@ __device__ __forceinline__ void pair_row_solves(const HelmItem& I, int q, int b,
     const double d = fma(s, hs, fma((double)i, ddg, dg0));
     const double on = -s * ho;
     const double Pj = fma(d, P1, -(oprev * oprev) * P2);
-    const double rj = P1 * rcp64(Pj);
+    const double rj = P1 * rcp64_1(Pj);
     const double yj = fma(-oprev, yprev, g) * rj;
     cl = on * rj;
     cm[i] = (float)cl;
@@ -1343,7 +1351,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         const float* src = Wk + m * LDU;
 #pragma unroll
         for (int c = 0; c < 4; ++c)
-          if (ln + 64 * c < n) dst[(unsigned)(ln + 64 * c)] = (TIO)src[ln + 64 * c];
+          if (ln + 64 * c < n) __builtin_nontemporal_store((TIO)src[ln + 64 * c], dst + (unsigned)(ln + 64 * c));
       }
     }
     __syncthreads();
