@@ -1,12 +1,12 @@
 // FP64 per-axis contraction on int8 MFMA with exact integer accumulation:
 //     Y[p, m, q] = sum_k A[m, k] X[p, k, q],   M, K <= 128, X and Y FP64,
 // for the two column layouts of every axis of a C-order array (Q % 64 == 0:
-// a tile is 64 consecutive q of one p; Q == 1: a tile is 64 consecutive rows
+// a tile is 32 consecutive q of one p; Q == 1: a tile is 32 consecutive rows
 // p, each K contiguous values).
 //
 // Both operands are cut into S balanced base-128 digits (values in [-64, 64])
 // under a power-of-two scale: A once per row (jfx_split_matrix_i8), X per
-// column of each staged tile (in the kernel), so that
+// column of each tile (in the kernel), so that
 //     A[m, k] = 2^ea_m sum_s a_s[m, k] 2^(-6-7s),   X[k, c] = 2^ex_c sum_t x_t[k, c] 2^(-6-7t)
 // up to 2^(-6-7(S-1)) of the row / column maximum.  The S(S+1)/2 digit
 // products with s + t < S run on v_mfma_i32_16x16x64_i8; every product and
@@ -15,32 +15,23 @@
 //     Y[m, c] = 2^(ea_m + ex_c - 12) sum_l 2^(-7l) acc_l[m, c].
 //
 // One persistent workgroup of 8 waves per CU; wave w keeps the digit slices
-// of operator rows 16w..16w+15 in registers and splits columns 8w..8w+7 of
-// every tile (lane l: column 8w + l/8, rows 16 (l % 8) .. + 15; the column
-// maxima are combined with lane shuffles), straight from the registers the
-// tile was loaded into.  Digit tiles are double buffered, so one barrier per
-// tile separates the split of tile t+1 from the products of tile t, and the
-// two waves sharing a SIMD run the two in opposite order so that one's VALU
-// work overlaps the other's MFMAs.
+// of operator rows 16w..16w+15 in registers.  Tiles stream from global memory
+// straight into two LDS staging buffers two tiles ahead; every wave splits
+// four columns of a staged tile (lane l: column 4w + l/16, rows 8 (l % 16) ..
+// + 7; column maxima by lane shuffles) into one of two digit buffers, between
+// the MFMA blocks of the previous tile.
 #include "jfx_common.h"
 
 namespace jfx {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-constexpr int I8T = 64, I8NT = 512, I8KMAX = 128;
+constexpr int I8T = 64, I8NT = 512, I8KMAX = 128;  // I8T: the trailing-size granule (Q % 64 == 0)
 constexpr int I8DLD = I8KMAX + 16;  // digit tile column stride (bytes): Dg[buf][s][c][k]
 
 struct I8Map {
-  long long P, Q, ncols, ntiles, nqt;  // nqt: 64-column tiles per p (Q % 64 == 0)
+  long long P, Q, ncols, ntiles, nqt;  // nqt: 32-column tiles per p (Q % 64 == 0)
   int M, K, Kp, qone;
-};
-
-template <int S>
-struct I8Lds {
-  static constexpr size_t DG = (size_t)S * I8T * I8DLD;  // one digit tile
-  static constexpr size_t CE = (size_t)I8T * 4;          // its column exponents
-  static constexpr size_t TOTAL = 2 * (DG + CE);
 };
 
 // Balanced base-128 digits of the S-digit fixed-point value of xs (|xs| < 1),
@@ -77,24 +68,136 @@ __device__ __forceinline__ int pack4(int a, int b, int c, int d) {
   return (int)__builtin_amdgcn_perm(cd, ab, 0x05040100u);
 }
 
+// ---------------------------------------------------------------------------
+// The FP64 values of a 32-column tile go from global memory straight into one
+// of two LDS staging buffers (global_load_lds, 16 bytes per lane), issued two
+// tiles ahead, so no register holds a tile in flight.
+// Per tile: wait for the tile's own loads, barrier, then the split of tile
+// t + 1 (from LDS) runs between the MFMA blocks of tile t.
+//   Q % 64 == 0 staging: row k of the tile (32 doubles) at
+//     (k / 4) * D32G + (k % 4) * 256 bytes (a 16-byte pad after every 4 rows);
+//   Q == 1 staging: row c of the tile (K doubles) at c * D32G.
+constexpr int I8T2 = 32;               // columns per streamed tile
+constexpr int D32G = 1024 + 16;        // bytes per staged group (4 rows of 32 / one row of 128) + pad
+constexpr int D32B = 32 * D32G;        // one staging buffer
+
+template <int S>
+struct I8DLds {
+  static constexpr size_t DG = (size_t)S * I8T2 * I8DLD;  // one digit tile
+};
+
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+
+// Workgroup barrier over LDS only: this wave's LDS accesses complete, then
+// s_barrier.  The staged tiles' direct loads are waited for explicitly
+// (wait_vmcnt) by every wave before it, so the barrier need not also wait for
+// the loads of the tile after next, which __syncthreads' fence would.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ unsigned lds_off(const void* p) {
+  return (unsigned)(uintptr_t)(const __attribute__((address_space(3))) char*)p;
+}
+
+// Eight staged values of one column (Q % 64 == 0 layout): rows r..r+3 of one
+// group and the next, at byte offsets 0, 256, 512, 768, D32G + (0..3) 256.
+// Issued and waited for in one block, so no load is left in flight; read
+// through asm because the compiler cannot tell these LDS reads from the
+// direct loads still streaming into the other staging buffer.
+__device__ __forceinline__ void lds_read8_col(unsigned a, double (&v)[8]) {
+  static_assert(D32G == 1040, "offsets below assume 1040-byte groups");
+  asm volatile(
+      "ds_read_b64 %0, %8 offset:0\n\t"
+      "ds_read_b64 %1, %8 offset:256\n\t"
+      "ds_read_b64 %2, %8 offset:512\n\t"
+      "ds_read_b64 %3, %8 offset:768\n\t"
+      "ds_read_b64 %4, %8 offset:1040\n\t"
+      "ds_read_b64 %5, %8 offset:1296\n\t"
+      "ds_read_b64 %6, %8 offset:1552\n\t"
+      "ds_read_b64 %7, %8 offset:1808\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3]), "=&v"(v[4]), "=&v"(v[5]), "=&v"(v[6]), "=&v"(v[7])
+      : "v"(a)
+      : "memory");
+}
+
+// Eight consecutive staged values of one row (Q == 1 layout).
+__device__ __forceinline__ void lds_read8_row(unsigned a, double (&v)[8]) {
+  v2d r0, r1, r2, r3;
+  asm volatile(
+      "ds_read_b128 %0, %4 offset:0\n\t"
+      "ds_read_b128 %1, %4 offset:16\n\t"
+      "ds_read_b128 %2, %4 offset:32\n\t"
+      "ds_read_b128 %3, %4 offset:48\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(r0), "=&v"(r1), "=&v"(r2), "=&v"(r3)
+      : "v"(a)
+      : "memory");
+  v[0] = r0.x; v[1] = r0.y; v[2] = r1.x; v[3] = r1.y;
+  v[4] = r2.x; v[5] = r2.y; v[6] = r3.x; v[7] = r3.y;
+}
+
+// The S digit fragments of one (column block, k-step) for this lane: slices
+// 32 * I8DLD bytes apart, read and waited for in one block (same reason as above).
+template <int S>
+__device__ __forceinline__ void lds_read_frags(unsigned a, v4i (&b)[S]);
+template <>
+__device__ __forceinline__ void lds_read_frags<5>(unsigned a, v4i (&b)[5]) {
+  static_assert(I8T2 * I8DLD == 4608, "slice stride");
+  asm volatile(
+      "ds_read_b128 %0, %5 offset:0\n\tds_read_b128 %1, %5 offset:4608\n\t"
+      "ds_read_b128 %2, %5 offset:9216\n\tds_read_b128 %3, %5 offset:13824\n\t"
+      "ds_read_b128 %4, %5 offset:18432\n\ts_waitcnt lgkmcnt(0)"
+      : "=&v"(b[0]), "=&v"(b[1]), "=&v"(b[2]), "=&v"(b[3]), "=&v"(b[4])
+      : "v"(a)
+      : "memory");
+}
+template <>
+__device__ __forceinline__ void lds_read_frags<6>(unsigned a, v4i (&b)[6]) {
+  asm volatile(
+      "ds_read_b128 %0, %6 offset:0\n\tds_read_b128 %1, %6 offset:4608\n\t"
+      "ds_read_b128 %2, %6 offset:9216\n\tds_read_b128 %3, %6 offset:13824\n\t"
+      "ds_read_b128 %4, %6 offset:18432\n\tds_read_b128 %5, %6 offset:23040\n\ts_waitcnt lgkmcnt(0)"
+      : "=&v"(b[0]), "=&v"(b[1]), "=&v"(b[2]), "=&v"(b[3]), "=&v"(b[4]), "=&v"(b[5])
+      : "v"(a)
+      : "memory");
+}
+template <>
+__device__ __forceinline__ void lds_read_frags<7>(unsigned a, v4i (&b)[7]) {
+  asm volatile(
+      "ds_read_b128 %0, %7 offset:0\n\tds_read_b128 %1, %7 offset:4608\n\t"
+      "ds_read_b128 %2, %7 offset:9216\n\tds_read_b128 %3, %7 offset:13824\n\t"
+      "ds_read_b128 %4, %7 offset:18432\n\tds_read_b128 %5, %7 offset:23040\n\t"
+      "ds_read_b128 %6, %7 offset:27648\n\ts_waitcnt lgkmcnt(0)"
+      : "=&v"(b[0]), "=&v"(b[1]), "=&v"(b[2]), "=&v"(b[3]), "=&v"(b[4]), "=&v"(b[5]), "=&v"(b[6])
+      : "v"(a)
+      : "memory");
+}
+
 // FULL: K == 128 and M % 16 == 0, so no row or k bound needs a test.
 template <int S, bool FULL>
-__global__ __launch_bounds__(I8NT) void contract_i8_kernel(const signed char* __restrict__ Asl,
-                                                           const int* __restrict__ Aexp,
-                                                           const double* __restrict__ X, double* __restrict__ Y,
-                                                           I8Map mp) {
-  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-  signed char* Dg0 = reinterpret_cast<signed char*>(smem);                       // [2][S][64][I8DLD]
-  int* ce0 = reinterpret_cast<int*>(smem + 2 * I8Lds<S>::DG);                    // [2][64]
+__global__ __launch_bounds__(I8NT) void contract_i8s_kernel(const signed char* __restrict__ Asl,
+                                                            const int* __restrict__ Aexp,
+                                                            const double* __restrict__ X, double* __restrict__ Y,
+                                                            I8Map mp) {
+  __shared__ __attribute__((aligned(16))) unsigned char stage0[D32B];
+  __shared__ __attribute__((aligned(16))) unsigned char stage1[D32B];
+  __shared__ __attribute__((aligned(16))) signed char Dg0[2 * I8DLds<S>::DG];
+  __shared__ int ce0[2 * I8T2];
 
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int M = mp.M, K = mp.K, Kp = mp.Kp, nks = Kp >> 6;
   const bool wact = 16 * w < M;
-  const int cw = 8 * w + (lane >> 3), kg = lane & 7;  // split: column and 16-row group of this lane
-  const bool kact = 16 * kg < Kp;
+  const int cs = 4 * w + (lane >> 4), kg = lane & 15;  // split: column and 8-row group of this lane
+  const bool kact = 8 * kg < Kp;
 
-  // ---- operator digits of rows 16w..16w+15 (zero past M), resident for the launch
   v4i ad[S][2];
   {
     const int row = 16 * w + (lane & 15);
@@ -115,156 +218,172 @@ __global__ __launch_bounds__(I8NT) void contract_i8_kernel(const signed char* __
     rexp[r] = m < M ? Aexp[m] : 0;
   }
 
-  // ---- this lane's 16 values of a tile: column cw, rows 16 kg + e
-  double v[16];
-  auto load_tile = [&](long long t) {
+  // ---- four 16-byte direct loads per lane per tile (wave w: staged groups w + 8i)
+  auto issue_loads = [&](long long t, unsigned char* stage) {
     if (mp.qone) {
-      const long long col = t * I8T + cw;
-      const bool ok = col < mp.ncols && kact;
-      const double* src = X + col * K + 16 * kg;
+      const long long last = mp.ncols - 1;
 #pragma unroll
-      for (int e = 0; e < 16; e += 2) {
-        double2 d = {0.0, 0.0};
-        if (ok && (FULL || 16 * kg + e < K)) d = *(const double2*)(src + e);
-        v[e] = d.x;
-        v[e + 1] = d.y;
+      for (int i = 0; i < 4; ++i) {
+        const int c = w + 8 * i;                        // tile row (output column)
+        const long long rowg = min(t * I8T2 + c, last);  // tail rows re-read the last row (zeroed in the split)
+        const int ch = min(lane, (K >> 1) - 1);         // 16-byte chunk of the row (K even)
+        const double* src = X + rowg * K + 2 * ch;
+        __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)(stage + c * D32G), 16, 0, 0);
       }
     } else {
-      // uniform slab base and 32-bit lane offsets (a slab K Q is below 2^29 values)
       const long long p = t / mp.nqt;
-      const double* base = X + p * (long long)K * mp.Q + (t - p * mp.nqt) * I8T;
+      const double* base = X + p * (long long)K * mp.Q + (t - p * mp.nqt) * I8T2;
       const unsigned Qu = (unsigned)mp.Q;
-      unsigned off = (unsigned)(16 * kg) * Qu + (unsigned)cw;
 #pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        if (FULL)
-          v[e] = base[off];
-        else
-          v[e] = (kact && 16 * kg + e < K) ? base[off] : 0.0;
-        off += Qu;
+      for (int i = 0; i < 4; ++i) {
+        const int g = w + 8 * i;                              // group of 4 rows
+        const int k = min(4 * g + (lane >> 4), K - 1);       // rows past K re-read row K-1 (zeroed in the split)
+        const double* src = base + (unsigned)k * Qu + 2 * (lane & 15);
+        __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)(stage + g * D32G), 16, 0, 0);
       }
     }
   };
-  // split of the tile held in v, in four quarters of 4 values (quarter 0
-  // also reduces the column maximum); the packed digits of the 16 values go
-  // to LDS as one 16-byte store per slice after quarter 3
+
+  // ---- split of one staged tile into digit buffer `buf`, in two halves of 4 values
+  double v[8];
   int ex = 0;
-  int packed[S][4];
-  auto split_part = [&](int buf, int j) {
-    if (j == 0) {
+  int packed[S][2];
+  auto split_part = [&](long long t, const unsigned char* stage, int buf, int h) {
+    if (h == 0) {
+      const bool colok = !mp.qone || t * I8T2 + cs < mp.ncols;
+      if (mp.qone)
+        lds_read8_row(lds_off(stage + cs * D32G + 64 * kg), v);
+      else
+        lds_read8_col(lds_off(stage + 2 * kg * D32G + cs * 8), v);  // rows 8 kg .. 8 kg + 7
       double mx = 0.0;
 #pragma unroll
-      for (int e = 0; e < 16; ++e) mx = fmax(mx, fabs(v[e]));
+      for (int e = 0; e < 8; ++e) {
+        if (!(FULL || (kact && 8 * kg + e < K)) || !colok) v[e] = 0.0;
+        mx = fmax(mx, fabs(v[e]));
+      }
 #pragma unroll
-      for (int o = 1; o < 8; o <<= 1) mx = fmax(mx, __shfl_xor(mx, o));
-      ex = mx > 0.0 ? ilogb(mx) + 1 : 0;  // |X[:, cw]| < 2^ex
-      if (kg == 0) ce0[buf * I8T + cw] = ex;
+      for (int o = 1; o < 16; o <<= 1) mx = fmax(mx, __shfl_xor(mx, o));
+      ex = mx > 0.0 ? ilogb(mx) + 1 : 0;
+      if (kg == 0) ce0[buf * I8T2 + cs] = ex;
     }
     int dq[4][S];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) i8_digits<S>(ldexp(v[4 * j + e], 20 - ex), dq[e]);
+    for (int e = 0; e < 4; ++e) i8_digits<S>(ldexp(v[4 * h + e], 20 - ex), dq[e]);
 #pragma unroll
-    for (int s = 0; s < S; ++s) packed[s][j] = pack4(dq[0][s], dq[1][s], dq[2][s], dq[3][s]);
-    if (j == 3 && kact) {
-      signed char* dst = Dg0 + (size_t)buf * I8Lds<S>::DG + (size_t)cw * I8DLD + 16 * kg;
+    for (int s = 0; s < S; ++s) packed[s][h] = pack4(dq[0][s], dq[1][s], dq[2][s], dq[3][s]);
+    if (h == 1 && kact) {
+      signed char* dst = Dg0 + (size_t)buf * I8DLds<S>::DG + (size_t)cs * I8DLD + 8 * kg;
 #pragma unroll
-      for (int s = 0; s < S; ++s)
-        *(v4i*)(dst + (size_t)s * I8T * I8DLD) = v4i{packed[s][0], packed[s][1], packed[s][2], packed[s][3]};
+      for (int s = 0; s < S; ++s) {
+        typedef int v2i __attribute__((ext_vector_type(2)));
+        *(v2i*)(dst + (size_t)s * I8T2 * I8DLD) = v2i{packed[s][0], packed[s][1]};
+      }
     }
   };
-  // products of tile t (digits in buf) by column block; after block j the
-  // quarter j of the next tile's split (values in v) runs, and once v is
-  // consumed the tile after that (tload >= 0) is loaded into it
-  auto products = [&](long long t, int buf, bool split_next, long long tload) {
-    const signed char* Dg = Dg0 + (size_t)buf * I8Lds<S>::DG;
-    const int* ce = ce0 + buf * I8T;
+
+  auto products = [&](long long t, int buf, bool split_next, long long tn, const unsigned char* stage_n) {
+    const signed char* Dg = Dg0 + (size_t)buf * I8DLds<S>::DG;
+    const int* ce = ce0 + buf * I8T2;
     long long c0, ybase;
     if (mp.qone) {
-      c0 = t * I8T;
+      c0 = t * I8T2;
       ybase = 0;
     } else {
       const long long p = t / mp.nqt;
-      c0 = (t - p * mp.nqt) * I8T;
+      c0 = (t - p * mp.nqt) * I8T2;
       ybase = p * (long long)M * mp.Q + c0;
     }
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < 2; ++j) {
       if (wact) {
-      v4i acc[S];
+        v4i acc[S];
 #pragma unroll
-      for (int l = 0; l < S; ++l) acc[l] = v4i{0, 0, 0, 0};
-      const int col = 16 * j + (lane & 15);
+        for (int l = 0; l < S; ++l) acc[l] = v4i{0, 0, 0, 0};
+        const int col = 16 * j + (lane & 15);
 #pragma unroll
-      for (int ks = 0; ks < 2; ++ks) {
-        if (ks < nks) {
-          v4i bd[S];
+        for (int ks = 0; ks < 2; ++ks) {
+          if (ks < nks) {
+            v4i bd[S];
+            lds_read_frags<S>(lds_off(Dg + (size_t)col * I8DLD + 64 * ks + 16 * (lane >> 4)), bd);
 #pragma unroll
-          for (int s = 0; s < S; ++s)
-            bd[s] = *(const v4i*)(Dg + ((size_t)s * I8T + col) * I8DLD + 64 * ks + 16 * (lane >> 4));
-          // operator slice s against data slice l - s for every level l >= s:
-          // consecutive MFMAs go to different accumulators, so none waits on the last
+            for (int s = 0; s < S; ++s)
 #pragma unroll
-          for (int s = 0; s < S; ++s)
-#pragma unroll
-            for (int l = s; l < S; ++l)
-              acc[l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ad[s][ks], bd[l - s], acc[l], 0, 0, 0);
-        }
-      }
-      const int ec = ce[col];
-      double yv[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        // adjacent levels pair exactly in int32 (|acc_l| < 2^22): acc_l 2^7 + acc_{l+1}
-        // terms from the least significant: pairs (l-1, l) in units of 2^(-7l),
-        // each 2^14 above the previous; an odd S ends with level 0 alone
-        double y = 0.0;
-#pragma unroll
-        for (int l = S - 1; l >= 0; l -= 2) {
-          const int pr = l >= 1 ? (acc[l - 1][r] << 7) + acc[l][r] : acc[l][r];
-          y = fma(y, 0x1p-14, (double)pr);
-        }
-        yv[r] = ldexp(y, rexp[r] + ec - 12 - ((S & 1) ? 0 : 7));
-      }
-      const int m0 = 16 * w + 4 * (lane >> 4);
-      if (mp.qone) {
-        const long long col_g = c0 + col;
-        if (col_g < mp.ncols) {
-          double* dst = Y + col_g * M + m0;
-          if (FULL) {  // M even: 16-byte aligned pairs
-            *(double2*)dst = double2{yv[0], yv[1]};
-            *(double2*)(dst + 2) = double2{yv[2], yv[3]};
-          } else {
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-              if (m0 + r < M) dst[r] = yv[r];
+              for (int l = s; l < S; ++l)
+                acc[l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ad[s][ks], bd[l - s], acc[l], 0, 0, 0);
           }
         }
-      } else {
-        double* yb = Y + ybase;  // uniform; 32-bit lane offsets
-        const unsigned Qu = (unsigned)mp.Q;
+        const int ec = ce[col];
+        double yv[4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-          if (FULL || m0 + r < M) yb[(unsigned)(m0 + r) * Qu + (unsigned)col] = yv[r];
+        for (int r = 0; r < 4; ++r) {
+          double y = 0.0;
+#pragma unroll
+          for (int l = S - 1; l >= 0; l -= 2) {
+            const int pr = l >= 1 ? (acc[l - 1][r] << 7) + acc[l][r] : acc[l][r];
+            y = fma(y, 0x1p-14, (double)pr);
+          }
+          yv[r] = ldexp(y, rexp[r] + ec - 12 - ((S & 1) ? 0 : 7));
+        }
+        const int m0 = 16 * w + 4 * (lane >> 4);
+        if (mp.qone) {
+          const long long col_g = c0 + col;
+          if (col_g < mp.ncols) {
+            double* dst = Y + col_g * M + m0;
+            if (FULL) {
+              *(double2*)dst = double2{yv[0], yv[1]};
+              *(double2*)(dst + 2) = double2{yv[2], yv[3]};
+            } else {
+#pragma unroll
+              for (int r = 0; r < 4; ++r)
+                if (m0 + r < M) dst[r] = yv[r];
+            }
+          }
+        } else {
+          double* yb = Y + ybase;
+          const unsigned Qu = (unsigned)mp.Q;
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (FULL || m0 + r < M) yb[(unsigned)(m0 + r) * Qu + (unsigned)col] = yv[r];
+        }
       }
-      }
-      if (split_next) split_part(buf ^ 1, j);  // VALU work of the next tile between this tile's MFMA blocks
+      if (split_next) split_part(tn, stage_n, buf ^ 1, j);
     }
-    if (tload >= 0) load_tile(tload);
   };
 
   const long long nt = mp.ntiles, G = gridDim.x;
   long long t = blockIdx.x;
   if (t >= nt) return;
-  load_tile(t);
-#pragma unroll
-  for (int j = 0; j < 4; ++j) split_part(0, j);
-  if (t + G < nt) load_tile(t + G);
-  __syncthreads();
-  int buf = 0;
-  for (; t < nt; t += G) {
-    products(t, buf, t + G < nt, t + 2 * G < nt ? t + 2 * G : -1);
-    __syncthreads();
-    buf ^= 1;
+  issue_loads(t, stage0);
+  const bool has1 = t + G < nt;
+  if (has1) issue_loads(t + G, stage1);
+  if (has1)
+    wait_vmcnt<4>();  // tile t's four loads done, tile t + G's may fly
+  else
+    wait_vmcnt<0>();
+  lds_barrier();
+  split_part(t, stage0, 0, 0);
+  split_part(t, stage0, 0, 1);
+  lds_barrier();
+  // one step per tile: sn holds tile t + G, snn (split in the previous step)
+  // receives tile t + 2G; the loop is unrolled by two so that every access
+  // names one staging buffer, and the waits need not cover the other's loads
+  auto step = [&](long long tt, int buf, const unsigned char* sn, unsigned char* snn) -> bool {
+    const bool hasn = tt + G < nt, hasnn = tt + 2 * G < nt;
+    if (hasnn) issue_loads(tt + 2 * G, snn);
+    if (hasnn)
+      wait_vmcnt<4>();
+    else
+      wait_vmcnt<0>();
+    lds_barrier();
+    products(tt, buf, hasn, tt + G, sn);
+    lds_barrier();
+    return hasn;
+  };
+  for (;;) {
+    if (!step(t, 0, stage1, stage0)) break;
+    t += G;
+    if (!step(t, 1, stage0, stage1)) break;
+    t += G;
   }
 }
 
@@ -308,11 +427,11 @@ static int launch_contract_i8(const signed char* Asl, const int* Aexp, int M, in
   if (Q == 1) {
     mp.qone = 1;
     mp.ncols = P;
-    mp.ntiles = (P + I8T - 1) / I8T;
+    mp.ntiles = (P + I8T2 - 1) / I8T2;
     mp.nqt = 1;
   } else {
     mp.qone = 0;
-    mp.nqt = Q / I8T;
+    mp.nqt = Q / I8T2;
     mp.ncols = P * Q;
     mp.ntiles = P * mp.nqt;
   }
@@ -321,12 +440,9 @@ static int launch_contract_i8(const signed char* Asl, const int* Aexp, int M, in
   JFX_CHECK_HIP(hipGetDevice(&dev));
   JFX_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   const long long nb = mp.ntiles < cus ? mp.ntiles : cus;
-  const size_t shm = I8Lds<S>::TOTAL;
   const bool full = K == I8KMAX && M % 16 == 0;
-  auto kern = full ? contract_i8_kernel<S, true> : contract_i8_kernel<S, false>;
-  hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-  if (e != hipSuccess) return set_error("contract_i8: hipFuncSetAttribute: %s", hipGetErrorString(e)), JFX_EHIP;
-  hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(I8NT), shm, st, Asl, Aexp, X, Y, mp);
+  auto kern = full ? contract_i8s_kernel<S, true> : contract_i8s_kernel<S, false>;
+  hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(I8NT), 0, st, Asl, Aexp, X, Y, mp);
   JFX_LAUNCH_CHECK();
   return JFX_OK;
 }
