@@ -61,6 +61,7 @@ struct RMap {
   // launch's sums to Y (a split of K over launches)
   int lda, accum;
   long long xps;
+  int x16;  // bf16x6 with Q == 1: rows start 16-byte aligned and hold whole 8-value groups
 };
 
 // NS: k-steps run unconditionally (4 deep for FP64/FP32, 32 deep for bf16x6);
@@ -146,10 +147,20 @@ __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __res
           const int c = gi >> 4, k8 = (gi & 15) * 8;
           const bool ok = c < ncol && k8 < K;
           const float* src = X + xbase + (long long)c * rm.xps + k8;
-          float4 v0 = ok ? *(const float4*)src : float4{0, 0, 0, 0};
-          float4 v1 = ok ? *(const float4*)(src + 4) : float4{0, 0, 0, 0};
-          xr[8 * g + 0] = v0.x; xr[8 * g + 1] = v0.y; xr[8 * g + 2] = v0.z; xr[8 * g + 3] = v0.w;
-          xr[8 * g + 4] = v1.x; xr[8 * g + 5] = v1.y; xr[8 * g + 6] = v1.z; xr[8 * g + 7] = v1.w;
+          if (rm.x16) {
+            float4 v0 = ok ? *(const float4*)src : float4{0, 0, 0, 0};
+            float4 v1 = ok ? *(const float4*)(src + 4) : float4{0, 0, 0, 0};
+            xr[8 * g + 0] = v0.x; xr[8 * g + 1] = v0.y; xr[8 * g + 2] = v0.z; xr[8 * g + 3] = v0.w;
+            xr[8 * g + 4] = v1.x; xr[8 * g + 5] = v1.y; xr[8 * g + 6] = v1.z; xr[8 * g + 7] = v1.w;
+          } else {
+            // even K, rows 8-byte aligned: pairs, each inside the row or zero
+#pragma unroll
+            for (int e = 0; e < 8; e += 2) {
+              float2 v = (ok && k8 + e < K) ? *(const float2*)(src + e) : float2{0, 0};
+              xr[8 * g + e] = v.x;
+              xr[8 * g + e + 1] = v.y;
+            }
+          }
         } else {
           const int c = gi & 63, k8 = (gi >> 6) * 8;
 #pragma unroll
@@ -419,6 +430,7 @@ int launch_ra(const void* A, int M, int K, const void* X, void* Y, long long P, 
   rm.lda = lda > 0 ? lda : lda0;
   rm.xps = xps > 0 ? xps : (long long)K * Q;
   rm.accum = accum;
+  rm.x16 = (K % 8 == 0 && rm.xps % 4 == 0) ? 1 : 0;
   const bool ksplit = rm.lda != lda0 || rm.xps != (long long)K * Q || accum;
   if (ksplit && ((Q != 1 && Q % RTN) || rm.lda < lda0 || rm.xps < (long long)K * Q ||
                  (Q == 1 && rm.xps % (POL == 3 ? 8 : EPC))))
@@ -429,7 +441,10 @@ int launch_ra(const void* A, int M, int K, const void* X, void* Y, long long P, 
     // keep the zero fill (stores write rows < K only) and the operator
     // registers past K are zero, so FP64 takes any even K (126: a 128-point
     // Dirichlet composite)
-    if (K % (POL == 3 ? 8 : EPC) || M % EPC || rm.Mtot % EPC || rm.moff % EPC) return 0;
+    // bf16x6 takes any even K: rows that are not 16-byte aligned or end inside
+    // an 8-value group are read in 8-byte pairs
+    if ((POL == 3 ? K % 2 : K % EPC) || M % EPC || rm.Mtot % EPC || rm.moff % EPC) return 0;
+    if (POL == 3 && rm.xps % 2) return 0;
     rm.qone = 1;
     rm.nqt = 0;
     rm.ntiles = (P + RTN - 1) / RTN;

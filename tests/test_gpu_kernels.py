@@ -234,12 +234,13 @@ def test_contract_k_split(dev, prec, tol, M, Kc, shape, axis):
     assert O.rel_l2(Y.to(torch.float64).cpu().numpy(), ref) < tol
 
 
-@pytest.mark.parametrize("prec,tol", [("f64", 1e-14), ("f32", 2e-6)])
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-14), ("f32", 2e-6), ("bf16x6", 2e-6)])
 @pytest.mark.parametrize("M", [16, 96, 128, 192])
 @pytest.mark.parametrize("Kc", [10, 62, 126])
 def test_contract_last_axis_even_k(dev, prec, tol, M, Kc):
-    # last-axis contractions (Q == 1) with K even but not a multiple of 4: the
-    # register-resident FP64 kernel's k-steps past K read zeros
+    # last-axis contractions (Q == 1) with K even but not a multiple of 4 (8 for
+    # bf16x6): the register-resident kernels' k-steps past K read zeros, and
+    # bf16x6 reads such rows (8-byte aligned only) in pairs
     import torch
 
     from jaxfun_amd import kernels as K
