@@ -397,16 +397,15 @@ def bench_learned_pcg(ntrain: int = 20000, neval: int = 1024, chunk: int = 1024,
     xp, itp = Lm.pcg(A, b, op.plain_precond(op.kbar(kq)), rtol=1e-10)
     s_an = op.kfield(-0.5 * g)
     xa, ita = Lm.pcg(A, b, op.scaled_precond(s_an), x0=op.scaled_precond(s_an)(b), rtol=1e-10)
-    s_nn = lp.field(g)
-    P = op.scaled_precond(s_nn)
-    xl, itl = Lm.pcg(A, b, P, x0=P(b), rtol=1e-10)
-    _, itl0 = Lm.pcg(A, b, P, rtol=1e-10)
+    xl, itl, fb = Lm.solve_learned(op, lp, kq, g, b, rtol=1e-10)
+    _, itl0 = Lm.pcg(A, b, op.scaled_precond(lp.field(g)), rtol=1e-10)
     res = {k: float(Lm.true_residual(op, kq, x_, b).max().item()) for k, x_ in
            (("plain", xp), ("analytic_sqrt_k", xa), ("learned", xl))}
     itp, ita, itl, itl0 = (t_.double().mean().item() for t_ in (itp, ita, itl, itl0))
     return dict(train_systems=nsys, train_pairs=int(gh.shape[0]), eval_systems=neval,
-                mlp_params=lp.net.n_params(), loss="energy norm of P_s b - u, sigma = -g/2 + MLP(g)",
+                mlp_params=lp.net.n_params(), loss="energy norm of P_s b - u, sigma = -g/2 + D o MLP(g / D)",
                 gen_s=gen_s, train_s=train_s, train_loss_first=hist[0], train_loss_last=hist[-1],
+                train_steps_skipped=lp.skipped, plain_fallbacks=int(fb.sum().item()),
                 mean_iters_plain=itp, mean_iters_learned=itl, mean_iters_learned_zero_x0=itl0,
                 mean_iters_scaled_sqrt_k_no_network=ita,
                 iteration_ratio_plain_over_learned=itp / itl,

@@ -329,7 +329,8 @@ int jfx_pcg_update_xr(double* x, double* r, const double* p, const double* q, co
 /* p_b = z_b + (num_b/den_b) p_b for active b */
 int jfx_pcg_update_p(double* p, const double* z, const double* num, const double* den, const int* active,
                      int batch, long long n, hipStream_t stream);
-/* active_b &= rr_b > tol2 bb_b, counting iters_b for systems active on entry when count != 0 */
+/* active_b &= rr_b > tol2 bb_b, counting iters_b for systems active on entry when count != 0;
+ * a non-finite rr_b stops system b with iters_b = -1 (breakdown) */
 int jfx_pcg_check(const double* rr, const double* bb, double tol2, int* active, int* iters, int batch, int count,
                   hipStream_t stream);
 /* y_b = s_b x_b */

@@ -14,9 +14,13 @@ Preconditioners (all SPD, so PCG stays valid):
              on the quadrature mesh (B: coefficients -> mesh values,
              F: mesh values -> coefficients).  s = k^-1/2 is the classical
              choice; the learned preconditioner takes s = exp(sigma) with
-             sigma's 16 x 16 low Legendre modes = -g/2 + MLP(g) (a 4-layer
-             512-wide tanh MLP whose last layer starts at zero, so training
-             starts from the classical scaling), trained on (b, u) pairs that
+             sigma's 16 x 16 low Legendre modes = -g/2 + D o MLP(g / D) (a
+             4-layer 512-wide tanh MLP whose last layer starts at zero, so
+             training starts from the classical scaling; D_ij =
+             (1 + |(i, j)|)^-2 is the prior spectrum of g, which whitens the
+             input and keeps the correction's field bounded: an unweighted
+             correction can put coherent high modes into sigma whose sum at
+             the mesh corners overflows exp), trained on (b, u) pairs that
              plain PCG generates to minimise the energy-norm error of the
              initial guess, (P_s b - u)^T A (P_s b - u) / u^T A u.  Learned PCG
              starts from x0 = P_s b.
@@ -219,12 +223,17 @@ def batched_dot(x: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
 
 
 def pcg(apply, b: torch.Tensor, precond, x0: torch.Tensor | None = None, rtol: float = 1e-10,
-        maxiter: int = 2000, check_every: int = 1, snapshots=(), snap_out: dict | None = None):
+        maxiter: int = 2000, check_every: int = 1, snapshots=(), snap_out: dict | None = None,
+        breakdown: str = "raise"):
     """Batched preconditioned CG; every system stops on ||r|| <= rtol ||b||
     (recursive residual).  Returns (x, iterations per system).  For each k in
     ``snapshots`` the iterate and residual after k iterations are stored in
     ``snap_out[k] = (x_k, r_k)`` (systems that stopped earlier keep their
-    final values)."""
+    final values).  A system whose residual turns non-finite stops with
+    iterations -1; ``breakdown="raise"`` then raises FloatingPointError,
+    ``"mark"`` returns the marks for the caller's fallback."""
+    if breakdown not in ("raise", "mark"):
+        raise ValueError("breakdown must be 'raise' or 'mark'")
     B = b.shape[0]
     n = b[0].numel()
     x = torch.zeros_like(b) if x0 is None else x0.clone()
@@ -252,12 +261,38 @@ def pcg(apply, b: torch.Tensor, precond, x0: torch.Tensor | None = None, rtol: f
         rz_new = batched_dot(r, z)
         _lib.call("jfx_pcg_update_p", K._p(p), K._p(z), K._p(rz_new), K._p(rz), K._p(active), B, n, K._s())
         rz = rz_new
+    if breakdown == "raise":
+        nbad = int((iters < 0).sum().item())
+        if nbad:
+            raise FloatingPointError(f"PCG breakdown: non-finite residual in {nbad} of {B} systems")
     return x, iters
 
 
 def true_residual(op: VarCoeffPoisson2D, kq, x, b) -> torch.Tensor:
     r = K.lincomb([(1.0, b), (-1.0, op.apply(kq, x))])
     return torch.sqrt(batched_dot(r, r) / batched_dot(b, b))
+
+
+def solve_learned(op: VarCoeffPoisson2D, lp: "LearnedPreconditioner", kq: torch.Tensor, ghat: torch.Tensor,
+                  b: torch.Tensor, rtol: float = 1e-10, accept: float | None = None, maxiter: int = 2000):
+    """Learned PCG (x0 = P_s b, preconditioner P_s with the network's field)
+    with plain PCG as the fallback and the true Galerkin residual as the
+    judge: systems whose learned run broke down or whose true residual
+    ||b - A x|| / ||b|| exceeds ``accept`` (default 10 rtol) are solved again
+    by plain PCG from zero.  Returns (x, iterations, fallback mask); the
+    iterations of a fallback system count both runs."""
+    accept = 10.0 * rtol if accept is None else float(accept)
+    P = op.scaled_precond(lp.field(ghat))
+    x, it = pcg(lambda U: op.apply(kq, U), b, P, x0=P(b), rtol=rtol, maxiter=maxiter, breakdown="mark")
+    res = true_residual(op, kq, x, b)
+    bad = (it < 0) | ~(res <= accept)
+    if bool(bad.any().item()):
+        idx = torch.nonzero(bad).flatten()
+        kf, bf = kq[idx].contiguous(), b[idx].contiguous()
+        xf, itf = pcg(lambda U: op.apply(kf, U), bf, op.plain_precond(op.kbar(kf)), rtol=rtol, maxiter=maxiter)
+        x[idx] = xf
+        it[idx] = it[idx].clamp(min=0) + itf
+    return x, it, bad
 
 
 # ----------------------------------------------------------------------------- learned preconditioner
@@ -277,11 +312,17 @@ class LearnedPreconditioner:
         self.loss = loss
         self.net = MLP([gm2] + [hidden] * (layers - 1) + [gm2], seed=seed)
         self.net.params[-1][0].zero_()  # the untrained network adds nothing to the prior
+        m = np.arange(op.gm, dtype=np.float64)
+        D = (1.0 + np.sqrt(m[:, None] ** 2 + m[None, :] ** 2)) ** -2.0  # prior spectrum of ghat
+        self.D = torch.tensor(D, device=K.device())
+        self.Dinv = torch.tensor(1.0 / D, device=K.device())
+        self.skipped = 0  # training steps dropped for a non-finite loss
 
     def sigma_hat(self, ghat: torch.Tensor):
-        X = _f32(ghat.reshape(ghat.shape[0], -1))
+        X = _f32(mul_bcast(ghat.contiguous(), self.Dinv).reshape(ghat.shape[0], -1))
         Y, hs = self.net.forward(X)
-        sh = K.lincomb([(1.0, K.cast(Y, torch.float64).reshape(ghat.shape)), (self.prior, ghat.contiguous())])
+        corr = mul_bcast(K.cast(Y, torch.float64).reshape(ghat.shape), self.D)
+        sh = K.lincomb([(1.0, corr), (self.prior, ghat.contiguous())])
         return sh, hs
 
     def field(self, ghat):
@@ -310,7 +351,7 @@ class LearnedPreconditioner:
         gs = op.scaled_backward(s, parts, gz)
         gsig = mul_bcast(gs, s)
         gsh = op._ap(op.VgT, op._ap(op.VgT, gsig, 1), 2)
-        grads = self.net.backward(hs, _f32(gsh.reshape(Bn, -1)))
+        grads = self.net.backward(hs, _f32(mul_bcast(gsh, self.D).reshape(Bn, -1)))
         return float(per.mean().item()), grads
 
     def train(self, ghat, b, u, steps: int = 1000, batch: int = 256, lr: float = 1e-3, seed: int = 0,
@@ -321,8 +362,13 @@ class LearnedPreconditioner:
         for st in range(steps):
             idx = torch.tensor(rng.choice(N, size=min(batch, N), replace=False), device=ghat.device)
             loss, grads = self.loss_and_grad(ghat[idx].contiguous(), b[idx].contiguous(), u[idx].contiguous())
+            if not math.isfinite(loss):  # a batch whose loss overflowed leaves the weights alone
+                self.skipped += 1
+                continue
             self.net.adam(grads, lr=lr * (0.5 * (1 + math.cos(math.pi * st / steps))) + 1e-5)
             hist.append(loss)
             if log and (st % max(1, steps // 10) == 0 or st == steps - 1):
                 log(f"step {st} loss {loss:.4e}")
+        if not hist:
+            raise FloatingPointError("training produced no finite loss")
         return hist

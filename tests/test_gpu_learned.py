@@ -38,6 +38,63 @@ def test_gemm_bf16_all_transposes(dev):
             assert O.rel_l2(C.cpu().numpy(), ref) < 2e-2
 
 
+def _bf16(a):
+    """Round float32 values to bfloat16 (nearest even), returned as float64."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("M,N,K,offset", [
+    (70, 45, 130, 0),     # K % 8 != 0: element loads
+    (256, 512, 512, 0),   # an MLP layer: 16-byte fragment loads, 16 chunks over four waves
+    (33, 97, 600, 0),     # 19 chunks: a second pass of the chunk loop, ragged tiles
+    (512, 256, 256, 1),   # operands one element off 16-byte alignment: element loads
+])
+def test_gemm_bf16_vs_rounded_operands(dev, M, N, K, offset):
+    # The MFMA products of bf16 operands are exact in FP32, so the result is
+    # the FP64 product of the bf16-rounded operands up to FP32 accumulation
+    # error, far below the operand rounding a wrong index or transpose shows.
+    import torch
+
+    from jaxfun_amd import _lib
+    from jaxfun_amd import kernels as K_
+
+    rng = np.random.default_rng(M + N + K)
+    for ta in (0, 1):
+        for tb in (0, 1):
+            ash, bsh = ((K, M) if ta else (M, K)), ((N, K) if tb else (K, N))
+            A = rng.standard_normal(ash).astype(np.float32)
+            B = rng.standard_normal(bsh).astype(np.float32)
+            Ad = torch.empty(A.size + offset, dtype=torch.float32, device=dev)[offset:].view(ash)
+            Bd = torch.empty(B.size + offset, dtype=torch.float32, device=dev)[offset:].view(bsh)
+            Ad.copy_(torch.tensor(A))
+            Bd.copy_(torch.tensor(B))
+            C = torch.empty((M, N), dtype=torch.float32, device=dev)
+            _lib.call("jfx_gemm_bf16", ta, tb, M, N, K, 1.0, K_._p(Ad), ash[1], K_._p(Bd), bsh[1], 0.0, K_._p(C),
+                      N, None, 0, K_._s())
+            Ar, Br = _bf16(A), _bf16(B)
+            At, Bt = (Ar.T if ta else Ar), (Br.T if tb else Br)
+            ref = At @ Bt
+            bound = 2.0 * K * 2.0 ** -24 * (np.abs(At) @ np.abs(Bt)) + 1e-30
+            err = np.abs(C.cpu().numpy() - ref)
+            assert np.all(err <= bound), (ta, tb, float((err / bound).max()))
+
+
+def test_colsum_vs_numpy(dev):
+    import torch
+
+    from jaxfun_amd import _lib
+    from jaxfun_amd import kernels as K_
+
+    for rows, cols in ((256, 512), (1, 3), (1000, 130)):
+        X = np.random.default_rng(rows).standard_normal((rows, cols)).astype(np.float32)
+        out = torch.empty(cols, dtype=torch.float32, device=dev)
+        _lib.call("jfx_colsum", K_._p(torch.tensor(X, device=dev)), K_._p(out), rows, cols, K_._s())
+        ref = X.astype(np.float64).sum(0)
+        assert np.all(np.abs(out.cpu().numpy() - ref) <= rows * 2.0 ** -23 * np.abs(X).sum(0)), (rows, cols)
+
+
 def test_mlp_gradients_vs_oracle(dev):
     import torch
 
@@ -158,3 +215,38 @@ def test_learned_training_reduces_loss_and_pcg_converges(small_problem):
     # what the trained preconditioner gains over plain PCG (the method without it)
     _, it_plain = Lm.pcg(lambda U: op.apply(kq, U), b, op.plain_precond(op.kbar(kq)), rtol=1e-10)
     assert iters.double().mean().item() < it_plain.double().mean().item()
+
+
+def test_pcg_breakdown_marked_or_raised(small_problem):
+    import torch
+
+    from jaxfun_amd import learned as Lm
+
+    op, ghat, fhat, G, Fh = small_problem
+    kq = op.kfield(ghat)
+    b = op.rhs(fhat)
+    b[2, 0, 0] = float("nan")
+    A = lambda U: op.apply(kq, U)  # noqa: E731
+    with pytest.raises(FloatingPointError):
+        Lm.pcg(A, b, op.plain_precond(op.kbar(kq)), rtol=1e-10)
+    x, it = Lm.pcg(A, b, op.plain_precond(op.kbar(kq)), rtol=1e-10, breakdown="mark")
+    it = it.cpu().numpy()
+    assert it[2] == -1 and np.all(np.delete(it, 2) > 2)
+    del torch
+
+
+def test_solve_learned_falls_back_to_plain_pcg(small_problem):
+    from jaxfun_amd import learned as Lm
+
+    op, ghat, fhat, G, Fh = small_problem
+    kq = op.kfield(ghat)
+    b = op.rhs(fhat)
+    lp = Lm.LearnedPreconditioner(op, hidden=32, layers=3)
+    x, it, fb = Lm.solve_learned(op, lp, kq, ghat, b)  # untrained: the classical scaling
+    assert not fb.any().item() and np.all(Lm.true_residual(op, kq, x, b).cpu().numpy() < 1e-9)
+    lp.net.params[-1][1].fill_(1e4)  # a network whose field overflows exp: every system breaks down
+    x, it, fb = Lm.solve_learned(op, lp, kq, ghat, b)
+    assert fb.all().item()
+    assert np.all(Lm.true_residual(op, kq, x, b).cpu().numpy() < 1e-9)
+    _, it_plain = Lm.pcg(lambda U: op.apply(kq, U), b, op.plain_precond(op.kbar(kq)), rtol=1e-10)
+    assert np.array_equal(it.cpu().numpy(), it_plain.cpu().numpy())  # broken runs stop at once
