@@ -121,17 +121,19 @@ int jfx_split_matrix_i8(const double* A, int M, int K, int nslices, signed char*
 int jfx_contract_i8(const signed char* Aslices, const int* Arowexp, int nslices, int M, int K, const double* X,
                     double* Y, long long P, long long Q, hipStream_t stream);
 
-/* Fused two-axis contraction of 2D slabs: Y[b] = A0 X[b] A1^T (X: B x K0 x K1,
- * Y: B x M0 x M1, all dims <= 128), optionally followed by Y /= E (epi 1) or
- * Y *= E (epi 2) with E (M0 x M1).  prec 0: FP64 operators A0d, A1d; prec 1:
+/* Fused two-axis contraction of 2D slabs: Y[b] = alpha A0 X[b] A1^T (X: B x K0
+ * x K1, Y: B x M0 x M1, all dims <= 128), optionally followed by Y /= E (epi 1)
+ * or Y *= E (epi 2) with E (M0 x M1); accumulate = 1 adds the result to the
+ * previous Y (Y must not alias X).  prec 0: FP64 operators A0d, A1d; prec 1:
  * 3-term bf16 slices A0s, A1s ([3][M][Kp], Kp = 32*ceil(K/32)) on FP32 data.
  * Replaces two successive per-axis applications of TensorProductSpace
- * transforms (galerkin/tensorproductspace.py:330-417) or of the 2D
+ * transforms (galerkin/tensorproductspace.py:330-417), the 2D
  * fast-diagonalisation solve (la/tpmatrix.py:545-587, with the eigenvalue
- * division as the epilogue). */
+ * division as the epilogue) and, with alpha and accumulate, the scaled sum
+ * of a TPMatrices product (la/tpmatrix.py:158-166, 291-316). */
 int jfx_plane_contract(int prec, const double* A0d, const double* A1d, const unsigned short* A0s,
-                       const unsigned short* A1s, const void* X, void* Y, const void* E, int epi, long long B,
-                       int M0, int K0, int M1, int K1, hipStream_t stream);
+                       const unsigned short* A1s, const void* X, void* Y, const void* E, int epi, double alpha,
+                       int accumulate, long long B, int M0, int K0, int M1, int K1, hipStream_t stream);
 
 /* 2D fast-diagonalisation solve of n x n slabs in one launch:
  *     Y[b] = V0 ((W0 X[b] W1^T) / E) V1^T        (X, Y: B x n x n, E: n x n),

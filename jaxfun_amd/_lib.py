@@ -79,8 +79,8 @@ SIGNATURES: dict[str, tuple] = {
     "jfx_pcg_check": (c_int, [c_vp, c_vp, c_dbl, c_vp, c_vp, c_int, c_int, c_vp]),
     "jfx_scale_batch": (c_int, [c_vp, c_vp, c_vp, c_int, c_ll, c_vp]),
     "jfx_exp": (c_int, [c_vp, c_vp, c_dbl, c_ll, c_vp]),
-    "jfx_plane_contract": (c_int, [c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_ll, c_int, c_int,
-                                   c_int, c_int, c_vp]),
+    "jfx_plane_contract": (c_int, [c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_dbl, c_int, c_ll,
+                                   c_int, c_int, c_int, c_int, c_vp]),
     "jfx_plane_fd": (c_int, [c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_ll,
                              c_int, c_vp]),
     "jfx_scale_transpose": (c_int, [c_vp, c_int, c_int, c_vp, c_vp, c_dbl, c_vp, c_vp]),

@@ -519,15 +519,28 @@ def plane_fd(W0: AxisOperator, W1: AxisOperator, V0: AxisOperator, V1: AxisOpera
 
 
 def plane(op0: AxisOperator, op1: AxisOperator, X: torch.Tensor, prec: str = "f64",
-          epi: tuple | None = None) -> torch.Tensor:
-    """``Y[b] = A0 X[b] A1^T`` over the last two axes of X in one kernel
-    (optionally ``/ E`` or ``* E`` with ``epi=("div"|"mul", E)``)."""
+          epi: tuple | None = None, alpha: float = 1.0, out: torch.Tensor | None = None,
+          accumulate: bool = False) -> torch.Tensor:
+    """``Y[b] = alpha A0 X[b] A1^T`` over the last two axes of X in one kernel
+    (optionally ``/ E`` or ``* E`` with ``epi=("div"|"mul", E)``); with
+    ``accumulate`` the result is added to ``out``."""
     dt = io_dtype(prec)
     _req(X, dt, "X")
     if X.shape[-2] != op0.K or X.shape[-1] != op1.K:
         raise ValueError("plane: operator/array shape mismatch")
     B = math.prod(X.shape[:-2])
-    Y = torch.empty(X.shape[:-2] + (op0.M, op1.M), dtype=dt, device=X.device)
+    shape = X.shape[:-2] + (op0.M, op1.M)
+    if out is None:
+        if accumulate:
+            raise ValueError("plane: accumulate needs out=")
+        Y = torch.empty(shape, dtype=dt, device=X.device)
+    else:
+        _req(out, dt, "out")
+        if tuple(out.shape) != tuple(shape):
+            raise ValueError("plane: bad out shape")
+        if out.data_ptr() == X.data_ptr():
+            raise ValueError("plane: out must not alias X")
+        Y = out
     mode, E = 0, None
     if epi is not None:
         mode = {"div": 1, "mul": 2}[epi[0]]
@@ -535,11 +548,12 @@ def plane(op0: AxisOperator, op1: AxisOperator, X: torch.Tensor, prec: str = "f6
         _req(E, dt, "E")
         if tuple(E.shape) != (op0.M, op1.M):
             raise ValueError("plane: epilogue shape mismatch")
+    acc = 1 if accumulate else 0
     if prec == "f64":
         _lib.call("jfx_plane_contract", 0, _p(op0.A64), _p(op1.A64), _p(None), _p(None), _p(X), _p(Y), _p(E),
-                  mode, B, op0.M, op0.K, op1.M, op1.K, _s())
+                  mode, float(alpha), acc, B, op0.M, op0.K, op1.M, op1.K, _s())
     else:
         s0, s1 = op0.prepared("bf16x6")[0], op1.prepared("bf16x6")[0]
         _lib.call("jfx_plane_contract", 1, _p(None), _p(None), _p(s0), _p(s1), _p(X), _p(Y), _p(E),
-                  mode, B, op0.M, op0.K, op1.M, op1.K, _s())
+                  mode, float(alpha), acc, B, op0.M, op0.K, op1.M, op1.K, _s())
     return Y

@@ -55,18 +55,23 @@ class TPMatrix:
     def scale(self, alpha) -> "TPMatrix":
         return TPMatrix(self.mats, self.coefficient * alpha, self.global_indices)
 
-    def _matmul_array(self, w, precision: str = "f64") -> torch.Tensor:
-        """A0 along axis 0, A1 along axis 1, ... (la/tpmatrix.py:152-156)."""
+    def _plane_able(self, w, precision: str) -> bool:
+        return self.dims == 2 and not w.is_complex() and \
+            K.plane_ok(self.mats[0].operator(), self.mats[1].operator(), precision)
+
+    def _matmul_array(self, w, precision: str = "f64", out=None, accumulate: bool = False) -> torch.Tensor:
+        """A0 along axis 0, A1 along axis 1, ... (la/tpmatrix.py:152-156).  In
+        2D the coefficient is applied by the fused kernel, which also adds the
+        product to ``out`` when ``accumulate`` is set."""
         from .matrix import _coefficients
 
         w = _as_dev(_coefficients(w), K.io_dtype(precision))
         nb = w.ndim - self.dims
-        if self.dims == 2 and not w.is_complex() and \
-                K.plane_ok(self.mats[0].operator(), self.mats[1].operator(), precision):
-            w = K.plane(self.mats[0].operator(), self.mats[1].operator(), w.contiguous(), precision)
-            if self.coefficient != 1:
-                w = K.lincomb([(float(self.coefficient), w)])
-            return w
+        if self._plane_able(w, precision):
+            return K.plane(self.mats[0].operator(), self.mats[1].operator(), w.contiguous(), precision,
+                           alpha=float(self.coefficient), out=out, accumulate=accumulate)
+        if out is not None or accumulate:
+            raise ValueError("out/accumulate need the fused 2D path")
         for i, m in enumerate(self.mats):
             w = m.matvec(w, nb + i, precision)
         if self.coefficient != 1:
@@ -549,8 +554,17 @@ class TPMatrices:
         return self.tpmats[0].shape if self.tpmats else (0, 0)
 
     def _apply_array(self, u, precision: str = "f64") -> torch.Tensor:
-        parts = [tp._matmul_array(u, precision) for tp in self.tpmats]
-        return K.lincomb([(1.0, p) for p in parts])
+        from .matrix import _coefficients
+
+        w = _as_dev(_coefficients(u), K.io_dtype(precision)).contiguous()
+        if all(tp._plane_able(w, precision) for tp in self.tpmats):
+            # one fused launch per term, each adding into the first's output
+            out = None
+            for i, tp in enumerate(self.tpmats):
+                out = tp._matmul_array(w, precision, out=out, accumulate=i > 0)
+            return out
+        parts = [tp._matmul_array(w, precision) for tp in self.tpmats]
+        return parts[0] if len(parts) == 1 else K.lincomb([(1.0, p) for p in parts])
 
     def __call__(self, u, precision: str = "f64"):
         return self._apply_array(u, precision)

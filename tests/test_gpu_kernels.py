@@ -167,6 +167,32 @@ def test_plane_contract(dev, dims, prec, tol, epi):
     assert O.rel_l2(Y.to(torch.float64).cpu().numpy(), Yr) < tol
 
 
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-14), ("bf16x6", 2e-6)])
+@pytest.mark.parametrize("dims", [(126, 126, 126, 126), (30, 100, 70, 20)])
+def test_plane_alpha_accumulate(dev, prec, tol, dims):
+    # Y = alpha A0 X A1^T (/ E), then += beta-free accumulation of a second term into the same out
+    import torch
+
+    from jaxfun_amd import kernels as K
+
+    M0, K0, M1, K1 = dims
+    rng = np.random.default_rng(11)
+    A0, A1, B0, B1 = (rng.standard_normal(s) for s in ((M0, K0), (M1, K1), (M0, K0), (M1, K1)))
+    E = rng.uniform(1.0, 2.0, size=(M0, M1))
+    X = rng.standard_normal((4, K0, K1))
+    dt = K.io_dtype(prec)
+    Xd, Ed = _t(X, dev, dt), _t(E, dev, dt)
+    ops = [K.AxisOperator(_t(a, dev)) for a in (A0, A1, B0, B1)]
+    Y = K.plane(ops[0], ops[1], Xd, prec, ("div", Ed), alpha=-0.5)
+    Y2 = K.plane(ops[2], ops[3], Xd, prec, alpha=3.0, out=Y, accumulate=True)
+    assert Y2.data_ptr() == Y.data_ptr()
+    Xr = Xd.to(torch.float64).cpu().numpy()
+    Yr = -0.5 * np.einsum("mk,bkl,nl->bmn", A0, Xr, A1) / E + 3.0 * np.einsum("mk,bkl,nl->bmn", B0, Xr, B1)
+    assert O.rel_l2(Y.to(torch.float64).cpu().numpy(), Yr) < tol
+    with pytest.raises(ValueError):
+        K.plane(ops[0], ops[1], Xd, prec, accumulate=True)
+
+
 @pytest.mark.parametrize("prec,n", [("f64", 5), ("f64", 17), ("f64", 62), ("f64", 64), ("bf16x6", 30),
                                     ("bf16x6", 62), ("bf16x6", 126), ("bf16x6", 128)])
 @pytest.mark.parametrize("B", [1, 3, 300])
