@@ -1,4 +1,5 @@
-// Register-resident-operator contraction for M, K <= 128:
+// Register-resident-operator contraction for M <= 128, K <= 128 (bf16x6:
+// K <= 192):
 //     Y[p, m, q] = sum_k A[m, k] X[p, k, q],   X viewed as (P, K, Q)
 // for the two column layouts that cover every axis of a C-order array:
 // Q % 64 == 0 (a 64-column tile is 64 consecutive q of one p) and Q == 1
@@ -42,12 +43,18 @@ struct RT<3> {
   typedef v4f V;
 };
 
+// bf16x6 stages 8-wide k groups: 16 per column (K <= 128), 24 with six
+// k-steps (K <= 192, one column group of 8 waves).
+constexpr int RMAX3 = 192;
+template <int POL, int NS>
+constexpr int ra_kgroups() { return (POL == 3 && NS > 4) ? 4 * NS : 16; }
+
 // Two X-tile buffers: plain Bs[k][c] (row stride 72 doubles / 80 floats),
-// bf16x6 Bsl[3][c][k] (row stride 136 bf16).
-template <int POL>
+// bf16x6 Bsl[3][c][k] (row stride 8 KG + 8 bf16: 136 for K <= 128, 200 for 192).
+template <int POL, int NS>
 struct RLds {
   typedef typename RT<POL>::TX TX;
-  static constexpr int BLD = (POL == 3) ? 136 : (POL == 0 ? 72 : 80);
+  static constexpr int BLD = (POL == 3) ? 8 * ra_kgroups<POL, NS>() + 8 : (POL == 0 ? 72 : 80);
   static constexpr size_t B = (POL == 3) ? (size_t)3 * RTN * BLD * 2 : (size_t)RMAX * BLD * sizeof(TX);
   static constexpr size_t TOTAL = 2 * B;
 };
@@ -62,6 +69,7 @@ struct RMap {
   int lda, accum;
   long long xps;
   int x16;  // bf16x6 with Q == 1: rows start 16-byte aligned and hold whole 8-value groups
+  int o16;  // FP32 / bf16x6 with Q == 1: output columns in 16-byte chunks (else 8-byte pairs)
 };
 
 // NS: k-steps run unconditionally (4 deep for FP64/FP32, 32 deep for bf16x6);
@@ -73,14 +81,15 @@ __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __res
                                                           typename RT<POL>::TX* __restrict__ Y, RMap rm) {
   typedef typename RT<POL>::TX TX;
   typedef typename RT<POL>::V V;
-  constexpr int BLD = RLds<POL>::BLD;
+  constexpr int BLD = RLds<POL, NS>::BLD;
+  constexpr int KG = ra_kgroups<POL, NS>();        // bf16x6: 8-wide k groups staged per column
   constexpr int EPC = 16 / sizeof(TX);            // elements per 16-byte chunk
   constexpr int NT = RNT * CG, JB = 4 / CG;        // threads; 16-column blocks per wave
   constexpr int NCH = RMAX * RTN / EPC / NT;      // plain: chunks per thread per tile
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   TX* Bs0 = reinterpret_cast<TX*>(smem);  // buffer b at Bs0 + b * BSZ
   unsigned short* Bsl0 = reinterpret_cast<unsigned short*>(smem);
-  constexpr int BSZ = (int)(RLds<POL>::B / sizeof(TX)), BSL = (int)(RLds<POL>::B / 2);
+  constexpr int BSZ = (int)(RLds<POL, NS>::B / sizeof(TX)), BSL = (int)(RLds<POL, NS>::B / 2);
 
   const int tid = threadIdx.x, lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -120,7 +129,8 @@ __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __res
   // ---- X tile loads (registers) and LDS staging
   // plain: NCH 16-byte chunks per thread; bf16x6: 2 groups of 8 consecutive k of one column
   typedef typename std::conditional<POL == 3, float, TX>::type RV;
-  constexpr int NREG = (POL == 3) ? 16 / CG : NCH * EPC;
+  constexpr int GPT = KG * RTN / NT;  // bf16x6: k groups per thread per tile
+  constexpr int NREG = (POL == 3) ? 8 * GPT : NCH * EPC;
   RV xr[NREG];
   auto tile_cols = [&](long long t, long long& xbase, long long& c0, int& ncol) {
     if (rm.qone) {
@@ -141,10 +151,10 @@ __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __res
     tile_cols(t, xbase, c0, ncol);
     if constexpr (POL == 3) {
 #pragma unroll
-      for (int g = 0; g < 2 / CG; ++g) {
+      for (int g = 0; g < GPT; ++g) {
         const int gi = tid + NT * g;
         if (rm.qone) {
-          const int c = gi >> 4, k8 = (gi & 15) * 8;
+          const int c = gi / KG, k8 = (gi - c * KG) * 8;
           const bool ok = c < ncol && k8 < K;
           const float* src = X + xbase + (long long)c * rm.xps + k8;
           if (rm.x16) {
@@ -217,9 +227,9 @@ __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __res
     unsigned short* Bsl = Bsl0 + buf * BSL;
     if constexpr (POL == 3) {
 #pragma unroll
-      for (int g = 0; g < 2 / CG; ++g) {
+      for (int g = 0; g < GPT; ++g) {
         const int gi = tid + NT * g;
-        const int c = rm.qone ? (gi >> 4) : (gi & 63), k8 = rm.qone ? (gi & 15) * 8 : (gi >> 6) * 8;
+        const int c = rm.qone ? (gi / KG) : (gi & 63), k8 = rm.qone ? (gi - (gi / KG) * KG) * 8 : (gi >> 6) * 8;
         float xv[8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) xv[e] = xr[8 * g + e];
@@ -262,7 +272,8 @@ __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __res
       if (wact) {
         // B fragments of step s + 1 are read from LDS before step s's MFMAs issue
         if constexpr (POL == 3) {
-          constexpr int NBB = CG == 2 ? 1 : 2;  // B fragments double-buffered
+          // B fragments double-buffered when the registers allow (not with six k-steps)
+          constexpr int NBB = (CG == 2 || NS > 4) ? 1 : 2;
           v8u16 bf[NBB][JB][3];
           auto bload = [&](int s, v8u16(&dst)[JB][3]) {
   #pragma unroll
@@ -356,7 +367,7 @@ __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __res
               const int m = orow(r);
               if (m < M) dst[m] = rm.accum ? dst[m] + acc[j][r] : acc[j][r];
             }
-          } else {
+          } else if (rm.o16) {
             const int m = orow(0);  // rows m .. m+3, M % 4 == 0
             if (m < M) {
               float4 o = float4{acc[j][0], acc[j][1], acc[j][2], acc[j][3]};
@@ -365,6 +376,21 @@ __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __res
                 o.x += d.x; o.y += d.y; o.z += d.z; o.w += d.w;
               }
               *(float4*)(dst + m) = o;
+            }
+          } else {
+            // even M (Mtot, moff even): two 8-byte pairs, each inside the column or skipped
+#pragma unroll
+            for (int h2 = 0; h2 < 2; ++h2) {
+              const int m = orow(2 * h2);
+              if (m < M) {
+                float2 o = float2{acc[j][2 * h2], acc[j][2 * h2 + 1]};
+                if (rm.accum) {
+                  const float2 d = *(const float2*)(dst + m);
+                  o.x += d.x;
+                  o.y += d.y;
+                }
+                *(float2*)(dst + m) = o;
+              }
             }
           }
         }
@@ -414,7 +440,7 @@ int launch_ra(const void* A, int M, int K, const void* X, void* Y, long long P, 
               int* rc, int Mtot, int moff, int lda, long long xps, int accum) {
   typedef typename RT<POL>::TX TX;
   constexpr int EPC = 16 / sizeof(TX);
-  if (M <= 0 || K <= 0 || M > RMAX || K > RMAX || P <= 0 || Q <= 0) return 0;
+  if (M <= 0 || K <= 0 || M > RMAX || K > (POL == 3 ? RMAX3 : RMAX) || P <= 0 || Q <= 0) return 0;
   if (((uintptr_t)X & 15) || ((uintptr_t)Y & 15)) return 0;
   RMap rm;
   rm.P = P;
@@ -431,6 +457,7 @@ int launch_ra(const void* A, int M, int K, const void* X, void* Y, long long P, 
   rm.xps = xps > 0 ? xps : (long long)K * Q;
   rm.accum = accum;
   rm.x16 = (K % 8 == 0 && rm.xps % 4 == 0) ? 1 : 0;
+  rm.o16 = (M % 4 == 0 && rm.Mtot % 4 == 0 && rm.moff % 4 == 0) ? 1 : 0;
   const bool ksplit = rm.lda != lda0 || rm.xps != (long long)K * Q || accum;
   if (ksplit && ((Q != 1 && Q % RTN) || rm.lda < lda0 || rm.xps < (long long)K * Q ||
                  (Q == 1 && rm.xps % (POL == 3 ? 8 : EPC))))
@@ -443,7 +470,10 @@ int launch_ra(const void* A, int M, int K, const void* X, void* Y, long long P, 
     // Dirichlet composite)
     // bf16x6 takes any even K: rows that are not 16-byte aligned or end inside
     // an 8-value group are read in 8-byte pairs
-    if ((POL == 3 ? K % 2 : K % EPC) || M % EPC || rm.Mtot % EPC || rm.moff % EPC) return 0;
+    // bf16x6 also writes any even M (8-byte output pairs)
+    if (POL == 3 ? (K % 2 || M % 2 || rm.Mtot % 2 || rm.moff % 2)
+                 : (K % EPC || M % EPC || rm.Mtot % EPC || rm.moff % EPC))
+      return 0;
     if (POL == 3 && rm.xps % 2) return 0;
     rm.qone = 1;
     rm.nqt = 0;
@@ -459,18 +489,28 @@ int launch_ra(const void* A, int M, int K, const void* X, void* Y, long long P, 
   // bf16x6: 16 waves in two column groups (more waves per SIMD to hide the
   // staging; 128 VGPRs fit); FP64/FP32 keep 8 waves (their operator rows and
   // B fragments need more registers than 16 waves allow)
-  constexpr int RA_CG = (POL == 3) ? 2 : 1;
+  // K > 128 (six k-steps): one column group, whose 8 waves have the
+  // registers for the longer operator rows
+  int cgs = (POL == 3) ? 2 : 1;
   KernT kern;
+  size_t shm;
   if constexpr (POL == 3) {
     const int kst = (K + 31) / 32;  // 32-deep k-steps needed
-    kern = kst <= 1 ? contract_ra_kernel<POL, 1, RA_CG> : kst <= 2 ? contract_ra_kernel<POL, 2, RA_CG>
-                                                           : contract_ra_kernel<POL, 4, RA_CG>;
+    if (kst > 4) {
+      cgs = 1;
+      kern = contract_ra_kernel<POL, 6, 1>;
+      shm = RLds<POL, 6>::TOTAL;
+    } else {
+      kern = kst <= 1 ? contract_ra_kernel<POL, 1, 2> : kst <= 2 ? contract_ra_kernel<POL, 2, 2>
+                                                        : contract_ra_kernel<POL, 4, 2>;
+      shm = RLds<POL, 4>::TOTAL;
+    }
   } else {
     const int kst = (K + 3) / 4;
-    kern = kst <= 8 ? contract_ra_kernel<POL, 8, RA_CG> : kst <= 16 ? contract_ra_kernel<POL, 16, RA_CG>
-                                                            : contract_ra_kernel<POL, 32, RA_CG>;
+    kern = kst <= 8 ? contract_ra_kernel<POL, 8, 1> : kst <= 16 ? contract_ra_kernel<POL, 16, 1>
+                                                    : contract_ra_kernel<POL, 32, 1>;
+    shm = RLds<POL, 8>::TOTAL;
   }
-  const size_t shm = RLds<POL>::TOTAL;
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
   if (e != hipSuccess) {
     set_error("contract_ra: hipFuncSetAttribute: %s", hipGetErrorString(e));
@@ -479,7 +519,7 @@ int launch_ra(const void* A, int M, int K, const void* X, void* Y, long long P, 
   }
   long long grid = ra_grid();
   if (grid > rm.ntiles) grid = rm.ntiles;
-  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(RNT * RA_CG), shm, st, A, (const TX*)X, (TX*)Y, rm);
+  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(RNT * cgs), shm, st, A, (const TX*)X, (TX*)Y, rm);
   e = hipGetLastError();
   if (e != hipSuccess) {
     set_error("contract_ra launch: %s", hipGetErrorString(e));
