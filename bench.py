@@ -570,17 +570,25 @@ def main():
         "speedup_vs_fp64_gpu": mixed["solves_per_s"] / fp64["solves_per_s"],
         "roofline": helm_roofline(mixed, kname),
     }
+    t0 = time.perf_counter()
+
+    def run(key, fn):
+        line[key] = fn()
+        # progress on stderr (stdout carries the one JSON line)
+        print(f"bench: {key} done at {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+
+    print("bench: ragged Helmholtz batch done", file=sys.stderr, flush=True)
     if not args.no_extras:
-        line["transform_3d_16x128^3"] = bench_transform_3d()
-        line["poisson_chebyshev_N64"] = bench_poisson_chebyshev()
-        line["lifted_helmholtz_legendre_M50"] = bench_lifted_helmholtz()
-        line["periodic_poisson_fourier20_legendre80"] = bench_periodic_poisson()
+        run("transform_3d_16x128^3", bench_transform_3d)
+        run("poisson_chebyshev_N64", bench_poisson_chebyshev)
+        run("lifted_helmholtz_legendre_M50", bench_lifted_helmholtz)
+        run("periodic_poisson_fourier20_legendre80", bench_periodic_poisson)
     if not (args.skip_burgers or args.no_extras):
-        line["burgers_imex_128x128"] = bench_burgers()
+        run("burgers_imex_128x128", bench_burgers)
     if not (args.skip_pcg or args.no_extras):
-        line["learned_pcg_varcoeff_64"] = bench_learned_pcg()
+        run("learned_pcg_varcoeff_64", bench_learned_pcg)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(data)
+        run("cpu_baseline", lambda: cpu_baseline(data))
     print(json.dumps(line), flush=True)
 
 
