@@ -121,6 +121,16 @@ int jfx_split_matrix_i8(const double* A, int M, int K, int nslices, signed char*
 int jfx_contract_i8(const signed char* Aslices, const int* Arowexp, int nslices, int M, int K, const double* X,
                     double* Y, long long P, long long Q, hipStream_t stream);
 
+/* Fused two-axis FP64 contraction of 2D slabs on int8 MFMA: Y[b] = A0 X[b] A1^T
+ * (X: B x K0 x K1, Y: B x M0 x M1, FP64, all dims <= 128, K1 even, X and Y
+ * 16-byte aligned), with the operators as jfx_split_matrix_i8 digit slices
+ * (nslices 5 or 6).  One HBM pass for the last two axes of a transform
+ * (galerkin/tensorproductspace.py:330-417) instead of two jfx_contract_i8
+ * calls; same error bound per contraction. */
+int jfx_plane_i8(const signed char* A0slices, const int* A0rowexp, const signed char* A1slices, const int* A1rowexp,
+                 int nslices, const double* X, double* Y, long long B, int M0, int K0, int M1, int K1,
+                 hipStream_t stream);
+
 /* Fused two-axis contraction of 2D slabs: Y[b] = alpha A0 X[b] A1^T (X: B x K0
  * x K1, Y: B x M0 x M1, all dims <= 128), optionally followed by Y /= E (epi 1)
  * or Y *= E (epi 2) with E (M0 x M1); accumulate = 1 adds the result to the

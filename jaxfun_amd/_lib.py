@@ -39,6 +39,7 @@ SIGNATURES: dict[str, tuple] = {
     "jfx_contract_bf16x": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_vp, c_ll, c_ll, c_vp]),
     "jfx_contract_ozaki": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_vp, c_vp, c_ll, c_ll, c_vp]),
     "jfx_split_matrix_i8": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_vp, c_vp]),
+    "jfx_plane_i8": (c_int, [c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_ll, c_int, c_int, c_int, c_int, c_vp]),
     "jfx_contract_i8": (c_int, [c_vp, c_vp, c_int, c_int, c_int, c_vp, c_vp, c_ll, c_ll, c_vp]),
     "jfx_band_lu_factor_wide": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_vp, c_vp]),
     "jfx_band_lu_solve_wide": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_ll, c_vp]),

@@ -447,6 +447,266 @@ static int launch_contract_i8(const signed char* Asl, const int* Aexp, int M, in
   return JFX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Fused two-axis FP64 contraction of 2D slabs on int8 MFMA:
+//     Y[b] = A0 X[b] A1^T,   X[b]: K0 x K1, Y[b]: M0 x M1 (all <= 128, K1 even),
+// one HBM read and one write per slab instead of two of each.  With the
+// digit scheme above, per slab:
+//   * the rows of X are staged in two halves of 64: each row is scaled by a
+//     power of two from its maximum and cut into S digit slices in LDS (one
+//     split per element, by the staging thread);
+//   * phase A (wave w = column block w of T = X A1^T, A1's digit fragment of
+//     rows 16w.. resident in registers for every slab) sums the levels of
+//     T's rows of the half in int32 and combines them in FP64; T's column
+//     j is then scaled per half by its maximum over the half's 64 rows
+//     (lane shuffles) and cut into digits, stored transposed (T^T[j][i]);
+//   * phase B (wave w = row block w of Y, A0's fragment read from L2) contracts
+//     over i = T's rows: one 64-deep k-step per half, each combined in FP64
+//     with its half's column scale, so the halves need no common scale.
+// LDS rows are 128 bytes with 16-byte chunks XOR-swizzled by the row's low
+// three bits (conflict-free 16-byte fragment reads without padding):
+// S X-half slices of 64 rows and S T^T slices of 128 rows; S <= 6 fits.
+// The next half (or slab) is loaded into registers during the current phase.
+constexpr int PI8_XS = 64 * I8KMAX, PI8_TS = 128 * I8KMAX;  // bytes per digit slice (X half, T^T)
+
+template <int S>
+struct PI8Lds {
+  static constexpr size_t X = (size_t)S * PI8_XS, T = (size_t)S * PI8_TS;
+  static constexpr size_t TOTAL = X + T + (64 + 2 * 128) * sizeof(int);
+};
+
+__device__ __forceinline__ int pi8_addr(int row, int k) {  // byte k of a 128-byte swizzled row
+  return row * I8KMAX + ((((k >> 4) ^ (row & 7))) << 4) + (k & 15);
+}
+
+// rows 64 h .. 64 h + 63 of one slab: thread q takes rows 64 h + q / 16 and
+// + 32, values 8 (q % 16) .. + 7 (pairs; K1 even, the slab 16-byte aligned);
+// zero outside the slab.
+__device__ __forceinline__ void pi8_load(const double* __restrict__ Xb, int K0, int K1, int h, double (&v)[2][8]) {
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int i = 64 * h + (threadIdx.x >> 4) + 32 * u, k0 = 8 * (threadIdx.x & 15);
+    const double* p = Xb + (long long)i * K1 + k0;
+#pragma unroll
+    for (int e = 0; e < 8; e += 2) {
+      const double2 d = (i < K0 && k0 + e < K1) ? *(const double2*)(p + e) : double2{0.0, 0.0};
+      v[u][e] = d.x;
+      v[u][e + 1] = d.y;
+    }
+  }
+}
+
+template <int S>
+__device__ __forceinline__ void pi8_store(signed char* Xd, int* xe, const double (&v)[2][8]) {
+  typedef int v2i __attribute__((ext_vector_type(2)));
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int il = (threadIdx.x >> 4) + 32 * u, kg = threadIdx.x & 15;
+    double mx = 0.0;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) mx = fmax(mx, fabs(v[u][e]));
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) mx = fmax(mx, __shfl_xor(mx, o));  // the row's 16 lanes
+    const int ex = mx > 0.0 ? ilogb(mx) + 1 : 0;
+    if (kg == 0) xe[il] = ex;
+    int d[8][S];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) i8_digits<S>(ldexp(v[u][e], 20 - ex), d[e]);
+    signed char* dst = Xd + pi8_addr(il, 8 * kg);
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+      *(v2i*)(dst + s * PI8_XS) = v2i{pack4(d[0][s], d[1][s], d[2][s], d[3][s]), pack4(d[4][s], d[5][s], d[6][s], d[7][s])};
+  }
+}
+
+// sum_l 2^(-7l) acc_l for element r (adjacent levels paired in int32; an
+// even S leaves a factor 2^7, taken out by the caller's exponent)
+template <int S>
+__device__ __forceinline__ double pi8_levels(const v4i (&acc)[S], int r) {
+  double y = 0.0;
+#pragma unroll
+  for (int l = S - 1; l >= 0; l -= 2) {
+    const int pr = l >= 1 ? (acc[l - 1][r] << 7) + acc[l][r] : acc[l][r];
+    y = fma(y, 0x1p-14, (double)pr);
+  }
+  return y;
+}
+
+template <int S>
+__global__ __launch_bounds__(I8NT) void plane_i8_kernel(const signed char* __restrict__ A0sl,
+                                                        const int* __restrict__ A0exp,
+                                                        const signed char* __restrict__ A1sl,
+                                                        const int* __restrict__ A1exp, const double* __restrict__ X,
+                                                        double* __restrict__ Y, long long B, int M0, int K0, int M1,
+                                                        int K1) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  signed char* Xd = reinterpret_cast<signed char*>(smem);  // [S][64 rows i][128 k]
+  signed char* Td = Xd + PI8Lds<S>::X;                      // [S][128 columns j][128 rows i]
+  int* xe = reinterpret_cast<int*>(Td + PI8Lds<S>::T);      // row exponents of the staged half
+  int* te = xe + 64;                                         // [half][column] exponents of T
+  const int lane = threadIdx.x & 63, l16 = lane & 15, kq = lane >> 4;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int Kp0 = (K0 + 63) / 64 * 64, Kp1 = (K1 + 63) / 64 * 64;
+  const int nh = Kp0 >> 6, nks1 = Kp1 >> 6;
+  const bool actA = 16 * w < M1, actB = 16 * w < M0;
+  constexpr int CORR = 12 + ((S & 1) ? 0 : 7);
+  const long long slab = (long long)K0 * K1;
+
+  // A1 digits of T's column block w (all k, resident); A0 digits of Y's row
+  // block w are read from L2 at each phase B (registers)
+  v4i b1[S][2];
+  int e1, e0[4];
+  const int mrow = 16 * w + l16;
+  {
+    const int m = mrow;
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+      for (int ks = 0; ks < 2; ++ks)
+        b1[s][ks] = (m < M1 && ks < nks1) ? *(const v4i*)(A1sl + ((long long)s * M1 + m) * Kp1 + 64 * ks + 16 * kq)
+                                          : v4i{0, 0, 0, 0};
+    e1 = m < M1 ? A1exp[m] : 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int mo = 16 * w + 4 * kq + r;
+      e0[r] = mo < M0 ? A0exp[mo] : 0;
+    }
+  }
+
+  long long b = blockIdx.x;
+  if (b >= B) return;
+  double v[2][8];
+  pi8_load(X + b * slab, K0, K1, 0, v);
+  pi8_store<S>(Xd, xe, v);
+  __syncthreads();
+  for (;;) {
+    for (int h = 0; h < nh; ++h) {
+      const bool nexth = h + 1 < nh;
+      if (nexth) pi8_load(X + b * slab, K0, K1, h + 1, v);
+      // ---- phase A: T rows 64 h .. 64 h + 63, column block w
+      if (actA) {
+        double tv[4][4];
+        double mx = 0.0;
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+          v4i acc[S];
+#pragma unroll
+          for (int l = 0; l < S; ++l) acc[l] = v4i{0, 0, 0, 0};
+#pragma unroll
+          for (int ks = 0; ks < 2; ++ks) {
+            if (ks < nks1) {
+              const signed char* xr = Xd + pi8_addr(16 * rb + l16, 64 * ks + 16 * kq);
+              v4i xd[S];
+#pragma unroll
+              for (int s = 0; s < S; ++s) xd[s] = *(const v4i*)(xr + s * PI8_XS);
+              // level l = operator digit s + data digit l - s
+#pragma unroll
+              for (int s = 0; s < S; ++s)
+#pragma unroll
+                for (int l = s; l < S; ++l)
+                  acc[l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(xd[l - s], b1[s][ks], acc[l], 0, 0, 0);
+            }
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int il = 16 * rb + 4 * kq + r;  // T row within the half (mfma16_row_f32)
+            tv[rb][r] = ldexp(pi8_levels<S>(acc, r), xe[il] + e1 - CORR);
+            mx = fmax(mx, fabs(tv[rb][r]));
+          }
+        }
+        // the column's maximum over the half: the four lanes holding column l16
+        mx = fmax(mx, __shfl_xor(mx, 16));
+        mx = fmax(mx, __shfl_xor(mx, 32));
+        const int et = mx > 0.0 ? ilogb(mx) + 1 : 0;
+        const int j = 16 * w + l16;
+        if (kq == 0) te[128 * h + j] = et;
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+          int d[4][S];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) i8_digits<S>(ldexp(tv[rb][r], 20 - et), d[r]);
+          signed char* dst = Td + pi8_addr(j, 64 * h + 16 * rb + 4 * kq);
+#pragma unroll
+          for (int s = 0; s < S; ++s) *(int*)(dst + s * PI8_TS) = pack4(d[0][s], d[1][s], d[2][s], d[3][s]);
+        }
+      }
+      __syncthreads();
+      if (nexth) {
+        pi8_store<S>(Xd, xe, v);
+        __syncthreads();
+      }
+    }
+    // ---- phase B: Y row block w; the next slab's first half loads meanwhile
+    const long long nb = b + gridDim.x;
+    const bool more = nb < B;
+    if (more) pi8_load(X + nb * slab, K0, K1, 0, v);
+    if (actB) {
+      v4i a0[S][2];
+#pragma unroll
+      for (int s = 0; s < S; ++s)
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks)
+          a0[s][ks] = (mrow < M0 && ks < nh)
+                          ? *(const v4i*)(A0sl + ((long long)s * M0 + mrow) * Kp0 + 64 * ks + 16 * kq)
+                          : v4i{0, 0, 0, 0};
+      double* Yb = Y + b * (long long)M0 * M1;
+      const int ncb = (M1 + 15) >> 4;
+#pragma unroll 1
+      for (int t = 0; t < ncb; ++t) {
+        const int j = 16 * t + l16;
+        double y[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+          if (ks < nh) {
+            const signed char* tr = Td + pi8_addr(j, 64 * ks + 16 * kq);
+            v4i td[S], acc[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) td[s] = *(const v4i*)(tr + s * PI8_TS);
+#pragma unroll
+            for (int l = 0; l < S; ++l) acc[l] = v4i{0, 0, 0, 0};
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+#pragma unroll
+              for (int l = s; l < S; ++l)
+                acc[l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a0[s][ks], td[l - s], acc[l], 0, 0, 0);
+            const int ej = te[128 * ks + j];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) y[r] += ldexp(pi8_levels<S>(acc, r), e0[r] + ej - CORR);
+          }
+        }
+        if (j < M1) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int mo = 16 * w + 4 * kq + r;
+            if (mo < M0) Yb[(long long)mo * M1 + j] = y[r];
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (!more) break;
+    pi8_store<S>(Xd, xe, v);
+    __syncthreads();
+    b = nb;
+  }
+}
+
+template <int S>
+static int launch_plane_i8(const signed char* A0sl, const int* A0exp, const signed char* A1sl, const int* A1exp,
+                           const double* X, double* Y, long long B, int M0, int K0, int M1, int K1, hipStream_t st) {
+  int dev = 0, cus = 0;
+  JFX_CHECK_HIP(hipGetDevice(&dev));
+  JFX_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const size_t shm = PI8Lds<S>::TOTAL;
+  JFX_CHECK_HIP(hipFuncSetAttribute((const void*)plane_i8_kernel<S>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)shm));
+  const long long nb = B < cus ? B : cus;
+  hipLaunchKernelGGL(plane_i8_kernel<S>, dim3((unsigned)nb), dim3(I8NT), shm, st, A0sl, A0exp, A1sl, A1exp, X, Y, B,
+                     M0, K0, M1, K1);
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
+}
+
 }  // namespace jfx
 
 using namespace jfx;
@@ -495,6 +755,26 @@ int jfx_contract_i8(const signed char* Aslices, const int* Arowexp, int nslices,
       return launch_contract_i8<7>(Aslices, Arowexp, M, K, X, Y, P, Q, stream);
     default:
       return set_error("contract_i8: nslices must be 5, 6 or 7"), JFX_EARG;
+  }
+}
+
+int jfx_plane_i8(const signed char* A0slices, const int* A0rowexp, const signed char* A1slices, const int* A1rowexp,
+                 int nslices, const double* X, double* Y, long long B, int M0, int K0, int M1, int K1,
+                 hipStream_t stream) {
+  if (!A0slices || !A0rowexp || !A1slices || !A1rowexp || !X || !Y || B < 0)
+    return set_error("plane_i8: bad arguments"), JFX_EARG;
+  if (M0 < 1 || M0 > I8KMAX || K0 < 1 || K0 > I8KMAX || M1 < 1 || M1 > I8KMAX || K1 < 1 || K1 > I8KMAX)
+    return set_error("plane_i8: dims must lie in [1, %d]", I8KMAX), JFX_ESHAPE;
+  if (K1 & 1) return set_error("plane_i8: K1 must be even"), JFX_ESHAPE;
+  if (((uintptr_t)X & 15) || ((uintptr_t)Y & 15)) return set_error("plane_i8: X and Y must be 16-byte aligned"), JFX_EARG;
+  if (B == 0) return JFX_OK;
+  switch (nslices) {
+    case 5:
+      return launch_plane_i8<5>(A0slices, A0rowexp, A1slices, A1rowexp, X, Y, B, M0, K0, M1, K1, stream);
+    case 6:
+      return launch_plane_i8<6>(A0slices, A0rowexp, A1slices, A1rowexp, X, Y, B, M0, K0, M1, K1, stream);
+    default:
+      return set_error("plane_i8: nslices must be 5 or 6 (the LDS tiles of 7 do not fit)"), JFX_EARG;
   }
 }
 

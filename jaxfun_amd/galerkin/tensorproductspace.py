@@ -100,8 +100,10 @@ class TensorProductSpace:
         nplanes = 1
         for n_ in X.shape[:-2]:
             nplanes *= n_
-        if d >= 2 and self.fused and not X.is_complex() and nplanes < K.num_cus() and \
-                K.plane_ok(ops[-2], ops[-1], prec):
+        # the int8 plane pass saves an FP64 HBM round trip at any batch size;
+        # the others pay off for batches of fewer slabs than CUs
+        if d >= 2 and self.fused and not X.is_complex() and \
+                (nplanes < K.num_cus() or prec.startswith("i8x")) and K.plane_ok(ops[-2], ops[-1], prec):
             # leading axes one by one, the last two in one fused pass
             for ax, op in enumerate(ops[:-2]):
                 X = K.apply_along(op, X, nb + ax, prec)

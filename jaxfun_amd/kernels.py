@@ -465,7 +465,7 @@ def apply_along(op: AxisOperator, X: torch.Tensor, axis: int, prec: str = "f64",
     return Y
 
 
-PLANE_PRECISIONS = ("f64", "bf16x6")
+PLANE_PRECISIONS = ("f64", "bf16x6", "i8x5", "i8x6")
 
 
 _NUM_CUS = None
@@ -482,6 +482,8 @@ def num_cus() -> int:
 def plane_ok(op0: AxisOperator, op1: AxisOperator, prec: str) -> bool:
     if getattr(op0, "is_complex", False) or getattr(op1, "is_complex", False):
         return False
+    if prec.startswith("i8x") and op1.K % 2:
+        return False  # the int8 plane kernel reads rows in 16-byte pairs
     return prec in PLANE_PRECISIONS and max(op0.M, op0.K, op1.M, op1.K) <= 128
 
 
@@ -549,6 +551,15 @@ def plane(op0: AxisOperator, op1: AxisOperator, X: torch.Tensor, prec: str = "f6
         if tuple(E.shape) != (op0.M, op1.M):
             raise ValueError("plane: epilogue shape mismatch")
     acc = 1 if accumulate else 0
+    if prec.startswith("i8x"):
+        if mode or accumulate or alpha != 1.0:
+            raise ValueError("plane: the int8 path has no epilogue, scale or accumulation")
+        if not plane_ok(op0, op1, prec):
+            raise ValueError(f"plane: precision {prec} needs dims <= 128 and an even last axis")
+        p0, p1 = op0.prepared(prec), op1.prepared(prec)
+        _lib.call("jfx_plane_i8", _p(p0[0]), _p(p0[1]), _p(p1[0]), _p(p1[1]), p0[2], _p(X), _p(Y), B, op0.M, op0.K,
+                  op1.M, op1.K, _s())
+        return Y
     if prec == "f64":
         _lib.call("jfx_plane_contract", 0, _p(op0.A64), _p(op1.A64), _p(None), _p(None), _p(X), _p(Y), _p(E),
                   mode, float(alpha), acc, B, op0.M, op0.K, op1.M, op1.K, _s())

@@ -98,6 +98,42 @@ def test_contract_i8_rejects_unsupported(dev):
         K.apply_along(K.AxisOperator(_t(np.ones((64, 63)), dev)), _t(np.ones((5, 63)), dev), 1, "i8x6")
 
 
+@pytest.mark.parametrize("S", [5, 6])
+@pytest.mark.parametrize("dims,B", [((128, 128, 128, 128), 3), ((128, 128, 128, 128), 600), ((100, 70, 126, 64), 5),
+                                    ((33, 64, 48, 2), 7), ((126, 126, 126, 126), 257)])
+def test_plane_i8_vs_fp64(dev, S, dims, B):
+    # Y = A0 X A1^T in one pass (last two axes), against FP64 and against
+    # two per-axis int8 contractions
+    from jaxfun_amd import kernels as K
+
+    M0, K0, M1, K1 = dims
+    rng = np.random.default_rng(M0 + K0 + M1 + K1 + B)
+    A0, A1 = rng.standard_normal((M0, K0)), rng.standard_normal((M1, K1))
+    X = rng.standard_normal((B, K0, K1)) * np.logspace(0, -3, K1)[None, None, :]
+    ops = K.AxisOperator(_t(A0, dev)), K.AxisOperator(_t(A1, dev))
+    assert K.plane_ok(*ops, f"i8x{S}")
+    Y = K.plane(*ops, _t(X, dev), f"i8x{S}").cpu().numpy()
+    Yr = A0 @ X @ A1.T
+    err = O.rel_l2(Y, Yr)
+    assert err < {5: 1e-8, 6: 1e-10}[S], err
+    if K1 % 64 == 0:  # the per-axis int8 path takes a trailing size of 1 or a multiple of 64
+        Y2 = K.apply_along(ops[1], K.apply_along(ops[0], _t(X, dev), 1, f"i8x{S}"), 2, f"i8x{S}").cpu().numpy()
+        assert O.rel_l2(Y, Y2) < {5: 3e-8, 6: 3e-10}[S]
+
+
+def test_plane_i8_rejects(dev):
+    from jaxfun_amd import kernels as K
+
+    ops = K.AxisOperator(_t(np.ones((8, 8)), dev)), K.AxisOperator(_t(np.ones((8, 7)), dev))
+    assert not K.plane_ok(*ops, "i8x5")  # odd last axis
+    with pytest.raises(ValueError):
+        K.plane(*ops, _t(np.ones((2, 8, 7)), dev), "i8x5")
+    ops = K.AxisOperator(_t(np.ones((8, 8)), dev)), K.AxisOperator(_t(np.ones((8, 8)), dev))
+    assert not K.plane_ok(*ops, "i8x7")
+    with pytest.raises(ValueError):
+        K.plane(*ops, _t(np.ones((2, 8, 8)), dev), "i8x5", alpha=2.0)
+
+
 def test_legendre3d_16x128_roundtrip_i8x6(dev):
     # the batched 3D Legendre workload (seed 1) at its own size: backward vs
     # the committed oracle samples, round trip vs the input, both <= 1e-6
