@@ -304,6 +304,48 @@ def bench_periodic_poisson(reps: int = 50, batch: int = 1024):
                 **{f"batched_{batch}_solves_per_s": batch / bt})
 
 
+def bench_band_solve(reps: int = 20):
+    """DiaMatrix.solve on few long lines (la/diamatrix.py:592-770, LUFactors.solve
+    :1880-1925): the Legendre-Dirichlet mass matrix (offsets 0, +-2) of size n,
+    factored once, solved for `lines` right-hand sides with one wavefront per
+    line (eight for long lines; the rows split over the lanes, chunk maps
+    composed by a wave scan) and, for comparison, with one lane per line (a
+    sequential sweep).  Times are HIP-event means over back-to-back launches;
+    the error is the relative L2 difference of the two paths."""
+    import torch
+
+    from jaxfun_amd import kernels as K
+
+    out = {}
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    for n, lines in ((1024, 1), (20000, 1), (20000, 16), (100000, 1)):
+        k = np.arange(n, dtype=np.float64)
+        band = np.zeros((n, 5))
+        band[:, 2] = 2.0 / (2 * k + 1) + 2.0 / (2 * k + 5)
+        band[2:, 0] = -2.0 / (2 * k[:-2] + 5)
+        band[:-2, 4] = -2.0 / (2 * k[:-2] + 5)
+        lu = K.band_lu_factor(torch.tensor(band, device="cuda"), 2, 2)
+        b = torch.randn(n, lines, dtype=torch.float64, device="cuda", generator=gen)
+        r = {}
+        sol = {}
+        for m in ("wave", "lanes"):
+            K.band_lu_solve(lu, 2, 2, b.clone(), 0, method=m)  # warm-up
+            Xs = [b.clone() for _ in range(reps)]  # in-place solves: one copy per launch
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for X in Xs:
+                K.band_lu_solve(lu, 2, 2, X, 0, method=m)
+            e1.record()
+            torch.cuda.synchronize()
+            r[f"{m}_us"] = e0.elapsed_time(e1) / reps * 1e3
+            sol[m] = Xs[0]
+        r["speedup_wave_vs_lanes"] = r["lanes_us"] / r["wave_us"]
+        r["rel_diff_wave_vs_lanes"] = float(torch.linalg.norm(sol["wave"] - sol["lanes"]) / torch.linalg.norm(sol["lanes"]))
+        out[f"n{n}_lines{lines}"] = r
+    return out
+
+
 def bench_burgers(steps: int = 1000, batch: int = 256, N: int = 128):
     """IMEX ARS443 Burgers, Chebyshev-Dirichlet N x N, 3/2 padding,
     mixed precision vs FP64, drift after `steps` steps."""
@@ -583,6 +625,7 @@ def main():
         run("poisson_chebyshev_N64", bench_poisson_chebyshev)
         run("lifted_helmholtz_legendre_M50", bench_lifted_helmholtz)
         run("periodic_poisson_fourier20_legendre80", bench_periodic_poisson)
+        run("band_solve_legendre_mass", bench_band_solve)
     if not (args.skip_burgers or args.no_extras):
         run("burgers_imex_128x128", bench_burgers)
     if not (args.skip_pcg or args.no_extras):

@@ -167,6 +167,17 @@ int jfx_band_lu_factor(const double* band, int n, int p, int q, double* lu, int*
  * _forward_elimination/_backward_substitution (la/diamatrix.py:2079-2195). */
 int jfx_band_lu_solve(const double* lu, int n, int p, int q, double* X, long long P, long long Q,
                       hipStream_t stream);
+/* As jfx_band_lu_solve with the kernel chosen by mode: 1 = one lane per line
+ * (sequential sweep), 2 = wavefronts per line (the rows split into chunks over
+ * 64 lanes, or 512 lanes for at most 256 lines of n >= 4096; each chunk's
+ * affine map of the recurrence window is composed by a wave scan; p, q <= 4,
+ * else JFX_EARG), 0 = automatic (mode 2 for p, q <= 4, n >= 32 and at most
+ * 2048 lines, mode 1 otherwise; jfx_band_lu_solve is mode 0).  Replaces the
+ * same LUFactors.solve (la/diamatrix.py:1880-1925, 2079-2195) for a single
+ * system or a few lines: DiaMatrix.solve on a vector
+ * (la/diamatrix.py:592-770). */
+int jfx_band_lu_solve_mode(const double* lu, int n, int p, int q, double* X, long long P, long long Q, int mode,
+                           hipStream_t stream);
 
 /* nsys independent band matrices, band (nsys, n, p+q+1) "by row"; status_dev
  * receives 0 or 1 + the row of a zero / non-finite pivot in a failing system.

@@ -30,107 +30,191 @@ __device__ __forceinline__ int band_lu_factor_one(const double* __restrict__ ban
   return (piv == 0.0 || !isfinite(piv)) ? n : 0;
 }
 
-// One small banded matrix (setup).
-__global__ void band_lu_factor_kernel(const double* __restrict__ band, int n, int p, int q,
-                                      double* __restrict__ lu, int* __restrict__ status) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  *status = band_lu_factor_one(band, n, p, q, lu);
+// The same factorisation for bandwidths p, q <= B with the active part of the
+// elimination held in registers: at step k, w[r][c] = A_k[k + r, k + c] for
+// r, c in [0, B] (rows below k + p and columns beyond k + q are zero and stay
+// zero).  Each step divides out column 0, updates the window, stores row k of
+// U and the multipliers of column k, and slides the window by one row and
+// column; the entries entering it (column k + 1 + B of rows k + 1 .. k + B and
+// row k + 1 + B) have not been touched by any earlier step, so they are read
+// from the input.  Those reads are issued one block of FD steps ahead, so the
+// loop-carried chain is register arithmetic only (the plain version above
+// round-trips every update through memory).  The arithmetic and its order per
+// entry are the plain version's.
+__device__ __forceinline__ double band_entry(const double* __restrict__ band, int n, int p, int q, int W, int i,
+                                             int j) {
+  // the load is unconditional (an in-bounds address when the entry is outside
+  // the band) so that the memory-counter waits stay exact across the loop
+  const int o = j - i;
+  const bool in = i < n && j < n && o >= -p && o <= q;
+  const double v = band[in ? (long long)i * W + o + p : 0];
+  return in ? v : 0.0;
+}
+
+template <int B>
+__device__ int band_lu_factor_win(const double* __restrict__ band, int n, int p, int q, double* __restrict__ lu) {
+  constexpr int FD = B == 2 ? 8 : 4;  // steps per prefetch block (register budget)
+  constexpr int NE = 2 * B + 1;       // entries entering the window per step
+  const int W = p + q + 1;
+  // band-storage slots outside the matrix (first p rows, last q rows) are copied
+  for (int i = 0; i < min(n, p); ++i)
+    for (int o = -p; o < -i; ++o) lu[(long long)i * W + o + p] = band[(long long)i * W + o + p];
+  for (int i = max(0, n - q); i < n; ++i)
+    for (int o = n - i; o <= q; ++o) lu[(long long)i * W + o + p] = band[(long long)i * W + o + p];
+  double w[B + 1][B + 1];
+#pragma unroll
+  for (int r = 0; r <= B; ++r)
+#pragma unroll
+    for (int c = 0; c <= B; ++c) w[r][c] = band_entry(band, n, p, q, W, r, c);
+  // e[d]: what the slide after step k0 + d brings in: column k + 1 + B of rows
+  // k + 1 .. k + B (e[d][0 .. B-1]), then row k + 1 + B (e[d][B .. 2B])
+  double e[FD][NE];
+#pragma unroll
+  for (int d = 0; d < FD; ++d) {
+    const int k = d;
+#pragma unroll
+    for (int r = 0; r < B; ++r) e[d][r] = band_entry(band, n, p, q, W, k + 1 + r, k + 1 + B);
+#pragma unroll
+    for (int c = 0; c <= B; ++c) e[d][B + c] = band_entry(band, n, p, q, W, k + 1 + B, k + 1 + c);
+  }
+  for (int k0 = 0; k0 < n; k0 += FD) {
+    double en[FD][NE];
+#pragma unroll
+    for (int d = 0; d < FD; ++d) {
+      const int k = k0 + FD + d;
+#pragma unroll
+      for (int r = 0; r < B; ++r) en[d][r] = band_entry(band, n, p, q, W, k + 1 + r, k + 1 + B);
+#pragma unroll
+      for (int c = 0; c <= B; ++c) en[d][B + c] = band_entry(band, n, p, q, W, k + 1 + B, k + 1 + c);
+    }
+#pragma unroll
+    for (int d = 0; d < FD; ++d) {
+      const int k = k0 + d;
+      if (k >= n) break;
+      const double piv = w[0][0];
+      if (piv == 0.0 || !isfinite(piv)) return k + 1;
+      // Every store is unconditional: one that has no slot (a multiplier row
+      // below the band or past n, a U entry past the band or past n) goes to
+      // row k's pivot slot, which the last store of the step (c = 0)
+      // overwrites with the pivot.
+      const long long kp = (long long)k * W + p;
+#pragma unroll
+      for (int r = 1; r <= B; ++r) {
+        const double l = w[r][0] / piv;
+        lu[(r <= p && k + r < n) ? (long long)(k + r) * W + p - r : kp] = l;
+#pragma unroll
+        for (int c = 1; c <= B; ++c) w[r][c] -= l * w[0][c];
+      }
+#pragma unroll
+      for (int c = B; c >= 0; --c) lu[(c <= q && k + c < n) ? kp + c : kp] = w[0][c];
+#pragma unroll
+      for (int r = 0; r < B; ++r) {
+#pragma unroll
+        for (int c = 0; c < B; ++c) w[r][c] = w[r + 1][c + 1];
+        w[r][B] = e[d][r];
+      }
+#pragma unroll
+      for (int c = 0; c <= B; ++c) w[B][c] = e[d][B + c];
+    }
+#pragma unroll
+    for (int d = 0; d < FD; ++d)
+#pragma unroll
+      for (int t = 0; t < NE; ++t) e[d][t] = en[d][t];
+  }
+  return 0;
+}
+
+__device__ __forceinline__ int band_lu_factor_any(const double* __restrict__ band, int n, int p, int q,
+                                                  double* __restrict__ lu) {
+  if (p <= 2 && q <= 2) return band_lu_factor_win<2>(band, n, p, q, lu);
+  if (p <= 4 && q <= 4) return band_lu_factor_win<4>(band, n, p, q, lu);
+  return band_lu_factor_one(band, n, p, q, lu);
+}
+
+// One banded matrix, one lane.  The lane index enters the addresses (it is
+// zero) so that the factor's loads stay vector loads, whose completion
+// counter is in order: a uniform address would become a scalar load, and the
+// scalar counter has to be drained completely before each use.
+__global__ __launch_bounds__(64) void band_lu_factor_kernel(const double* __restrict__ band, int n, int p, int q,
+                                      double* __restrict__ lu, int* __restrict__ status, int nlanes) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nlanes) return;
+  const long long off = (long long)t * n * (p + q + 1);
+  *status = band_lu_factor_any(band + off, n, p, q, lu + off);
 }
 
 // nsys independent banded matrices (band (nsys, n, W)), one lane each; the
 // first failure's 1 + row lands in *status.
-__global__ void band_lu_factor_batched_kernel(const double* __restrict__ band, int nsys, int n, int p, int q,
+__global__ __launch_bounds__(64) void band_lu_factor_batched_kernel(const double* __restrict__ band, int nsys, int n, int p, int q,
                                               double* __restrict__ lu, int* __restrict__ status) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nsys) return;
   const long long off = (long long)s * n * (p + q + 1);
-  const int st = band_lu_factor_one(band + off, n, p, q, lu + off);
+  const int st = band_lu_factor_any(band + off, n, p, q, lu + off);
   if (st) atomicCAS(status, 0, st);
 }
 
 // Forward / back substitution of one line x (stride Q) with one factor.
-// For bandwidths <= RW the last solved values stay in a register window, so
-// the recurrence carries no memory round trip (the right-hand side and factor
-// loads do not depend on it and are issued ahead).
-constexpr int RW = 8;
-__device__ __forceinline__ void band_lu_solve_one(const double* __restrict__ lu, int n, int p, int q, double* __restrict__ x,
-                                  long long Q) {
-  const int W = p + q + 1;
-  if (p <= RW && q <= RW) {
-    // rows go in chunks of RC: the chunk's right-hand sides and factor rows
-    // are loaded together (one memory latency per chunk), then the chunk's
-    // recurrence runs from registers
-    constexpr int RC = 8;
-    double w[RW];
+// For bandwidths <= RWT the last solved values stay in a register window, so
+// the recurrence carries no memory round trip.  Rows go in batches of RC: a
+// batch's right-hand sides and factor entries are loaded together, from
+// clamped in-bounds rows (masked afterwards) so that every load is
+// unconditional, and the next batch is loaded before the current one is
+// solved, so one memory latency is spread over RC rows and overlapped.
+template <int RWT, int RC>
+struct LaneRows {
+  double xv[RC], dg[RC], cf[RC][RWT];
+};
+
+template <int RWT, int RC, bool FWD>
+__device__ __forceinline__ void lane_load(const double* __restrict__ lu, const double* __restrict__ x, long long Q,
+                                          int n, int W, int p, int q, int i0, LaneRows<RWT, RC>& r) {
 #pragma unroll
-    for (int t = 0; t < RW; ++t) w[t] = 0.0;
-    for (int i0 = 0; i0 < n; i0 += RC) {
-      double xv[RC], cf[RC][RW];
+  for (int c = 0; c < RC; ++c) {
+    const int i = FWD ? min(i0 + c, n - 1) : max(i0 - c, 0);
+    const double* row = lu + (long long)i * W;
+    r.xv[c] = x[(long long)i * Q];
+    r.dg[c] = FWD ? 1.0 : row[p];
 #pragma unroll
-      for (int c = 0; c < RC; ++c) {
-        const int i = i0 + c;
-        xv[c] = 0.0;
-#pragma unroll
-        for (int t = 0; t < RW; ++t) cf[c][t] = 0.0;
-        if (i < n) {
-          const double* row = lu + (long long)i * W;
-          xv[c] = x[(long long)i * Q];
-#pragma unroll
-          for (int t = 0; t < RW; ++t)
-            if (t < p && i - 1 - t >= 0) cf[c][t] = row[p - 1 - t];
-        }
-      }
-#pragma unroll
-      for (int c = 0; c < RC; ++c) {
-        const int i = i0 + c;
-        if (i < n) {
-          double s = xv[c];
-#pragma unroll
-          for (int t = 0; t < RW; ++t) s -= cf[c][t] * w[t];
-#pragma unroll
-          for (int t = RW - 1; t > 0; --t) w[t] = w[t - 1];
-          w[0] = s;
-          x[(long long)i * Q] = s;
-        }
-      }
+    for (int t = 0; t < RWT; ++t) {
+      const bool in = FWD ? (t < p && i - 1 - t >= 0) : (t < q && i + 1 + t < n);
+      const double v = row[in ? (FWD ? p - 1 - t : p + 1 + t) : 0];
+      r.cf[c][t] = in ? v : 0.0;
     }
-#pragma unroll
-    for (int t = 0; t < RW; ++t) w[t] = 0.0;
-    for (int i0 = n - 1; i0 >= 0; i0 -= RC) {
-      double xv[RC], dg[RC], cf[RC][RW];
-#pragma unroll
-      for (int c = 0; c < RC; ++c) {
-        const int i = i0 - c;
-        xv[c] = 0.0;
-        dg[c] = 1.0;
-#pragma unroll
-        for (int t = 0; t < RW; ++t) cf[c][t] = 0.0;
-        if (i >= 0) {
-          const double* row = lu + (long long)i * W;
-          xv[c] = x[(long long)i * Q];
-          dg[c] = row[p];
-#pragma unroll
-          for (int t = 0; t < RW; ++t)
-            if (t < q && i + 1 + t < n) cf[c][t] = row[p + 1 + t];
-        }
-      }
-#pragma unroll
-      for (int c = 0; c < RC; ++c) {
-        const int i = i0 - c;
-        if (i >= 0) {
-          double s = xv[c];
-#pragma unroll
-          for (int t = 0; t < RW; ++t) s -= cf[c][t] * w[t];
-          s /= dg[c];
-#pragma unroll
-          for (int t = RW - 1; t > 0; --t) w[t] = w[t - 1];
-          w[0] = s;
-          x[(long long)i * Q] = s;
-        }
-      }
-    }
-    return;
   }
+}
+
+template <int RWT, int RC, bool FWD>
+__device__ __forceinline__ void lane_sweep(const double* __restrict__ lu, int n, int p, int q, double* __restrict__ x,
+                                           long long Q) {
+  const int W = p + q + 1;
+  double w[RWT];
+#pragma unroll
+  for (int t = 0; t < RWT; ++t) w[t] = 0.0;
+  LaneRows<RWT, RC> cur, nxt;
+  lane_load<RWT, RC, FWD>(lu, x, Q, n, W, p, q, FWD ? 0 : n - 1, cur);
+  for (int b = 0; b < n; b += RC) {
+    lane_load<RWT, RC, FWD>(lu, x, Q, n, W, p, q, FWD ? b + RC : n - 1 - b - RC, nxt);
+#pragma unroll
+    for (int c = 0; c < RC; ++c) {
+      if (b + c >= n) break;
+      double s = cur.xv[c];
+#pragma unroll
+      for (int t = 0; t < RWT; ++t) s -= cur.cf[c][t] * w[t];
+      if (!FWD) s /= cur.dg[c];
+#pragma unroll
+      for (int t = RWT - 1; t > 0; --t) w[t] = w[t - 1];
+      w[0] = s;
+      x[(long long)(FWD ? b + c : n - 1 - b - c) * Q] = s;
+    }
+    cur = nxt;
+  }
+}
+
+// Any bandwidth: the plain loops.
+__device__ __forceinline__ void band_lu_solve_general(const double* __restrict__ lu, int n, int p, int q,
+                                                      double* __restrict__ x, long long Q) {
+  const int W = p + q + 1;
   for (int i = 1; i < n; ++i) {
     double s = x[(long long)i * Q];
     int j0 = max(0, i - p);
@@ -145,6 +229,22 @@ __device__ __forceinline__ void band_lu_solve_one(const double* __restrict__ lu,
   }
 }
 
+__device__ __forceinline__ void band_lu_solve_one(const double* __restrict__ lu, int n, int p, int q,
+                                                  double* __restrict__ x, long long Q) {
+  if (p <= 2 && q <= 2) {
+    lane_sweep<2, 8, true>(lu, n, p, q, x, Q);
+    lane_sweep<2, 8, false>(lu, n, p, q, x, Q);
+  } else if (p <= 4 && q <= 4) {
+    lane_sweep<4, 8, true>(lu, n, p, q, x, Q);
+    lane_sweep<4, 8, false>(lu, n, p, q, x, Q);
+  } else if (p <= 8 && q <= 8) {
+    lane_sweep<8, 4, true>(lu, n, p, q, x, Q);
+    lane_sweep<8, 4, false>(lu, n, p, q, x, Q);
+  } else {
+    band_lu_solve_general(lu, n, p, q, x, Q);
+  }
+}
+
 // Solve L U x = b in place along axis of length n of a (P, n, Q) batch; one
 // thread per line (coalesced over q).
 __global__ __launch_bounds__(256) void band_lu_solve_kernel(const double* __restrict__ lu, int n, int p, int q,
@@ -153,6 +253,250 @@ __global__ __launch_bounds__(256) void band_lu_solve_kernel(const double* __rest
   if (line >= P * Q) return;
   long long pp = line / Q, qq = line - pp * Q;
   band_lu_solve_one(lu, n, p, q, X + pp * (long long)n * Q + qq, Q);
+}
+
+// ---- wavefronts per line: partitioned substitution
+// The 64 * NW lanes of a workgroup split the n rows of one line into
+// contiguous chunks of c = ceil(n / (64 NW)).  Forward substitution with the
+// unit lower factor and back substitution with the upper factor are linear
+// recurrences over a window of the last B solved values (bandwidths p, q <=
+// B), so a chunk maps the window entering it to the window leaving it by an
+// affine map w -> M w + v.  Pass 1: each lane builds its chunk's map (v from a
+// zero window, column j of M from the unit window e_j).  A Kogge-Stone scan
+// over the wave composes the maps (six shuffle steps); with several waves, the
+// waves' total maps are chained through LDS.  That gives every lane the window
+// entering its chunk; pass 2 re-runs the chunk from it and stores the
+// solution.  The work is about (B + 2) times the sequential sweep's, spread
+// over 64 NW lanes; a lane whose chunk is empty carries the identity map.
+template <int B>
+__device__ __forceinline__ void wave_push(double (&w)[B], double s) {
+#pragma unroll
+  for (int t = B - 1; t > 0; --t) w[t] = w[t - 1];
+  w[0] = s;
+}
+
+template <int B, bool UP>
+__device__ __forceinline__ void shfl_map(const double (&M)[B][B], const double (&v)[B], int d, double (&Mp)[B][B],
+                                         double (&vp)[B]) {
+#pragma unroll
+  for (int r = 0; r < B; ++r) {
+    vp[r] = UP ? __shfl_up(v[r], d) : __shfl_down(v[r], d);
+#pragma unroll
+    for (int k = 0; k < B; ++k) Mp[r][k] = UP ? __shfl_up(M[r][k], d) : __shfl_down(M[r][k], d);
+  }
+}
+
+// (M, v) <- (M, v) o (Mp, vp) = (M Mp, M vp + v): the chunk of (Mp, vp) is
+// traversed first.
+template <int B>
+__device__ __forceinline__ void compose_map(double (&M)[B][B], double (&v)[B], const double (&Mp)[B][B],
+                                            const double (&vp)[B]) {
+  double nv[B], nM[B][B];
+#pragma unroll
+  for (int r = 0; r < B; ++r) {
+    double s = v[r];
+#pragma unroll
+    for (int k = 0; k < B; ++k) s += M[r][k] * vp[k];
+    nv[r] = s;
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      double a = 0.0;
+#pragma unroll
+      for (int m = 0; m < B; ++m) a += M[r][m] * Mp[m][k];
+      nM[r][k] = a;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < B; ++r) {
+    v[r] = nv[r];
+#pragma unroll
+    for (int k = 0; k < B; ++k) M[r][k] = nM[r][k];
+  }
+}
+
+// Window entering this lane's chunk.  UP: lanes (and waves) are traversed in
+// ascending order, else descending.  sm / ss: NW * (B * B + B) and NW * B
+// doubles of LDS (unused for NW = 1).
+template <int B, bool UP, int NW>
+__device__ __forceinline__ void line_scan(double (&M)[B][B], double (&v)[B], int lane, int wv, double* sm, double* ss,
+                                          double (&win)[B]) {
+  // inclusive scan within the wave
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    double Mp[B][B], vp[B];
+    shfl_map<B, UP>(M, v, d, Mp, vp);
+    if (UP ? lane >= d : lane + d < 64) compose_map<B>(M, v, Mp, vp);
+  }
+  // exclusive: the previous lane's inclusive map (identity at the first lane)
+  double Me[B][B], ve[B];
+  shfl_map<B, UP>(M, v, 1, Me, ve);
+  const bool first = UP ? lane == 0 : lane == 63;
+  if (first) {
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+      ve[r] = 0.0;
+#pragma unroll
+      for (int k = 0; k < B; ++k) Me[r][k] = r == k ? 1.0 : 0.0;
+    }
+  }
+  double S[B];
+#pragma unroll
+  for (int r = 0; r < B; ++r) S[r] = 0.0;
+  if constexpr (NW > 1) {
+    // the wave's last lane in traversal order holds the wave's total map
+    if (UP ? lane == 63 : lane == 0) {
+#pragma unroll
+      for (int r = 0; r < B; ++r) {
+        sm[wv * (B * B + B) + B * B + r] = v[r];
+#pragma unroll
+        for (int k = 0; k < B; ++k) sm[wv * (B * B + B) + r * B + k] = M[r][k];
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double st[B];
+#pragma unroll
+      for (int r = 0; r < B; ++r) st[r] = 0.0;
+      for (int t = 0; t < NW; ++t) {
+        const int k = UP ? t : NW - 1 - t;
+        const double* m = sm + k * (B * B + B);
+        double nt[B];
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+          ss[k * B + r] = st[r];
+          double a = m[B * B + r];
+#pragma unroll
+          for (int c = 0; c < B; ++c) a += m[r * B + c] * st[c];
+          nt[r] = a;
+        }
+#pragma unroll
+        for (int r = 0; r < B; ++r) st[r] = nt[r];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < B; ++r) S[r] = ss[wv * B + r];
+  }
+#pragma unroll
+  for (int r = 0; r < B; ++r) {
+    double a = ve[r];
+#pragma unroll
+    for (int k = 0; k < B; ++k) a += Me[r][k] * S[k];
+    win[r] = a;
+  }
+}
+
+// A batch of RC consecutive rows of one lane's chunk (in traversal order:
+// ascending for the forward pass, descending for the backward pass) with their
+// right-hand sides and factor entries.  Every load is issued unconditionally
+// from an in-bounds address (rows past the chunk read a clamped row and are
+// masked afterwards), so the memory-counter waits stay exact and a batch
+// prefetched one step ahead overlaps the current batch.
+template <int B, int RC>
+struct WaveRows {
+  double xv[RC], cf[RC][B], dg[RC];
+  bool ok[RC];
+};
+
+template <int B, int RC, bool FWD>
+__device__ __forceinline__ void wave_load(const double* __restrict__ lu, const double* __restrict__ x, long long Q,
+                                          int n, int W, int p, int q, int i0, int i1, int ib, WaveRows<B, RC>& r) {
+#pragma unroll
+  for (int u = 0; u < RC; ++u) {
+    const int i = FWD ? i0 + ib + u : i1 - 1 - ib - u;
+    const bool ok = FWD ? i < i1 : i >= i0;
+    const int ic = min(max(i, 0), n - 1);
+    const double* row = lu + (long long)ic * W;
+    r.ok[u] = ok;
+    r.xv[u] = x[(long long)ic * Q];
+#pragma unroll
+    for (int t = 0; t < B; ++t) {
+      const bool in = FWD ? (t < p && ic - 1 - t >= 0) : (t < q && ic + 1 + t < n);
+      const double v = row[in ? (FWD ? p - 1 - t : p + 1 + t) : 0];
+      r.cf[u][t] = in ? v : 0.0;
+    }
+    r.dg[u] = FWD ? 1.0 : row[p];
+  }
+}
+
+template <int B>
+__device__ __forceinline__ double wave_step(const double (&cf)[B], const double (&w)[B], double rhs, double dg,
+                                            bool FWD) {
+  double s = rhs;
+#pragma unroll
+  for (int t = 0; t < B; ++t) s -= cf[t] * w[t];
+  return FWD ? s : s / dg;
+}
+
+// One direction of the partitioned substitution on this lane's chunk
+// [i0, i1): pass 1 (chunk map), scan, pass 2 (solution stored to x).
+template <int B, int RC, int NW, bool FWD>
+__device__ __forceinline__ void wave_sweep(const double* __restrict__ lu, double* __restrict__ x, long long Q, int n,
+                                           int W, int p, int q, int i0, int i1, int lane, int wv, double* sm,
+                                           double* ss) {
+  const int len = i1 - i0;
+  double v[B], h[B][B], M[B][B], win[B];
+#pragma unroll
+  for (int t = 0; t < B; ++t) {
+    v[t] = 0.0;
+#pragma unroll
+    for (int j = 0; j < B; ++j) h[j][t] = j == t ? 1.0 : 0.0;
+  }
+  WaveRows<B, RC> cur, nxt;
+  wave_load<B, RC, FWD>(lu, x, Q, n, W, p, q, i0, i1, 0, cur);
+  for (int ib = 0; ib < len; ib += RC) {
+    wave_load<B, RC, FWD>(lu, x, Q, n, W, p, q, i0, i1, ib + RC, nxt);
+#pragma unroll
+    for (int u = 0; u < RC; ++u) {
+      if (!cur.ok[u]) break;
+      wave_push<B>(v, wave_step<B>(cur.cf[u], v, cur.xv[u], cur.dg[u], FWD));
+#pragma unroll
+      for (int j = 0; j < B; ++j) wave_push<B>(h[j], wave_step<B>(cur.cf[u], h[j], 0.0, cur.dg[u], FWD));
+    }
+    cur = nxt;
+  }
+#pragma unroll
+  for (int t = 0; t < B; ++t)
+#pragma unroll
+    for (int j = 0; j < B; ++j) M[t][j] = h[j][t];
+  line_scan<B, FWD, NW>(M, v, lane, wv, sm, ss, win);
+  wave_load<B, RC, FWD>(lu, x, Q, n, W, p, q, i0, i1, 0, cur);
+  for (int ib = 0; ib < len; ib += RC) {
+    wave_load<B, RC, FWD>(lu, x, Q, n, W, p, q, i0, i1, ib + RC, nxt);
+#pragma unroll
+    for (int u = 0; u < RC; ++u) {
+      if (!cur.ok[u]) break;
+      const double s = wave_step<B>(cur.cf[u], win, cur.xv[u], cur.dg[u], FWD);
+      wave_push<B>(win, s);
+      x[(long long)(FWD ? i0 + ib + u : i1 - 1 - ib - u) * Q] = s;
+    }
+    cur = nxt;
+  }
+}
+
+// Solve L U x = b in place along the axis of length n of a (P, n, Q) batch;
+// NW wavefronts per line, one line per workgroup (grid = P * Q).  p, q <= B.
+// Each lane streams its own chunk, so the lanes' accesses do not coalesce;
+// the chunks are short (n / (64 NW) rows) and the rows stay in L1/L2 between
+// the passes.
+template <int B, int NW>
+__global__ __launch_bounds__(64 * NW) void band_lu_solve_wave_kernel(const double* __restrict__ lu, int n, int p,
+                                                                     int q, double* __restrict__ X, long long Q) {
+  constexpr int RC = (B == 4 && NW > 1) ? 4 : 8;  // 256 VGPRs at 512 threads
+  __shared__ double sm[NW * (B * B + B)];
+  __shared__ double ss[NW * B];
+  const long long line = blockIdx.x;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long long pp = line / Q, qq = line - pp * Q;
+  double* __restrict__ x = X + pp * (long long)n * Q + qq;
+  const int W = p + q + 1;
+  const int c = (n + 64 * NW - 1) / (64 * NW);
+  const int i0 = min(n, (int)threadIdx.x * c), i1 = min(n, i0 + c);
+  // forward: y_i = b_i - sum_t L[i, i-1-t] y_{i-1-t}
+  wave_sweep<B, RC, NW, true>(lu, x, Q, n, W, p, q, i0, i1, lane, wv, sm, ss);
+  if constexpr (NW > 1) __syncthreads();  // sm / ss are reused
+  // backward: x_i = (y_i - sum_t U[i, i+1+t] x_{i+1+t}) / U[i, i]
+  wave_sweep<B, RC, NW, false>(lu, x, Q, n, W, p, q, i0, i1, lane, wv, sm, ss);
 }
 
 // As above with one factor per line group: line (pp, qq) uses system
@@ -342,6 +686,14 @@ __global__ void cast_kernel(const TI* __restrict__ x, TO* __restrict__ y, long l
   for (; i < n; i += stride) y[i] = (TO)x[i];
 }
 
+// Automatic choice between one lane per line and one wavefront per line: the
+// wavefront solve pays (B + 2) x the arithmetic to cut the sequential depth
+// 64-fold, which wins while the lines alone leave most of the GPU's lanes idle.
+constexpr int WAVE_MAXB = 4;
+constexpr int WAVE_MIN_N = 32;
+constexpr long long WAVE_MAX_LINES = 2048;
+constexpr long long WAVE_WIDE_LINES = 256;  // 8 waves per line: 2048 waves, two per SIMD
+
 static unsigned grid_for(long long n, int block) {
   long long g = (n + block - 1) / block;
   if (g > 65536) g = 65536;
@@ -359,21 +711,51 @@ int jfx_band_lu_factor(const double* band, int n, int p, int q, double* lu, int*
                        hipStream_t stream) {
   if (!band || !lu || !status_dev || n <= 0 || p < 0 || q < 0)
     return set_error("band_lu_factor: bad arguments"), JFX_EARG;
-  hipLaunchKernelGGL(band_lu_factor_kernel, dim3(1), dim3(64), 0, stream, band, n, p, q, lu, status_dev);
+  hipLaunchKernelGGL(band_lu_factor_kernel, dim3(1), dim3(64), 0, stream, band, n, p, q, lu, status_dev, 1);
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
+}
+
+int jfx_band_lu_solve_mode(const double* lu, int n, int p, int q, double* X, long long P, long long Q, int mode,
+                           hipStream_t stream) {
+  if (!lu || !X || n <= 0 || p < 0 || q < 0 || P < 0 || Q <= 0 || mode < 0 || mode > 2)
+    return set_error("band_lu_solve: bad arguments"), JFX_EARG;
+  const long long lines = P * Q;
+  const bool narrow = p <= WAVE_MAXB && q <= WAVE_MAXB;
+  if (mode == 2 && !narrow)
+    return set_error("band_lu_solve: the wavefront solve takes bandwidths <= %d (p=%d, q=%d)", WAVE_MAXB, p, q),
+           JFX_EARG;
+  if (lines == 0) return JFX_OK;
+  if (mode == 0)
+    mode = (narrow && n >= WAVE_MIN_N && lines <= WAVE_MAX_LINES) ? 2 : 1;
+  if (mode == 2) {
+    if (lines > 0x7fffffffLL) return set_error("band_lu_solve: too many lines for the wavefront solve"), JFX_EARG;
+    // eight waves per line while the lines alone leave most of the GPU idle
+    // and the chunks stay at least a batch of rows long
+    const bool wide = lines <= WAVE_WIDE_LINES && n >= 64 * 8 * 8;
+    const dim3 grid((unsigned)lines);
+    if (p <= 2 && q <= 2) {
+      if (wide)
+        hipLaunchKernelGGL((band_lu_solve_wave_kernel<2, 8>), grid, dim3(512), 0, stream, lu, n, p, q, X, Q);
+      else
+        hipLaunchKernelGGL((band_lu_solve_wave_kernel<2, 1>), grid, dim3(64), 0, stream, lu, n, p, q, X, Q);
+    } else {
+      if (wide)
+        hipLaunchKernelGGL((band_lu_solve_wave_kernel<4, 8>), grid, dim3(512), 0, stream, lu, n, p, q, X, Q);
+      else
+        hipLaunchKernelGGL((band_lu_solve_wave_kernel<4, 1>), grid, dim3(64), 0, stream, lu, n, p, q, X, Q);
+    }
+  } else {
+    hipLaunchKernelGGL(band_lu_solve_kernel, dim3((unsigned)((lines + 255) / 256)), dim3(256), 0, stream, lu, n, p,
+                       q, X, P, Q);
+  }
   JFX_LAUNCH_CHECK();
   return JFX_OK;
 }
 
 int jfx_band_lu_solve(const double* lu, int n, int p, int q, double* X, long long P, long long Q,
                       hipStream_t stream) {
-  if (!lu || !X || n <= 0 || p < 0 || q < 0 || P < 0 || Q <= 0)
-    return set_error("band_lu_solve: bad arguments"), JFX_EARG;
-  long long lines = P * Q;
-  if (lines == 0) return JFX_OK;
-  hipLaunchKernelGGL(band_lu_solve_kernel, dim3((unsigned)((lines + 255) / 256)), dim3(256), 0, stream, lu,
-                     n, p, q, X, P, Q);
-  JFX_LAUNCH_CHECK();
-  return JFX_OK;
+  return jfx_band_lu_solve_mode(lu, n, p, q, X, P, Q, 0, stream);
 }
 
 int jfx_band_lu_factor_batched(const double* band, int nsys, int n, int p, int q, double* lu, int* status_dev,

@@ -284,20 +284,31 @@ def band_lu_solve_batched(lu: torch.Tensor, p: int, q: int, X: torch.Tensor, axi
     return X
 
 
-def band_lu_solve(lu: torch.Tensor, p: int, q: int, X: torch.Tensor, axis: int) -> torch.Tensor:
+BAND_SOLVE_METHODS = {"auto": 0, "lanes": 1, "wave": 2}
+
+
+def band_lu_solve(lu: torch.Tensor, p: int, q: int, X: torch.Tensor, axis: int, method: str = "auto") -> torch.Tensor:
     """Solve in place along ``axis`` of X with a factored band matrix (a complex
-    X is solved through its real view: the same real factor for re and im)."""
+    X is solved through its real view: the same real factor for re and im).
+
+    ``method``: "lanes" runs one lane per line (a sequential sweep), "wave"
+    one wavefront per line (rows split over the 64 lanes, p, q <= 4), "auto"
+    the wavefront solve for few long lines and the per-lane sweep otherwise."""
+    if method not in BAND_SOLVE_METHODS:
+        raise ValueError(f"band solve: method must be one of {sorted(BAND_SOLVE_METHODS)}, got {method!r}")
     if X.is_complex():
         _req(X, torch.complex128, "X")
-        band_lu_solve(lu, p, q, torch.view_as_real(X), axis % X.ndim)
+        band_lu_solve(lu, p, q, torch.view_as_real(X), axis % X.ndim, method)
         return X
     _req(X, torch.float64, "X")
     n = lu.shape[0]
     if X.shape[axis] != n:
         raise ValueError(f"band solve: axis {axis} has length {X.shape[axis]}, matrix is {n}")
+    if method == "wave" and max(p, q) > 4:
+        raise ValueError(f"band solve: the wavefront solve takes bandwidths <= 4, got p={p}, q={q}")
     P = math.prod(X.shape[:axis])
     Q = math.prod(X.shape[axis + 1:])
-    _lib.call("jfx_band_lu_solve", _p(lu), n, p, q, _p(X), P, Q, _s())
+    _lib.call("jfx_band_lu_solve_mode", _p(lu), n, p, q, _p(X), P, Q, BAND_SOLVE_METHODS[method], _s())
     return X
 
 

@@ -124,6 +124,136 @@ def test_band_lu(dev):
         K.band_lu_factor(_t(band, dev), p, q)
 
 
+def _banded(rng, n, p, q, dominance=4.0):
+    A_band = np.zeros((n, p + q + 1))
+    for o in range(-p, q + 1):
+        lo, hi = max(0, -o), min(n, n - o)
+        A_band[lo:hi, o + p] = rng.standard_normal(hi - lo)
+    A_band[:, p] += dominance * (p + q + 1) * np.sign(A_band[:, p] + 0.5)
+    return A_band
+
+
+def _band_to_ab(A_band, p, q):
+    """The LAPACK diagonal-ordered form scipy.linalg.solve_banded takes."""
+    n = A_band.shape[0]
+    ab = np.zeros((p + q + 1, n))
+    for o in range(-p, q + 1):
+        lo, hi = max(0, -o), min(n, n - o)
+        ab[q - o, lo + o:hi + o] = A_band[lo:hi, o + p]
+    return ab
+
+
+@pytest.mark.parametrize(
+    "n,p,q,shape,axis",
+    [
+        (5, 2, 2, (5,), 0),  # fewer rows than lanes: most chunks empty
+        (63, 1, 1, (63,), 0),
+        (64, 2, 2, (64,), 0),
+        (1000, 2, 2, (1000,), 0),
+        (4097, 4, 4, (4097,), 0),
+        (3000, 0, 2, (3000,), 0),  # upper factor only
+        (3000, 3, 0, (3000,), 0),  # lower factor only
+        (2001, 1, 3, (2, 2001, 3), 1),  # lines across leading and trailing axes
+        (700, 2, 2, (5, 700), 1),
+        (100000, 2, 2, (100000,), 0),
+    ],
+)
+def test_band_solve_wave_vs_banded_lapack(dev, n, p, q, shape, axis):
+    """One wavefront per line (rows split over the lanes, chunk maps composed by
+    a wave scan) against LAPACK's banded solve and the per-lane sweep."""
+    import scipy.linalg as sla
+
+    from jaxfun_amd import kernels as K
+
+    rng = np.random.default_rng(n + 10 * p + q)
+    A_band = _banded(rng, n, p, q)
+    lu = K.band_lu_factor(_t(A_band, dev), p, q)
+    B = rng.standard_normal(shape)
+    Bl = np.moveaxis(B, axis, 0).reshape(n, -1)
+    Xr = np.moveaxis(sla.solve_banded((p, q), _band_to_ab(A_band, p, q), Bl).reshape(
+        (n,) + tuple(np.delete(shape, axis))), 0, axis)
+    Xw = K.band_lu_solve(lu, p, q, _t(B, dev).contiguous(), axis, method="wave").cpu().numpy()
+    Xl = K.band_lu_solve(lu, p, q, _t(B, dev).contiguous(), axis, method="lanes").cpu().numpy()
+    Xa = K.band_lu_solve(lu, p, q, _t(B, dev).contiguous(), axis).cpu().numpy()
+    scale = np.abs(Xr).max()
+    assert np.abs(Xw - Xr).max() <= 1e-13 * scale
+    assert np.abs(Xl - Xr).max() <= 1e-13 * scale
+    assert np.abs(Xw - Xl).max() <= 1e-13 * scale
+    assert np.array_equal(Xa, Xw) or np.array_equal(Xa, Xl)
+
+
+def test_band_solve_wave_legendre_mass_large(dev):
+    """DiaMatrix.solve of one long vector with the Legendre-Dirichlet mass matrix
+    (offsets 0, +-2) goes to the wavefront solve and matches LAPACK."""
+    import scipy.linalg as sla
+
+    from jaxfun_amd import kernels as K
+
+    n = 20000
+    k = np.arange(n, dtype=np.float64)
+    A_band = np.zeros((n, 5))
+    A_band[:, 2] = 2.0 / (2 * k + 1) + 2.0 / (2 * k + 5)
+    A_band[2:, 0] = -2.0 / (2 * k[:-2] + 5)  # A[i, i-2] = M[i-2, i]
+    A_band[:-2, 4] = -2.0 / (2 * k[:-2] + 5)
+    lu = K.band_lu_factor(_t(A_band, dev), 2, 2)
+    ab = _band_to_ab(A_band, 2, 2)
+
+    def matvec(x):
+        y = A_band[:, 2] * x
+        y[2:] += A_band[2:, 0] * x[:-2]
+        y[:-2] += A_band[:-2, 4] * x[2:]
+        return y
+
+    rng = np.random.default_rng(7)
+    x_true = rng.standard_normal(n)
+    for b in (matvec(x_true), rng.standard_normal(n)):
+        x = K.band_lu_solve(lu, 2, 2, _t(b, dev).contiguous(), 0).cpu().numpy()
+        xl = K.band_lu_solve(lu, 2, 2, _t(b, dev).contiguous(), 0, method="lanes").cpu().numpy()
+        xr = sla.solve_banded((2, 2), ab, b)
+        # backward error of a no-pivot LU of this SPD matrix: a few ulps
+        absA = np.abs(A_band).sum(axis=1).max()
+        for y in (x, xl, xr):
+            assert np.abs(matvec(y) - b).max() <= 1e-14 * (absA * np.abs(y).max() + np.abs(b).max())
+        # the mass matrix's condition number grows like n (about 1e5 here)
+        assert np.linalg.norm(x - xl) <= 1e-10 * np.linalg.norm(xl)
+        assert np.linalg.norm(x - xr) <= 1e-10 * np.linalg.norm(xr)
+    assert np.linalg.norm(x_true - K.band_lu_solve(lu, 2, 2, _t(matvec(x_true), dev), 0).cpu().numpy()) <= 1e-10 * (
+        np.linalg.norm(x_true))
+
+
+@pytest.mark.parametrize("p,q", [(1, 1), (2, 2), (3, 4), (6, 5)])
+def test_band_lu_factor_reports_interior_zero_pivot(dev, p, q):
+    """A pivot that becomes exactly zero in the middle of the elimination (the
+    2 x 2 all-ones block at rows 5-6) is reported with its row, as
+    DiaMatrix.lu_factor raises (la/diamatrix.py:461-471)."""
+    from jaxfun_amd import kernels as K
+
+    n = 12
+    A = 2.0 * np.eye(n)
+    A[5:7, 5:7] = 1.0
+    band = np.zeros((n, p + q + 1))
+    for i in range(n):
+        for o in range(-p, q + 1):
+            if 0 <= i + o < n:
+                band[i, o + p] = A[i, i + o]
+    with pytest.raises(ValueError, match="row 6"):
+        K.band_lu_factor(_t(band, dev), p, q)
+    with pytest.raises(ValueError):
+        K.band_lu_factor_batched(_t(np.stack([band, band]), dev), p, q)
+
+
+def test_band_solve_wave_rejects_wide_bands(dev):
+    from jaxfun_amd import kernels as K
+
+    A_band = _banded(np.random.default_rng(1), 40, 5, 1)
+    lu = K.band_lu_factor(_t(A_band, dev), 5, 1)
+    x = _t(np.ones(40), dev)
+    with pytest.raises(ValueError):
+        K.band_lu_solve(lu, 5, 1, x, 0, method="wave")
+    with pytest.raises(ValueError):
+        K.band_lu_solve(lu, 5, 1, x, 0, method="scan")
+
+
 def test_elementwise(dev):
     import torch
 
