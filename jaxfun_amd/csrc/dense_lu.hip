@@ -1,0 +1,380 @@
+// Dense LU with partial pivoting and the matching solve, FP64 real or
+// complex (re/im interleaved), row-major n x n, in place.
+//
+// Blocked right-looking factorisation, panels of NB = 32 columns (16 complex):
+//   1. panel:  one 1024-thread workgroup factors columns k0 .. k0+NB-1 of rows
+//              k0 .. n-1 (pivot = first row of largest |re| + |im| in the
+//              column, row swap inside the panel, multipliers, rank-1 update
+//              of the panel's remaining columns);
+//   2. rows:   every column outside the panel applies the panel's row swaps;
+//              the columns right of it then solve L11 U12 = A12 (L11 unit
+//              lower, staged in LDS);
+//   3. update: A22 -= L21 U12 by a 64 x 64-tiled FP64 GEMM.
+// Pivots are recorded as ipiv[j] (row swapped with row j at step j) and,
+// at the end, as the permutation perm (row i of P A is row perm[i] of A).
+// The solve gathers P b, then runs blocked forward (unit L) and backward (U)
+// substitution: per 32-row block (16 complex) a triangular solve of the
+// block, then a GEMM update of the remaining rows.
+// Replaces jax.scipy.linalg.lu_factor / lu_solve as used by Matrix.lu_factor
+// (la/matrix.py:138-214), TPMatricesDenseLUFactors (la/tpmatrix.py:1016-1086)
+// and TensorMatrix.solve (la/tensormatrix.py:76-80).
+#include "jfx_common.h"
+
+namespace jfx {
+
+struct zd {
+  double re, im;
+};
+
+__device__ __forceinline__ double dl_abs1(double a) { return fabs(a); }
+__device__ __forceinline__ double dl_abs1(zd a) { return fabs(a.re) + fabs(a.im); }
+__device__ __forceinline__ double dl_fms(double c, double a, double b) { return fma(-a, b, c); }  // c - a b
+__device__ __forceinline__ zd dl_fms(zd c, zd a, zd b) {
+  return {fma(-a.re, b.re, fma(a.im, b.im, c.re)), fma(-a.re, b.im, fma(-a.im, b.re, c.im))};
+}
+__device__ __forceinline__ double dl_div(double a, double b) { return a / b; }
+__device__ __forceinline__ zd dl_div(zd a, zd b) {
+  // Smith's algorithm (no overflow for |b| near the range limits)
+  if (fabs(b.re) >= fabs(b.im)) {
+    const double r = b.im / b.re, d = b.re + b.im * r;
+    return {(a.re + a.im * r) / d, (a.im - a.re * r) / d};
+  }
+  const double r = b.re / b.im, d = b.re * r + b.im;
+  return {(a.re * r + a.im) / d, (a.im * r - a.re) / d};
+}
+template <typename T>
+__device__ __forceinline__ T dl_zero();
+template <>
+__device__ __forceinline__ double dl_zero<double>() { return 0.0; }
+template <>
+__device__ __forceinline__ zd dl_zero<zd>() { return {0.0, 0.0}; }
+__device__ __forceinline__ bool dl_ok(double a) { return a != 0.0 && isfinite(a); }
+__device__ __forceinline__ bool dl_ok(zd a) { return (a.re != 0.0 || a.im != 0.0) && isfinite(a.re) && isfinite(a.im); }
+
+constexpr int DL_NB = 32;     // largest panel width
+constexpr int DL_PT = 1024;   // panel workgroup
+// Panel width and solve block: a column's (right-hand side's) entries of one
+// block live in registers, 32 doubles or 16 complex values.
+template <typename T>
+constexpr int dl_nb() {
+  return sizeof(T) == sizeof(double) ? 32 : 16;
+}
+template <typename T>
+constexpr int dl_nbs() {
+  return dl_nb<T>();
+}
+constexpr int DL_MAXN = 16384;
+
+// ---- 1. panel factorisation (one workgroup).
+template <typename T>
+__global__ __launch_bounds__(DL_PT) void dl_panel_kernel(T* __restrict__ A, int n, int k0, int nb,
+                                                          int* __restrict__ ipiv, int* __restrict__ status) {
+  __shared__ double wv[DL_PT / 64];
+  __shared__ int wi[DL_PT / 64];
+  __shared__ int piv_row;
+  __shared__ T prow[DL_NB];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  for (int j = k0; j < k0 + nb; ++j) {
+    // pivot: the first row of largest |a_ij| over i >= j
+    double best = -1.0;
+    int bi = n;
+    for (int i = j + tid; i < n; i += DL_PT) {
+      const double v = dl_abs1(A[(long long)i * n + j]);
+      if (v > best) best = v, bi = i;  // rows ascend within a thread: ties keep the first
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(best, o);
+      const int oi = __shfl_xor(bi, o);
+      if (ov > best || (ov == best && oi < bi)) best = ov, bi = oi;
+    }
+    if (lane == 0) wv[w] = best, wi[w] = bi;
+    __syncthreads();
+    if (tid == 0) {
+      double b = -1.0;
+      int r = n;
+      for (int k = 0; k < DL_PT / 64; ++k)
+        if (wv[k] > b || (wv[k] == b && wi[k] < r)) b = wv[k], r = wi[k];
+      r = r < n ? r : j;
+      piv_row = r;
+      ipiv[j] = r;
+      if (!dl_ok(A[(long long)r * n + j]) && *status == 0) *status = j + 1;
+    }
+    __syncthreads();
+    const int r = piv_row;
+    // swap rows j and r over the panel's columns, keep the pivot row in LDS
+    if (tid < nb) {
+      const int c = k0 + tid;
+      const T a = A[(long long)j * n + c], b = A[(long long)r * n + c];
+      if (r != j) A[(long long)r * n + c] = a;
+      A[(long long)j * n + c] = b;
+      prow[tid] = b;
+    }
+    __syncthreads();
+    const T p = prow[j - k0];
+    if (dl_ok(p)) {
+      // multipliers and the rank-1 update of the panel's remaining columns
+      for (int i = j + 1 + tid; i < n; i += DL_PT) {
+        T* row = A + (long long)i * n;
+        const T l = dl_div(row[j], p);
+        row[j] = l;
+        for (int c = j + 1; c < k0 + nb; ++c) row[c] = dl_fms(row[c], l, prow[c - k0]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---- 2. row swaps outside the panel, then U12 = L11^-1 A12 right of it.
+// One thread per column; L11 staged in LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void dl_rows_kernel(T* __restrict__ A, int n, int k0, int nb,
+                                                       const int* __restrict__ ipiv) {
+  constexpr int NB = dl_nb<T>();
+  __shared__ T L[NB][NB + 1];
+  __shared__ int pv[NB];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < nb * nb; e += 256) L[e / nb][e % nb] = A[(long long)(k0 + e / nb) * n + k0 + e % nb];
+  if (tid < nb) pv[tid] = ipiv[k0 + tid];
+  __syncthreads();
+  // columns outside [k0, k0 + nb): c = idx for idx < k0, else idx + nb
+  const int idx = blockIdx.x * 256 + tid;
+  if (idx >= n - nb) return;
+  const int c = idx < k0 ? idx : idx + nb;
+  for (int t = 0; t < nb; ++t) {
+    const int j = k0 + t, r = pv[t];
+    if (r != j) {
+      const T a = A[(long long)j * n + c];
+      A[(long long)j * n + c] = A[(long long)r * n + c];
+      A[(long long)r * n + c] = a;
+    }
+  }
+  if (c < k0) return;
+  T x[NB];
+#pragma unroll
+  for (int t = 0; t < NB; ++t) x[t] = t < nb ? A[(long long)(k0 + t) * n + c] : dl_zero<T>();
+  // constant trip counts (both loops unroll, x stays in registers); the
+  // per-row branch keeps each row's LDS reads next to their use
+#pragma unroll
+  for (int i = 1; i < NB; ++i) {
+    if (i < nb) {
+#pragma unroll
+      for (int t = 0; t < NB; ++t)
+        if (t < i) x[i] = dl_fms(x[i], L[i][t], x[t]);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NB; ++t)
+    if (t < nb) A[(long long)(k0 + t) * n + c] = x[t];
+}
+
+// ---- 3. C (m x nc, ldc) -= A (m x k, lda) B (k x nc, ldb): 64 x 64 output
+// tile per 256-thread workgroup, 4 x 4 per thread, k in LDS slices of 16.
+constexpr int DL_KS = 16;
+template <typename T>
+__global__ __launch_bounds__(256) void dl_gemm_sub_kernel(const T* __restrict__ A, long long lda,
+                                                           const T* __restrict__ B, long long ldb, T* __restrict__ C,
+                                                           long long ldc, int m, int nc, int k) {
+  __shared__ T As[DL_KS][65];  // [k][row]
+  __shared__ T Bs[DL_KS][65];  // [k][col]
+  const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
+  const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  T acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = dl_zero<T>();
+  for (int k0 = 0; k0 < k; k0 += DL_KS) {
+    __syncthreads();  // the previous slice is consumed
+    for (int e = tid; e < 64 * DL_KS; e += 256) {
+      const int rr = e / DL_KS, kk = e % DL_KS;  // A: row rr, k index kk (k contiguous in memory)
+      As[kk][rr] = (r0 + rr < m && k0 + kk < k) ? A[(long long)(r0 + rr) * lda + k0 + kk] : dl_zero<T>();
+      const int k2 = e >> 6, cc = e & 63;  // B: row k2, column cc
+      Bs[k2][cc] = (k0 + k2 < k && c0 + cc < nc) ? B[(long long)(k0 + k2) * ldb + c0 + cc] : dl_zero<T>();
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < DL_KS; ++kk) {
+      T av[4], bv[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) av[a] = As[kk][tr + 16 * a];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) bv[b] = Bs[kk][tc + 16 * b];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = dl_fms(acc[a][b], av[a], bv[b]);  // acc -= a b
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int r = r0 + tr + 16 * a;
+    if (r >= m) continue;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int c = c0 + tc + 16 * b;
+      if (c < nc) {
+        T* p = C + (long long)r * ldc + c;
+        if constexpr (sizeof(T) == sizeof(double))
+          *p = *p + acc[a][b];
+        else
+          *p = zd{p->re + acc[a][b].re, p->im + acc[a][b].im};
+      }
+    }
+  }
+}
+
+// ---- permutation from the recorded swaps (one thread, LDS)
+__global__ void dl_perm_kernel(const int* __restrict__ ipiv, int n, int* __restrict__ perm) {
+  __shared__ int pm[DL_MAXN];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) pm[i] = i;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int j = 0; j < n; ++j) {
+      const int r = ipiv[j];
+      const int t = pm[j];
+      pm[j] = pm[r];
+      pm[r] = t;
+    }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += blockDim.x) perm[i] = pm[i];
+}
+
+// ---- solve
+template <typename T>
+__global__ void dl_gather_kernel(const T* __restrict__ B, const int* __restrict__ perm, T* __restrict__ X, int n,
+                                 long long nrhs) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n * nrhs) return;
+  const long long i = e / nrhs, r = e - i * nrhs;
+  X[e] = B[(long long)perm[i] * nrhs + r];
+}
+
+// Rows kb .. kb + nbs - 1 of X: X_k <- T_kk^-1 X_k with T_kk the unit lower
+// (LOWER) or upper (else) diagonal block of LU; one thread per right-hand
+// side, the block in LDS.
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void dl_trsv_block_kernel(const T* __restrict__ LU, int n, int kb, int nbs,
+                                                             T* __restrict__ X, long long nrhs) {
+  constexpr int NBS = dl_nbs<T>();
+  __shared__ T D[NBS][NBS + 1];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < nbs * nbs; e += 256)
+    D[e / nbs][e % nbs] = LU[(long long)(kb + e / nbs) * n + kb + e % nbs];
+  __syncthreads();
+  const long long r = (long long)blockIdx.x * 256 + tid;
+  if (r >= nrhs) return;
+  T x[NBS];
+#pragma unroll
+  for (int t = 0; t < NBS; ++t) x[t] = t < nbs ? X[(long long)(kb + t) * nrhs + r] : dl_zero<T>();
+  if (LOWER) {
+#pragma unroll
+    for (int i = 1; i < NBS; ++i) {
+      if (i < nbs) {
+#pragma unroll
+        for (int t = 0; t < NBS; ++t)
+          if (t < i) x[i] = dl_fms(x[i], D[i][t], x[t]);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int i = NBS - 1; i >= 0; --i) {
+      if (i < nbs) {
+#pragma unroll
+        for (int t = 0; t < NBS; ++t)
+          if (t > i && t < nbs) x[i] = dl_fms(x[i], D[i][t], x[t]);
+        x[i] = dl_div(x[i], D[i][i]);
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NBS; ++t)
+    if (t < nbs) X[(long long)(kb + t) * nrhs + r] = x[t];
+}
+
+template <typename T>
+static int dl_gemm_sub(const T* A, long long lda, const T* B, long long ldb, T* C, long long ldc, int m, int nc,
+                       int k, hipStream_t st) {
+  if (m <= 0 || nc <= 0 || k <= 0) return JFX_OK;
+  const dim3 grid((unsigned)((nc + 63) / 64), (unsigned)((m + 63) / 64));
+  hipLaunchKernelGGL(dl_gemm_sub_kernel<T>, grid, dim3(256), 0, st, A, lda, B, ldb, C, ldc, m, nc, k);
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
+}
+
+template <typename T>
+static int dl_factor(T* A, int n, int* ipiv, int* perm, int* status, hipStream_t st) {
+  constexpr int NB = dl_nb<T>();
+  for (int k0 = 0; k0 < n; k0 += NB) {
+    const int nb = n - k0 < NB ? n - k0 : NB;
+    hipLaunchKernelGGL(dl_panel_kernel<T>, dim3(1), dim3(DL_PT), 0, st, A, n, k0, nb, ipiv, status);
+    JFX_LAUNCH_CHECK();
+    if (n > nb) {
+      hipLaunchKernelGGL(dl_rows_kernel<T>, dim3((unsigned)((n - nb + 255) / 256)), dim3(256), 0, st, A, n, k0, nb,
+                         ipiv);
+      JFX_LAUNCH_CHECK();
+    }
+    const int m = n - k0 - nb;
+    const long long base = (long long)(k0 + nb) * n;
+    const int r = dl_gemm_sub<T>(A + base + k0, n, A + (long long)k0 * n + k0 + nb, n, A + base + k0 + nb, n, m, m,
+                                 nb, st);
+    if (r) return r;
+  }
+  hipLaunchKernelGGL(dl_perm_kernel, dim3(1), dim3(1024), 0, st, ipiv, n, perm);
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
+}
+
+template <typename T>
+static int dl_solve(const T* LU, int n, const int* perm, const T* B, T* X, long long nrhs, hipStream_t st) {
+  constexpr int NBS = dl_nbs<T>();
+  const long long tot = (long long)n * nrhs;
+  hipLaunchKernelGGL(dl_gather_kernel<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, B, perm, X, n,
+                     nrhs);
+  JFX_LAUNCH_CHECK();
+  const dim3 g((unsigned)((nrhs + 255) / 256));
+  for (int kb = 0; kb < n; kb += NBS) {  // forward: L y = P b
+    const int nbs = n - kb < NBS ? n - kb : NBS;
+    hipLaunchKernelGGL((dl_trsv_block_kernel<T, true>), g, dim3(256), 0, st, LU, n, kb, nbs, X, nrhs);
+    JFX_LAUNCH_CHECK();
+    const int r = dl_gemm_sub<T>(LU + (long long)(kb + nbs) * n + kb, n, X + (long long)kb * nrhs, nrhs,
+                                 X + (long long)(kb + nbs) * nrhs, nrhs, n - kb - nbs, (int)nrhs, nbs, st);
+    if (r) return r;
+  }
+  const int last = (n - 1) / NBS * NBS;
+  for (int kb = last; kb >= 0; kb -= NBS) {  // backward: U x = y
+    const int nbs = n - kb < NBS ? n - kb : NBS;
+    hipLaunchKernelGGL((dl_trsv_block_kernel<T, false>), g, dim3(256), 0, st, LU, n, kb, nbs, X, nrhs);
+    JFX_LAUNCH_CHECK();
+    const int r = dl_gemm_sub<T>(LU + kb, n, X + (long long)kb * nrhs, nrhs, X, nrhs, kb, (int)nrhs, nbs, st);
+    if (r) return r;
+  }
+  return JFX_OK;
+}
+
+}  // namespace jfx
+
+using namespace jfx;
+
+extern "C" {
+
+int jfx_dense_lu_factor(int cplx, void* A, int n, int* ipiv, int* perm, int* status_dev, hipStream_t stream) {
+  if (!A || !ipiv || !perm || !status_dev || n <= 0 || (cplx != 0 && cplx != 1))
+    return set_error("dense_lu_factor: bad arguments"), JFX_EARG;
+  if (n > DL_MAXN) return set_error("dense_lu_factor: n = %d exceeds %d", n, DL_MAXN), JFX_ESHAPE;
+  return cplx ? dl_factor<zd>((zd*)A, n, ipiv, perm, status_dev, stream)
+              : dl_factor<double>((double*)A, n, ipiv, perm, status_dev, stream);
+}
+
+int jfx_dense_lu_solve(int cplx, const void* LU, int n, const int* perm, const void* B, void* X, long long nrhs,
+                       hipStream_t stream) {
+  if (!LU || !perm || !B || !X || n <= 0 || nrhs < 0 || B == X || (cplx != 0 && cplx != 1))
+    return set_error("dense_lu_solve: bad arguments (B and X must differ)"), JFX_EARG;
+  if (n > DL_MAXN) return set_error("dense_lu_solve: n = %d exceeds %d", n, DL_MAXN), JFX_ESHAPE;
+  if (nrhs == 0) return JFX_OK;
+  if (nrhs > 0x7fffffffLL) return set_error("dense_lu_solve: too many right-hand sides"), JFX_ESHAPE;
+  return cplx ? dl_solve<zd>((const zd*)LU, n, perm, (const zd*)B, (zd*)X, nrhs, stream)
+              : dl_solve<double>((const double*)LU, n, perm, (const double*)B, (double*)X, nrhs, stream);
+}
+
+}  // extern "C"

@@ -296,18 +296,16 @@ class TensorMatrix:
         return self.todense_host().reshape(self.shape_dofs + self.shape_dofs)
 
     def solve(self, rhs, axis: int = 0):
-        """Dense LU solve (la/tensormatrix.py:76-80); rocSOLVER factorisation."""
+        """Dense LU solve (la/tensormatrix.py:76-80) by the library's blocked LU
+        with partial pivoting."""
         from .orthogonal import as_device
 
         if self._lu is None:
-            self._lu = torch.linalg.lu_factor(self.todense())
+            self._lu = K.dense_lu_factor(self.todense())
         b = as_device(rhs)
-        LU, piv = self._lu
-        if b.is_complex():
-            LU = LU.to(torch.complex128)
-        nb = b.ndim - len(self.shape_dofs)
+        LU, perm = self._lu
         flat = b.reshape((-1, self.shape[0])).T.contiguous()
-        x = torch.linalg.lu_solve(LU, piv, flat)
+        x = K.dense_lu_solve(LU, perm, flat)
         return x.T.reshape(b.shape).contiguous()
 
 

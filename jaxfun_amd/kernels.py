@@ -312,6 +312,59 @@ def band_lu_solve(lu: torch.Tensor, p: int, q: int, X: torch.Tensor, axis: int, 
     return X
 
 
+# ---------------------------------------------------------------- dense LU
+
+def dense_lu_factor(A: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
+    """LU with partial pivoting of a square FP64 or complex128 matrix (row
+    pivoting on the largest |re| + |im|), blocked over 32-column panels
+    (16 complex).  Returns (LU, perm): L unit lower and U in one array, and
+    row i of P A = row perm[i] of A.  Raises ValueError on a zero pivot, as
+    Matrix.lu_factor does (la/matrix.py:137-167)."""
+    if A.ndim != 2 or A.shape[0] != A.shape[1]:
+        raise ValueError(f"dense LU needs a square matrix, got shape {tuple(A.shape)}")
+    cplx = A.is_complex()
+    _req(A, torch.complex128 if cplx else torch.float64, "A")
+    n = A.shape[0]
+    LU = A.contiguous().clone()
+    ipiv = torch.empty(n, dtype=torch.int32, device=A.device)
+    perm = torch.empty(n, dtype=torch.int32, device=A.device)
+    st = torch.zeros(1, dtype=torch.int32, device=A.device)
+    _lib.call("jfx_dense_lu_factor", int(cplx), _p(LU), n, _p(ipiv), _p(perm), _p(st), _s())
+    s = int(st.item())
+    if s != 0:
+        raise ValueError(f"Matrix is singular: zero pivot in LU factorisation (column {s - 1})")
+    return LU, perm
+
+
+def dense_lu_solve(LU: torch.Tensor, perm: torch.Tensor, B: torch.Tensor) -> torch.Tensor:
+    """X = A^-1 B for B of shape (n,) or (n, nrhs), with (LU, perm) from
+    ``dense_lu_factor``.  A complex B with a real factor is solved as its
+    real and imaginary parts; a real B with a complex factor is promoted."""
+    n = LU.shape[0]
+    if B.shape[0] != n:
+        raise ValueError(f"dense LU solve: B has {B.shape[0]} rows, the matrix is {n} x {n}")
+    vec = B.ndim == 1
+    B2 = B.reshape(n, -1)
+    if LU.is_complex():
+        B2 = (B2 if B2.is_complex() else to_complex(B2.contiguous())).contiguous()
+        _req(B2, torch.complex128, "B")
+        X = torch.empty_like(B2)
+        _lib.call("jfx_dense_lu_solve", 1, _p(LU), n, _p(perm), _p(B2), _p(X), B2.shape[1], _s())
+    elif B2.is_complex():
+        B2 = B2.contiguous()
+        _req(B2, torch.complex128, "B")
+        Br = torch.view_as_real(B2).reshape(n, -1)  # (n, 2 nrhs): re, im interleaved
+        Xr = torch.empty_like(Br)
+        _lib.call("jfx_dense_lu_solve", 0, _p(LU), n, _p(perm), _p(Br), _p(Xr), Br.shape[1], _s())
+        X = torch.view_as_complex(Xr.reshape(n, -1, 2))
+    else:
+        B2 = B2.contiguous()
+        _req(B2, torch.float64, "B")
+        X = torch.empty_like(B2)
+        _lib.call("jfx_dense_lu_solve", 0, _p(LU), n, _p(perm), _p(B2), _p(X), B2.shape[1], _s())
+    return X.reshape(n) if vec else X
+
+
 # ---------------------------------------------------------------- per-axis contraction
 
 

@@ -189,22 +189,20 @@ class Matrix(BaseMatrix):
     """Dense matrix (la/matrix.py:23-214)."""
 
     def lu_factor(self):
+        """LU with partial pivoting on the device (la/matrix.py:137-167),
+        cached; raises ValueError on a zero pivot."""
         if self._lu is None:
-            self._lu = torch.linalg.lu_factor(self.todense())
+            self._lu = K.dense_lu_factor(self.todense())
         return self._lu
 
     def solve(self, b, axis: int = 0):
-        """Dense solve along an axis (la/matrix.py:187-214) through rocSOLVER."""
+        """Dense solve along an axis (la/matrix.py:187-214)."""
         from ..galerkin.orthogonal import as_device
 
         b = as_device(b)
-        LU, piv = self.lu_factor()
-        if b.is_complex() and not LU.is_complex():
-            LU = LU.to(torch.complex128)
-        elif LU.is_complex() and not b.is_complex():
-            b = K.to_complex(b)
+        LU, perm = self.lu_factor()
         x = b.movedim(axis, 0).reshape(b.shape[axis], -1)
-        y = torch.linalg.lu_solve(LU, piv, x)
+        y = K.dense_lu_solve(LU, perm, x)
         return y.reshape(b.movedim(axis, 0).shape).movedim(0, axis).contiguous()
 
 

@@ -222,21 +222,19 @@ class TPMatricesLUFactors:
 
 
 class TPMatricesDenseLUFactors:
-    """Dense Kronecker LU (la/tpmatrix.py:1016-1049), factored by rocSOLVER."""
+    """Dense Kronecker LU (la/tpmatrix.py:1016-1049) with partial pivoting,
+    factored and solved by the library's blocked LU kernels."""
 
     def __init__(self, A, shape):
         self.shape = tuple(shape)
         A = A if isinstance(A, torch.Tensor) else torch.tensor(A, device=K.device())
-        self.LU, self.piv = torch.linalg.lu_factor(A)
+        self.LU, self.perm = K.dense_lu_factor(A)
 
     def solve(self, rhs, precision: str = "f64") -> torch.Tensor:
         b = _as_dev(rhs)
         nb = b.ndim - len(self.shape)
         flat = b.reshape(b.shape[:nb] + (-1,))
-        LU = self.LU
-        if b.is_complex() and not LU.is_complex():
-            LU = self.LU.to(torch.complex128)
-        x = torch.linalg.lu_solve(LU, self.piv, flat.reshape(-1, flat.shape[-1]).T.contiguous())
+        x = K.dense_lu_solve(self.LU, self.perm, flat.reshape(-1, flat.shape[-1]).T.contiguous())
         return x.T.reshape(b.shape).contiguous()
 
 
