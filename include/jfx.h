@@ -181,8 +181,9 @@ int jfx_band_lu_solve_mode(const double* lu, int n, int p, int q, double* X, lon
 
 /* Dense LU with partial pivoting, in place, of a row-major n x n matrix A
  * (cplx = 0: FP64, cplx = 1: complex128 as re/im pairs), n <= 16384:
- * blocked over 32-column panels (16 complex; one workgroup per panel, the
- * pivot the first row of largest |re| + |im|), row swaps and the block-row
+ * blocked over 32-column panels (16 complex; one workgroup per panel with
+ * the panel's rows in registers while at most 4096 rows remain, the pivot
+ * the first row of largest |re| + |im|), row swaps and the block-row
  * triangular solve per panel, then a tiled FP64 GEMM update of the trailing
  * matrix.  ipiv (n ints): row swapped with row j at step j; perm (n ints):
  * row i of P A is row perm[i] of A; status_dev (zeroed by the caller)
@@ -192,8 +193,8 @@ int jfx_band_lu_solve_mode(const double* lu, int n, int p, int q, double* X, lon
  * and TensorMatrix.solve (la/tensormatrix.py:76-80). */
 int jfx_dense_lu_factor(int cplx, void* A, int n, int* ipiv, int* perm, int* status_dev, hipStream_t stream);
 /* X (n x nrhs, row-major) = A^-1 B from the factors above: X = P B, then
- * blocked forward (unit L) and backward (U) substitution, 32-row blocks (16
- * complex) solved in registers with GEMM updates of the remaining rows.
+ * blocked forward (unit L) and backward (U) substitution, 32-row blocks
+ * solved in registers with GEMM updates of the remaining rows.
  * B and X must not overlap.  Replaces jax.scipy.linalg.lu_solve in
  * Matrix.lu_solve (la/matrix.py:169-214). */
 int jfx_dense_lu_solve(int cplx, const void* LU, int n, const int* perm, const void* B, void* X, long long nrhs,

@@ -13,7 +13,7 @@
 // Pivots are recorded as ipiv[j] (row swapped with row j at step j) and,
 // at the end, as the permutation perm (row i of P A is row perm[i] of A).
 // The solve gathers P b, then runs blocked forward (unit L) and backward (U)
-// substitution: per 32-row block (16 complex) a triangular solve of the
+// substitution: per 32-row block a triangular solve of the
 // block, then a GEMM update of the remaining rows.
 // Replaces jax.scipy.linalg.lu_factor / lu_solve as used by Matrix.lu_factor
 // (la/matrix.py:138-214), TPMatricesDenseLUFactors (la/tpmatrix.py:1016-1086)
@@ -54,14 +54,14 @@ __device__ __forceinline__ bool dl_ok(zd a) { return (a.re != 0.0 || a.im != 0.0
 constexpr int DL_NB = 32;     // largest panel width
 constexpr int DL_PT = 1024;   // panel workgroup
 // Panel width and solve block: a column's (right-hand side's) entries of one
-// block live in registers, 32 doubles or 16 complex values.
+// block live in registers (panel: 32 doubles or 16 complex values; solve: 32).
 template <typename T>
 constexpr int dl_nb() {
   return sizeof(T) == sizeof(double) ? 32 : 16;
 }
 template <typename T>
 constexpr int dl_nbs() {
-  return dl_nb<T>();
+  return 32;
 }
 constexpr int DL_MAXN = 16384;
 
@@ -119,6 +119,184 @@ __global__ __launch_bounds__(DL_PT) void dl_panel_kernel(T* __restrict__ A, int 
         const T l = dl_div(row[j], p);
         row[j] = l;
         for (int c = j + 1; c < k0 + nb; ++c) row[c] = dl_fms(row[c], l, prow[c - k0]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---- 1'. the same panel with its rows in registers (n - k0 <= NT R).
+// Thread t owns panel rows k0 + t + NT m (m < R) for the whole panel.  The
+// panel goes in sub-panels of SB columns: a sub-panel's entries of the owned
+// rows are held in registers while its SB columns are pivoted and
+// eliminated (pivot search, row exchange through LDS, rank-1 updates all in
+// registers; only the panel's other columns are swapped in memory), then
+// written back; the rest of the panel is updated once per sub-panel (unit
+// lower solve of the sub-panel's rows, then a rank-SB update of the rows
+// below).  Memory traffic per panel falls from (n - k0) NB^2 to about
+// (n - k0) (2 NB + NB^2 / SB) values, and no column step waits on memory.
+template <typename T, int R, int NT>
+__global__ __launch_bounds__(NT) void dl_panel_reg_kernel(T* __restrict__ A, int n, int k0, int nb,
+                                                              int* __restrict__ ipiv, int* __restrict__ status) {
+  constexpr int SB = sizeof(T) == sizeof(double) ? 8 : 4;
+  __shared__ double wv[NT / 64];
+  __shared__ int wi[NT / 64];
+  __shared__ int piv_sh[2];
+  __shared__ T srow[2][2][SB];  // [column parity][row j, row r] sub-panel entries
+  __shared__ T Lss[SB][SB];
+  __shared__ T Us[SB][DL_NB];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  for (int s0 = k0; s0 < k0 + nb; s0 += SB) {
+    const int sb = min(SB, k0 + nb - s0);
+    T v[R][SB];
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      const int i = k0 + tid + NT * m;
+#pragma unroll
+      for (int c = 0; c < SB; ++c) v[m][c] = (i < n && i >= s0 && c < sb) ? A[(long long)i * n + s0 + c] : dl_zero<T>();
+    }
+    for (int jj = 0; jj < sb; ++jj) {
+      const int j = s0 + jj, par = jj & 1;
+      // pivot: the first row of largest |a_ij| over i >= j
+      double best = -1.0;
+      int bi = n;
+#pragma unroll
+      for (int m = 0; m < R; ++m) {
+        const int i = k0 + tid + NT * m;
+        if (i >= j && i < n) {
+          T x = v[m][0];
+#pragma unroll
+          for (int c = 1; c < SB; ++c)
+            if (c == jj) x = v[m][c];
+          const double a = dl_abs1(x);
+          if (a > best) best = a, bi = i;
+        }
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(best, o);
+        const int oi = __shfl_xor(bi, o);
+        if (ov > best || (ov == best && oi < bi)) best = ov, bi = oi;
+      }
+      if (lane == 0) wv[w] = best, wi[w] = bi;
+      __syncthreads();
+      if (tid == 0) {
+        double b = -1.0;
+        int r = n;
+        for (int k = 0; k < NT / 64; ++k)
+          if (wv[k] > b || (wv[k] == b && wi[k] < r)) b = wv[k], r = wi[k];
+        r = r < n ? r : j;
+        piv_sh[par] = r;
+        ipiv[j] = r;
+      }
+      __syncthreads();
+      const int r = piv_sh[par];
+      // rows j and r of the sub-panel through LDS; the panel's other columns in memory
+      const int oj = (j - k0) % NT, mj = (j - k0) / NT, orr = (r - k0) % NT, mr = (r - k0) / NT;
+#pragma unroll
+      for (int m = 0; m < R; ++m) {
+        if (tid == oj && m == mj)
+#pragma unroll
+          for (int c = 0; c < SB; ++c) srow[par][0][c] = v[m][c];
+        if (tid == orr && m == mr)
+#pragma unroll
+          for (int c = 0; c < SB; ++c) srow[par][1][c] = v[m][c];
+      }
+      if (r != j && tid < nb) {
+        const int c = k0 + tid;
+        if (c < s0 || c >= s0 + sb) {
+          const T a = A[(long long)j * n + c];
+          A[(long long)j * n + c] = A[(long long)r * n + c];
+          A[(long long)r * n + c] = a;
+        }
+      }
+      __syncthreads();
+      if (r != j) {
+#pragma unroll
+        for (int m = 0; m < R; ++m) {
+          if (tid == oj && m == mj)
+#pragma unroll
+            for (int c = 0; c < SB; ++c) v[m][c] = srow[par][1][c];
+          if (tid == orr && m == mr)
+#pragma unroll
+            for (int c = 0; c < SB; ++c) v[m][c] = srow[par][0][c];
+        }
+      }
+      // srow[par][1] is the pivot row now
+      T p = srow[par][1][0];
+#pragma unroll
+      for (int c = 1; c < SB; ++c)
+        if (c == jj) p = srow[par][1][c];
+      if (tid == 0 && !dl_ok(p) && *status == 0) *status = j + 1;
+      if (dl_ok(p)) {
+#pragma unroll
+        for (int m = 0; m < R; ++m) {
+          const int i = k0 + tid + NT * m;
+          if (i > j && i < n) {
+#pragma unroll
+            for (int c = 0; c < SB; ++c)
+              if (c == jj) v[m][c] = dl_div(v[m][c], p);
+            T l = v[m][0];
+#pragma unroll
+            for (int c = 1; c < SB; ++c)
+              if (c == jj) l = v[m][c];
+#pragma unroll
+            for (int c = 0; c < SB; ++c)
+              if (c > jj) v[m][c] = dl_fms(v[m][c], l, srow[par][1][c]);
+          }
+        }
+      }
+      // srow[par] is rewritten two columns later, after two more barriers
+    }
+    // the sub-panel back to memory; L of the sub-panel's own rows to LDS
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      const int i = k0 + tid + NT * m;
+      if (i >= s0 && i < n) {
+#pragma unroll
+        for (int c = 0; c < SB; ++c)
+          if (c < sb) A[(long long)i * n + s0 + c] = v[m][c];
+      }
+      if (i >= s0 && i < s0 + sb) {
+#pragma unroll
+        for (int c = 0; c < SB; ++c) Lss[i - s0][c] = v[m][c];
+      }
+    }
+    __syncthreads();
+    const int rc = k0 + nb - s0 - sb;  // panel columns right of the sub-panel
+    if (rc > 0) {
+      if (tid < rc) {  // U rows of the sub-panel: L_ss^-1 A[s0 .. s0+sb-1, c]
+        const int c = s0 + sb + tid;
+        T x[SB];
+#pragma unroll
+        for (int a = 0; a < SB; ++a) x[a] = a < sb ? A[(long long)(s0 + a) * n + c] : dl_zero<T>();
+#pragma unroll
+        for (int a = 1; a < SB; ++a)
+#pragma unroll
+          for (int t = 0; t < SB; ++t)
+            if (t < a) x[a] = dl_fms(x[a], Lss[a][t], x[t]);
+#pragma unroll
+        for (int a = 0; a < SB; ++a)
+          if (a < sb) {
+            A[(long long)(s0 + a) * n + c] = x[a];
+            Us[a][tid] = x[a];
+          }
+      }
+      __syncthreads();
+      // rank-sb update of the rows below, multipliers from registers
+#pragma unroll
+      for (int m = 0; m < R; ++m) {
+        const int i = k0 + tid + NT * m;
+        if (i >= s0 + sb && i < n) {
+          T* row = A + (long long)i * n + s0 + sb;
+          for (int cc = 0; cc < rc; ++cc) {
+            T a = row[cc];
+#pragma unroll
+            for (int t = 0; t < SB; ++t)
+              if (t < sb) a = dl_fms(a, v[m][t], Us[t][cc]);
+            row[cc] = a;
+          }
+        }
       }
     }
     __syncthreads();
@@ -307,7 +485,15 @@ static int dl_factor(T* A, int n, int* ipiv, int* perm, int* status, hipStream_t
   constexpr int NB = dl_nb<T>();
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int nb = n - k0 < NB ? n - k0 : NB;
-    hipLaunchKernelGGL(dl_panel_kernel<T>, dim3(1), dim3(DL_PT), 0, st, A, n, k0, nb, ipiv, status);
+    const int rows = n - k0;
+    if (rows <= 1024)
+      hipLaunchKernelGGL((dl_panel_reg_kernel<T, 1, 1024>), dim3(1), dim3(1024), 0, st, A, n, k0, nb, ipiv, status);
+    else if (rows <= 2048)
+      hipLaunchKernelGGL((dl_panel_reg_kernel<T, 2, 1024>), dim3(1), dim3(1024), 0, st, A, n, k0, nb, ipiv, status);
+    else if (rows <= 4096)  // 512 threads: room for 8 rows of registers each
+      hipLaunchKernelGGL((dl_panel_reg_kernel<T, 8, 512>), dim3(1), dim3(512), 0, st, A, n, k0, nb, ipiv, status);
+    else
+      hipLaunchKernelGGL(dl_panel_kernel<T>, dim3(1), dim3(DL_PT), 0, st, A, n, k0, nb, ipiv, status);
     JFX_LAUNCH_CHECK();
     if (n > nb) {
       hipLaunchKernelGGL(dl_rows_kernel<T>, dim3((unsigned)((n - nb + 255) / 256)), dim3(256), 0, st, A, n, k0, nb,

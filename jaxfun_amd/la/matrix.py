@@ -243,7 +243,7 @@ class DiaMatrix(BaseMatrix):
 
     def lu_factor(self):
         """No-pivot banded LU on the device (la/diamatrix.py:394-504); a complex
-        matrix is factored densely (rocSOLVER)."""
+        matrix is factored densely (complex LU with partial pivoting)."""
         if self.is_complex:
             return Matrix.lu_factor(self)
         if self._lu is None:
