@@ -130,3 +130,32 @@ def test_matrix_and_kron_paths_use_it():
     As[:, 7] = 0.0
     with pytest.raises(ValueError):
         Matrix(As).solve(_t(b), axis=0)
+
+
+@pytest.mark.parametrize("n,cplx", [(4500, False), (2100, True)])
+def test_large_panels(n, cplx):
+    """Panels past 4096 rows (memory-resident panel) and the 512-thread
+    register panel (up to 4096 rows; complex from 2049)."""
+    from jaxfun_amd import kernels as K
+
+    rng = np.random.default_rng(300 + n)
+    A = _rand(rng, n, cplx) + 2 * n ** 0.5 * np.eye(n)
+    B = rng.standard_normal((n, 2))
+    LU, perm = K.dense_lu_factor(_t(A))
+    X = K.dense_lu_solve(LU, perm, _t(B)).cpu().numpy()
+    R = A @ X - B
+    assert np.abs(R).max() <= 1e-12 * (np.abs(A).sum(1).max() * np.abs(X).max())
+    Xr = np.linalg.solve(A, B)
+    assert np.abs(X - Xr).max() <= 1e-10 * np.abs(Xr).max()
+
+
+def test_size_limit_rejected():
+    import torch
+
+    from jaxfun_amd import _lib
+
+    buf = torch.zeros(4, dtype=torch.float64, device="cuda")
+    piv = torch.zeros(4, dtype=torch.int32, device="cuda")
+    with pytest.raises(Exception, match="exceeds"):
+        _lib.call("jfx_dense_lu_factor", 0, buf.data_ptr(), 20000, piv.data_ptr(), piv.data_ptr(), piv.data_ptr(),
+                  torch.cuda.current_stream().cuda_stream)
