@@ -72,10 +72,12 @@ struct RMap {
   int o16;  // FP32 / bf16x6 with Q == 1: output columns in 16-byte chunks (else 8-byte pairs)
 };
 
+// RB: row blocks of 16 per wave (2: operators of up to 256 rows, bf16x6
+// with one column group).
 // NS: k-steps run unconditionally (4 deep for FP64/FP32, 32 deep for bf16x6);
 // the operator registers and the staged rows past K are zero.  CG: column
 // groups of 8 waves (1 or 2).
-template <int POL, int NS, int CG>
+template <int POL, int NS, int CG, int RB = 1>
 __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __restrict__ Aptr,
                                                           const typename RT<POL>::TX* __restrict__ X,
                                                           typename RT<POL>::TX* __restrict__ Y, RMap rm) {
@@ -96,13 +98,16 @@ __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __res
   const int w = wv & 7, cg = wv >> 3;  // row block, column group
   const int M = rm.M, K = rm.K;
   const long long Q = rm.Q;
-  const bool wact = 16 * w < M;  // this wave owns output rows
+  static_assert(RB == 1 || (POL == 3 && CG == 1), "two row blocks per wave: bf16x6, one column group");
+  const bool wact = 16 * w < M;  // this wave owns output rows (row block 0)
 
-  // ---- operator rows of this wave, resident in registers
+  // ---- operator rows of this wave, resident in registers: row block rb
+  // holds rows 16 w + 128 rb .. + 15
   typedef typename std::conditional<POL == 3, v8u16, TX>::type AR;
-  AR ar[POL == 3 ? 3 * NS : NS];
-  {
-    const int row = 16 * w + (lane & 15);
+  AR ar[RB][POL == 3 ? 3 * NS : NS];
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb) {
+    const int row = 16 * w + RMAX * rb + (lane & 15);
     if constexpr (POL == 3) {
       // slices of Mtot rows of lda bf16 values each (a row split passes Aptr
       // at its first row, a column split at its first column)
@@ -114,14 +119,14 @@ __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __res
         for (int sl = 0; sl < 3; ++sl) {
           v8u16 v = {0, 0, 0, 0, 0, 0, 0, 0};
           if (row < M && 32 * s < Kp) v = *(const v8u16*)(Asl + ((long long)sl * rm.Mtot + row) * rm.lda + 32 * s + 8 * (lane >> 4));
-          ar[3 * s + sl] = v;
+          ar[rb][3 * s + sl] = v;
         }
     } else {
       const TX* A = reinterpret_cast<const TX*>(Aptr);
 #pragma unroll
       for (int s = 0; s < NS; ++s) {
         const int k = 4 * s + (lane >> 4);
-        ar[s] = (row < M && k < K) ? A[(long long)row * rm.lda + k] : TX(0);
+        ar[rb][s] = (row < M && k < K) ? A[(long long)row * rm.lda + k] : TX(0);
       }
     }
   }
@@ -263,17 +268,49 @@ __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __res
     // with Q == 1 only rows < K are staged: the rest of the k-steps reads zeros
     for (int e = tid; e < 2 * BSZ; e += NT) Bs0[e] = TX(0);
   }
-  typedef V AccT[JB];
+  typedef V AccT[RB][JB];
   auto compute = [&](AccT& acc, int buf) {
     const TX* Bs = Bs0 + buf * BSZ;
     const unsigned short* Bsl = Bsl0 + buf * BSL;
 #pragma unroll
-    for (int j = 0; j < JB; ++j) acc[j] = V{0, 0, 0, 0};
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int j = 0; j < JB; ++j) acc[rb][j] = V{0, 0, 0, 0};
       if (wact) {
         // B fragments of step s + 1 are read from LDS before step s's MFMAs issue
-        if constexpr (POL == 3) {
+        if constexpr (POL == 3 && RB > 1) {
+          // two row blocks: each B fragment feeds 12 MFMAs, so fragments are
+          // read one (k-step, column block) ahead instead of a whole k-step
+          v8u16 bq[2][3];
+          auto bl1 = [&](int idx, v8u16(&dst)[3]) {
+            const int s = idx / JB, j = idx - s * JB;
+            const int off = (16 * j + (lane & 15)) * BLD + 32 * s + 8 * (lane >> 4);
+            dst[0] = *(const v8u16*)&Bsl[off];
+            dst[1] = *(const v8u16*)&Bsl[RTN * BLD + off];
+            dst[2] = *(const v8u16*)&Bsl[2 * RTN * BLD + off];
+          };
+          bl1(0, bq[0]);
+  #pragma unroll
+          for (int idx = 0; idx < NS * JB; ++idx) {
+            const int s = idx / JB, j = idx - s * JB;
+            if (idx + 1 < NS * JB) bl1(idx + 1, bq[(idx + 1) & 1]);
+            __builtin_amdgcn_sched_barrier(0);
+            const v8u16 bh = bq[idx & 1][0], bm = bq[idx & 1][1], bl = bq[idx & 1][2];
+  #pragma unroll
+            for (int rb = 0; rb < RB; ++rb) {
+              if (rb > 0 && 16 * w + RMAX * rb >= M) break;  // wave-uniform
+              const v8u16 ah = ar[rb][3 * s], am = ar[rb][3 * s + 1], al = ar[rb][3 * s + 2];
+              acc[rb][j] = mfma_bf16(al, bh, acc[rb][j]);
+              acc[rb][j] = mfma_bf16(am, bm, acc[rb][j]);
+              acc[rb][j] = mfma_bf16(ah, bl, acc[rb][j]);
+              acc[rb][j] = mfma_bf16(am, bh, acc[rb][j]);
+              acc[rb][j] = mfma_bf16(ah, bm, acc[rb][j]);
+              acc[rb][j] = mfma_bf16(ah, bh, acc[rb][j]);
+            }
+          }
+        } else if constexpr (POL == 3) {
           // B fragments double-buffered when the registers allow (not with six k-steps)
-          constexpr int NBB = (CG == 2 || NS > 4) ? 1 : 2;
+          constexpr int NBB = (CG == 2 || NS > 4 || RB > 1) ? 1 : 2;
           v8u16 bf[NBB][JB][3];
           auto bload = [&](int s, v8u16(&dst)[JB][3]) {
   #pragma unroll
@@ -293,18 +330,22 @@ __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __res
               bload(s, bf[0]);
             }
             __builtin_amdgcn_sched_barrier(0);  // the next step's reads issue before this step's MFMAs
-            const v8u16 ah = ar[3 * s], am = ar[3 * s + 1], al = ar[3 * s + 2];
   #pragma unroll
-            for (int j = 0; j < JB; ++j) {
-              const int sb = NBB == 2 ? (s & 1) : 0;
-              const v8u16 bh = bf[sb][j][0], bm = bf[sb][j][1], bl = bf[sb][j][2];
-              // products ordered small to large
-              acc[j] = mfma_bf16(al, bh, acc[j]);
-              acc[j] = mfma_bf16(am, bm, acc[j]);
-              acc[j] = mfma_bf16(ah, bl, acc[j]);
-              acc[j] = mfma_bf16(am, bh, acc[j]);
-              acc[j] = mfma_bf16(ah, bm, acc[j]);
-              acc[j] = mfma_bf16(ah, bh, acc[j]);
+            for (int rb = 0; rb < RB; ++rb) {
+              if (rb > 0 && 16 * w + RMAX * rb >= M) break;  // wave-uniform
+              const v8u16 ah = ar[rb][3 * s], am = ar[rb][3 * s + 1], al = ar[rb][3 * s + 2];
+  #pragma unroll
+              for (int j = 0; j < JB; ++j) {
+                const int sb = NBB == 2 ? (s & 1) : 0;
+                const v8u16 bh = bf[sb][j][0], bm = bf[sb][j][1], bl = bf[sb][j][2];
+                // products ordered small to large
+                acc[rb][j] = mfma_bf16(al, bh, acc[rb][j]);
+                acc[rb][j] = mfma_bf16(am, bm, acc[rb][j]);
+                acc[rb][j] = mfma_bf16(ah, bl, acc[rb][j]);
+                acc[rb][j] = mfma_bf16(am, bh, acc[rb][j]);
+                acc[rb][j] = mfma_bf16(ah, bm, acc[rb][j]);
+                acc[rb][j] = mfma_bf16(ah, bh, acc[rb][j]);
+              }
             }
           }
         } else {
@@ -327,9 +368,9 @@ __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __res
   #pragma unroll
               for (int j = 0; j < JB; ++j) {
                 if constexpr (POL == 0)
-                  acc[j] = mfma_f64(ar[s0 + u], bv[u][j], acc[j]);
+                  acc[0][j] = mfma_f64(ar[0][s0 + u], bv[u][j], acc[0][j]);
                 else
-                  acc[j] = mfma_f32(ar[s0 + u], bv[u][j], acc[j]);
+                  acc[0][j] = mfma_f32(ar[0][s0 + u], bv[u][j], acc[0][j]);
               }
           }
         }
@@ -341,55 +382,60 @@ __global__ __launch_bounds__(RNT * CG) void contract_ra_kernel(const void* __res
     int ncol;
     tile_cols(tt, xbase, c0, ncol);
     if (!wact) return;
-    if (!rm.qone) {
-      TX* dst = Y + (tt / rm.nqt) * (long long)rm.Mtot * Q + (long long)rm.moff * Q + c0 + (lane & 15);
 #pragma unroll
-      for (int j = 0; j < JB; ++j)
+    for (int rb = 0; rb < RB; ++rb) {
+      const int mb = RMAX * rb;  // row offset of this row block
+      if (rb > 0 && 16 * w + mb >= M) break;
+      if (!rm.qone) {
+        TX* dst = Y + (tt / rm.nqt) * (long long)rm.Mtot * Q + (long long)rm.moff * Q + c0 + (lane & 15);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int m = orow(r);
-          if (m < M) {
-            TX& d = dst[(long long)m * Q + 16 * (cg * JB + j)];
-            d = rm.accum ? d + (TX)acc[j][r] : (TX)acc[j][r];
-          }
-        }
-    } else {
-      // output column c is the contiguous M-vector Y[(c0 + c) M ...]; with FP32
-      // accumulators a lane holds 4 consecutive rows of it (one 16-byte store)
+        for (int j = 0; j < JB; ++j)
 #pragma unroll
-      for (int j = 0; j < JB; ++j) {
-        const int c = 16 * (cg * JB + j) + (lane & 15);
-        if (c < ncol) {
-          TX* dst = Y + (c0 + c) * rm.Mtot + rm.moff;
-          if constexpr (POL == 0) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const int m = orow(r);
-              if (m < M) dst[m] = rm.accum ? dst[m] + acc[j][r] : acc[j][r];
-            }
-          } else if (rm.o16) {
-            const int m = orow(0);  // rows m .. m+3, M % 4 == 0
+          for (int r = 0; r < 4; ++r) {
+            const int m = orow(r) + mb;
             if (m < M) {
-              float4 o = float4{acc[j][0], acc[j][1], acc[j][2], acc[j][3]};
-              if (rm.accum) {
-                const float4 d = *(const float4*)(dst + m);
-                o.x += d.x; o.y += d.y; o.z += d.z; o.w += d.w;
-              }
-              *(float4*)(dst + m) = o;
+              TX& d = dst[(long long)m * Q + 16 * (cg * JB + j)];
+              d = rm.accum ? d + (TX)acc[rb][j][r] : (TX)acc[rb][j][r];
             }
-          } else {
-            // even M (Mtot, moff even): two 8-byte pairs, each inside the column or skipped
+          }
+      } else {
+        // output column c is the contiguous M-vector Y[(c0 + c) M ...]; with FP32
+        // accumulators a lane holds 4 consecutive rows of it (one 16-byte store)
 #pragma unroll
-            for (int h2 = 0; h2 < 2; ++h2) {
-              const int m = orow(2 * h2);
+        for (int j = 0; j < JB; ++j) {
+          const int c = 16 * (cg * JB + j) + (lane & 15);
+          if (c < ncol) {
+            TX* dst = Y + (c0 + c) * rm.Mtot + rm.moff;
+            if constexpr (POL == 0) {
+#pragma unroll
+              for (int r = 0; r < 4; ++r) {
+                const int m = orow(r) + mb;
+                if (m < M) dst[m] = rm.accum ? dst[m] + acc[rb][j][r] : acc[rb][j][r];
+              }
+            } else if (rm.o16) {
+              const int m = orow(0) + mb;  // rows m .. m+3, M % 4 == 0
               if (m < M) {
-                float2 o = float2{acc[j][2 * h2], acc[j][2 * h2 + 1]};
+                float4 o = float4{acc[rb][j][0], acc[rb][j][1], acc[rb][j][2], acc[rb][j][3]};
                 if (rm.accum) {
-                  const float2 d = *(const float2*)(dst + m);
-                  o.x += d.x;
-                  o.y += d.y;
+                  const float4 d = *(const float4*)(dst + m);
+                  o.x += d.x; o.y += d.y; o.z += d.z; o.w += d.w;
                 }
-                *(float2*)(dst + m) = o;
+                *(float4*)(dst + m) = o;
+              }
+            } else {
+              // even M (Mtot, moff even): two 8-byte pairs, each inside the column or skipped
+#pragma unroll
+              for (int h2 = 0; h2 < 2; ++h2) {
+                const int m = orow(2 * h2) + mb;
+                if (m < M) {
+                  float2 o = float2{acc[rb][j][2 * h2], acc[rb][j][2 * h2 + 1]};
+                  if (rm.accum) {
+                    const float2 d = *(const float2*)(dst + m);
+                    o.x += d.x;
+                    o.y += d.y;
+                  }
+                  *(float2*)(dst + m) = o;
+                }
               }
             }
           }
@@ -440,7 +486,9 @@ int launch_ra(const void* A, int M, int K, const void* X, void* Y, long long P, 
               int* rc, int Mtot, int moff, int lda, long long xps, int accum) {
   typedef typename RT<POL>::TX TX;
   constexpr int EPC = 16 / sizeof(TX);
-  if (M <= 0 || K <= 0 || M > RMAX || K > (POL == 3 ? RMAX3 : RMAX) || P <= 0 || Q <= 0) return 0;
+  // bf16x6 with K <= 128 takes M <= 256 in one launch (two row blocks per wave)
+  const int mmax = (POL == 3 && K <= RMAX) ? 2 * RMAX : RMAX;
+  if (M <= 0 || K <= 0 || M > mmax || K > (POL == 3 ? RMAX3 : RMAX) || P <= 0 || Q <= 0) return 0;
   if (((uintptr_t)X & 15) || ((uintptr_t)Y & 15)) return 0;
   RMap rm;
   rm.P = P;
@@ -500,6 +548,13 @@ int launch_ra(const void* A, int M, int K, const void* X, void* Y, long long P, 
       cgs = 1;
       kern = contract_ra_kernel<POL, 6, 1>;
       shm = RLds<POL, 6>::TOTAL;
+    } else if (M > RMAX) {
+      // one launch reads X once for all M rows: 8 waves, each with row
+      // blocks w and w + 8 sharing every staged B fragment
+      cgs = 1;
+      kern = kst <= 1 ? contract_ra_kernel<POL, 1, 1, 2> : kst <= 2 ? contract_ra_kernel<POL, 2, 1, 2>
+                                                           : contract_ra_kernel<POL, 4, 1, 2>;
+      shm = RLds<POL, 4>::TOTAL;
     } else {
       kern = kst <= 1 ? contract_ra_kernel<POL, 1, 2> : kst <= 2 ? contract_ra_kernel<POL, 2, 2>
                                                         : contract_ra_kernel<POL, 4, 2>;

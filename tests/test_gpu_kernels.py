@@ -430,3 +430,25 @@ def test_contract_row_split(dev, prec, tol, M, shape, axis):
     ref = np.moveaxis(np.tensordot(A, Xr, axes=([1], [axis])), 0, axis)
     assert Y.shape == ref.shape
     assert O.rel_l2(Y.to(torch.float64).cpu().numpy(), ref) < tol
+
+
+@pytest.mark.parametrize("M", [130, 192, 200, 256])
+@pytest.mark.parametrize("Kc", [30, 64, 126, 128])
+@pytest.mark.parametrize("shape,axis", [((70, None), 1), ((3, None, 64), 1), ((2, None, 192), 1), ((5, 33, None), 2)])
+def test_contract_bf16x6_two_row_blocks(dev, M, Kc, shape, axis):
+    """bf16x6 with 128 < M <= 256 and K <= 128: one launch, each wave holding
+    two 16-row blocks of the operator (the 3/2-padded backward transforms)."""
+    import torch
+
+    from jaxfun_amd import kernels as K
+
+    shape = tuple(Kc if d is None else d for d in shape)
+    rng = np.random.default_rng(M + 7 * Kc + axis)
+    A = rng.standard_normal((M, Kc))
+    X = rng.standard_normal(shape)
+    Xd = _t(X, dev, torch.float32)
+    Y = K.apply_along(K.AxisOperator(_t(A, dev)), Xd, axis, "bf16x6")
+    Xr = Xd.to(torch.float64).cpu().numpy()
+    ref = np.moveaxis(np.tensordot(A, Xr, axes=([1], [axis])), 0, axis)
+    assert Y.shape == ref.shape
+    assert O.rel_l2(Y.to(torch.float64).cpu().numpy(), ref) < 2e-6
