@@ -651,16 +651,24 @@ struct SumProd {
   const T* b[4];
   double c[4];
 };
-template <typename T>
+// COMMON: every term has the same first factor (u u_x + u u_y): it is read
+// once and the sum becomes a[i] * sum_t c_t b_t[i].
+template <typename T, bool COMMON>
 __global__ void sum_products_kernel(SumProd<T> sp, int nterms, T* __restrict__ out, long long n) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long stride = (long long)gridDim.x * blockDim.x;
   for (; i < n; i += stride) {
     T s = 0;
-    for (int t = 0; t < nterms; ++t) s += (T)sp.c[t] * sp.a[t][i] * sp.b[t][i];
+    if constexpr (COMMON) {
+      for (int t = 0; t < nterms; ++t) s += (T)sp.c[t] * sp.b[t][i];
+      s *= sp.a[0][i];
+    } else {
+      for (int t = 0; t < nterms; ++t) s += (T)sp.c[t] * sp.a[t][i] * sp.b[t][i];
+    }
     out[i] = s;
   }
 }
+
 
 // y = sum_j c_j x_j (n <= 8 inputs), optionally accumulating into y.
 template <typename T>
@@ -699,6 +707,27 @@ static unsigned grid_for(long long n, int block) {
   if (g > 65536) g = 65536;
   if (g < 1) g = 1;
   return (unsigned)g;
+}
+
+template <typename T>
+static int launch_sum_products(int nterms, const void* const* As, const void* const* Bs, const double* coeffs,
+                               void* out, long long n, hipStream_t stream) {
+  SumProd<T> sp{};
+  bool common = nterms > 1;
+  for (int t = 0; t < nterms; ++t) {
+    sp.a[t] = (const T*)As[t];
+    sp.b[t] = (const T*)Bs[t];
+    sp.c[t] = coeffs[t];
+    common = common && As[t] == As[0];
+  }
+  if (common)
+    hipLaunchKernelGGL((sum_products_kernel<T, true>), dim3(grid_for(n, 256)), dim3(256), 0, stream, sp, nterms,
+                       (T*)out, n);
+  else
+    hipLaunchKernelGGL((sum_products_kernel<T, false>), dim3(grid_for(n, 256)), dim3(256), 0, stream, sp, nterms,
+                       (T*)out, n);
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
 }
 
 }  // namespace jfx
@@ -867,29 +896,9 @@ int jfx_sum_products(int dtype, int nterms, const void* const* As, const void* c
                      const double* coeffs, void* out, long long n, hipStream_t stream) {
   if (nterms < 1 || nterms > 4 || !As || !Bs || !coeffs || !out) return set_error("sum_products: bad arguments"), JFX_EARG;
   if (n == 0) return JFX_OK;
-  if (dtype == 0) {
-    SumProd<double> sp{};
-    for (int t = 0; t < nterms; ++t) {
-      sp.a[t] = (const double*)As[t];
-      sp.b[t] = (const double*)Bs[t];
-      sp.c[t] = coeffs[t];
-    }
-    hipLaunchKernelGGL(sum_products_kernel<double>, dim3(grid_for(n, 256)), dim3(256), 0, stream, sp, nterms,
-                       (double*)out, n);
-  } else if (dtype == 1) {
-    SumProd<float> sp{};
-    for (int t = 0; t < nterms; ++t) {
-      sp.a[t] = (const float*)As[t];
-      sp.b[t] = (const float*)Bs[t];
-      sp.c[t] = coeffs[t];
-    }
-    hipLaunchKernelGGL(sum_products_kernel<float>, dim3(grid_for(n, 256)), dim3(256), 0, stream, sp, nterms,
-                       (float*)out, n);
-  } else {
-    return set_error("sum_products: bad dtype"), JFX_EARG;
-  }
-  JFX_LAUNCH_CHECK();
-  return JFX_OK;
+  if (dtype == 0) return launch_sum_products<double>(nterms, As, Bs, coeffs, out, n, stream);
+  if (dtype == 1) return launch_sum_products<float>(nterms, As, Bs, coeffs, out, n, stream);
+  return set_error("sum_products: bad dtype"), JFX_EARG;
 }
 
 // direction 0: f64 -> f32 (round to nearest), 1: f32 -> f64

@@ -154,7 +154,13 @@ class IMEXRungeKutta:
             if ky not in half:
                 half[ky] = K.apply_along(T[1].backward_operator(self.Np[1], ky), U, nb + 1, prec)
             fields[(kx, ky)] = K.apply_along(T[0].backward_operator(self.Np[0], kx), half[ky], nb, prec)
-        w = K.sum_products([(c, fields[o1], fields[o2]) for c, o1, o2 in self.nonlinear])
+        # a factor shared by every term goes first: the kernel then reads it once
+        terms = list(self.nonlinear)
+        shared = set.intersection(*[{o1, o2} for _, o1, o2 in terms])
+        if shared:
+            f = min(shared)
+            terms = [(c, o1, o2) if o1 == f else (c, o2, o1) for c, o1, o2 in terms]
+        w = K.sum_products([(c, fields[o1], fields[o2]) for c, o1, o2 in terms])
         return T.scalar_product(w, precision=prec)
 
     # -------------------------------------------------------------- stepping

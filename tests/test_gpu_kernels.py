@@ -452,3 +452,24 @@ def test_contract_bf16x6_two_row_blocks(dev, M, Kc, shape, axis):
     ref = np.moveaxis(np.tensordot(A, Xr, axes=([1], [axis])), 0, axis)
     assert Y.shape == ref.shape
     assert O.rel_l2(Y.to(torch.float64).cpu().numpy(), ref) < 2e-6
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_sum_products(dev, dtype):
+    """out = sum_t c_t a_t b_t, with the shared-first-factor path
+    (u u_x + u u_y reads u once) and the general path."""
+    import torch
+
+    from jaxfun_amd import kernels as K
+
+    dt = torch.float64 if dtype == "f64" else torch.float32
+    tol = 1e-14 if dtype == "f64" else 2e-6
+    rng = np.random.default_rng(21)
+    u, ux, uy, v = (rng.standard_normal((3, 50, 70)) for _ in range(4))
+    tu, tux, tuy, tv = (_t(a, dev, dt) for a in (u, ux, uy, v))
+    w = K.sum_products([(-1.0, tu, tux), (-1.0, tu, tuy)]).to(torch.float64).cpu().numpy()
+    assert O.rel_l2(w, -(u * ux + u * uy)) < tol
+    w = K.sum_products([(2.0, tu, tux), (-0.5, tv, tuy), (1.5, tux, tuy)]).to(torch.float64).cpu().numpy()
+    assert O.rel_l2(w, 2 * u * ux - 0.5 * v * uy + 1.5 * ux * uy) < tol
+    w = K.sum_products([(3.0, tu, tv)]).to(torch.float64).cpu().numpy()
+    assert O.rel_l2(w, 3 * u * v) < tol
