@@ -651,24 +651,32 @@ struct SumProd {
   const T* b[4];
   double c[4];
 };
-// COMMON: every term has the same first factor (u u_x + u u_y): it is read
-// once and the sum becomes a[i] * sum_t c_t b_t[i].
-template <typename T, bool COMMON>
-__global__ void sum_products_kernel(SumProd<T> sp, int nterms, T* __restrict__ out, long long n) {
+// NT terms (compile time, so every load of an element issues before its
+// arithmetic).  COMMON: every term has the same first factor (u u_x + u u_y):
+// it is read once and the sum becomes a[i] * sum_t c_t b_t[i].
+template <typename T, int NT, bool COMMON>
+__global__ void sum_products_kernel(SumProd<T> sp, T* __restrict__ out, long long n) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long stride = (long long)gridDim.x * blockDim.x;
   for (; i < n; i += stride) {
+    T a[NT], b[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      if (!COMMON || t == 0) a[t] = sp.a[t][i];
+      b[t] = sp.b[t][i];
+    }
     T s = 0;
     if constexpr (COMMON) {
-      for (int t = 0; t < nterms; ++t) s += (T)sp.c[t] * sp.b[t][i];
-      s *= sp.a[0][i];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) s += (T)sp.c[t] * b[t];
+      s *= a[0];
     } else {
-      for (int t = 0; t < nterms; ++t) s += (T)sp.c[t] * sp.a[t][i] * sp.b[t][i];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) s += (T)sp.c[t] * a[t] * b[t];
     }
     out[i] = s;
   }
 }
-
 
 // y = sum_j c_j x_j (n <= 8 inputs), optionally accumulating into y.
 template <typename T>
@@ -676,13 +684,18 @@ struct LinComb {
   const T* x[8];
   double c[8];
 };
-template <typename T>
-__global__ void lincomb_kernel(LinComb<T> lc, int nin, T* __restrict__ y, long long n, int accumulate) {
+// NIN inputs (compile time, so an element's loads all issue before its sum).
+template <typename T, int NIN>
+__global__ void lincomb_kernel(LinComb<T> lc, T* __restrict__ y, long long n, int accumulate) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long stride = (long long)gridDim.x * blockDim.x;
   for (; i < n; i += stride) {
+    T x[NIN > 0 ? NIN : 1];
+#pragma unroll
+    for (int j = 0; j < NIN; ++j) x[j] = lc.x[j][i];
     double s = accumulate ? (double)y[i] : 0.0;
-    for (int j = 0; j < nin; ++j) s += lc.c[j] * (double)lc.x[j][i];
+#pragma unroll
+    for (int j = 0; j < NIN; ++j) s += lc.c[j] * (double)x[j];
     y[i] = (T)s;
   }
 }
@@ -709,6 +722,14 @@ static unsigned grid_for(long long n, int block) {
   return (unsigned)g;
 }
 
+template <typename T, int NT>
+static void launch_sum_products_nt(const SumProd<T>& sp, bool common, T* out, long long n, hipStream_t stream) {
+  if (common)
+    hipLaunchKernelGGL((sum_products_kernel<T, NT, true>), dim3(grid_for(n, 256)), dim3(256), 0, stream, sp, out, n);
+  else
+    hipLaunchKernelGGL((sum_products_kernel<T, NT, false>), dim3(grid_for(n, 256)), dim3(256), 0, stream, sp, out, n);
+}
+
 template <typename T>
 static int launch_sum_products(int nterms, const void* const* As, const void* const* Bs, const double* coeffs,
                                void* out, long long n, hipStream_t stream) {
@@ -720,12 +741,41 @@ static int launch_sum_products(int nterms, const void* const* As, const void* co
     sp.c[t] = coeffs[t];
     common = common && As[t] == As[0];
   }
-  if (common)
-    hipLaunchKernelGGL((sum_products_kernel<T, true>), dim3(grid_for(n, 256)), dim3(256), 0, stream, sp, nterms,
-                       (T*)out, n);
-  else
-    hipLaunchKernelGGL((sum_products_kernel<T, false>), dim3(grid_for(n, 256)), dim3(256), 0, stream, sp, nterms,
-                       (T*)out, n);
+  switch (nterms) {
+    case 1: launch_sum_products_nt<T, 1>(sp, false, (T*)out, n, stream); break;
+    case 2: launch_sum_products_nt<T, 2>(sp, common, (T*)out, n, stream); break;
+    case 3: launch_sum_products_nt<T, 3>(sp, common, (T*)out, n, stream); break;
+    default: launch_sum_products_nt<T, 4>(sp, common, (T*)out, n, stream); break;
+  }
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
+}
+
+template <typename T, int NIN>
+static void launch_lincomb_n(const LinComb<T>& lc, T* y, long long n, int accumulate, hipStream_t stream) {
+  hipLaunchKernelGGL((lincomb_kernel<T, NIN>), dim3(grid_for(n, 256)), dim3(256), 0, stream, lc, y, n, accumulate);
+}
+
+template <typename T>
+static int launch_lincomb(int nin, const void* const* xs, const double* coeffs, void* y, long long n, int accumulate,
+                          hipStream_t stream) {
+  LinComb<T> lc{};
+  for (int j = 0; j < nin; ++j) {
+    lc.x[j] = (const T*)xs[j];
+    lc.c[j] = coeffs[j];
+  }
+  T* yo = (T*)y;
+  switch (nin) {
+    case 0: launch_lincomb_n<T, 0>(lc, yo, n, accumulate, stream); break;
+    case 1: launch_lincomb_n<T, 1>(lc, yo, n, accumulate, stream); break;
+    case 2: launch_lincomb_n<T, 2>(lc, yo, n, accumulate, stream); break;
+    case 3: launch_lincomb_n<T, 3>(lc, yo, n, accumulate, stream); break;
+    case 4: launch_lincomb_n<T, 4>(lc, yo, n, accumulate, stream); break;
+    case 5: launch_lincomb_n<T, 5>(lc, yo, n, accumulate, stream); break;
+    case 6: launch_lincomb_n<T, 6>(lc, yo, n, accumulate, stream); break;
+    case 7: launch_lincomb_n<T, 7>(lc, yo, n, accumulate, stream); break;
+    default: launch_lincomb_n<T, 8>(lc, yo, n, accumulate, stream); break;
+  }
   JFX_LAUNCH_CHECK();
   return JFX_OK;
 }
@@ -923,27 +973,9 @@ int jfx_lincomb(int dtype, int nin, const void* const* xs, const double* coeffs,
   if (nin < 0 || nin > 8 || !y || (nin > 0 && (!xs || !coeffs)))
     return set_error("lincomb: bad arguments"), JFX_EARG;
   if (n == 0) return JFX_OK;
-  if (dtype == 0) {
-    LinComb<double> lc{};
-    for (int j = 0; j < nin; ++j) {
-      lc.x[j] = (const double*)xs[j];
-      lc.c[j] = coeffs[j];
-    }
-    hipLaunchKernelGGL(lincomb_kernel<double>, dim3(grid_for(n, 256)), dim3(256), 0, stream, lc, nin,
-                       (double*)y, n, accumulate);
-  } else if (dtype == 1) {
-    LinComb<float> lc{};
-    for (int j = 0; j < nin; ++j) {
-      lc.x[j] = (const float*)xs[j];
-      lc.c[j] = coeffs[j];
-    }
-    hipLaunchKernelGGL(lincomb_kernel<float>, dim3(grid_for(n, 256)), dim3(256), 0, stream, lc, nin,
-                       (float*)y, n, accumulate);
-  } else {
-    return set_error("lincomb: bad dtype"), JFX_EARG;
-  }
-  JFX_LAUNCH_CHECK();
-  return JFX_OK;
+  if (dtype == 0) return launch_lincomb<double>(nin, xs, coeffs, y, n, accumulate, stream);
+  if (dtype == 1) return launch_lincomb<float>(nin, xs, coeffs, y, n, accumulate, stream);
+  return set_error("lincomb: bad dtype"), JFX_EARG;
 }
 
 }  // extern "C"

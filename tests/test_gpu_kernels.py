@@ -473,3 +473,26 @@ def test_sum_products(dev, dtype):
     assert O.rel_l2(w, 2 * u * ux - 0.5 * v * uy + 1.5 * ux * uy) < tol
     w = K.sum_products([(3.0, tu, tv)]).to(torch.float64).cpu().numpy()
     assert O.rel_l2(w, 3 * u * v) < tol
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("nin", [1, 2, 3, 5, 8, 11])
+def test_lincomb_counts(dev, dtype, nin):
+    """y = sum_j c_j x_j for every unrolled input count (more than 8 inputs go
+    in chunks of 8 accumulating into y), with and without accumulation."""
+    import torch
+
+    from jaxfun_amd import kernels as K
+
+    dt = torch.float64 if dtype == "f64" else torch.float32
+    tol = 1e-14 if dtype == "f64" else 2e-6
+    rng = np.random.default_rng(nin)
+    xs = [rng.standard_normal(1001) for _ in range(nin)]
+    cs = rng.standard_normal(nin)
+    y = K.lincomb([(float(c), _t(x, dev, dt)) for c, x in zip(cs, xs)]).to(torch.float64).cpu().numpy()
+    ref = sum(c * x for c, x in zip(cs, xs))
+    assert O.rel_l2(y, ref) < tol
+    y0 = rng.standard_normal(1001)
+    out = _t(y0, dev, dt)
+    K.lincomb([(float(c), _t(x, dev, dt)) for c, x in zip(cs, xs)], out=out, accumulate=True)
+    assert O.rel_l2(out.to(torch.float64).cpu().numpy(), ref + y0) < tol
