@@ -193,12 +193,13 @@ int jfx_band_lu_solve_mode(const double* lu, int n, int p, int q, double* X, lon
  * and TensorMatrix.solve (la/tensormatrix.py:76-80). */
 int jfx_dense_lu_factor(int cplx, void* A, int n, int* ipiv, int* perm, int* status_dev, hipStream_t stream);
 /* X (n x nrhs, row-major) = A^-1 B from the factors above: X = P B, then
- * blocked forward (unit L) and backward (U) substitution, 32-row blocks
- * solved in registers with GEMM updates of the remaining rows.
- * B and X must not overlap.  Replaces jax.scipy.linalg.lu_solve in
- * Matrix.lu_solve (la/matrix.py:169-214). */
+ * blocked forward (unit L) and backward (U) substitution, one launch per
+ * 32-row block (16 complex) that solves the block in registers and updates
+ * the remaining rows; work holds n x nrhs values of the intermediate
+ * L^-1 P B.  B, X and work must not overlap.  Replaces
+ * jax.scipy.linalg.lu_solve in Matrix.lu_solve (la/matrix.py:169-214). */
 int jfx_dense_lu_solve(int cplx, const void* LU, int n, const int* perm, const void* B, void* X, long long nrhs,
-                       hipStream_t stream);
+                       void* work, hipStream_t stream);
 
 /* nsys independent band matrices, band (nsys, n, p+q+1) "by row"; status_dev
  * receives 0 or 1 + the row of a zero / non-finite pivot in a failing system.

@@ -13,8 +13,8 @@
 // Pivots are recorded as ipiv[j] (row swapped with row j at step j) and,
 // at the end, as the permutation perm (row i of P A is row perm[i] of A).
 // The solve gathers P b, then runs blocked forward (unit L) and backward (U)
-// substitution: per 32-row block a triangular solve of the
-// block, then a GEMM update of the remaining rows.
+// substitution, one launch per 32-row block (16 complex; the block's triangular solve
+// and the update of the remaining rows fused, with a work buffer in between).
 // Replaces jax.scipy.linalg.lu_factor / lu_solve as used by Matrix.lu_factor
 // (la/matrix.py:138-214), TPMatricesDenseLUFactors (la/tpmatrix.py:1016-1086)
 // and TensorMatrix.solve (la/tensormatrix.py:76-80).
@@ -54,14 +54,14 @@ __device__ __forceinline__ bool dl_ok(zd a) { return (a.re != 0.0 || a.im != 0.0
 constexpr int DL_NB = 32;     // largest panel width
 constexpr int DL_PT = 1024;   // panel workgroup
 // Panel width and solve block: a column's (right-hand side's) entries of one
-// block live in registers (panel: 32 doubles or 16 complex values; solve: 32).
+// block live in registers (32 doubles or 16 complex values).
 template <typename T>
 constexpr int dl_nb() {
   return sizeof(T) == sizeof(double) ? 32 : 16;
 }
 template <typename T>
-constexpr int dl_nbs() {
-  return 32;
+constexpr int dl_nbs() {  // the fused solve step's LDS (3 tiles) stays under 64 KB
+  return sizeof(T) == sizeof(double) ? 32 : 16;
 }
 constexpr int DL_MAXN = 16384;
 
@@ -431,43 +431,102 @@ __global__ void dl_gather_kernel(const T* __restrict__ B, const int* __restrict_
 // Rows kb .. kb + nbs - 1 of X: X_k <- T_kk^-1 X_k with T_kk the unit lower
 // (LOWER) or upper (else) diagonal block of LU; one thread per right-hand
 // side, the block in LDS.
+// One block step of the substitution, fused: every workgroup solves the
+// diagonal block T_kk x_k = Tb[block k] for its 64 right-hand sides (in
+// registers; the unit lower L_kk (LOWER) or U_kk), workgroup row 0 writes x_k
+// to Out, and workgroup row r >= 1 updates its 64 rows of Tb below (LOWER)
+// or above the block: Tb[rows] -= T[rows, block k] x_k.  Tb's block-k rows
+// are only read in this launch, so the redundant solves agree and no
+// workgroup waits for another.
 template <typename T, bool LOWER>
-__global__ __launch_bounds__(256) void dl_trsv_block_kernel(const T* __restrict__ LU, int n, int kb, int nbs,
-                                                             T* __restrict__ X, long long nrhs) {
+__global__ __launch_bounds__(256) void dl_solve_step_kernel(const T* __restrict__ LU, int n, int kb, int nbs,
+                                                             T* __restrict__ Tb, T* __restrict__ Out,
+                                                             long long nrhs) {
   constexpr int NBS = dl_nbs<T>();
   __shared__ T D[NBS][NBS + 1];
+  __shared__ T ys[NBS][65];
+  __shared__ T Ls[64][NBS + 1];
   const int tid = threadIdx.x;
+  const long long c0 = (long long)blockIdx.x * 64;
+  const int rt = blockIdx.y;
   for (int e = tid; e < nbs * nbs; e += 256)
     D[e / nbs][e % nbs] = LU[(long long)(kb + e / nbs) * n + kb + e % nbs];
+  // rows this workgroup updates: below the block (LOWER) or above it
+  const int r0 = LOWER ? kb + nbs + (rt - 1) * 64 : (rt - 1) * 64;
+  const int rend = LOWER ? n : kb;
+  if (rt > 0)
+    for (int e = tid; e < 64 * nbs; e += 256) {
+      const int r = e / nbs, k = e % nbs;
+      Ls[r][k] = r0 + r < rend ? LU[(long long)(r0 + r) * n + kb + k] : dl_zero<T>();
+    }
   __syncthreads();
-  const long long r = (long long)blockIdx.x * 256 + tid;
-  if (r >= nrhs) return;
-  T x[NBS];
+  if (tid < 64) {
+    const long long c = c0 + tid;
+    T x[NBS];
 #pragma unroll
-  for (int t = 0; t < NBS; ++t) x[t] = t < nbs ? X[(long long)(kb + t) * nrhs + r] : dl_zero<T>();
-  if (LOWER) {
+    for (int t = 0; t < NBS; ++t) x[t] = (t < nbs && c < nrhs) ? Tb[(long long)(kb + t) * nrhs + c] : dl_zero<T>();
+    if (LOWER) {
 #pragma unroll
-    for (int i = 1; i < NBS; ++i) {
-      if (i < nbs) {
+      for (int i = 1; i < NBS; ++i) {
+        if (i < nbs) {
 #pragma unroll
-        for (int t = 0; t < NBS; ++t)
-          if (t < i) x[i] = dl_fms(x[i], D[i][t], x[t]);
+          for (int t = 0; t < NBS; ++t)
+            if (t < i) x[i] = dl_fms(x[i], D[i][t], x[t]);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int i = NBS - 1; i >= 0; --i) {
+        if (i < nbs) {
+#pragma unroll
+          for (int t = 0; t < NBS; ++t)
+            if (t > i && t < nbs) x[i] = dl_fms(x[i], D[i][t], x[t]);
+          x[i] = dl_div(x[i], D[i][i]);
+        }
       }
     }
-  } else {
 #pragma unroll
-    for (int i = NBS - 1; i >= 0; --i) {
-      if (i < nbs) {
+    for (int t = 0; t < NBS; ++t) {
+      ys[t][tid] = x[t];
+      if (rt == 0 && t < nbs && c < nrhs) Out[(long long)(kb + t) * nrhs + c] = x[t];
+    }
+  }
+  if (rt == 0) return;
+  __syncthreads();
+  // Tb[r0 + r][c0 + c] -= sum_k Ls[r][k] ys[k][c]: 4 rows x 4 columns per thread
+  const int tr = tid >> 4, tc = tid & 15;
+  T acc[4][4];
 #pragma unroll
-        for (int t = 0; t < NBS; ++t)
-          if (t > i && t < nbs) x[i] = dl_fms(x[i], D[i][t], x[t]);
-        x[i] = dl_div(x[i], D[i][i]);
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = dl_zero<T>();
+  for (int k = 0; k < nbs; ++k) {
+    T av[4], bv[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) av[a] = Ls[tr + 16 * a][k];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) bv[b] = ys[k][tc + 16 * b];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = dl_fms(acc[a][b], av[a], bv[b]);
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int r = r0 + tr + 16 * a;
+    if (r >= rend) continue;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const long long c = c0 + tc + 16 * b;
+      if (c < nrhs) {
+        T* p = Tb + (long long)r * nrhs + c;
+        if constexpr (sizeof(T) == sizeof(double))
+          *p = *p + acc[a][b];
+        else
+          *p = zd{p->re + acc[a][b].re, p->im + acc[a][b].im};
       }
     }
   }
-#pragma unroll
-  for (int t = 0; t < NBS; ++t)
-    if (t < nbs) X[(long long)(kb + t) * nrhs + r] = x[t];
 }
 
 template <typename T>
@@ -512,28 +571,25 @@ static int dl_factor(T* A, int n, int* ipiv, int* perm, int* status, hipStream_t
 }
 
 template <typename T>
-static int dl_solve(const T* LU, int n, const int* perm, const T* B, T* X, long long nrhs, hipStream_t st) {
+static int dl_solve(const T* LU, int n, const int* perm, const T* B, T* X, T* W, long long nrhs, hipStream_t st) {
   constexpr int NBS = dl_nbs<T>();
   const long long tot = (long long)n * nrhs;
   hipLaunchKernelGGL(dl_gather_kernel<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, B, perm, X, n,
                      nrhs);
   JFX_LAUNCH_CHECK();
-  const dim3 g((unsigned)((nrhs + 255) / 256));
-  for (int kb = 0; kb < n; kb += NBS) {  // forward: L y = P b
+  const unsigned gx = (unsigned)((nrhs + 63) / 64);
+  for (int kb = 0; kb < n; kb += NBS) {  // forward, L y = P b: X (updated) -> W
     const int nbs = n - kb < NBS ? n - kb : NBS;
-    hipLaunchKernelGGL((dl_trsv_block_kernel<T, true>), g, dim3(256), 0, st, LU, n, kb, nbs, X, nrhs);
+    const unsigned gy = 1 + (unsigned)((n - kb - nbs + 63) / 64);
+    hipLaunchKernelGGL((dl_solve_step_kernel<T, true>), dim3(gx, gy), dim3(256), 0, st, LU, n, kb, nbs, X, W, nrhs);
     JFX_LAUNCH_CHECK();
-    const int r = dl_gemm_sub<T>(LU + (long long)(kb + nbs) * n + kb, n, X + (long long)kb * nrhs, nrhs,
-                                 X + (long long)(kb + nbs) * nrhs, nrhs, n - kb - nbs, (int)nrhs, nbs, st);
-    if (r) return r;
   }
   const int last = (n - 1) / NBS * NBS;
-  for (int kb = last; kb >= 0; kb -= NBS) {  // backward: U x = y
+  for (int kb = last; kb >= 0; kb -= NBS) {  // backward, U x = y: W (updated) -> X
     const int nbs = n - kb < NBS ? n - kb : NBS;
-    hipLaunchKernelGGL((dl_trsv_block_kernel<T, false>), g, dim3(256), 0, st, LU, n, kb, nbs, X, nrhs);
+    const unsigned gy = 1 + (unsigned)((kb + 63) / 64);
+    hipLaunchKernelGGL((dl_solve_step_kernel<T, false>), dim3(gx, gy), dim3(256), 0, st, LU, n, kb, nbs, W, X, nrhs);
     JFX_LAUNCH_CHECK();
-    const int r = dl_gemm_sub<T>(LU + kb, n, X + (long long)kb * nrhs, nrhs, X, nrhs, kb, (int)nrhs, nbs, st);
-    if (r) return r;
   }
   return JFX_OK;
 }
@@ -553,14 +609,16 @@ int jfx_dense_lu_factor(int cplx, void* A, int n, int* ipiv, int* perm, int* sta
 }
 
 int jfx_dense_lu_solve(int cplx, const void* LU, int n, const int* perm, const void* B, void* X, long long nrhs,
-                       hipStream_t stream) {
-  if (!LU || !perm || !B || !X || n <= 0 || nrhs < 0 || B == X || (cplx != 0 && cplx != 1))
-    return set_error("dense_lu_solve: bad arguments (B and X must differ)"), JFX_EARG;
+                       void* work, hipStream_t stream) {
+  if (!LU || !perm || !B || !X || !work || n <= 0 || nrhs < 0 || B == X || work == X || work == B ||
+      (cplx != 0 && cplx != 1))
+    return set_error("dense_lu_solve: bad arguments (B, X and work must be distinct)"), JFX_EARG;
   if (n > DL_MAXN) return set_error("dense_lu_solve: n = %d exceeds %d", n, DL_MAXN), JFX_ESHAPE;
   if (nrhs == 0) return JFX_OK;
   if (nrhs > 0x7fffffffLL) return set_error("dense_lu_solve: too many right-hand sides"), JFX_ESHAPE;
-  return cplx ? dl_solve<zd>((const zd*)LU, n, perm, (const zd*)B, (zd*)X, nrhs, stream)
-              : dl_solve<double>((const double*)LU, n, perm, (const double*)B, (double*)X, nrhs, stream);
+  return cplx ? dl_solve<zd>((const zd*)LU, n, perm, (const zd*)B, (zd*)X, (zd*)work, nrhs, stream)
+              : dl_solve<double>((const double*)LU, n, perm, (const double*)B, (double*)X, (double*)work, nrhs,
+                                 stream);
 }
 
 }  // extern "C"

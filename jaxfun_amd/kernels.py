@@ -349,19 +349,22 @@ def dense_lu_solve(LU: torch.Tensor, perm: torch.Tensor, B: torch.Tensor) -> tor
         B2 = (B2 if B2.is_complex() else to_complex(B2.contiguous())).contiguous()
         _req(B2, torch.complex128, "B")
         X = torch.empty_like(B2)
-        _lib.call("jfx_dense_lu_solve", 1, _p(LU), n, _p(perm), _p(B2), _p(X), B2.shape[1], _s())
+        W = torch.empty_like(B2)
+        _lib.call("jfx_dense_lu_solve", 1, _p(LU), n, _p(perm), _p(B2), _p(X), B2.shape[1], _p(W), _s())
     elif B2.is_complex():
         B2 = B2.contiguous()
         _req(B2, torch.complex128, "B")
         Br = torch.view_as_real(B2).reshape(n, -1)  # (n, 2 nrhs): re, im interleaved
         Xr = torch.empty_like(Br)
-        _lib.call("jfx_dense_lu_solve", 0, _p(LU), n, _p(perm), _p(Br), _p(Xr), Br.shape[1], _s())
+        W = torch.empty_like(Br)
+        _lib.call("jfx_dense_lu_solve", 0, _p(LU), n, _p(perm), _p(Br), _p(Xr), Br.shape[1], _p(W), _s())
         X = torch.view_as_complex(Xr.reshape(n, -1, 2))
     else:
         B2 = B2.contiguous()
         _req(B2, torch.float64, "B")
         X = torch.empty_like(B2)
-        _lib.call("jfx_dense_lu_solve", 0, _p(LU), n, _p(perm), _p(B2), _p(X), B2.shape[1], _s())
+        W = torch.empty_like(B2)
+        _lib.call("jfx_dense_lu_solve", 0, _p(LU), n, _p(perm), _p(B2), _p(X), B2.shape[1], _p(W), _s())
     return X.reshape(n) if vec else X
 
 
