@@ -491,7 +491,8 @@ __global__ __launch_bounds__(64 * NW) void band_lu_solve_wave_kernel(const doubl
   double* __restrict__ x = X + pp * (long long)n * Q + qq;
   const int W = p + q + 1;
   const int c = (n + 64 * NW - 1) / (64 * NW);
-  const int i0 = min(n, (int)threadIdx.x * c), i1 = min(n, i0 + c);
+  const int i0 = (int)min((long long)n, (long long)threadIdx.x * c);
+  const int i1 = (int)min((long long)n, (long long)i0 + c);
   // forward: y_i = b_i - sum_t L[i, i-1-t] y_{i-1-t}
   wave_sweep<B, RC, NW, true>(lu, x, Q, n, W, p, q, i0, i1, lane, wv, sm, ss);
   if constexpr (NW > 1) __syncthreads();  // sm / ss are reused
