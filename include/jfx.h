@@ -214,6 +214,13 @@ int jfx_band_lu_factor_batched(const double* band, int nsys, int n, int p, int q
  * TPMatricesWavenumberSolver.solve (la/tpmatrix.py:590-690, 800-1011). */
 int jfx_band_lu_solve_batched(const double* lu, int nsys, int n, int p, int q, double* X, long long P, long long Q,
                               int qdiv, hipStream_t stream);
+/* As jfx_band_lu_solve_batched with the kernel chosen by mode, as in
+ * jfx_band_lu_solve_mode (0 automatic, 1 one lane per line, 2 wavefronts per
+ * line with each line's own factor; jfx_band_lu_solve_batched is mode 0).
+ * The same per-wavenumber solve (la/tpmatrix.py:590-690, 800-1011) for few
+ * lines. */
+int jfx_band_lu_solve_batched_mode(const double* lu, int nsys, int n, int p, int q, double* X, long long P,
+                                   long long Q, int qdiv, int mode, hipStream_t stream);
 
 /* One large band matrix (the Kronecker matrix of banded factors), same band
  * storage: LU without pivoting by one 1024-thread workgroup (each

@@ -476,12 +476,15 @@ __device__ __forceinline__ void wave_sweep(const double* __restrict__ lu, double
 
 // Solve L U x = b in place along the axis of length n of a (P, n, Q) batch;
 // NW wavefronts per line, one line per workgroup (grid = P * Q).  p, q <= B.
+// With nsys > 1 line (pp, qq) uses factor (pp * (Q / qdiv) + qq / qdiv) mod
+// nsys of lu (nsys, n, p + q + 1), as the per-lane batched solve does.
 // Each lane streams its own chunk, so the lanes' accesses do not coalesce;
 // the chunks are short (n / (64 NW) rows) and the rows stay in L1/L2 between
 // the passes.
 template <int B, int NW>
 __global__ __launch_bounds__(64 * NW) void band_lu_solve_wave_kernel(const double* __restrict__ lu, int n, int p,
-                                                                     int q, double* __restrict__ X, long long Q) {
+                                                                     int q, double* __restrict__ X, long long Q,
+                                                                     int nsys, int qdiv) {
   constexpr int RC = (B == 4 && NW > 1) ? 4 : 8;  // 256 VGPRs at 512 threads
   __shared__ double sm[NW * (B * B + B)];
   __shared__ double ss[NW * B];
@@ -490,6 +493,7 @@ __global__ __launch_bounds__(64 * NW) void band_lu_solve_wave_kernel(const doubl
   const long long pp = line / Q, qq = line - pp * Q;
   double* __restrict__ x = X + pp * (long long)n * Q + qq;
   const int W = p + q + 1;
+  if (nsys > 1) lu += ((pp * (Q / qdiv) + qq / qdiv) % nsys) * (long long)n * W;
   const int c = (n + 64 * NW - 1) / (64 * NW);
   const int i0 = (int)min((long long)n, (long long)threadIdx.x * c);
   const int i1 = (int)min((long long)n, (long long)i0 + c);
@@ -785,6 +789,29 @@ static int launch_lincomb(int nin, const void* const* xs, const double* coeffs, 
 
 using namespace jfx;
 
+// Wavefronts per line (p, q <= WAVE_MAXB): eight waves per line while the
+// lines alone leave most of the GPU idle and the chunks stay at least a batch
+// of rows long, else one.
+static int launch_band_wave(const double* lu, int n, int p, int q, double* X, long long lines, long long Q, int nsys,
+                            int qdiv, hipStream_t stream) {
+  if (lines > 0x7fffffffLL) return set_error("band_lu_solve: too many lines for the wavefront solve"), JFX_EARG;
+  const bool wide = lines <= WAVE_WIDE_LINES && n >= 64 * 8 * 8;
+  const dim3 grid((unsigned)lines);
+  if (p <= 2 && q <= 2) {
+    if (wide)
+      hipLaunchKernelGGL((band_lu_solve_wave_kernel<2, 8>), grid, dim3(512), 0, stream, lu, n, p, q, X, Q, nsys, qdiv);
+    else
+      hipLaunchKernelGGL((band_lu_solve_wave_kernel<2, 1>), grid, dim3(64), 0, stream, lu, n, p, q, X, Q, nsys, qdiv);
+  } else {
+    if (wide)
+      hipLaunchKernelGGL((band_lu_solve_wave_kernel<4, 8>), grid, dim3(512), 0, stream, lu, n, p, q, X, Q, nsys, qdiv);
+    else
+      hipLaunchKernelGGL((band_lu_solve_wave_kernel<4, 1>), grid, dim3(64), 0, stream, lu, n, p, q, X, Q, nsys, qdiv);
+  }
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
+}
+
 extern "C" {
 
 int jfx_band_lu_factor(const double* band, int n, int p, int q, double* lu, int* status_dev,
@@ -809,22 +836,8 @@ int jfx_band_lu_solve_mode(const double* lu, int n, int p, int q, double* X, lon
   if (mode == 0)
     mode = (narrow && n >= WAVE_MIN_N && lines <= WAVE_MAX_LINES) ? 2 : 1;
   if (mode == 2) {
-    if (lines > 0x7fffffffLL) return set_error("band_lu_solve: too many lines for the wavefront solve"), JFX_EARG;
-    // eight waves per line while the lines alone leave most of the GPU idle
-    // and the chunks stay at least a batch of rows long
-    const bool wide = lines <= WAVE_WIDE_LINES && n >= 64 * 8 * 8;
-    const dim3 grid((unsigned)lines);
-    if (p <= 2 && q <= 2) {
-      if (wide)
-        hipLaunchKernelGGL((band_lu_solve_wave_kernel<2, 8>), grid, dim3(512), 0, stream, lu, n, p, q, X, Q);
-      else
-        hipLaunchKernelGGL((band_lu_solve_wave_kernel<2, 1>), grid, dim3(64), 0, stream, lu, n, p, q, X, Q);
-    } else {
-      if (wide)
-        hipLaunchKernelGGL((band_lu_solve_wave_kernel<4, 8>), grid, dim3(512), 0, stream, lu, n, p, q, X, Q);
-      else
-        hipLaunchKernelGGL((band_lu_solve_wave_kernel<4, 1>), grid, dim3(64), 0, stream, lu, n, p, q, X, Q);
-    }
+    const int rc = launch_band_wave(lu, n, p, q, X, lines, Q, 1, 1, stream);
+    if (rc) return rc;
   } else {
     hipLaunchKernelGGL(band_lu_solve_kernel, dim3((unsigned)((lines + 255) / 256)), dim3(256), 0, stream, lu, n, p,
                        q, X, P, Q);
@@ -848,19 +861,31 @@ int jfx_band_lu_factor_batched(const double* band, int nsys, int n, int p, int q
   return JFX_OK;
 }
 
-int jfx_band_lu_solve_batched(const double* lu, int nsys, int n, int p, int q, double* X, long long P, long long Q,
-                              int qdiv, hipStream_t stream) {
+int jfx_band_lu_solve_batched_mode(const double* lu, int nsys, int n, int p, int q, double* X, long long P,
+                                   long long Q, int qdiv, int mode, hipStream_t stream) {
   if (!lu || !X || nsys <= 0 || n <= 0 || p < 0 || q < 0 || P < 0 || Q <= 0 || qdiv < 1 || Q % qdiv ||
-      (P * (Q / qdiv)) % nsys)
+      (P * (Q / qdiv)) % nsys || mode < 0 || mode > 2)
     return set_error("band_lu_solve_batched: bad arguments (%d systems for P=%lld, Q=%lld, qdiv=%d)", nsys, P, Q,
                      qdiv),
            JFX_EARG;
+  const bool narrow = p <= WAVE_MAXB && q <= WAVE_MAXB;
+  if (mode == 2 && !narrow)
+    return set_error("band_lu_solve_batched: the wavefront solve takes bandwidths <= %d (p=%d, q=%d)", WAVE_MAXB,
+                     p, q),
+           JFX_EARG;
   long long lines = P * Q;
   if (lines == 0) return JFX_OK;
+  if (mode == 0) mode = (narrow && n >= WAVE_MIN_N && lines <= WAVE_MAX_LINES) ? 2 : 1;
+  if (mode == 2) return launch_band_wave(lu, n, p, q, X, lines, Q, nsys, qdiv, stream);
   hipLaunchKernelGGL(band_lu_solve_batched_kernel, dim3((unsigned)((lines + 255) / 256)), dim3(256), 0, stream,
                      lu, nsys, n, p, q, X, P, Q, qdiv);
   JFX_LAUNCH_CHECK();
   return JFX_OK;
+}
+
+int jfx_band_lu_solve_batched(const double* lu, int nsys, int n, int p, int q, double* X, long long P, long long Q,
+                              int qdiv, hipStream_t stream) {
+  return jfx_band_lu_solve_batched_mode(lu, nsys, n, p, q, X, P, Q, qdiv, 0, stream);
 }
 
 int jfx_band_lu_factor_batched_z(const double* band, int nsys, int n, int p, int q, double* lu, int* status_dev,

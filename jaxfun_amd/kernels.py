@@ -254,9 +254,17 @@ def band_lu_factor_batched(band: torch.Tensor, p: int, q: int) -> torch.Tensor:
     return lu
 
 
-def band_lu_solve_batched(lu: torch.Tensor, p: int, q: int, X: torch.Tensor, axis: int) -> torch.Tensor:
+BAND_SOLVE_METHODS = {"auto": 0, "lanes": 1, "wave": 2}
+
+
+def band_lu_solve_batched(lu: torch.Tensor, p: int, q: int, X: torch.Tensor, axis: int,
+                          method: str = "auto") -> torch.Tensor:
     """Solve in place along ``axis`` of X; the systems run over the other axes
-    in C order (one factor per line, the re/im pair of complex X sharing one)."""
+    in C order (one factor per line, the re/im pair of complex X sharing one).
+    ``method`` as for ``band_lu_solve`` (real factors; complex factors use
+    one lane per line)."""
+    if method not in BAND_SOLVE_METHODS:
+        raise ValueError(f"band solve: method must be one of {sorted(BAND_SOLVE_METHODS)}, got {method!r}")
     nsys, n = lu.shape[0], lu.shape[1]
     axis = axis % X.ndim
     if X.shape[axis] != n:
@@ -280,11 +288,11 @@ def band_lu_solve_batched(lu: torch.Tensor, p: int, q: int, X: torch.Tensor, axi
     _req(Xr, torch.float64, "X")
     P = math.prod(X.shape[:axis])
     Q = math.prod(X.shape[axis + 1:]) * qdiv
-    _lib.call("jfx_band_lu_solve_batched", _p(lu), nsys, n, p, q, _p(Xr), P, Q, qdiv, _s())
+    if method == "wave" and max(p, q) > 4:
+        raise ValueError(f"band solve: the wavefront solve takes bandwidths <= 4, got p={p}, q={q}")
+    _lib.call("jfx_band_lu_solve_batched_mode", _p(lu), nsys, n, p, q, _p(Xr), P, Q, qdiv, BAND_SOLVE_METHODS[method],
+              _s())
     return X
-
-
-BAND_SOLVE_METHODS = {"auto": 0, "lanes": 1, "wave": 2}
 
 
 def band_lu_solve(lu: torch.Tensor, p: int, q: int, X: torch.Tensor, axis: int, method: str = "auto") -> torch.Tensor:
