@@ -168,9 +168,30 @@ def bench_poisson_chebyshev(reps: int = 50):
         step(bb)
     torch.cuda.synchronize()
     bt = (time.perf_counter() - t0) / 5
+    # the exact Kronecker solve of the same system (the reference's dense path,
+    # la/tpmatrix.py:1052-1086): device assembly, LU with partial pivoting
+    from jaxfun_amd.la.tpmatrix import TPMatricesDenseLUFactors, kron_device
+
+    shape = tuple(int(m.shape[0]) for m in A.tpmats[0].mats)
+    Ak = kron_device(A.tpmats)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dense = TPMatricesDenseLUFactors(Ak, shape)
+    torch.cuda.synchronize()
+    t_factor = time.perf_counter() - t0
+    Ud = dense.solve(b)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        Ud = dense.solve(b)
+    torch.cuda.synchronize()
+    t_solve = (time.perf_counter() - t0) / 5
+    err_dense = float(np.linalg.norm(Ud.cpu().numpy() - fx["U"]) / np.linalg.norm(fx["U"]))
     return dict(single_solve_us=lat * 1e6, single_solves_per_s=1.0 / lat, graph_matches_eager=same,
                 batched_4096_solves_per_s=4096 / bt,
-                rel_err_coeffs_vs_oracle_kron_solve=err_U, rel_err_mesh_values_vs_oracle=err_u)
+                rel_err_coeffs_vs_oracle_kron_solve=err_U, rel_err_mesh_values_vs_oracle=err_u,
+                dense_kronecker_lu={"n": int(Ak.shape[0]), "factor_ms": t_factor * 1e3, "solve_ms": t_solve * 1e3,
+                                    "rel_err_vs_oracle_kron_solve": err_dense})
 
 
 def bench_lifted_helmholtz(reps: int = 50, batch: int = 4096):

@@ -42,6 +42,12 @@ __device__ __forceinline__ zd dl_div(zd a, zd b) {
   const double r = b.re / b.im, d = b.re * r + b.im;
   return {(a.re * r + a.im) / d, (a.im * r - a.re) / d};
 }
+__device__ __forceinline__ double dl_mul(double a, double b) { return a * b; }
+__device__ __forceinline__ zd dl_mul(zd a, zd b) {
+  return {fma(a.re, b.re, -a.im * b.im), fma(a.re, b.im, a.im * b.re)};
+}
+__device__ __forceinline__ double dl_rcp(double a) { return 1.0 / a; }
+__device__ __forceinline__ zd dl_rcp(zd a) { return dl_div(zd{1.0, 0.0}, a); }
 template <typename T>
 __device__ __forceinline__ T dl_zero();
 template <>
@@ -142,10 +148,12 @@ __global__ __launch_bounds__(NT) void dl_panel_reg_kernel(T* __restrict__ A, int
   __shared__ double wv[NT / 64];
   __shared__ int wi[NT / 64];
   __shared__ int piv_sh[2];
+  __shared__ int pvs[DL_NB];  // the panel's pivot rows, stored to ipiv at the end
   __shared__ T srow[2][2][SB];  // [column parity][row j, row r] sub-panel entries
   __shared__ T Lss[SB][SB];
   __shared__ T Us[SB][DL_NB];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  int bad = 0;  // thread 0: 1 + the first column with a zero pivot
   for (int s0 = k0; s0 < k0 + nb; s0 += SB) {
     const int sb = min(SB, k0 + nb - s0);
     T v[R][SB];
@@ -155,7 +163,11 @@ __global__ __launch_bounds__(NT) void dl_panel_reg_kernel(T* __restrict__ A, int
 #pragma unroll
       for (int c = 0; c < SB; ++c) v[m][c] = (i < n && i >= s0 && c < sb) ? A[(long long)i * n + s0 + c] : dl_zero<T>();
     }
-    for (int jj = 0; jj < sb; ++jj) {
+    // the column index within the sub-panel is a compile-time constant, so
+    // the register slots it selects need no runtime selects
+#pragma unroll
+    for (int jj = 0; jj < SB; ++jj) {
+      if (jj >= sb) break;  // uniform
       const int j = s0 + jj, par = jj & 1;
       // pivot: the first row of largest |a_ij| over i >= j
       double best = -1.0;
@@ -187,7 +199,7 @@ __global__ __launch_bounds__(NT) void dl_panel_reg_kernel(T* __restrict__ A, int
           if (wv[k] > b || (wv[k] == b && wi[k] < r)) b = wv[k], r = wi[k];
         r = r < n ? r : j;
         piv_sh[par] = r;
-        ipiv[j] = r;
+        pvs[j - k0] = r;
       }
       __syncthreads();
       const int r = piv_sh[par];
@@ -201,14 +213,6 @@ __global__ __launch_bounds__(NT) void dl_panel_reg_kernel(T* __restrict__ A, int
         if (tid == orr && m == mr)
 #pragma unroll
           for (int c = 0; c < SB; ++c) srow[par][1][c] = v[m][c];
-      }
-      if (r != j && tid < nb) {
-        const int c = k0 + tid;
-        if (c < s0 || c >= s0 + sb) {
-          const T a = A[(long long)j * n + c];
-          A[(long long)j * n + c] = A[(long long)r * n + c];
-          A[(long long)r * n + c] = a;
-        }
       }
       __syncthreads();
       if (r != j) {
@@ -227,26 +231,46 @@ __global__ __launch_bounds__(NT) void dl_panel_reg_kernel(T* __restrict__ A, int
 #pragma unroll
       for (int c = 1; c < SB; ++c)
         if (c == jj) p = srow[par][1][c];
-      if (tid == 0 && !dl_ok(p) && *status == 0) *status = j + 1;
+      if (tid == 0 && !dl_ok(p) && !bad) bad = j + 1;
       if (dl_ok(p)) {
+        // multipliers by one reciprocal of the pivot (as LAPACK's getf2
+        // scales the column), one division per column instead of one per
+        // register slot the column index could select
+        const T rp = dl_rcp(p);
 #pragma unroll
         for (int m = 0; m < R; ++m) {
           const int i = k0 + tid + NT * m;
           if (i > j && i < n) {
-#pragma unroll
-            for (int c = 0; c < SB; ++c)
-              if (c == jj) v[m][c] = dl_div(v[m][c], p);
-            T l = v[m][0];
+            T x = v[m][0];
 #pragma unroll
             for (int c = 1; c < SB; ++c)
-              if (c == jj) l = v[m][c];
+              if (c == jj) x = v[m][c];
+            const T l = dl_mul(x, rp);
 #pragma unroll
-            for (int c = 0; c < SB; ++c)
+            for (int c = 0; c < SB; ++c) {
+              if (c == jj) v[m][c] = l;
               if (c > jj) v[m][c] = dl_fms(v[m][c], l, srow[par][1][c]);
+            }
           }
         }
       }
       // srow[par] is rewritten two columns later, after two more barriers
+    }
+    // the sub-panel's row exchanges on the panel's other columns, in order
+    // (kept out of the column loop, where each would cost a memory round
+    // trip before a barrier)
+    if (tid < nb) {
+      const int c = k0 + tid;
+      if (c < s0 || c >= s0 + sb) {
+        for (int jj = 0; jj < sb; ++jj) {
+          const int j = s0 + jj, r = pvs[j - k0];
+          if (r != j) {
+            const T a = A[(long long)j * n + c];
+            A[(long long)j * n + c] = A[(long long)r * n + c];
+            A[(long long)r * n + c] = a;
+          }
+        }
+      }
     }
     // the sub-panel back to memory; L of the sub-panel's own rows to LDS
 #pragma unroll
@@ -283,24 +307,37 @@ __global__ __launch_bounds__(NT) void dl_panel_reg_kernel(T* __restrict__ A, int
           }
       }
       __syncthreads();
-      // rank-sb update of the rows below, multipliers from registers
+      // rank-sb update of the rows below, multipliers from registers; a
+      // row's remaining entries are loaded together, updated, then stored
+      // (one memory round trip per row, not one per entry)
+      constexpr int RCM = dl_nb<T>() - SB;  // the panel columns right of a sub-panel, at most
 #pragma unroll
       for (int m = 0; m < R; ++m) {
         const int i = k0 + tid + NT * m;
         if (i >= s0 + sb && i < n) {
           T* row = A + (long long)i * n + s0 + sb;
-          for (int cc = 0; cc < rc; ++cc) {
-            T a = row[cc];
+          T a[RCM];
 #pragma unroll
-            for (int t = 0; t < SB; ++t)
-              if (t < sb) a = dl_fms(a, v[m][t], Us[t][cc]);
-            row[cc] = a;
+          for (int cc = 0; cc < RCM; ++cc) a[cc] = cc < rc ? row[cc] : dl_zero<T>();
+#pragma unroll
+          for (int t = 0; t < SB; ++t) {
+            if (t < sb) {
+#pragma unroll
+              for (int cc = 0; cc < RCM; ++cc) a[cc] = dl_fms(a[cc], v[m][t], Us[t][cc]);
+            }
+            asm volatile("" ::: "memory");  // one row of Us in registers at a time
           }
+#pragma unroll
+          for (int cc = 0; cc < RCM; ++cc)
+            if (cc < rc) row[cc] = a[cc];
         }
+        asm volatile("" ::: "memory");  // one row's entries in registers at a time
       }
     }
     __syncthreads();
   }
+  if (tid < nb) ipiv[k0 + tid] = pvs[tid];
+  if (tid == 0 && bad && *status == 0) *status = bad;
 }
 
 // ---- 2. row swaps outside the panel, then U12 = L11^-1 A12 right of it.
