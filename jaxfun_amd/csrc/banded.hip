@@ -143,6 +143,136 @@ __global__ __launch_bounds__(64) void band_lu_factor_kernel(const double* __rest
   *status = band_lu_factor_any(band + off, n, p, q, lu + off);
 }
 
+// One banded matrix (p, q <= B) by one wave: lane 0 runs the elimination
+// window of band_lu_factor_win (with one reciprocal of the pivot per step
+// instead of a division per multiplier), but reads the entries entering the
+// window from LDS and writes its results to an LDS ring of output rows; the wave
+// streams the input in and the results out in chunks of FCH rows with
+// consecutive lanes on consecutive addresses (the next chunk's input is
+// loaded into registers while lane 0 works).  A single lane's own loads and
+// stores would be bound by its memory counter, not by the recurrence.
+constexpr int FCH = 64;
+template <int B>
+__global__ __launch_bounds__(64) void band_lu_factor_wave_kernel(const double* __restrict__ band, int n, int p,
+                                                                 int q, double* __restrict__ lu,
+                                                                 int* __restrict__ status) {
+  constexpr int WM = 2 * B + 1;            // widest band row
+  constexpr int IR = FCH + B + 1;          // input rows per chunk
+  constexpr int NIN = (IR * WM + 63) / 64;  // input values per lane and chunk
+  __shared__ double inb[2][IR * WM];
+  __shared__ double outb[2 * FCH * WM];     // ring: row i at slot i mod 2 FCH
+  __shared__ int fail;
+  const int lane = threadIdx.x;
+  const int W = p + q + 1;
+  const long long nW = (long long)n * W;
+  // band-storage slots outside the matrix (first p rows, last q rows) are copied
+  for (int e = lane; e < p * W; e += 64) {
+    const int i = e / W, o = e % W - p;
+    if (i < n && i + o < 0) lu[(long long)i * W + o + p] = band[(long long)i * W + o + p];
+  }
+  for (int e = lane; e < q * W; e += 64) {
+    const int i = n - 1 - e / W, o = e % W - p;
+    if (i >= 0 && i + o >= n) lu[(long long)i * W + o + p] = band[(long long)i * W + o + p];
+  }
+  if (lane == 0) fail = 0;
+  // input of chunk c: band rows c FCH + 1 .. c FCH + IR (flat, contiguous)
+  auto load_in = [&](int c, double (&r)[NIN]) {
+    const long long base = ((long long)c * FCH + 1) * W;
+#pragma unroll
+    for (int t = 0; t < NIN; ++t) {
+      const long long e = base + lane + 64 * t;
+      const bool ok = lane + 64 * t < IR * W && e < nW;
+      const double v = band[ok ? e : 0];
+      r[t] = ok ? v : 0.0;
+    }
+  };
+  auto store_in = [&](int buf, const double (&r)[NIN]) {
+#pragma unroll
+    for (int t = 0; t < NIN; ++t)
+      if (lane + 64 * t < IR * W) inb[buf][lane + 64 * t] = r[t];
+  };
+  double rin[NIN];
+  load_in(0, rin);
+  store_in(0, rin);
+  // lane 0's elimination window: w[r][c] = A_k[k + r, k + c]
+  double w[B + 1][B + 1];
+  if (lane == 0) {
+#pragma unroll
+    for (int r = 0; r <= B; ++r)
+#pragma unroll
+      for (int c = 0; c <= B; ++c) w[r][c] = band_entry(band, n, p, q, W, r, c);
+  }
+  __syncthreads();
+  const int nch = (n + FCH - 1) / FCH;
+  for (int c = 0; c < nch; ++c) {
+    const int buf = c & 1;
+    if (c + 1 < nch) load_in(c + 1, rin);
+    if (lane == 0 && !fail) {
+      // row k + 1 of the matrix is local row k - c FCH of the chunk's input;
+      // input rows past n are zero (load_in), the other entries outside the
+      // band are masked here
+      const double* pr = inb[buf];
+      const int kend = min(n, (c + 1) * FCH);
+      for (int k = c * FCH; k < kend; ++k, pr += W) {
+        const double piv = w[0][0];
+        if (piv == 0.0 || !isfinite(piv)) {
+          *status = k + 1;
+          fail = 1;
+          break;
+        }
+        // the entering entries, read before the arithmetic
+        double ecol[B], erow[B + 1];
+        const bool colin = k + 1 + B < n;
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+          const double v = pr[r * W + (B - r <= q ? B - r + p : 0)];
+          ecol[r] = (B - r <= q && colin) ? v : 0.0;
+        }
+#pragma unroll
+        for (int cc = 0; cc <= B; ++cc) {
+          const double v = pr[B * W + (cc - B >= -p ? cc - B + p : 0)];
+          erow[cc] = cc - B >= -p ? v : 0.0;
+        }
+        const double rp = 1.0 / piv;  // one division per step; multipliers by the reciprocal
+        double* ob = outb + (k & (2 * FCH - 1)) * WM;
+#pragma unroll
+        for (int r = 1; r <= B; ++r) {
+          const double l = w[r][0] * rp;
+          if (r <= p && k + r < n) outb[((k + r) & (2 * FCH - 1)) * WM + p - r] = l;
+#pragma unroll
+          for (int cc = 1; cc <= B; ++cc) w[r][cc] -= l * w[0][cc];
+        }
+#pragma unroll
+        for (int cc = 0; cc <= B; ++cc)
+          if (cc <= q && k + cc < n) ob[p + cc] = w[0][cc];
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+#pragma unroll
+          for (int cc = 0; cc < B; ++cc) w[r][cc] = w[r + 1][cc + 1];
+          w[r][B] = ecol[r];
+        }
+#pragma unroll
+        for (int cc = 0; cc <= B; ++cc) w[B][cc] = erow[cc];
+      }
+    }
+    __syncthreads();
+    // rows of chunk c are complete: out to memory, in-matrix entries only
+    {
+      const long long row0 = (long long)c * FCH;
+      const int nrow = (int)min((long long)FCH, n - row0);
+      for (int e = lane; e < nrow * W; e += 64) {
+        const int rr = e / W, o = e % W - p;
+        const long long i = row0 + rr;
+        if (i + o >= 0 && i + o < n) lu[i * W + o + p] = outb[(i % (2 * FCH)) * WM + o + p];
+      }
+    }
+    if (c + 1 < nch) store_in(buf ^ 1, rin);
+    __syncthreads();
+    if (fail) break;  // uniform: read after the barrier
+  }
+  if (lane == 0 && !fail) *status = 0;
+}
+
 // nsys independent banded matrices (band (nsys, n, W)), one lane each; the
 // first failure's 1 + row lands in *status.
 __global__ __launch_bounds__(64) void band_lu_factor_batched_kernel(const double* __restrict__ band, int nsys, int n, int p, int q,
@@ -818,7 +948,12 @@ int jfx_band_lu_factor(const double* band, int n, int p, int q, double* lu, int*
                        hipStream_t stream) {
   if (!band || !lu || !status_dev || n <= 0 || p < 0 || q < 0)
     return set_error("band_lu_factor: bad arguments"), JFX_EARG;
-  hipLaunchKernelGGL(band_lu_factor_kernel, dim3(1), dim3(64), 0, stream, band, n, p, q, lu, status_dev, 1);
+  if (p <= 2 && q <= 2)
+    hipLaunchKernelGGL(band_lu_factor_wave_kernel<2>, dim3(1), dim3(64), 0, stream, band, n, p, q, lu, status_dev);
+  else if (p <= 4 && q <= 4)
+    hipLaunchKernelGGL(band_lu_factor_wave_kernel<4>, dim3(1), dim3(64), 0, stream, band, n, p, q, lu, status_dev);
+  else
+    hipLaunchKernelGGL(band_lu_factor_kernel, dim3(1), dim3(64), 0, stream, band, n, p, q, lu, status_dev, 1);
   JFX_LAUNCH_CHECK();
   return JFX_OK;
 }
