@@ -1,11 +1,12 @@
 """Benchmark: ragged batch of 4096 Legendre-Dirichlet Helmholtz solves on one MI355X.
 
-Headline (``value``): solves/s of the mixed-precision batched path (3-term
-bf16 split MFMA, FP32 accumulation and I/O, FP64 tridiagonal solves), one
-persistent HIP-graph-replayed launch per step, at <= 1e-6 relative L2 error
-against the plain FP64 GPU path (both measured here).  Extra fields report
-the FP64 path, the 3D Legendre transform throughput per precision (16 x 128^3
-round trip) and the Chebyshev-Dirichlet Poisson solve (N = 64).
+Headline (``value``): solves/s of the mixed-precision batched path (2-term
+fp16 split MFMA of power-of-two-scaled blocks, FP32 accumulation and I/O,
+FP64 tridiagonal solves), one persistent HIP-graph-replayed launch per step,
+at <= 1e-6 relative L2 error against the plain FP64 GPU path (both measured
+here).  Extra fields report the 3-term bf16 split and FP64 paths, the 3D
+Legendre transform throughput per precision (16 x 128^3 round trip), the
+Chebyshev-Dirichlet Poisson solve (N = 64) and the other workloads.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-extras] [--no-cpu]
 """
@@ -43,7 +44,7 @@ def bench_helmholtz(steps: int, warmup: int):
     Ns, alphas, loads = W.helmholtz_ragged_batch()
     res = {}
     outs = {}
-    for mode in ("mixed", "fp64"):
+    for mode in ("mixed16", "mixed", "fp64"):
         hb = HelmholtzBatch(Ns, alphas, mode=mode)
         F = hb.pack(loads)
         U = torch.empty_like(F)
@@ -68,8 +69,10 @@ def bench_helmholtz(steps: int, warmup: int):
                          mfma_flops=hb.mfma_flops(), nitems=hb.nitems, nblocks=hb.nblocks,
                          nwaves=hb.nwaves, queues=hb.nqueues)
         outs[mode] = U.to(torch.float64)
-    err = float(torch.linalg.vector_norm(outs["mixed"] - outs["fp64"]) / torch.linalg.vector_norm(outs["fp64"]))
-    return res, err, (Ns, alphas, loads)
+    for mode in ("mixed16", "mixed"):
+        res[mode]["rel_err_vs_fp64_gpu"] = float(torch.linalg.vector_norm(outs[mode] - outs["fp64"]) /
+                                                 torch.linalg.vector_norm(outs["fp64"]))
+    return res, (Ns, alphas, loads)
 
 
 PRECISIONS_3D = ("f64", "f32", "bf16x6", "ozaki4", "i8x5", "i8x6")
@@ -550,13 +553,14 @@ def _kernel_src_sha(name: str) -> str:
         return hashlib.sha256(fh.read()).hexdigest()[:16]
 
 
-def helm_roofline(mixed, kname):
+def helm_roofline(mixed, kname, m):
     """Roofline of the ragged-batch kernel by the workload's algorithmic count:
-    F = sum(4 n^3 + 14 n^2) flop per batch, m = 6 bf16 products per logical
-    product, bytes = F in + U out once (FP32).  Roofline time is
-    max(F m / 2.5 PFLOP/s, bytes / 8 TB/s); the binding roof is reported."""
+    F = sum(4 n^3 + 14 n^2) flop per batch, m 16-bit MFMA products per
+    logical product (6 for the 3-term bf16 split, 3 for the 2-term fp16
+    split; both run at the same 2.5 PFLOP/s dense peak), bytes = F in + U out
+    once (FP32).  Roofline time is max(F m / 2.5 PFLOP/s, bytes / 8 TB/s);
+    the binding roof is reported."""
     ks = mixed["kernel_ms"] * 1e-3
-    m = 6.0
     t_mfma = mixed["alg_flops"] * m / (BF16_PEAK_TFS * 1e12)
     t_hbm = mixed["io_bytes"] / (HBM_PEAK_GBS * 1e9)
     traffic = None
@@ -608,9 +612,11 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    res, err, data = bench_helmholtz(args.steps, args.warmup)
-    mixed, fp64 = res["mixed"], res["fp64"]
-    kname = "helm_compact_kernel<float>" if mixed["nwaves"] == 4 else f"helm_ragged_kernel<1,{mixed['nwaves']},float>"
+    res, data = bench_helmholtz(args.steps, args.warmup)
+    mixed, bf6, fp64 = res["mixed16"], res["mixed"], res["fp64"]
+    kname = "helm_compact_kernel<float, 1>"
+    kname6 = "helm_compact_kernel<float, 0>" if bf6["nwaves"] == 4 else "helm_pair_kernel<float>"
+    err = mixed["rel_err_vs_fp64_gpu"]
     line = {
         "metric": "solves/s + 3D transform GB/s at <=1e-6 rel err vs plain FP64 GPU; PCG iters",
         "value": mixed["solves_per_s"],
@@ -622,7 +628,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16x6-split/fp32-acc (fp64 tridiagonal recurrences, fp32-stored sweep values)",
+        "dtype": "fp16x2-split (3 products, power-of-two block scaling)/fp32-acc "
+                 "(fp64 tridiagonal recurrences, fp32-stored sweep values)",
         "data": "synthetic (seeded generators, jaxfun_amd/workloads.py)",
         "config": {"workload": "ragged batch of 4096 2D Legendre-Dirichlet Helmholtz solves, "
                                "N~U{32..256}, alpha=10^U(-2,4), seed 2", "io_dtype": "float32",
@@ -632,7 +639,11 @@ def main():
         "fp64_path": {"solves_per_s": fp64["solves_per_s"], "ms_per_step": fp64["ms"],
                       "kernel_ms": fp64["kernel_ms"]},
         "speedup_vs_fp64_gpu": mixed["solves_per_s"] / fp64["solves_per_s"],
-        "roofline": helm_roofline(mixed, kname),
+        "roofline": helm_roofline(mixed, kname, 3.0),
+        "bf16x6_path": {"solves_per_s": bf6["solves_per_s"], "ms_per_step": bf6["ms"],
+                        "kernel_ms": bf6["kernel_ms"], "rel_err_vs_fp64_gpu": bf6["rel_err_vs_fp64_gpu"],
+                        "speedup_vs_fp64_gpu": bf6["solves_per_s"] / fp64["solves_per_s"],
+                        "roofline": helm_roofline(bf6, kname6, 6.0)},
     }
     t0 = time.perf_counter()
 

@@ -324,7 +324,9 @@ int jfx_cast(int direction, const void* x, void* y, long long n, hipStream_t str
  *   with one queue per XCD, items that share tables are best kept in one queue.
  * Vd/VTd: FP64 eigenvector tables V_p / V_p^T (AP x AP, zero padded) of the
  *   parity blocks of K_p V = M_p V diag(lam); Vs/VTs: the same as 3 bf16
- *   float-split slices each ([3][AP][AP] per table at 3*voff).  lam at loff.
+ *   float-split slices each ([3][AP][AP] per table at 3*voff) in mode 1, or
+ *   as the fp16 high and low terms of V_p * 2^texp ([2][AP][AP] at 2*voff,
+ *   texp in the item, max |V_p| 2^texp < 2^14) in mode 2.  lam at loff.
  * mode 0: FP64 MFMA (fp64 I/O); mode 1: 3-term BF16 split MFMA with FP32
  *   accumulation/storage and FP64 tridiagonal recurrences whose sweep values
  *   and multipliers are stored in FP32 (io_f32 selects float or double F/U).
@@ -332,6 +334,10 @@ int jfx_cast(int direction, const void* x, void* y, long long n, hipStream_t str
  *   parity at a time; 8 = one 8-wave workgroup per CU, both column parities
  *   at once with whole-row F reads and U writes); scratch holds
  *   nblocks * jfx_helm_scratch_doubles_per_block() doubles (mode 0 only).
+ * mode 2: as mode 1 with a 2-term fp16 split (three MFMA products instead of
+ *   six): each operand block is scaled by a power of two from its largest
+ *   entry before the split and the products are scaled back exactly; nwaves 4
+ *   only. */
  * Replaces TPMatrices.solve -> tpmats_lu_factor / TPMatricesLUFactors.solve
  * (la/tpmatrix.py:429-587, 1089-1233) applied once per solve, and the banded
  * DiaMatrix solve along an axis (la/diamatrix.py:592-770). */

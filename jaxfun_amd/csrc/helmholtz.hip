@@ -19,6 +19,8 @@
 //   MODE 0: FP64 MFMA, FP64 storage
 //   MODE 1: BF16 3-term float split (6 products) MFMA, FP32 accumulation and
 //           storage, FP64 tridiagonal solve.
+//   mode 2 (compact kernel only): FP16 2-term split (3 products) of operands
+//           scaled by powers of two, otherwise as MODE 1.
 #include <mutex>
 #include <type_traits>
 
@@ -30,7 +32,8 @@ struct HelmItem {
   long long foff;   // offset of this solve's F / U (n*n, row-major)
   long long voff;   // offset of the (AP x AP) eigenvector table (parity p)
   double alpha;
-  int n, p, q, a, b, AP, loff, pad;
+  int n, p, q, a, b, AP, loff;
+  int texp;         // fp16 split: the table holds V * 2^texp (else 0)
 };
 
 constexpr int HMAXA = 128;  // a, b <= 128  (n <= 256)
@@ -474,37 +477,37 @@ struct SlabRegs {
   v8u16 v[3];
 };
 
-// Issue the loads of slab (k0, half h) of table T (three AP x AP bf16 slices).
+// Issue the loads of slab (k0, half h) of table T (NS AP x AP 16-bit slices:
+// three bf16 terms, or the fp16 high and low terms).
+template <int NS>
 __device__ __forceinline__ void slab_fetch(SlabRegs& P, const unsigned short* __restrict__ T, int AP, int k0, int h,
                                            int tid) {
+  const int row = (tid & 255) >> 2, ck = tid & 3, gr = h * 64 + row;
 #pragma unroll
-  for (int u = 0; u < 3; ++u) {
-    const int e = tid + u * 256, sl = e >> 8, rem = e & 255, row = rem >> 2, ck = rem & 3;
-    const int gr = h * 64 + row;
-    if (gr < AP) P.v[u] = *(const v8u16*)(T + (long long)sl * AP * AP + (long long)gr * AP + k0 + ck * 8);
-  }
+  for (int u = 0; u < NS; ++u)
+    if (gr < AP) P.v[u] = *(const v8u16*)(T + (long long)u * AP * AP + (long long)gr * AP + k0 + ck * 8);
 }
 
+template <int NS>
 __device__ __forceinline__ void slab_store(const SlabRegs& P, unsigned short* slab, int AP, int h, int tid) {
+  const int row = (tid & 255) >> 2, ck = tid & 3;
 #pragma unroll
-  for (int u = 0; u < 3; ++u) {
-    const int e = tid + u * 256, sl = e >> 8, rem = e & 255, row = rem >> 2, ck = rem & 3;
-    if (h * 64 + row < AP) *(v8u16*)&slab[slab_off(sl, row, ck)] = P.v[u];
-  }
+  for (int u = 0; u < NS; ++u)
+    if (h * 64 + row < AP) *(v8u16*)&slab[slab_off(u, row, ck)] = P.v[u];
 }
 
 // The first two slabs of a GEMM phase, issued one phase ahead.  NH = number of
 // 64-row halves of the table (AP > 64 -> 2); with NH = 1 the two slabs are
 // the table's only k-slabs (AP <= 64).
-template <int NH>
+template <int NH, int NS>
 __device__ __forceinline__ void slab_prefetch2(SlabRegs (&P)[2], const unsigned short* __restrict__ T, int AP,
                                                int tid) {
   if constexpr (NH == 2) {
-    slab_fetch(P[0], T, AP, 0, 0, tid);
-    slab_fetch(P[1], T, AP, 0, 1, tid);
+    slab_fetch<NS>(P[0], T, AP, 0, 0, tid);
+    slab_fetch<NS>(P[1], T, AP, 0, 1, tid);
   } else {
-    slab_fetch(P[0], T, AP, 0, 0, tid);
-    if (AP > 32) slab_fetch(P[1], T, AP, 32, 0, tid);
+    slab_fetch<NS>(P[0], T, AP, 0, 0, tid);
+    if (AP > 32) slab_fetch<NS>(P[1], T, AP, 32, 0, tid);
   }
 }
 
@@ -517,10 +520,15 @@ __device__ __forceinline__ void slab_prefetch2(SlabRegs (&P)[2], const unsigned 
 // zeros and columns >= b only feed discarded outputs, so B is read unmasked.
 // `between` runs after the last k-slab (Wk and P are free from then on) and
 // before the epilogue; `after` runs after it.
-template <typename TIO, bool TO_GLOBAL, int NH, typename Between, typename After>
+// SP 0: T holds three bf16 terms, B is split into three (six products).
+// SP 1: T holds the fp16 high and low terms of V * 2^eA, B is scaled by sb =
+//       2^eB and split into two fp16 terms (three products, the low x low
+//       product dropped); the outputs are scaled back by 2^-es, es = eA + eB.
+template <typename TIO, bool TO_GLOBAL, int NH, int SP, typename Between, typename After>
 __device__ __forceinline__ void helmc_gemm(const HelmItem& I, int q, int b, const unsigned short* __restrict__ T,
                                            float* Wk, unsigned short* slab, TIO* __restrict__ Ub,
-                                           SlabRegs (&P)[2], Between&& between, After&& after) {
+                                           SlabRegs (&P)[2], float sb, int es, Between&& between, After&& after) {
+  constexpr int NS = SP ? 2 : 3;
   constexpr int LDW = HelmCompact::LDW;
   // opaque copy of the thread id: keeps the compiler from hoisting every
   // lane-dependent address of the epilogue out of the item loop into registers
@@ -545,7 +553,13 @@ __device__ __forceinline__ void helmc_gemm(const HelmItem& I, int q, int b, cons
       float xv[8];
 #pragma unroll
       for (int e = 0; e < 8; ++e) xv[e] = src[e * LDW];
-      split3_pk8(xv, bh[pp], bm[pp], bl[pp]);
+      if constexpr (SP) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) xv[e] *= sb;
+        split2_f16_pk8(xv, bh[pp], bl[pp]);
+      } else {
+        split3_pk8(xv, bh[pp], bm[pp], bl[pp]);
+      }
     }
   };
   auto mult = [&](auto hc) {
@@ -554,27 +568,33 @@ __device__ __forceinline__ void helmc_gemm(const HelmItem& I, int q, int b, cons
     v8u16 A[2][3];
     auto aload = [&](v8u16(&dst)[3], int rl) {
       const int off = off0 + rl * 16 * 4 * 8;  // 16 rows further; the swizzle depends on row bits 2-3 only
-      dst[0] = *(const v8u16*)&slab[off];
-      dst[1] = *(const v8u16*)&slab[off + 64 * 32];
-      dst[2] = *(const v8u16*)&slab[off + 2 * 64 * 32];
+#pragma unroll
+      for (int u = 0; u < NS; ++u) dst[u] = *(const v8u16*)&slab[off + u * 64 * 32];
     };
     aload(A[0], 0);
 #pragma unroll
     for (int rl = 0; rl < 4; ++rl) {
       if (rl + 1 < 4) aload(A[(rl + 1) & 1], rl + 1);
       const int rb = h * 4 + rl;
-      const v8u16 ah = A[rl & 1][0], am = A[rl & 1][1], al = A[rl & 1][2];
       if (rb < nrb) {
 #pragma unroll
         for (int pp = 0; pp < 2; ++pp) {
           if (pp < npp) {
             // products ordered small to large
-            acc[pp][rb] = mfma_bf16(al, bh[pp], acc[pp][rb]);
-            acc[pp][rb] = mfma_bf16(am, bm[pp], acc[pp][rb]);
-            acc[pp][rb] = mfma_bf16(ah, bl[pp], acc[pp][rb]);
-            acc[pp][rb] = mfma_bf16(am, bh[pp], acc[pp][rb]);
-            acc[pp][rb] = mfma_bf16(ah, bm[pp], acc[pp][rb]);
-            acc[pp][rb] = mfma_bf16(ah, bh[pp], acc[pp][rb]);
+            if constexpr (SP) {
+              const v8u16 ah = A[rl & 1][0], al = A[rl & 1][1];
+              acc[pp][rb] = mfma_f16(al, bh[pp], acc[pp][rb]);
+              acc[pp][rb] = mfma_f16(ah, bl[pp], acc[pp][rb]);
+              acc[pp][rb] = mfma_f16(ah, bh[pp], acc[pp][rb]);
+            } else {
+              const v8u16 ah = A[rl & 1][0], am = A[rl & 1][1], al = A[rl & 1][2];
+              acc[pp][rb] = mfma_bf16(al, bh[pp], acc[pp][rb]);
+              acc[pp][rb] = mfma_bf16(am, bm[pp], acc[pp][rb]);
+              acc[pp][rb] = mfma_bf16(ah, bl[pp], acc[pp][rb]);
+              acc[pp][rb] = mfma_bf16(am, bh[pp], acc[pp][rb]);
+              acc[pp][rb] = mfma_bf16(ah, bm[pp], acc[pp][rb]);
+              acc[pp][rb] = mfma_bf16(ah, bh[pp], acc[pp][rb]);
+            }
           }
         }
       }
@@ -584,14 +604,14 @@ __device__ __forceinline__ void helmc_gemm(const HelmItem& I, int q, int b, cons
   if constexpr (NH == 2) {
     for (int k0 = 0; k0 < AP; k0 += 32) {
       bfrag(k0);
-      slab_store(P[0], slab, AP, 0, tid);
+      slab_store<NS>(P[0], slab, AP, 0, tid);
       __syncthreads();
-      if (k0 + 32 < AP) slab_fetch(P[0], T, AP, k0 + 32, 0, tid);
+      if (k0 + 32 < AP) slab_fetch<NS>(P[0], T, AP, k0 + 32, 0, tid);
       mult(std::integral_constant<int, 0>());
       __syncthreads();
-      slab_store(P[1], slab, AP, 1, tid);
+      slab_store<NS>(P[1], slab, AP, 1, tid);
       __syncthreads();
-      if (k0 + 32 < AP) slab_fetch(P[1], T, AP, k0 + 32, 1, tid);
+      if (k0 + 32 < AP) slab_fetch<NS>(P[1], T, AP, k0 + 32, 1, tid);
       mult(std::integral_constant<int, 1>());
       __syncthreads();
     }
@@ -600,7 +620,7 @@ __device__ __forceinline__ void helmc_gemm(const HelmItem& I, int q, int b, cons
     for (int s = 0; s < 2; ++s) {
       if (32 * s < AP) {
         bfrag(32 * s);
-        slab_store(P[s], slab, AP, 0, tid);
+        slab_store<NS>(P[s], slab, AP, 0, tid);
         __syncthreads();
         mult(std::integral_constant<int, 0>());
         __syncthreads();
@@ -608,6 +628,12 @@ __device__ __forceinline__ void helmc_gemm(const HelmItem& I, int q, int b, cons
     }
   }
   between();
+  // outputs scaled back by 2^-es (exact) in the fp16 split
+  auto outv = [&](float v) -> TIO {
+    if constexpr (!SP) return (TIO)v;
+    else if constexpr (sizeof(TIO) == 4) return __builtin_ldexpf(v, -es);
+    else return __builtin_ldexp((double)v, -es);
+  };
   // epilogue: output row m = rb*16 + lr + r, lr = 4 (lane / 16)
   const int lr = 4 * (lane >> 4), lim = a - lr;
 #pragma unroll
@@ -623,7 +649,7 @@ __device__ __forceinline__ void helmc_gemm(const HelmItem& I, int q, int b, cons
         if (rb >= nrb) continue;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          if (cok && rb * 16 + r < lim) dst[0] = (TIO)acc[pp][rb][r];
+          if (cok && rb * 16 + r < lim) dst[0] = outv(acc[pp][rb][r]);
           dst += rs;
         }
         dst += 12 * rs;
@@ -635,7 +661,8 @@ __device__ __forceinline__ void helmc_gemm(const HelmItem& I, int q, int b, cons
         if (rb >= nrb) continue;
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          if (cok && rb * 16 + r < lim) dst[(rb * 16 + r) * LDW] = acc[pp][rb][r];
+          if (cok && rb * 16 + r < lim)
+            dst[(rb * 16 + r) * LDW] = SP ? __builtin_ldexpf(acc[pp][rb][r], -es) : acc[pp][rb][r];
       }
     }
   }
@@ -671,10 +698,27 @@ __device__ __forceinline__ void gather_issue(GatherRegs<TIO>& G, const TIO* __re
   }
 }
 
-template <typename TIO>
-__device__ __forceinline__ void gather_store(const GatherRegs<TIO>& G, const TIO* __restrict__ Fb, int n, int p, int q,
-                                             int a, int b, float* Wk) {
+// Largest |entry| of the block entries this thread gathered (rows w + 4i,
+// columns lane and lane + 64), read back from LDS (its own stores: no barrier).
+__device__ __forceinline__ float own_block_absmax(const float* Wk, int a, int b) {
   constexpr int LDW = HelmCompact::LDW;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const bool ok0 = lane < b, ok1 = lane + 64 < b;
+  float m = 0.f;
+  for (int r = w; r < a; r += 4) {
+    if (ok0) m = fmaxf(m, fabsf(Wk[r * LDW + lane]));
+    if (ok1) m = fmaxf(m, fabsf(Wk[r * LDW + lane + 64]));
+  }
+  return m;
+}
+
+// Stores the next block's gathered registers into the work block; with MAX,
+// returns the largest |entry| this thread stored (else 0).
+template <typename TIO, bool MAX>
+__device__ __forceinline__ float gather_store(const GatherRegs<TIO>& G, const TIO* __restrict__ Fb, int n, int p,
+                                              int q, int a, int b, float* Wk) {
+  constexpr int LDW = HelmCompact::LDW;
+  float m = 0.f;
   if constexpr (sizeof(TIO) == 4) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const bool ok0 = lane < b, ok1 = lane + 64 < b;
@@ -682,15 +726,37 @@ __device__ __forceinline__ void gather_store(const GatherRegs<TIO>& G, const TIO
 #pragma unroll
     for (int i = 0; i < 32; ++i) {
       if (w + 4 * i < a) {
-        if (ok0) Wk[o] = G.v[i][0];
-        if (ok1) Wk[o + 64] = G.v[i][1];
+        if (ok0) {
+          Wk[o] = G.v[i][0];
+          if constexpr (MAX) m = fmaxf(m, fabsf(G.v[i][0]));
+        }
+        if (ok1) {
+          Wk[o + 64] = G.v[i][1];
+          if constexpr (MAX) m = fmaxf(m, fabsf(G.v[i][1]));
+        }
       }
       o += 4 * LDW;
       asm volatile("" : "+v"(o));
     }
   } else {
     helm_gather<4, LDW>(Fb, n, p, q, a, b, Wk);
+    if constexpr (MAX) m = own_block_absmax(Wk, a, b);
   }
+  return m;
+}
+
+// Wave-wide maximum of m, written by lane 0 to slot[wave].
+__device__ __forceinline__ void publish_wave_max(float m, float* slot) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) slot[threadIdx.x >> 6] = m;
+}
+
+// Exponent e with 2^e * max(slot[0..3]) < 2^14 (0 for an all-zero block; at
+// most 100, so that tiny blocks do not overflow the scale).
+__device__ __forceinline__ int block_split_exp(const float* slot) {
+  const float m = fmaxf(fmaxf(slot[0], slot[1]), fmaxf(slot[2], slot[3]));
+  return m > 0.f ? min(14 - __builtin_amdgcn_frexp_expf(m), 100) : 0;
 }
 
 // One pass (column parity q) of work item I.  On entry Wk rows < a hold F_pq
@@ -698,18 +764,21 @@ __device__ __forceinline__ void gather_store(const GatherRegs<TIO>& G, const TIO
 // is set up during this pass's last epilogue: its F block is loaded into
 // registers before the output stores and written to Wk after them, and its
 // first table slabs are fetched into P.
-template <typename TIO, int NH>
+// SP selects the split (see helmc_gemm); with SP 1, smx[0..3] hold the wave
+// maxima of |F_pq| on entry (published before the pass's first barrier) and
+// smx[4..7] receive those of U~; the next pass's F maxima go to smx[0..3].
+template <typename TIO, int NH, int SP>
 __device__ __forceinline__ void helmc_pass(const HelmItem& I, int q, const HelmItem& In, int qn, bool has_next,
                                            const unsigned short* __restrict__ Vs,
                                            const unsigned short* __restrict__ VTs, const double* __restrict__ lam,
                                            const TIO* __restrict__ F, TIO* __restrict__ U, float* Wk,
-                                           unsigned short* slab, double* H, SlabRegs (&P)[2]) {
-  constexpr int LDW = HelmCompact::LDW, NT = HelmCompact::NT;
+                                           unsigned short* slab, double* H, SlabRegs (&P)[2], float* smx) {
+  constexpr int LDW = HelmCompact::LDW, NT = HelmCompact::NT, NS = SP ? 2 : 3;
   const int tid = threadIdx.x, wu = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int a = I.a, AP = I.AP;
-  double* Hs = H + HMAXA + 2;                    // h_c + h_{c+2}
-  const unsigned short* TV = Vs + 3 * I.voff;    // V_p slices    (phase 3)
-  const unsigned short* TVT = VTs + 3 * I.voff;  // V_p^T slices  (phase 1)
+  double* Hs = H + HMAXA + 2;                     // h_c + h_{c+2}
+  const unsigned short* TV = Vs + NS * I.voff;    // V_p slices    (phase 3)
+  const unsigned short* TVT = VTs + NS * I.voff;  // V_p^T slices  (phase 1)
   const int b = (I.n - q + 1) >> 1;
   {
     // ---------------- phase 0: pad rows, h tables (F_pq is already in Wk)
@@ -722,8 +791,11 @@ __device__ __forceinline__ void helmc_pass(const HelmItem& I, int q, const HelmI
     __syncthreads();
 
     // ---------------- phase 1: G = V_p^T F_pq (in place)
-    helmc_gemm<TIO, false, NH>(I, q, b, TVT, Wk, slab, nullptr, P, [] {}, [] {});
-    slab_prefetch2<NH>(P, TV, AP, tid);  // phase 3's first slabs fly during the row solves
+    int e1 = 0;
+    if constexpr (SP) e1 = block_split_exp(smx);
+    helmc_gemm<TIO, false, NH, SP>(I, q, b, TVT, Wk, slab, nullptr, P, __builtin_ldexpf(1.f, e1), I.texp + e1,
+                                   [] {}, [] {});
+    slab_prefetch2<NH, NS>(P, TV, AP, tid);  // phase 3's first slabs fly during the row solves
     __syncthreads();
 
     // ---------------- phase 2: FP64 row solves, two lanes per row (twisted)
@@ -735,6 +807,7 @@ __device__ __forceinline__ void helmc_pass(const HelmItem& I, int q, const HelmI
     // row, solved from both sides' last multipliers; even b leaves a 2 x 2 system
     // between the two sides' last rows.  Full chunks of 8 steps run without
     // branches so their loads and reciprocals overlap.
+    float umax = 0.f;  // largest |u~| this lane wrote (fp16 split only)
     if (wu * 32 < a) {
       // opaque lane index: stops per-step offsets (multiples of the lane's
       // direction) from being hoisted out of the item loop into registers
@@ -805,6 +878,7 @@ __device__ __forceinline__ void helmc_pass(const HelmItem& I, int q, const HelmI
         const double cmm = (m < b - 1) ? -s * H[m + 1] : 0.0;
         const double dm = fma(s, Hs[m], fma(8.0, (double)m, 4.0 * q + 6.0));
         own = ((double)row[m] - am * yt - cmm * yb) / (dm - am * ct - cmm * cb);
+        if constexpr (SP) umax = fabsf((float)own);
         start = len - 1;
         if (side == 0) row[m] = (float)own;  // both lanes have read row[m] above
       } else {
@@ -813,6 +887,7 @@ __device__ __forceinline__ void helmc_pass(const HelmItem& I, int q, const HelmI
         const double cb = side == 0 ? co : cl, yb = side == 0 ? yo : yprev;
         const double ut = (yt - ct * yb) / (1.0 - ct * cb);
         own = side == 0 ? ut : yb - cb * ut;
+        if constexpr (SP) umax = fabsf((float)own);
         start = len - 2;
         row[j0 + dj * (len - 1)] = (float)own;
       }
@@ -821,6 +896,7 @@ __device__ __forceinline__ void helmc_pass(const HelmItem& I, int q, const HelmI
       auto bwd = [&](int i, float y) {
         const double v = fma(-(double)cm[i], uprev, (double)y);
         row[j0 + dj * i] = (float)v;
+        if constexpr (SP) umax = fmaxf(umax, fabsf((float)v));
         uprev = v;
       };
 #pragma unroll
@@ -838,30 +914,37 @@ __device__ __forceinline__ void helmc_pass(const HelmItem& I, int q, const HelmI
             if (ch * 8 + u <= start) bwd(ch * 8 + u, row[j0 + dj * (ch * 8 + u)]);
         }
       }
+      if (!act) umax = 0.f;
     }
+    if constexpr (SP) publish_wave_max(umax, smx + 4);
     __syncthreads();
 
     // ---------------- phase 3: U_pq = V_p U~ -> global, next pass set up around the epilogue
     GatherRegs<TIO> G;
     const int bn = (In.n - qn + 1) >> 1;
-    helmc_gemm<TIO, true, NH>(
-        I, q, b, TV, Wk, slab, U + I.foff, P,
+    int e2 = 0;
+    if constexpr (SP) e2 = block_split_exp(smx + 4);
+    helmc_gemm<TIO, true, NH, SP>(
+        I, q, b, TV, Wk, slab, U + I.foff, P, __builtin_ldexpf(1.f, e2), I.texp + e2,
         [&] {
           if (has_next) {
             if (In.AP > 64)
-              slab_prefetch2<2>(P, VTs + 3 * In.voff, In.AP, tid);
+              slab_prefetch2<2, NS>(P, VTs + NS * In.voff, In.AP, tid);
             else
-              slab_prefetch2<1>(P, VTs + 3 * In.voff, In.AP, tid);
+              slab_prefetch2<1, NS>(P, VTs + NS * In.voff, In.AP, tid);
             gather_issue(G, F + In.foff, In.n, In.p, qn, In.a, bn);
           }
         },
         [&] {
-          if (has_next) gather_store(G, F + In.foff, In.n, In.p, qn, In.a, bn, Wk);
+          if (has_next) {
+            const float m = gather_store<TIO, SP == 1>(G, F + In.foff, In.n, In.p, qn, In.a, bn, Wk);
+            if constexpr (SP) publish_wave_max(m, smx);
+          }
         });
   }
 }
 
-template <typename TIO>
+template <typename TIO, int SP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void helm_compact_kernel(
     const HelmItem* __restrict__ items, int nitems, HelmQueues Q, const unsigned short* __restrict__ Vs,
     const unsigned short* __restrict__ VTs, const double* __restrict__ lam, const TIO* __restrict__ F,
@@ -871,7 +954,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   unsigned short* slab = reinterpret_cast<unsigned short*>(smem_raw + HelmCompact::WK);
   double* H = reinterpret_cast<double*>(smem_raw + HelmCompact::WK + HelmCompact::SLAB);
   __shared__ int s_nx[2];
+  __shared__ float s_mx[8];  // wave maxima of |F_pq| and |U~| (fp16 split)
   const int tid = threadIdx.x;
+  constexpr int NS = SP ? 2 : 3;
 
   // the next item's index is claimed at the start of the current one (two
   // slots: slot k is rewritten only after every thread has passed the
@@ -886,11 +971,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   if (it0 >= nitems) return;
   HelmItem I = items[it0];
   helm_gather<4, HelmCompact::LDW>(F + I.foff, I.n, I.p, 0, I.a, (I.n + 1) >> 1, Wk);
+  if constexpr (SP) publish_wave_max(own_block_absmax(Wk, I.a, (I.n + 1) >> 1), s_mx);
   SlabRegs P[2];
   if (I.AP > 64)
-    slab_prefetch2<2>(P, VTs + 3 * I.voff, I.AP, tid);
+    slab_prefetch2<2, NS>(P, VTs + NS * I.voff, I.AP, tid);
   else
-    slab_prefetch2<1>(P, VTs + 3 * I.voff, I.AP, tid);
+    slab_prefetch2<1, NS>(P, VTs + NS * I.voff, I.AP, tid);
   int slot = 1;
   for (;;) {
     // the next item is claimed now; its index is published after this item's
@@ -899,9 +985,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     int nxt = 0;
     if (tid < 64) nxt = Q.claim(qi);
     if (I.AP > 64)
-      helmc_pass<TIO, 2>(I, 0, I, 1, true, Vs, VTs, lam, F, U, Wk, slab, H, P);
+      helmc_pass<TIO, 2, SP>(I, 0, I, 1, true, Vs, VTs, lam, F, U, Wk, slab, H, P, s_mx);
     else
-      helmc_pass<TIO, 1>(I, 0, I, 1, true, Vs, VTs, lam, F, U, Wk, slab, H, P);
+      helmc_pass<TIO, 1, SP>(I, 0, I, 1, true, Vs, VTs, lam, F, U, Wk, slab, H, P, s_mx);
     if (tid == 0) s_nx[slot] = nxt;
     __syncthreads();
     const int itn = __builtin_amdgcn_readfirstlane(s_nx[slot]);
@@ -909,9 +995,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const bool has = itn < nitems;
     const HelmItem In = items[has ? itn : it0];
     if (I.AP > 64)
-      helmc_pass<TIO, 2>(I, 1, In, 0, has, Vs, VTs, lam, F, U, Wk, slab, H, P);
+      helmc_pass<TIO, 2, SP>(I, 1, In, 0, has, Vs, VTs, lam, F, U, Wk, slab, H, P, s_mx);
     else
-      helmc_pass<TIO, 1>(I, 1, In, 0, has, Vs, VTs, lam, F, U, Wk, slab, H, P);
+      helmc_pass<TIO, 1, SP>(I, 1, In, 0, has, Vs, VTs, lam, F, U, Wk, slab, H, P, s_mx);
     if (!has) break;
     I = In;
     __syncthreads();
@@ -1383,12 +1469,12 @@ static int launch_helm_pair(const HelmItem* items, int nitems, HelmQueues Q, con
   return JFX_OK;
 }
 
-template <typename TIO>
+template <typename TIO, int SP = 0>
 static int launch_helm_compact(const HelmItem* items, int nitems, HelmQueues Q, const unsigned short* Vs,
                                const unsigned short* VTs, const double* lam, const TIO* F, TIO* U, int nblocks,
                                hipStream_t st) {
   const size_t shm = HelmCompact::BYTES;
-  auto kern = helm_compact_kernel<TIO>;
+  auto kern = helm_compact_kernel<TIO, SP>;
   hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
   if (e != hipSuccess) return set_error("helm: hipFuncSetAttribute: %s", hipGetErrorString(e)), JFX_EHIP;
   hipLaunchKernelGGL(helm_reset, dim3(1), dim3(64), 0, st, Q.counters, Q.nq);
@@ -1490,6 +1576,8 @@ int jfx_helm_scratch_doubles_per_block(void) { return HMAXA * HMAXA; }
 
 // mode 0: FP64 (F, U double; Vd/VTd used)
 // mode 1: BF16 3-term split, FP32 storage (F, U float if io_f32 else double; Vs/VTs used)
+// mode 2: FP16 2-term split (tables V 2^eA in two fp16 slices, eA in the
+//         item's pad field; blocks scaled by a power of two), nwaves 4
 // nwaves: 4 or 8 waves per workgroup; nblocks: persistent grid size
 //   (mult_scratch must hold nblocks * jfx_helm_scratch_doubles_per_block() doubles).
 int jfx_helm_ragged(int mode, int io_f32, int nwaves, const void* items, int nitems, int* counters,
@@ -1527,6 +1615,14 @@ int jfx_helm_ragged(int mode, int io_f32, int nwaves, const void* items, int nit
         return launch_helm_pair<double>(it, nitems, queues, Vs, VTs, lam, (const double*)F, (double*)U, nblocks,
                                         stream);
     }
+  } else if (mode == 2) {
+    if (!Vs || !VTs) return set_error("helm_ragged: mode 2 needs fp16 slice tables"), JFX_EARG;
+    if (nwaves != 4) return set_error("helm_ragged: mode 2 runs with nwaves 4 only"), JFX_EARG;
+    if (io_f32)
+      return launch_helm_compact<float, 1>(it, nitems, queues, Vs, VTs, lam, (const float*)F, (float*)U, nblocks,
+                                           stream);
+    return launch_helm_compact<double, 1>(it, nitems, queues, Vs, VTs, lam, (const double*)F, (double*)U, nblocks,
+                                          stream);
   }
   return set_error("helm_ragged: bad mode/nwaves"), JFX_EARG;
 }

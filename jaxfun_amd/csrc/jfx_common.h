@@ -80,6 +80,28 @@ __device__ __forceinline__ void split3_pk8(const float (&x)[8], v8u16& H, v8u16&
   L = __builtin_bit_cast(v8u16, lw);
 }
 
+// Split 8 fp32 values into two fp16 terms x = h + l + r (packed hardware
+// conversions, round to nearest even; x - h is exact in fp32):
+// |r| <= max(2^-22 |x|, 2^-25), the second bound where x - h falls below
+// fp16's normal range.  The caller scales x so that |x| < 2^15.
+__device__ __forceinline__ void split2_f16_pk8(const float (&x)[8], v8u16& H, v8u16& L) {
+  typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  u4 hw, lw;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const float x0 = x[2 * e], x1 = x[2 * e + 1];
+    const h2 h = __builtin_convertvector((f2){x0, x1}, h2);
+    const f2 hb = __builtin_convertvector(h, f2);
+    const h2 l = __builtin_convertvector((f2){x0 - hb[0], x1 - hb[1]}, h2);
+    hw[e] = __builtin_bit_cast(unsigned, h);
+    lw[e] = __builtin_bit_cast(unsigned, l);
+  }
+  H = __builtin_bit_cast(v8u16, hw);
+  L = __builtin_bit_cast(v8u16, lw);
+}
+
 // Map a linear workgroup id so that consecutive logical tiles land on the
 // same XCD (workgroups are dispatched round-robin over the 8 XCDs).
 __device__ __forceinline__ unsigned xcd_remap(unsigned bid, unsigned nblk) {
@@ -99,6 +121,11 @@ __device__ __forceinline__ v4d mfma_f64(double a, double b, v4d c) {
 }
 __device__ __forceinline__ v4f mfma_f32(float a, float b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ v4f mfma_f16(v8u16 a, v8u16 b, v4f c) {
+  typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(v8h, a), __builtin_bit_cast(v8h, b), c, 0, 0,
+                                                0);
 }
 __device__ __forceinline__ v4f mfma_bf16(v8u16 a, v8u16 b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a),

@@ -30,9 +30,9 @@ from . import kernels as K
 
 ITEM_DTYPE = np.dtype([("foff", "<i8"), ("voff", "<i8"), ("alpha", "<f8"), ("n", "<i4"), ("p", "<i4"),
                        ("q", "<i4"), ("a", "<i4"), ("b", "<i4"), ("AP", "<i4"), ("loff", "<i4"),
-                       ("pad", "<i4")])
+                       ("texp", "<i4")])
 
-MODES = {"fp64": 0, "mixed": 1}
+MODES = {"fp64": 0, "mixed": 1, "mixed16": 2}
 NQUEUES = 8  # work queues: one per XCD of an MI355X (workgroups are dispatched round-robin over them)
 
 
@@ -63,9 +63,11 @@ class HelmholtzBatch:
     Args:
         Ns: per-solve number of Legendre modes N_b (unknowns (N_b-2)^2), 4 <= N_b <= 258.
         alphas: per-solve Helmholtz shifts alpha_b >= 0.
-        mode: ``"fp64"`` (FP64 MFMA) or ``"mixed"`` (3-term bf16 split MFMA,
+        mode: ``"fp64"`` (FP64 MFMA), ``"mixed"`` (3-term bf16 split MFMA,
             FP32 accumulation, FP64 tridiagonal recurrences with FP32-stored
-            sweep values and multipliers).
+            sweep values and multipliers) or ``"mixed16"`` (as ``mixed`` with
+            a 2-term fp16 split of operands scaled by powers of two: three
+            MFMA products instead of six; 4-wave launches only).
         io_dtype: dtype of F/U (``mixed`` defaults to float32, ``fp64`` to float64).
         nwaves, blocks_per_cu: launch shape (see :func:`autotune`); in mixed mode
             nwaves 4 runs one column parity at a time (two workgroups fit a CU),
@@ -88,7 +90,7 @@ class HelmholtzBatch:
         if mode not in MODES:
             raise ValueError(f"mode must be one of {list(MODES)}")
         self.mode = mode
-        self.io_dtype = io_dtype or (torch.float32 if mode == "mixed" else torch.float64)
+        self.io_dtype = io_dtype or (torch.float64 if mode == "fp64" else torch.float32)
         if mode == "fp64" and self.io_dtype != torch.float64:
             raise ValueError("fp64 mode needs float64 I/O")
         self.nwaves = nwaves
@@ -108,6 +110,8 @@ class HelmholtzBatch:
         """Choose the launch shape (waves per workgroup, workgroups per CU)."""
         if nwaves not in (4, 8) or blocks_per_cu < 1:
             raise ValueError("nwaves must be 4 or 8 and blocks_per_cu >= 1")
+        if self.mode == "mixed16" and nwaves != 4:
+            raise ValueError("mixed16 runs with nwaves 4 only")
         self.nwaves, self.blocks_per_cu = nwaves, blocks_per_cu
         cus = torch.cuda.get_device_properties(self.dev).multi_processor_count
         self.nblocks = int(cus * blocks_per_cu)
@@ -135,6 +139,7 @@ class HelmholtzBatch:
                 vcur += AP * AP
                 lcur += a
         self._voffs, self._loffs = voffs, loffs
+        self._texp = {}
         self.Vd = torch.tensor(np.concatenate(Vs), device=self.dev)
         self.VTd = torch.tensor(np.concatenate(VTs), device=self.dev)
         self.lam = torch.tensor(np.concatenate(lams), device=self.dev)
@@ -146,6 +151,24 @@ class HelmholtzBatch:
                 for src, dst in ((self.Vd, self.Vs), (self.VTd, self.VTs)):
                     _lib.call("jfx_split_matrix", K._p(src[off:off + AP * AP]), AP, AP, 0, 3,
                               K._p(dst[3 * off:3 * off + 3 * AP * AP]), K._p(None), K._s())
+        elif self.mode == "mixed16":
+            # V * 2^texp (max below 2^14) as fp16 high and low terms, per table
+            Vh = np.empty((2 * vcur,), dtype=np.float16)
+            VTh = np.empty((2 * vcur,), dtype=np.float16)
+            Vall = np.concatenate(Vs)
+            for (N, p), (off, AP) in voffs.items():
+                V = Vall[off:off + AP * AP]
+                m = float(np.max(np.abs(V)))
+                texp = 14 - int(np.frexp(m)[1]) if m > 0 else 0
+                self._texp[(N, p)] = texp
+                for src, dst in ((V.reshape(AP, AP), Vh), (V.reshape(AP, AP).T, VTh)):
+                    x = np.ldexp(np.ascontiguousarray(src).ravel(), texp)
+                    hi = x.astype(np.float16)
+                    lo = (x - hi.astype(np.float64)).astype(np.float16)
+                    dst[2 * off:2 * off + AP * AP] = hi
+                    dst[2 * off + AP * AP:2 * off + 2 * AP * AP] = lo
+            self.Vs = torch.tensor(Vh.view(np.int16), device=self.dev)
+            self.VTs = torch.tensor(VTh.view(np.int16), device=self.dev)
         else:
             self.Vs = self.VTs = None
 
@@ -160,7 +183,7 @@ class HelmholtzBatch:
                 off, AP = self._voffs[(int(N), p)]
                 # one item per (solve, row parity); the kernel runs both column parities
                 items.append((int(self.offsets[bi]), off, float(alpha), n, p, 0, a, (n + 1) // 2, AP,
-                              self._loffs[(int(N), p)], 0))
+                              self._loffs[(int(N), p)], self._texp.get((int(N), p), 0)))
         arr = np.array(items, dtype=ITEM_DTYPE)
         cost = arr["a"].astype(np.float64) ** 2 * arr["n"] + 40.0 * arr["a"] * arr["n"]
         # One queue per XCD: the items of one N (which share eigenvector tables)
@@ -261,15 +284,17 @@ class HelmholtzBatch:
     def mfma_flops(self) -> float:
         """Flops of the MFMA instructions one launch issues: per (item, column
         parity) two GEMMs of ceil(a/16) x ceil(b/16) output tiles over AP/32
-        k-slabs (bf16 16x16x32, six products each, in mixed mode; FP64 16x16x4
-        over AP/4 k-steps in fp64 mode), padding included."""
+        k-slabs (bf16 16x16x32, six products each, in mixed mode; fp16
+        16x16x32, three products each, in mixed16 mode; FP64 16x16x4 over AP/4
+        k-steps in fp64 mode), padding included."""
         it = self.items_host
         tot = 0.0
         for q in (0, 1):
             b = (it["n"] - q + 1) // 2
             tiles = ((it["a"] + 15) // 16) * ((b + 15) // 16)
-            if self.mode == "mixed":
-                tot += float(np.sum(2 * (it["AP"] // 32) * tiles * 6)) * 16 * 16 * 32 * 2
+            if self.mode != "fp64":
+                nprod = 6 if self.mode == "mixed" else 3
+                tot += float(np.sum(2 * (it["AP"] // 32) * tiles * nprod)) * 16 * 16 * 32 * 2
             else:
                 tot += float(np.sum(2 * (it["AP"] // 4) * tiles)) * 16 * 16 * 4 * 2
         return tot
@@ -362,9 +387,11 @@ def autotune(batch: HelmholtzBatch, F: torch.Tensor, candidates=None, reps: int 
     batch itself, keep the fastest on ``batch`` and return it.  In mixed mode
     the shapes are the two kernels at their occupancy: (4, 2) two 4-wave
     workgroups per CU (one column parity at a time) and (8, 1) one 8-wave
-    workgroup per CU (both column parities at once), plus (4, 1)."""
+    workgroup per CU (both column parities at once), plus (4, 1); mixed16
+    has the 4-wave kernel only."""
     if candidates is None:
-        candidates = ((4, 2), (8, 1), (4, 1)) if batch.mode == "mixed" else ((4, 1), (8, 1), (4, 2), (8, 2))
+        candidates = {"mixed": ((4, 2), (8, 1), (4, 1)), "mixed16": ((4, 2), (4, 1))}.get(
+            batch.mode, ((4, 1), (8, 1), (4, 2), (8, 2)))
     U = torch.empty_like(F)
     best, best_t = None, float("inf")
     for nw, bpc in candidates:

@@ -10,7 +10,10 @@
   256 seed-3 initial conditions: five ARS443 steps vs the oracle, and the
   mixed-precision drift;
 * variable-coefficient Poisson 64^2 with 16 x 16 g-modes: operator, load and
-  PCG vs a dense solve (3844^2), and the 4 x 512 MLP's gradients.
+  PCG vs a dense solve (3844^2), and the 4 x 512 MLP's gradients;
+* the ragged Helmholtz batch, 4096 Legendre-Dirichlet solves, seed 2: both
+  mixed splits vs the FP64 path over the whole batch and an N-stratified
+  sample vs the oracle.
 """
 
 import os
@@ -317,3 +320,25 @@ def test_learned_pcg_at_config_size_beats_plain_and_scaling(dev):
     assert not bool(fb.any())
     assert itp / itl >= 2.0, (itp, itl)
     assert itl <= ita, (itl, ita)
+
+
+# ----------------------------------------------------------------------------- ragged Helmholtz batch
+@pytest.mark.parametrize("mode", ["mixed16", "mixed"])
+def test_helmholtz_ragged_4096_vs_fp64_and_oracle(dev, mode):
+    import torch
+
+    from jaxfun_amd import workloads as W
+    from jaxfun_amd.helmholtz import HelmholtzBatch
+
+    Ns, alphas, loads = W.helmholtz_ragged_batch()
+    ref = HelmholtzBatch(Ns, alphas, mode="fp64", nwaves=8, blocks_per_cu=1)
+    U64 = ref.solve(ref.pack(loads))
+    hb = HelmholtzBatch(Ns, alphas, mode=mode)
+    U = hb.solve(hb.pack(loads)).to(torch.float64)
+    err = float(torch.linalg.vector_norm(U - U64) / torch.linalg.vector_norm(U64))
+    assert err < 5e-7, (mode, err)
+    Uh = hb.unpack(U)
+    order = np.argsort(Ns, kind="stable")
+    for i in order[::256]:  # 16 solves, N from the smallest to the largest
+        e = O.rel_l2(Uh[i], O.helmholtz_partial_fd_lapack(int(Ns[i]), float(alphas[i]), loads[i]))
+        assert e < 1e-6, (mode, int(Ns[i]), float(alphas[i]), e)

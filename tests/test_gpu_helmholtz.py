@@ -38,9 +38,11 @@ def _residual(N, alpha, U, F):
 # two FP64 implementations of the same solve differ by ~cond * eps (cond ~ N^4 / alpha);
 # the FP64 residual is bounded by 10x the oracle's own residual
 # nwaves selects the kernel: fp64 4/8 waves; mixed 4 = compact (two workgroups per CU, one column parity at
-# a time), 8 = pair (one workgroup per CU, both column parities at once, whole-row I/O)
+# a time), 8 = pair (one workgroup per CU, both column parities at once, whole-row I/O); mixed16 = the
+# compact kernel with the 2-term fp16 split
 @pytest.mark.parametrize("mode,nwaves,tol,rtol", [("fp64", 4, 1e-8, None), ("fp64", 8, 1e-8, None),
-                                                  ("mixed", 4, 1e-6, 1e-6), ("mixed", 8, 1e-6, 1e-6)])
+                                                  ("mixed", 4, 1e-6, 1e-6), ("mixed", 8, 1e-6, 1e-6),
+                                                  ("mixed16", 4, 1e-6, 1e-6)])
 def test_ragged_vs_oracle(dev, mode, nwaves, tol, rtol):
     import torch
 
@@ -108,18 +110,18 @@ def test_rejects_bad_sizes(dev):
         HelmholtzBatch([40], [-1.0])
 
 
-@pytest.mark.parametrize("nwaves", [4, 8])
-def test_mixed_fp64_io_vs_oracle(dev, nwaves):
+@pytest.mark.parametrize("mode,nwaves", [("mixed", 4), ("mixed", 8), ("mixed16", 4)])
+def test_mixed_fp64_io_vs_oracle(dev, mode, nwaves):
     import torch
 
     from jaxfun_amd.helmholtz import HelmholtzBatch
 
     Ns, alphas, loads = _case([4, 5, 34, 97, 129, 256, 258], seed=14)
-    hb = HelmholtzBatch(Ns, alphas, mode="mixed", io_dtype=torch.float64, nwaves=nwaves, blocks_per_cu=1)
+    hb = HelmholtzBatch(Ns, alphas, mode=mode, io_dtype=torch.float64, nwaves=nwaves, blocks_per_cu=1)
     U = hb.unpack(hb.solve(hb.pack(loads)))
     for N, a, F, Ub in zip(Ns, alphas, loads, U):
         err = O.rel_l2(Ub, _ref(int(N), a, F))
-        assert err < 1e-6, (nwaves, N, a, err)
+        assert err < 1e-6, (mode, nwaves, N, a, err)
 
 
 def test_io_checks(dev):
@@ -164,3 +166,55 @@ def test_solve_ragged_from_inner_systems(dev, mode, tol):
     for Ub, Ur in zip(U, refs):
         assert Ub.dtype.is_floating_point and Ub.shape == Ur.shape
         assert O.rel_l2(Ub.cpu().numpy(), Ur) < tol
+
+
+@pytest.mark.parametrize("io", ["f32", "f64"])
+def test_mixed16_power_of_two_scaling(dev, io):
+    # the fp16 split scales every operand block by a power of two from its
+    # largest entry, so scaling the loads by 2^k scales the solution by 2^k
+    # bit for bit (FP64 recurrences and FP32 storage commute with it too);
+    # a zero load gives exactly zero, and loads far from unit size keep the
+    # 1e-6 accuracy
+    import torch
+
+    from jaxfun_amd.helmholtz import HelmholtzBatch
+
+    Ns, alphas, loads = _case([6, 40, 129, 256], seed=17)
+    loads[1] = np.zeros_like(loads[1])
+    dt = torch.float32 if io == "f32" else torch.float64
+    hb = HelmholtzBatch(Ns, alphas, mode="mixed16", io_dtype=dt)
+    base = hb.solve(hb.pack(loads)).cpu().numpy()
+    assert not np.any(base[hb.offsets[1]:hb.offsets[2]])
+    for k in (-60, -20, 40):  # FP32 I/O stays in its normal range
+        scaled = hb.solve(hb.pack([np.ldexp(F, k) for F in loads])).cpu().numpy()
+        assert np.array_equal(scaled, np.ldexp(base, k)), k
+    U = hb.unpack(hb.solve(hb.pack([np.ldexp(F, -60) for F in loads])))
+    for i, (N, a, F) in enumerate(zip(Ns, alphas, loads)):
+        if i != 1:
+            assert O.rel_l2(np.ldexp(U[i], 60), _ref(int(N), a, F)) < 1e-6
+
+
+def test_mixed16_launch_shapes_and_graph(dev):
+    import torch
+
+    from jaxfun_amd.helmholtz import HelmholtzBatch
+
+    Ns, alphas, loads = _case([32, 77, 128, 256, 190], seed=18)
+    hb = HelmholtzBatch(Ns, alphas, mode="mixed16", blocks_per_cu=1)
+    F = hb.pack(loads)
+    a = hb.solve(F).cpu().numpy()
+    hb.set_launch(4, 3)
+    assert np.array_equal(hb.solve(F).cpu().numpy(), a)
+    with pytest.raises(ValueError):
+        hb.set_launch(8, 1)
+    # the two splits agree to their accuracy
+    ref = HelmholtzBatch(Ns, alphas, mode="mixed").solve(F).cpu().numpy()
+    assert np.linalg.norm(a - ref) <= 2e-6 * np.linalg.norm(ref)
+    U = torch.empty_like(F)
+    hb.capture(F, U)
+    for _ in range(3):
+        U.fill_(0)
+        hb.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(U.cpu().numpy(), a)
+        hb.solve(F)
