@@ -193,11 +193,16 @@ int jfx_band_lu_solve_mode(const double* lu, int n, int p, int q, double* X, lon
  * and TensorMatrix.solve (la/tensormatrix.py:76-80). */
 int jfx_dense_lu_factor(int cplx, void* A, int n, int* ipiv, int* perm, int* status_dev, hipStream_t stream);
 /* X (n x nrhs, row-major) = A^-1 B from the factors above: X = P B, then
- * blocked forward (unit L) and backward (U) substitution, one launch per
+ * forward (unit L) and backward (U) substitution.  Real factors with
+ * nrhs <= 2: one launch per direction, one workgroup per 64-row block that
+ * applies the earlier blocks' results as they are published (device flags,
+ * bounded waits; a block that gives up waiting is written as NaN) and solves
+ * its diagonal block with reciprocal pivots.  Otherwise one launch per
  * 32-row block (16 complex) that solves the block in registers and updates
- * the remaining rows; work holds n x nrhs values of the intermediate
- * L^-1 P B.  B, X and work must not overlap.  Replaces
- * jax.scipy.linalg.lu_solve in Matrix.lu_solve (la/matrix.py:169-214). */
+ * the remaining rows.  work holds n x nrhs values of the intermediate
+ * L^-1 P B followed by 2 ceil(n/64) ints (the block flags).  B, X and work
+ * must not overlap.  Replaces jax.scipy.linalg.lu_solve in Matrix.lu_solve
+ * (la/matrix.py:169-214). */
 int jfx_dense_lu_solve(int cplx, const void* LU, int n, const int* perm, const void* B, void* X, long long nrhs,
                        void* work, hipStream_t stream);
 

@@ -18,6 +18,8 @@
 // Replaces jax.scipy.linalg.lu_factor / lu_solve as used by Matrix.lu_factor
 // (la/matrix.py:138-214), TPMatricesDenseLUFactors (la/tpmatrix.py:1016-1086)
 // and TensorMatrix.solve (la/tensormatrix.py:76-80).
+#include <type_traits>
+
 #include "jfx_common.h"
 
 namespace jfx {
@@ -458,11 +460,172 @@ __global__ void dl_perm_kernel(const int* __restrict__ ipiv, int n, int* __restr
 // ---- solve
 template <typename T>
 __global__ void dl_gather_kernel(const T* __restrict__ B, const int* __restrict__ perm, T* __restrict__ X, int n,
-                                 long long nrhs) {
+                                 long long nrhs, int* __restrict__ flags, int nflags) {
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < nflags) flags[e] = 0;  // the single-launch solve's block flags (nflags <= n)
   if (e >= (long long)n * nrhs) return;
   const long long i = e / nrhs, r = e - i * nrhs;
   X[e] = B[(long long)perm[i] * nrhs + r];
+}
+
+// ---------------------------------------------------------------------------
+// Single-launch triangular solve for up to 16 right-hand sides (FP64; B and
+// X are n x nc row-major, nc <= NR).  Workgroup g owns the 64-row block blk
+// (blk = g forward, nb - 1 - g backward, so that workgroups are dispatched in
+// dependency order and a waiting workgroup only waits for workgroups
+// dispatched before it).  It subtracts T[block, block j] x_j for every block
+// j it depends on (j < blk forward, j > blk backward) as soon as workgroup j
+// has published x_j (flags[j], release / acquire at agent scope), wave w
+// taking columns 16w .. 16w + 15 of each block, then solves the diagonal
+// block in registers (unit lower L forward; U with reciprocal pivots
+// backward; wave 0 alone for NR <= 2, else each wave NR / 4 right-hand
+// sides), stores x_blk and sets flags[blk].  The wait for a flag is bounded:
+// a workgroup that gives up writes NaN to its block (and still sets its
+// flag), so a failure shows in the result instead of hanging.
+constexpr int DL_SF_BS = 64;
+constexpr int DL_SF_MAXR = 16;
+constexpr int DL_SF_SPIN = 1 << 20;  // flag polls (~1 us each with the sleep) before giving up
+
+// v broadcast from lane t (t a compile-time constant after unrolling)
+__device__ __forceinline__ double dl_bcast(double v, int t) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(unsigned)b, t);
+  const int hi = __builtin_amdgcn_readlane((int)(unsigned)(b >> 32), t);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ bool dl_sf_wait(const int* f) {
+  int it = 0;
+  while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+    if (++it >= DL_SF_SPIN) return false;
+    __builtin_amdgcn_s_sleep(2);
+  }
+  return true;
+}
+
+// LDS (doubles): Ld[64][64] (the diagonal block, transposed: Ld[t][r] =
+// T[r0 + r][r0 + t]), xs[4][64 NR] (per wave: the block being applied, then
+// the wave's partial sums), rdg[64] (1 / U[r][r], backward).
+template <int NR>
+constexpr size_t dl_sf_lds() {
+  return (size_t)(DL_SF_BS * DL_SF_BS + 4 * DL_SF_BS * NR + DL_SF_BS) * sizeof(double);
+}
+
+template <int NR, bool LOWER>
+__global__ __launch_bounds__(256) void dl_trsv_sf_kernel(const double* __restrict__ LU, int n, int nc,
+                                                          const double* __restrict__ Bv, double* Xv, int* flags) {
+  constexpr int BS = DL_SF_BS;
+  constexpr int CW = NR <= 2 ? NR : NR / 4;  // right-hand sides per solving wave
+  extern __shared__ double sm_sf[];
+  double* Ld = sm_sf;                         // [t * BS + r]
+  double* xs = Ld + BS * BS;                  // [w][BS * NR]
+  double* rdg = xs + 4 * BS * NR;             // [BS]
+  __shared__ int sbad;
+  const int nb = (n + BS - 1) / BS;
+  const int blk = LOWER ? (int)blockIdx.x : nb - 1 - (int)blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int r0 = blk * BS;
+  const int row = min(r0 + lane, n - 1);  // rows past n reread row n - 1 (results unused)
+  double* xw = xs + w * BS * NR;
+  if (tid == 0) sbad = 0;
+  for (int e = tid; e < BS * BS; e += 256) {
+    const int r = e / BS, t = e % BS;
+    const bool in = r0 + r < n && r0 + t < n;
+    Ld[t * BS + r] = in ? LU[(long long)(r0 + r) * n + r0 + t] : 0.0;
+  }
+  if (!LOWER && tid < BS) rdg[tid] = r0 + tid < n ? 1.0 / LU[(long long)(r0 + tid) * n + r0 + tid] : 1.0;
+  __syncthreads();  // sbad is initialised before any wave can set it
+  double acc[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) acc[c] = 0.0;
+  bool bad = false;
+  const double* Lrow = LU + (long long)row * n;
+  const int ndep = LOWER ? blk : nb - 1 - blk;
+  for (int s = 0; s < ndep; ++s) {
+    const int j = LOWER ? s : nb - 1 - s;  // blocks in the order they are published
+    if (!dl_sf_wait(&flags[j])) {
+      bad = true;
+      break;
+    }
+    const int jr = j * BS + lane;
+#pragma unroll
+    for (int c = 0; c < NR; ++c) xw[lane * NR + c] = (jr < n && c < nc) ? Xv[(long long)jr * nc + c] : 0.0;
+    double lv[16];
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) lv[kk] = Lrow[min(j * BS + 16 * w + kk, n - 1)];
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk)
+#pragma unroll
+      for (int c = 0; c < NR; ++c) acc[c] = fma(-lv[kk], xw[(16 * w + kk) * NR + c], acc[c]);
+  }
+  if (bad) sbad = 1;
+  // the wave's partial sums replace its staged block (same wave: in order)
+#pragma unroll
+  for (int c = 0; c < NR; ++c) xw[lane * NR + c] = acc[c];
+  __syncthreads();
+  if (NR <= 2 && w != 0) return;
+  const int cb = NR <= 2 ? 0 : w * CW;  // this wave's first right-hand side
+  double x[CW];
+#pragma unroll
+  for (int c = 0; c < CW; ++c) {
+    const int cc = cb + c;
+    const double b = cc < nc ? Bv[(long long)row * nc + cc] : 0.0;
+    x[c] = b + xs[lane * NR + cc] + xs[BS * NR + lane * NR + cc] + xs[2 * BS * NR + lane * NR + cc] +
+           xs[3 * BS * NR + lane * NR + cc];
+  }
+  if (LOWER) {
+#pragma unroll
+    for (int t = 0; t < BS - 1; ++t) {
+      const double l = Ld[t * BS + lane];
+#pragma unroll
+      for (int c = 0; c < CW; ++c) {
+        const double xt = dl_bcast(x[c], t);
+        if (lane > t) x[c] = fma(-l, xt, x[c]);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int t = BS - 1; t >= 0; --t) {
+      const double u = Ld[t * BS + lane], rt = rdg[t];
+#pragma unroll
+      for (int c = 0; c < CW; ++c) {
+        const double xt = dl_bcast(x[c], t) * rt;
+        if (lane == t) x[c] = xt;
+        else if (lane < t) x[c] = fma(-u, xt, x[c]);
+      }
+    }
+  }
+  const bool failed = sbad != 0;
+  if (r0 + lane < n) {
+#pragma unroll
+    for (int c = 0; c < CW; ++c)
+      if (cb + c < nc) Xv[(long long)(r0 + lane) * nc + cb + c] = failed ? __builtin_nan("") : x[c];
+  }
+  // publish x_blk once every solving wave's stores are done: each such wave
+  // releases its stores, then the barrier, then one flag store (release)
+  if constexpr (NR > 2) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+  }
+  if (tid == 0) __hip_atomic_store(&flags[blk], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int NR>
+static int dl_trsv_sf(const double* LU, int n, int nc, double* X, double* W, int* flags, hipStream_t st) {
+  const int nb = (n + DL_SF_BS - 1) / DL_SF_BS;
+  const size_t shm = dl_sf_lds<NR>();
+  auto kf = dl_trsv_sf_kernel<NR, true>;
+  auto kb = dl_trsv_sf_kernel<NR, false>;
+  for (auto k : {kf, kb}) {
+    hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    if (e != hipSuccess) return set_error("dense_lu_solve: hipFuncSetAttribute: %s", hipGetErrorString(e)), JFX_EHIP;
+  }
+  hipLaunchKernelGGL(kf, dim3(nb), dim3(256), shm, st, LU, n, nc, X, W, flags);
+  JFX_LAUNCH_CHECK();
+  hipLaunchKernelGGL(kb, dim3(nb), dim3(256), shm, st, LU, n, nc, W, X, flags + nb);
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
 }
 
 // Rows kb .. kb + nbs - 1 of X: X_k <- T_kk^-1 X_k with T_kk the unit lower
@@ -611,9 +774,24 @@ template <typename T>
 static int dl_solve(const T* LU, int n, const int* perm, const T* B, T* X, T* W, long long nrhs, hipStream_t st) {
   constexpr int NBS = dl_nbs<T>();
   const long long tot = (long long)n * nrhs;
-  hipLaunchKernelGGL(dl_gather_kernel<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, B, perm, X, n,
-                     nrhs);
+  const int nb = (n + DL_SF_BS - 1) / DL_SF_BS;
+  // flags of the single-launch solve: 2 nb ints after the n x nrhs work values
+  int* flags = reinterpret_cast<int*>(W + tot);
+  const bool single = std::is_same<T, double>::value && nrhs <= DL_SF_MAXR;
+  const long long nthreads = tot > 2LL * nb ? tot : 2LL * nb;
+  hipLaunchKernelGGL(dl_gather_kernel<T>, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, st, B, perm, X,
+                     n, nrhs, flags, single ? 2 * nb : 0);
   JFX_LAUNCH_CHECK();
+  if constexpr (std::is_same<T, double>::value) {
+    if (single) {
+      const int nc = (int)nrhs;
+      if (nc == 1) return dl_trsv_sf<1>(LU, n, nc, X, W, flags, st);
+      if (nc == 2) return dl_trsv_sf<2>(LU, n, nc, X, W, flags, st);
+      if (nc <= 4) return dl_trsv_sf<4>(LU, n, nc, X, W, flags, st);
+      if (nc <= 8) return dl_trsv_sf<8>(LU, n, nc, X, W, flags, st);
+      return dl_trsv_sf<16>(LU, n, nc, X, W, flags, st);
+    }
+  }
   const unsigned gx = (unsigned)((nrhs + 63) / 64);
   for (int kb = 0; kb < n; kb += NBS) {  // forward, L y = P b: X (updated) -> W
     const int nbs = n - kb < NBS ? n - kb : NBS;

@@ -344,6 +344,14 @@ def dense_lu_factor(A: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
     return LU, perm
 
 
+def _dense_solve_work(n: int, nrhs: int, like: torch.Tensor) -> torch.Tensor:
+    """Work buffer of ``jfx_dense_lu_solve``: n x nrhs values, then
+    2 ceil(n / 64) ints of block flags."""
+    elt = like.element_size()
+    extra = (2 * ((n + 63) // 64) * 4 + elt - 1) // elt
+    return torch.empty(n * nrhs + extra, dtype=like.dtype, device=like.device)
+
+
 def dense_lu_solve(LU: torch.Tensor, perm: torch.Tensor, B: torch.Tensor) -> torch.Tensor:
     """X = A^-1 B for B of shape (n,) or (n, nrhs), with (LU, perm) from
     ``dense_lu_factor``.  A complex B with a real factor is solved as its
@@ -357,21 +365,21 @@ def dense_lu_solve(LU: torch.Tensor, perm: torch.Tensor, B: torch.Tensor) -> tor
         B2 = (B2 if B2.is_complex() else to_complex(B2.contiguous())).contiguous()
         _req(B2, torch.complex128, "B")
         X = torch.empty_like(B2)
-        W = torch.empty_like(B2)
+        W = _dense_solve_work(n, B2.shape[1], B2)
         _lib.call("jfx_dense_lu_solve", 1, _p(LU), n, _p(perm), _p(B2), _p(X), B2.shape[1], _p(W), _s())
     elif B2.is_complex():
         B2 = B2.contiguous()
         _req(B2, torch.complex128, "B")
         Br = torch.view_as_real(B2).reshape(n, -1)  # (n, 2 nrhs): re, im interleaved
         Xr = torch.empty_like(Br)
-        W = torch.empty_like(Br)
+        W = _dense_solve_work(n, Br.shape[1], Br)
         _lib.call("jfx_dense_lu_solve", 0, _p(LU), n, _p(perm), _p(Br), _p(Xr), Br.shape[1], _p(W), _s())
         X = torch.view_as_complex(Xr.reshape(n, -1, 2))
     else:
         B2 = B2.contiguous()
         _req(B2, torch.float64, "B")
         X = torch.empty_like(B2)
-        W = torch.empty_like(B2)
+        W = _dense_solve_work(n, B2.shape[1], B2)
         _lib.call("jfx_dense_lu_solve", 0, _p(LU), n, _p(perm), _p(B2), _p(X), B2.shape[1], _p(W), _s())
     return X.reshape(n) if vec else X
 

@@ -159,3 +159,32 @@ def test_size_limit_rejected():
     with pytest.raises(Exception, match="exceeds"):
         _lib.call("jfx_dense_lu_factor", 0, buf.data_ptr(), 20000, piv.data_ptr(), piv.data_ptr(), piv.data_ptr(),
                   torch.cuda.current_stream().cuda_stream)
+
+
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 129, 640, 3844])
+def test_single_launch_solve_matches_blocked(n):
+    """Up to 16 right-hand sides take the single-launch solve (one workgroup
+    per 64-row block, device flags between blocks; templates for 1, 2, 4, 8
+    and 16 columns); 17 take the per-block launches.  Both agree with LAPACK
+    and with each other."""
+    from jaxfun_amd import kernels as K
+
+    rng = np.random.default_rng(500 + n)
+    A = _rand(rng, n) + n ** 0.5 * np.eye(n)
+    B = rng.standard_normal((n, 17))
+    LU, perm = K.dense_lu_factor(_t(A))
+    Xr = np.linalg.solve(A, B)
+    scale = np.abs(Xr).max()
+    xb = K.dense_lu_solve(LU, perm, _t(B)).cpu().numpy()  # blocked
+    x1 = K.dense_lu_solve(LU, perm, _t(B[:, 0])).cpu().numpy()
+    for X, k in [(xb, 17), (x1[:, None], 1)] + [
+            (K.dense_lu_solve(LU, perm, _t(B[:, :k])).cpu().numpy(), k) for k in (2, 3, 5, 8, 16)]:
+        assert np.isfinite(X).all(), k
+        assert np.abs(X - Xr[:, :k]).max() <= 1e-11 * scale, k
+        R = A @ X - B[:, :k]
+        assert np.abs(R).max() <= 1e-13 * (np.abs(A).sum(1).max() * np.abs(X).max()) * max(1, n ** 0.5), k
+        assert np.abs(X - xb[:, :k]).max() <= 1e-12 * scale, k
+    # repeated solves clear the work buffer's flags each time
+    for _ in range(3):
+        again = K.dense_lu_solve(LU, perm, _t(B[:, 0])).cpu().numpy()
+        assert np.array_equal(again, x1)
