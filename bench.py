@@ -331,18 +331,20 @@ def bench_periodic_poisson(reps: int = 50, batch: int = 1024):
 def bench_band_solve(reps: int = 20):
     """DiaMatrix.solve on few long lines (la/diamatrix.py:592-770, LUFactors.solve
     :1880-1925): the Legendre-Dirichlet mass matrix (offsets 0, +-2) of size n,
-    factored once, solved for `lines` right-hand sides with one wavefront per
-    line (eight for long lines; the rows split over the lanes, chunk maps
-    composed by a wave scan) and, for comparison, with one lane per line (a
+    factored once, solved for `lines` right-hand sides with each line's rows
+    over several workgroups that hand the recurrence window on (`grid`, what
+    `auto` takes for lines of 8192 rows and more), one workgroup per line
+    (`wave`: eight wavefronts for long lines, the rows split over the lanes,
+    chunk maps composed by a wave scan) and one lane per line (`lanes`: a
     sequential sweep).  Times are HIP-event means over back-to-back launches;
-    the error is the relative L2 difference of the two paths."""
+    the errors are relative L2 differences from the per-lane sweep."""
     import torch
 
     from jaxfun_amd import kernels as K
 
     out = {}
     gen = torch.Generator(device="cuda").manual_seed(5)
-    for n, lines in ((1024, 1), (20000, 1), (20000, 16), (100000, 1)):
+    for n, lines in ((1024, 1), (20000, 1), (20000, 16), (100000, 1), (1000000, 1)):
         k = np.arange(n, dtype=np.float64)
         band = np.zeros((n, 5))
         band[:, 2] = 2.0 / (2 * k + 1) + 2.0 / (2 * k + 5)
@@ -352,9 +354,12 @@ def bench_band_solve(reps: int = 20):
         b = torch.randn(n, lines, dtype=torch.float64, device="cuda", generator=gen)
         r = {}
         sol = {}
-        for m in ("wave", "lanes"):
+        methods = ("grid", "wave", "lanes") if n >= 8192 else ("wave", "lanes")
+        if n > 100000:
+            methods = ("grid", "wave")  # one lane per line: ~0.4 s per solve
+        for m in methods:
             K.band_lu_solve(lu, 2, 2, b.clone(), 0, method=m)  # warm-up
-            nrep = reps if (m == "wave" or n < 20000) else max(3, reps // 5)  # the slow path: fewer launches
+            nrep = reps if (m != "lanes" or n < 20000) else max(3, reps // 5)  # the slow path: fewer launches
             Xs = [b.clone() for _ in range(nrep)]  # in-place solves: one copy per launch
             torch.cuda.synchronize()
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -365,8 +370,15 @@ def bench_band_solve(reps: int = 20):
             torch.cuda.synchronize()
             r[f"{m}_us"] = e0.elapsed_time(e1) / nrep * 1e3
             sol[m] = Xs[0]
-        r["speedup_wave_vs_lanes"] = r["lanes_us"] / r["wave_us"]
-        r["rel_diff_wave_vs_lanes"] = float(torch.linalg.norm(sol["wave"] - sol["lanes"]) / torch.linalg.norm(sol["lanes"]))
+        ref = "lanes" if "lanes" in sol else "wave"
+        if "lanes" in sol:
+            r["speedup_wave_vs_lanes"] = r["lanes_us"] / r["wave_us"]
+            r["rel_diff_wave_vs_lanes"] = float(torch.linalg.norm(sol["wave"] - sol["lanes"]) /
+                                                torch.linalg.norm(sol["lanes"]))
+        if "grid" in sol:
+            r["speedup_grid_vs_wave"] = r["wave_us"] / r["grid_us"]
+            r[f"rel_diff_grid_vs_{ref}"] = float(torch.linalg.norm(sol["grid"] - sol[ref]) /
+                                                 torch.linalg.norm(sol[ref]))
         out[f"n{n}_lines{lines}"] = r
     return out
 

@@ -178,6 +178,18 @@ int jfx_band_lu_solve(const double* lu, int n, int p, int q, double* X, long lon
  * (la/diamatrix.py:592-770). */
 int jfx_band_lu_solve_mode(const double* lu, int n, int p, int q, double* X, long long P, long long Q, int mode,
                            hipStream_t stream);
+/* The same solve for few long lines (p, q <= 4, n >= 8192, lines x chunks
+ * <= 1024) with each line's rows split over several 8-wave workgroups (chunks
+ * of >= 4096 rows, at most 64 per line), one launch per direction: each
+ * workgroup takes the recurrence window entering its rows from its
+ * predecessor through device flags (release / acquire at agent scope;
+ * bounded waits, a chunk that gives up propagates NaN instead of hanging).
+ * work holds jfx_band_lu_solve_grid_work_bytes(...) bytes (0: the shape does
+ * not suit the grid solve); calls sharing a work buffer must not overlap.
+ * Same reference interface as jfx_band_lu_solve. */
+long long jfx_band_lu_solve_grid_work_bytes(int n, int p, int q, long long P, long long Q);
+int jfx_band_lu_solve_grid(const double* lu, int n, int p, int q, double* X, long long P, long long Q, void* work,
+                           long long work_bytes, hipStream_t stream);
 
 /* Dense LU with partial pivoting, in place, of a row-major n x n matrix A
  * (cplx = 0: FP64, cplx = 1: complex128 as re/im pairs), n <= 16384:

@@ -48,6 +48,8 @@ SIGNATURES: dict[str, tuple] = {
     "jfx_band_lu_factor": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_vp, c_vp]),
     "jfx_band_lu_solve": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_ll, c_ll, c_vp]),
     "jfx_band_lu_solve_mode": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_ll, c_ll, c_int, c_vp]),
+    "jfx_band_lu_solve_grid_work_bytes": (c_ll, [c_int, c_int, c_int, c_ll, c_ll]),
+    "jfx_band_lu_solve_grid": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_ll, c_ll, c_vp, c_ll, c_vp]),
     "jfx_band_lu_solve_batched_mode": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_vp, c_ll, c_ll, c_int, c_int,
                                                c_vp]),
     "jfx_dense_lu_factor": (c_int, [c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_vp]),

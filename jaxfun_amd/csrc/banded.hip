@@ -447,9 +447,32 @@ __device__ __forceinline__ void compose_map(double (&M)[B][B], double (&v)[B], c
 // Window entering this lane's chunk.  UP: lanes (and waves) are traversed in
 // ascending order, else descending.  sm / ss: NW * (B * B + B) and NW * B
 // doubles of LDS (unused for NW = 1).
+// Chunks of one line spread over several workgroups (the grid solve): the
+// window entering this workgroup's rows is the state its predecessor in
+// traversal order published (pred_flag null for the line's first chunk), and
+// the window leaving them is published for the successor.
+struct GridLink {
+  const int* pred_flag;
+  const double* pred_state;
+  int* own_flag;
+  double* own_state;
+};
+
+constexpr int GRID_SPIN = 1 << 20;  // flag polls (~1 us each with the sleep) before giving up
+
+// Waits (bounded) for a predecessor's flag with acquire ordering at agent scope.
+__device__ __forceinline__ bool grid_wait(const int* f) {
+  int it = 0;
+  while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+    if (++it >= GRID_SPIN) return false;
+    __builtin_amdgcn_s_sleep(2);
+  }
+  return true;
+}
+
 template <int B, bool UP, int NW>
 __device__ __forceinline__ void line_scan(double (&M)[B][B], double (&v)[B], int lane, int wv, double* sm, double* ss,
-                                          double (&win)[B]) {
+                                          double (&win)[B], const GridLink* gl = nullptr) {
   // inclusive scan within the wave
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
@@ -487,6 +510,13 @@ __device__ __forceinline__ void line_scan(double (&M)[B][B], double (&v)[B], int
       double st[B];
 #pragma unroll
       for (int r = 0; r < B; ++r) st[r] = 0.0;
+      if (gl && gl->pred_flag) {
+        // a predecessor that never publishes leaves NaN, which the rest of
+        // the line inherits, instead of a hang
+        const bool ok = grid_wait(gl->pred_flag);
+#pragma unroll
+        for (int r = 0; r < B; ++r) st[r] = ok ? gl->pred_state[r] : __builtin_nan("");
+      }
       for (int t = 0; t < NW; ++t) {
         const int k = UP ? t : NW - 1 - t;
         const double* m = sm + k * (B * B + B);
@@ -501,6 +531,11 @@ __device__ __forceinline__ void line_scan(double (&M)[B][B], double (&v)[B], int
         }
 #pragma unroll
         for (int r = 0; r < B; ++r) st[r] = nt[r];
+      }
+      if (gl) {
+#pragma unroll
+        for (int r = 0; r < B; ++r) gl->own_state[r] = st[r];
+        __hip_atomic_store(gl->own_flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
     __syncthreads();
@@ -563,7 +598,7 @@ __device__ __forceinline__ double wave_step(const double (&cf)[B], const double 
 template <int B, int RC, int NW, bool FWD>
 __device__ __forceinline__ void wave_sweep(const double* __restrict__ lu, double* __restrict__ x, long long Q, int n,
                                            int W, int p, int q, int i0, int i1, int lane, int wv, double* sm,
-                                           double* ss) {
+                                           double* ss, const GridLink* gl = nullptr) {
   const int len = i1 - i0;
   double v[B], h[B][B], M[B][B], win[B];
 #pragma unroll
@@ -589,7 +624,7 @@ __device__ __forceinline__ void wave_sweep(const double* __restrict__ lu, double
   for (int t = 0; t < B; ++t)
 #pragma unroll
     for (int j = 0; j < B; ++j) M[t][j] = h[j][t];
-  line_scan<B, FWD, NW>(M, v, lane, wv, sm, ss, win);
+  line_scan<B, FWD, NW>(M, v, lane, wv, sm, ss, win, gl);
   wave_load<B, RC, FWD>(lu, x, Q, n, W, p, q, i0, i1, 0, cur);
   for (int ib = 0; ib < len; ib += RC) {
     wave_load<B, RC, FWD>(lu, x, Q, n, W, p, q, i0, i1, ib + RC, nxt);
@@ -632,6 +667,46 @@ __global__ __launch_bounds__(64 * NW) void band_lu_solve_wave_kernel(const doubl
   if constexpr (NW > 1) __syncthreads();  // sm / ss are reused
   // backward: x_i = (y_i - sum_t U[i, i+1+t] x_{i+1+t}) / U[i, i]
   wave_sweep<B, RC, NW, false>(lu, x, Q, n, W, p, q, i0, i1, lane, wv, sm, ss);
+}
+
+// The same solve with each line's rows split over G workgroups of 8 waves
+// (rows [g R, (g + 1) R) for chunk g), one launch per direction: each
+// workgroup runs both passes of its rows and takes the window entering them
+// from its predecessor in traversal order through `flags` / `states` (one
+// flag and B doubles per line and chunk, flags zeroed before the launch).
+// Workgroups are numbered in traversal order (the backward launch numbers the
+// chunks from the end), so a workgroup only waits for one dispatched before
+// it.  The forward launch writes y into X, the backward launch x.
+template <int B, bool FWD>
+__global__ __launch_bounds__(512) void band_lu_solve_grid_kernel(const double* __restrict__ lu, int n, int p, int q,
+                                                                 double* __restrict__ X, long long Q, int G, int R,
+                                                                 int* __restrict__ flags,
+                                                                 double* __restrict__ states) {
+  constexpr int NW = 8, RC = B == 4 ? 4 : 8;
+  __shared__ double sm[NW * (B * B + B)];
+  __shared__ double ss[NW * B];
+  const long long bid = blockIdx.x;
+  const long long line = bid / G;
+  const int gi = (int)(bid - line * G), g = FWD ? gi : G - 1 - gi;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long long pp = line / Q, qq = line - pp * Q;
+  double* __restrict__ x = X + pp * (long long)n * Q + qq;
+  const int W = p + q + 1;
+  const int lo = g * R, hi = min(n, lo + R);
+  const int c = (hi - lo + 64 * NW - 1) / (64 * NW);
+  const int i0 = min(hi, lo + (int)threadIdx.x * c), i1 = min(hi, i0 + c);
+  int* lf = flags + line * G;
+  double* lst = states + line * G * B;
+  const int pred = FWD ? g - 1 : g + 1;
+  const bool haspred = pred >= 0 && pred < G;
+  const GridLink gl{haspred ? lf + pred : nullptr, haspred ? lst + (long long)pred * B : nullptr, lf + g,
+                    lst + (long long)g * B};
+  wave_sweep<B, RC, NW, FWD>(lu, x, Q, n, W, p, q, i0, i1, lane, wv, sm, ss, &gl);
+}
+
+__global__ void band_grid_reset(int* __restrict__ flags, long long nflags) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nflags; i += (long long)gridDim.x * blockDim.x)
+    flags[i] = 0;
 }
 
 // As above with one factor per line group: line (pp, qq) uses system
@@ -849,6 +924,24 @@ constexpr int WAVE_MAXB = 4;
 constexpr int WAVE_MIN_N = 32;
 constexpr long long WAVE_MAX_LINES = 2048;
 constexpr long long WAVE_WIDE_LINES = 256;  // 8 waves per line: 2048 waves, two per SIMD
+constexpr int GRID_ROWS = 4096;             // rows per workgroup of the grid solve (at least)
+constexpr int GRID_MAX_CHUNKS = 64;         // workgroups per line
+// lines x chunks.  Beyond the workgroups the GPU holds at once, the rest wait
+// for slots; no workgroup can block one it depends on (each waits only for a
+// lower-numbered one, and a slot's holders with lower numbers keep running).
+constexpr long long GRID_MAX_WGS = 1024;
+
+// Chunks per line and rows per chunk of the grid solve (G = 0: not applicable).
+static void grid_shape(int n, long long lines, int& G, int& R) {
+  G = 0;
+  R = GRID_ROWS;
+  if (n < 2 * GRID_ROWS) return;
+  const long long per = ((long long)n + GRID_MAX_CHUNKS - 1) / GRID_MAX_CHUNKS;
+  if (per > R) R = (int)((per + 511) / 512 * 512);
+  const int g = (n + R - 1) / R;
+  if (lines * g > GRID_MAX_WGS) return;
+  G = g;
+}
 
 static unsigned grid_for(long long n, int block) {
   long long g = (n + block - 1) / block;
@@ -984,6 +1077,55 @@ int jfx_band_lu_solve_mode(const double* lu, int n, int p, int q, double* X, lon
 int jfx_band_lu_solve(const double* lu, int n, int p, int q, double* X, long long P, long long Q,
                       hipStream_t stream) {
   return jfx_band_lu_solve_mode(lu, n, p, q, X, P, Q, 0, stream);
+}
+
+long long jfx_band_lu_solve_grid_work_bytes(int n, int p, int q, long long P, long long Q) {
+  if (n <= 0 || p < 0 || q < 0 || P < 0 || Q <= 0 || p > WAVE_MAXB || q > WAVE_MAXB) return 0;
+  int G, R;
+  grid_shape(n, P * Q, G, R);
+  if (G == 0) return 0;
+  const long long chunks = 2 * P * Q * G;  // forward and backward
+  return (chunks * 4 + 15) / 16 * 16 + chunks * WAVE_MAXB * 8;
+}
+
+int jfx_band_lu_solve_grid(const double* lu, int n, int p, int q, double* X, long long P, long long Q, void* work,
+                           long long work_bytes, hipStream_t stream) {
+  if (!lu || !X || !work || n <= 0 || p < 0 || q < 0 || P < 0 || Q <= 0)
+    return set_error("band_lu_solve_grid: bad arguments"), JFX_EARG;
+  if (p > WAVE_MAXB || q > WAVE_MAXB)
+    return set_error("band_lu_solve_grid: bandwidths must be <= %d (p=%d, q=%d)", WAVE_MAXB, p, q), JFX_EARG;
+  const long long lines = P * Q;
+  if (lines == 0) return JFX_OK;
+  int G, R;
+  grid_shape(n, lines, G, R);
+  if (G == 0)
+    return set_error("band_lu_solve_grid: %lld lines of %d rows do not suit the grid solve (n >= %d, lines x "
+                     "chunks <= %lld)", lines, n, 2 * GRID_ROWS, GRID_MAX_WGS),
+           JFX_ESHAPE;
+  const long long need = jfx_band_lu_solve_grid_work_bytes(n, p, q, P, Q);
+  if (work_bytes < need)
+    return set_error("band_lu_solve_grid: work holds %lld bytes, %lld needed", work_bytes, need), JFX_ESHAPE;
+  const long long chunks = lines * G;
+  int* flags = static_cast<int*>(work);
+  double* states = reinterpret_cast<double*>(static_cast<char*>(work) + (2 * chunks * 4 + 15) / 16 * 16);
+  hipLaunchKernelGGL(band_grid_reset, dim3(1), dim3(256), 0, stream, flags, 2 * chunks);
+  JFX_LAUNCH_CHECK();
+  const dim3 grid((unsigned)chunks);
+  if (p <= 2 && q <= 2) {
+    hipLaunchKernelGGL((band_lu_solve_grid_kernel<2, true>), grid, dim3(512), 0, stream, lu, n, p, q, X, Q, G, R,
+                       flags, states);
+    JFX_LAUNCH_CHECK();
+    hipLaunchKernelGGL((band_lu_solve_grid_kernel<2, false>), grid, dim3(512), 0, stream, lu, n, p, q, X, Q, G, R,
+                       flags + chunks, states + chunks * WAVE_MAXB);
+  } else {
+    hipLaunchKernelGGL((band_lu_solve_grid_kernel<4, true>), grid, dim3(512), 0, stream, lu, n, p, q, X, Q, G, R,
+                       flags, states);
+    JFX_LAUNCH_CHECK();
+    hipLaunchKernelGGL((band_lu_solve_grid_kernel<4, false>), grid, dim3(512), 0, stream, lu, n, p, q, X, Q, G, R,
+                       flags + chunks, states + chunks * WAVE_MAXB);
+  }
+  JFX_LAUNCH_CHECK();
+  return JFX_OK;
 }
 
 int jfx_band_lu_factor_batched(const double* band, int nsys, int n, int p, int q, double* lu, int* status_dev,

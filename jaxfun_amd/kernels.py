@@ -254,7 +254,7 @@ def band_lu_factor_batched(band: torch.Tensor, p: int, q: int) -> torch.Tensor:
     return lu
 
 
-BAND_SOLVE_METHODS = {"auto": 0, "lanes": 1, "wave": 2}
+BAND_SOLVE_METHODS = {"auto": 0, "lanes": 1, "wave": 2, "grid": 3}
 
 
 def band_lu_solve_batched(lu: torch.Tensor, p: int, q: int, X: torch.Tensor, axis: int,
@@ -300,8 +300,11 @@ def band_lu_solve(lu: torch.Tensor, p: int, q: int, X: torch.Tensor, axis: int, 
     X is solved through its real view: the same real factor for re and im).
 
     ``method``: "lanes" runs one lane per line (a sequential sweep), "wave"
-    one wavefront per line (rows split over the 64 lanes, p, q <= 4), "auto"
-    the wavefront solve for few long lines and the per-lane sweep otherwise."""
+    one wavefront per line (rows split over the 64 lanes, p, q <= 4), "grid"
+    each line's rows over several workgroups that hand the boundary window on
+    (few lines of at least 8192 rows, p, q <= 4), "auto" the grid solve where
+    it applies, else the wavefront solve for few long lines and the per-lane
+    sweep otherwise."""
     if method not in BAND_SOLVE_METHODS:
         raise ValueError(f"band solve: method must be one of {sorted(BAND_SOLVE_METHODS)}, got {method!r}")
     if X.is_complex():
@@ -316,6 +319,15 @@ def band_lu_solve(lu: torch.Tensor, p: int, q: int, X: torch.Tensor, axis: int, 
         raise ValueError(f"band solve: the wavefront solve takes bandwidths <= 4, got p={p}, q={q}")
     P = math.prod(X.shape[:axis])
     Q = math.prod(X.shape[axis + 1:])
+    if method in ("auto", "grid"):
+        nbytes = _lib.load().jfx_band_lu_solve_grid_work_bytes(n, p, q, P, Q)
+        if nbytes > 0:
+            work = torch.empty(nbytes, dtype=torch.uint8, device=X.device)
+            _lib.call("jfx_band_lu_solve_grid", _p(lu), n, p, q, _p(X), P, Q, _p(work), nbytes, _s())
+            return X
+        if method == "grid":
+            raise ValueError(f"band solve: the grid solve takes few lines of >= 8192 rows and p, q <= 4 "
+                             f"(n={n}, lines={P * Q}, p={p}, q={q})")
     _lib.call("jfx_band_lu_solve_mode", _p(lu), n, p, q, _p(X), P, Q, BAND_SOLVE_METHODS[method], _s())
     return X
 

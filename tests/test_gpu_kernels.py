@@ -179,12 +179,66 @@ def test_band_solve_wave_vs_banded_lapack(dev, n, p, q, shape, axis):
     assert np.abs(Xw - Xr).max() <= 1e-13 * scale
     assert np.abs(Xl - Xr).max() <= 1e-13 * scale
     assert np.abs(Xw - Xl).max() <= 1e-13 * scale
-    assert np.array_equal(Xa, Xw) or np.array_equal(Xa, Xl)
+    same = [np.array_equal(Xa, Xw), np.array_equal(Xa, Xl)]
+    if n >= 8192:  # long enough for the grid solve, which auto then takes
+        Xg = K.band_lu_solve(lu, p, q, _t(B, dev).contiguous(), axis, method="grid").cpu().numpy()
+        assert np.abs(Xg - Xr).max() <= 1e-13 * scale
+        same.append(np.array_equal(Xa, Xg))
+    assert any(same)
+
+
+@pytest.mark.parametrize(
+    "n,p,q,shape,axis",
+    [
+        (8192, 2, 2, (8192,), 0),  # two chunks
+        (8193, 1, 1, (8193,), 0),  # a last chunk of one row
+        (50000, 4, 4, (50000,), 0),
+        (16384, 0, 2, (16384,), 0),  # upper factor only
+        (16384, 3, 0, (16384,), 0),  # lower factor only
+        (30000, 1, 3, (2, 30000, 3), 1),  # six lines of eight chunks
+        (300000, 2, 2, (300000,), 0),  # 59 chunks of 5120 rows
+    ],
+)
+def test_band_solve_grid_vs_banded_lapack(dev, n, p, q, shape, axis):
+    """Each line's rows over several workgroups that hand the recurrence
+    window on through device flags, against LAPACK's banded solve and the
+    one-workgroup wavefront solve; "auto" takes it for these shapes."""
+    import scipy.linalg as sla
+
+    from jaxfun_amd import kernels as K
+
+    rng = np.random.default_rng(n + 10 * p + q + 1)
+    A_band = _banded(rng, n, p, q)
+    lu = K.band_lu_factor(_t(A_band, dev), p, q)
+    B = rng.standard_normal(shape)
+    Bl = np.moveaxis(B, axis, 0).reshape(n, -1)
+    Xr = np.moveaxis(sla.solve_banded((p, q), _band_to_ab(A_band, p, q), Bl).reshape(
+        (n,) + tuple(np.delete(shape, axis))), 0, axis)
+    Xg = K.band_lu_solve(lu, p, q, _t(B, dev).contiguous(), axis, method="grid").cpu().numpy()
+    Xw = K.band_lu_solve(lu, p, q, _t(B, dev).contiguous(), axis, method="wave").cpu().numpy()
+    Xa = K.band_lu_solve(lu, p, q, _t(B, dev).contiguous(), axis).cpu().numpy()
+    scale = np.abs(Xr).max()
+    assert np.isfinite(Xg).all()
+    assert np.abs(Xg - Xr).max() <= 1e-13 * scale
+    assert np.abs(Xg - Xw).max() <= 1e-13 * scale
+    assert np.array_equal(Xa, Xg)
+    # repeated calls start from cleared flags
+    again = K.band_lu_solve(lu, p, q, _t(B, dev).contiguous(), axis, method="grid").cpu().numpy()
+    assert np.array_equal(again, Xg)
+
+
+def test_band_solve_grid_rejects_unsuited_shapes(dev):
+    from jaxfun_amd import kernels as K
+
+    rng = np.random.default_rng(3)
+    lu = K.band_lu_factor(_t(_banded(rng, 1000, 2, 2), dev), 2, 2)
+    with pytest.raises(ValueError, match="grid solve"):
+        K.band_lu_solve(lu, 2, 2, _t(rng.standard_normal(1000), dev), 0, method="grid")
 
 
 def test_band_solve_wave_legendre_mass_large(dev):
     """DiaMatrix.solve of one long vector with the Legendre-Dirichlet mass matrix
-    (offsets 0, +-2) goes to the wavefront solve and matches LAPACK."""
+    (offsets 0, +-2) goes to the grid solve and matches LAPACK."""
     import scipy.linalg as sla
 
     from jaxfun_amd import kernels as K
