@@ -194,8 +194,9 @@ int jfx_band_lu_solve_grid(const double* lu, int n, int p, int q, double* X, lon
 /* Dense LU with partial pivoting, in place, of a row-major n x n matrix A
  * (cplx = 0: FP64, cplx = 1: complex128 as re/im pairs), n <= 16384:
  * blocked over 32-column panels (16 complex; one workgroup per panel with
- * the panel's rows in registers while at most 4096 rows remain, the pivot
- * the first row of largest |re| + |im|), row swaps and the block-row
+ * the panel's rows in registers while at most 4096 rows remain, working on a
+ * column-major copy of the panel in work (n x 256 bytes), the pivot the
+ * first row of largest |re| + |im|), row swaps and the block-row
  * triangular solve per panel, then a tiled FP64 GEMM update of the trailing
  * matrix.  ipiv (n ints): row swapped with row j at step j; perm (n ints):
  * row i of P A is row perm[i] of A; status_dev (zeroed by the caller)
@@ -203,7 +204,8 @@ int jfx_band_lu_solve_grid(const double* lu, int n, int p, int q, double* X, lon
  * Replaces jax.scipy.linalg.lu_factor in Matrix.lu_factor
  * (la/matrix.py:137-167), TPMatricesDenseLUFactors (la/tpmatrix.py:1016-1086)
  * and TensorMatrix.solve (la/tensormatrix.py:76-80). */
-int jfx_dense_lu_factor(int cplx, void* A, int n, int* ipiv, int* perm, int* status_dev, hipStream_t stream);
+int jfx_dense_lu_factor(int cplx, void* A, int n, int* ipiv, int* perm, int* status_dev, void* work,
+                        hipStream_t stream);
 /* X (n x nrhs, row-major) = A^-1 B from the factors above: X = P B, then
  * forward (unit L) and backward (U) substitution.  Real factors with
  * nrhs <= 2: one launch per direction, one workgroup per 64-row block that

@@ -52,7 +52,7 @@ SIGNATURES: dict[str, tuple] = {
     "jfx_band_lu_solve_grid": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_ll, c_ll, c_vp, c_ll, c_vp]),
     "jfx_band_lu_solve_batched_mode": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_vp, c_ll, c_ll, c_int, c_int,
                                                c_vp]),
-    "jfx_dense_lu_factor": (c_int, [c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_vp]),
+    "jfx_dense_lu_factor": (c_int, [c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "jfx_dense_lu_solve": (c_int, [c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_ll, c_vp, c_vp]),
     "jfx_div_bcast_f64": (c_int, [c_vp, c_vp, c_vp, c_ll, c_ll, c_vp]),
     "jfx_div_bcast_f32": (c_int, [c_vp, c_vp, c_vp, c_ll, c_ll, c_vp]),

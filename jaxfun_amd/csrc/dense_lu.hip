@@ -145,8 +145,11 @@ __global__ __launch_bounds__(DL_PT) void dl_panel_kernel(T* __restrict__ A, int 
 // (n - k0) (2 NB + NB^2 / SB) values, and no column step waits on memory.
 template <typename T, int R, int NT>
 __global__ __launch_bounds__(NT) void dl_panel_reg_kernel(T* __restrict__ A, int n, int k0, int nb,
-                                                              int* __restrict__ ipiv, int* __restrict__ status) {
+                                                              int* __restrict__ ipiv, int* __restrict__ status,
+                                                              T* __restrict__ S) {
   constexpr int SB = sizeof(T) == sizeof(double) ? 8 : 4;
+  constexpr int TC = 64 / sizeof(T);  // columns per transpose tile (64-byte row pieces)
+  __shared__ T tt[NT / 64][32][TC + 1];  // per-wave transpose tile: 32 rows x TC columns
   __shared__ double wv[NT / 64];
   __shared__ int wi[NT / 64];
   __shared__ int piv_sh[2];
@@ -155,6 +158,29 @@ __global__ __launch_bounds__(NT) void dl_panel_reg_kernel(T* __restrict__ A, int
   __shared__ T Lss[SB][SB];
   __shared__ T Us[SB][DL_NB];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  // The panel A[k0.., k0..k0+nb) is worked on column-major in S (ld rows per
+  // column): a thread's rows are then consecutive with its neighbours', so
+  // every access below coalesces, where the row-major matrix would put each
+  // lane on its own cache line.  It is copied in and out through per-wave
+  // LDS tiles (32 rows x one 64-byte piece of each row), coalesced both ways.
+  const int ld = n - k0;
+  // (c - k0) * ld + (i - k0) < 32 n <= 2^19: 32-bit offsets
+  auto SP = [&](int i, int c) -> T& { return S[(c - k0) * ld + (i - k0)]; };
+  for (int rb = k0 + 32 * w; rb < n; rb += 32 * (NT / 64)) {
+    for (int cg = 0; cg < nb; cg += TC) {
+#pragma unroll
+      for (int t = 0; t < 32 * TC / 64; ++t) {
+        const int e = lane + 64 * t, rr = e / TC, cc = e % TC;
+        if (rb + rr < n && cg + cc < nb) tt[w][rr][cc] = A[(long long)(rb + rr) * n + k0 + cg + cc];
+      }
+#pragma unroll
+      for (int t = 0; t < 32 * TC / 64; ++t) {
+        const int e = lane + 64 * t, cc = e / 32, rr = e % 32;
+        if (rb + rr < n && cg + cc < nb) SP(rb + rr, k0 + cg + cc) = tt[w][rr][cc];
+      }
+    }
+  }
+  __syncthreads();
   int bad = 0;  // thread 0: 1 + the first column with a zero pivot
   for (int s0 = k0; s0 < k0 + nb; s0 += SB) {
     const int sb = min(SB, k0 + nb - s0);
@@ -163,7 +189,7 @@ __global__ __launch_bounds__(NT) void dl_panel_reg_kernel(T* __restrict__ A, int
     for (int m = 0; m < R; ++m) {
       const int i = k0 + tid + NT * m;
 #pragma unroll
-      for (int c = 0; c < SB; ++c) v[m][c] = (i < n && i >= s0 && c < sb) ? A[(long long)i * n + s0 + c] : dl_zero<T>();
+      for (int c = 0; c < SB; ++c) v[m][c] = (i < n && i >= s0 && c < sb) ? SP(i, s0 + c) : dl_zero<T>();
     }
     // the column index within the sub-panel is a compile-time constant, so
     // the register slots it selects need no runtime selects
@@ -267,9 +293,9 @@ __global__ __launch_bounds__(NT) void dl_panel_reg_kernel(T* __restrict__ A, int
         for (int jj = 0; jj < sb; ++jj) {
           const int j = s0 + jj, r = pvs[j - k0];
           if (r != j) {
-            const T a = A[(long long)j * n + c];
-            A[(long long)j * n + c] = A[(long long)r * n + c];
-            A[(long long)r * n + c] = a;
+            const T a = SP(j, c);
+            SP(j, c) = SP(r, c);
+            SP(r, c) = a;
           }
         }
       }
@@ -281,7 +307,7 @@ __global__ __launch_bounds__(NT) void dl_panel_reg_kernel(T* __restrict__ A, int
       if (i >= s0 && i < n) {
 #pragma unroll
         for (int c = 0; c < SB; ++c)
-          if (c < sb) A[(long long)i * n + s0 + c] = v[m][c];
+          if (c < sb) SP(i, s0 + c) = v[m][c];
       }
       if (i >= s0 && i < s0 + sb) {
 #pragma unroll
@@ -295,7 +321,7 @@ __global__ __launch_bounds__(NT) void dl_panel_reg_kernel(T* __restrict__ A, int
         const int c = s0 + sb + tid;
         T x[SB];
 #pragma unroll
-        for (int a = 0; a < SB; ++a) x[a] = a < sb ? A[(long long)(s0 + a) * n + c] : dl_zero<T>();
+        for (int a = 0; a < SB; ++a) x[a] = a < sb ? SP(s0 + a, c) : dl_zero<T>();
 #pragma unroll
         for (int a = 1; a < SB; ++a)
 #pragma unroll
@@ -304,7 +330,7 @@ __global__ __launch_bounds__(NT) void dl_panel_reg_kernel(T* __restrict__ A, int
 #pragma unroll
         for (int a = 0; a < SB; ++a)
           if (a < sb) {
-            A[(long long)(s0 + a) * n + c] = x[a];
+            SP(s0 + a, c) = x[a];
             Us[a][tid] = x[a];
           }
       }
@@ -317,10 +343,9 @@ __global__ __launch_bounds__(NT) void dl_panel_reg_kernel(T* __restrict__ A, int
       for (int m = 0; m < R; ++m) {
         const int i = k0 + tid + NT * m;
         if (i >= s0 + sb && i < n) {
-          T* row = A + (long long)i * n + s0 + sb;
           T a[RCM];
 #pragma unroll
-          for (int cc = 0; cc < RCM; ++cc) a[cc] = cc < rc ? row[cc] : dl_zero<T>();
+          for (int cc = 0; cc < RCM; ++cc) a[cc] = cc < rc ? SP(i, s0 + sb + cc) : dl_zero<T>();
 #pragma unroll
           for (int t = 0; t < SB; ++t) {
             if (t < sb) {
@@ -331,12 +356,27 @@ __global__ __launch_bounds__(NT) void dl_panel_reg_kernel(T* __restrict__ A, int
           }
 #pragma unroll
           for (int cc = 0; cc < RCM; ++cc)
-            if (cc < rc) row[cc] = a[cc];
+            if (cc < rc) SP(i, s0 + sb + cc) = a[cc];
         }
         asm volatile("" ::: "memory");  // one row's entries in registers at a time
       }
     }
     __syncthreads();
+  }
+  // the factored panel back to A (the barrier above ends the last sub-panel)
+  for (int rb = k0 + 32 * w; rb < n; rb += 32 * (NT / 64)) {
+    for (int cg = 0; cg < nb; cg += TC) {
+#pragma unroll
+      for (int t = 0; t < 32 * TC / 64; ++t) {
+        const int e = lane + 64 * t, cc = e / 32, rr = e % 32;
+        if (rb + rr < n && cg + cc < nb) tt[w][rr][cc] = SP(rb + rr, k0 + cg + cc);
+      }
+#pragma unroll
+      for (int t = 0; t < 32 * TC / 64; ++t) {
+        const int e = lane + 64 * t, rr = e / TC, cc = e % TC;
+        if (rb + rr < n && cg + cc < nb) A[(long long)(rb + rr) * n + k0 + cg + cc] = tt[w][rr][cc];
+      }
+    }
   }
   if (tid < nb) ipiv[k0 + tid] = pvs[tid];
   if (tid == 0 && bad && *status == 0) *status = bad;
@@ -740,17 +780,20 @@ static int dl_gemm_sub(const T* A, long long lda, const T* B, long long ldb, T* 
 }
 
 template <typename T>
-static int dl_factor(T* A, int n, int* ipiv, int* perm, int* status, hipStream_t st) {
+static int dl_factor(T* A, int n, int* ipiv, int* perm, int* status, T* S, hipStream_t st) {
   constexpr int NB = dl_nb<T>();
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int nb = n - k0 < NB ? n - k0 : NB;
     const int rows = n - k0;
     if (rows <= 1024)
-      hipLaunchKernelGGL((dl_panel_reg_kernel<T, 1, 1024>), dim3(1), dim3(1024), 0, st, A, n, k0, nb, ipiv, status);
+      hipLaunchKernelGGL((dl_panel_reg_kernel<T, 1, 1024>), dim3(1), dim3(1024), 0, st, A, n, k0, nb, ipiv, status,
+                         S);
     else if (rows <= 2048)
-      hipLaunchKernelGGL((dl_panel_reg_kernel<T, 2, 1024>), dim3(1), dim3(1024), 0, st, A, n, k0, nb, ipiv, status);
+      hipLaunchKernelGGL((dl_panel_reg_kernel<T, 2, 1024>), dim3(1), dim3(1024), 0, st, A, n, k0, nb, ipiv, status,
+                         S);
     else if (rows <= 4096)  // 512 threads: room for 8 rows of registers each
-      hipLaunchKernelGGL((dl_panel_reg_kernel<T, 8, 512>), dim3(1), dim3(512), 0, st, A, n, k0, nb, ipiv, status);
+      hipLaunchKernelGGL((dl_panel_reg_kernel<T, 8, 512>), dim3(1), dim3(512), 0, st, A, n, k0, nb, ipiv, status,
+                         S);
     else
       hipLaunchKernelGGL(dl_panel_kernel<T>, dim3(1), dim3(DL_PT), 0, st, A, n, k0, nb, ipiv, status);
     JFX_LAUNCH_CHECK();
@@ -815,12 +858,13 @@ using namespace jfx;
 
 extern "C" {
 
-int jfx_dense_lu_factor(int cplx, void* A, int n, int* ipiv, int* perm, int* status_dev, hipStream_t stream) {
-  if (!A || !ipiv || !perm || !status_dev || n <= 0 || (cplx != 0 && cplx != 1))
+int jfx_dense_lu_factor(int cplx, void* A, int n, int* ipiv, int* perm, int* status_dev, void* work,
+                        hipStream_t stream) {
+  if (!A || !ipiv || !perm || !status_dev || !work || work == A || n <= 0 || (cplx != 0 && cplx != 1))
     return set_error("dense_lu_factor: bad arguments"), JFX_EARG;
   if (n > DL_MAXN) return set_error("dense_lu_factor: n = %d exceeds %d", n, DL_MAXN), JFX_ESHAPE;
-  return cplx ? dl_factor<zd>((zd*)A, n, ipiv, perm, status_dev, stream)
-              : dl_factor<double>((double*)A, n, ipiv, perm, status_dev, stream);
+  return cplx ? dl_factor<zd>((zd*)A, n, ipiv, perm, status_dev, (zd*)work, stream)
+              : dl_factor<double>((double*)A, n, ipiv, perm, status_dev, (double*)work, stream);
 }
 
 int jfx_dense_lu_solve(int cplx, const void* LU, int n, const int* perm, const void* B, void* X, long long nrhs,

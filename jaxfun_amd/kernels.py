@@ -349,7 +349,8 @@ def dense_lu_factor(A: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
     ipiv = torch.empty(n, dtype=torch.int32, device=A.device)
     perm = torch.empty(n, dtype=torch.int32, device=A.device)
     st = torch.zeros(1, dtype=torch.int32, device=A.device)
-    _lib.call("jfx_dense_lu_factor", int(cplx), _p(LU), n, _p(ipiv), _p(perm), _p(st), _s())
+    work = torch.empty(n * 32, dtype=torch.float64, device=A.device)  # a column-major copy of one panel
+    _lib.call("jfx_dense_lu_factor", int(cplx), _p(LU), n, _p(ipiv), _p(perm), _p(st), _p(work), _s())
     s = int(st.item())
     if s != 0:
         raise ValueError(f"Matrix is singular: zero pivot in LU factorisation (column {s - 1})")

@@ -158,7 +158,7 @@ def test_size_limit_rejected():
     piv = torch.zeros(4, dtype=torch.int32, device="cuda")
     with pytest.raises(Exception, match="exceeds"):
         _lib.call("jfx_dense_lu_factor", 0, buf.data_ptr(), 20000, piv.data_ptr(), piv.data_ptr(), piv.data_ptr(),
-                  torch.cuda.current_stream().cuda_stream)
+                  buf.data_ptr() + 8, torch.cuda.current_stream().cuda_stream)
 
 
 @pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 129, 640, 3844])
