@@ -208,10 +208,11 @@ int jfx_dense_lu_factor(int cplx, void* A, int n, int* ipiv, int* perm, int* sta
                         hipStream_t stream);
 /* X (n x nrhs, row-major) = A^-1 B from the factors above: X = P B, then
  * forward (unit L) and backward (U) substitution.  Real factors with
- * nrhs <= 2: one launch per direction, one workgroup per 64-row block that
+ * nrhs <= 16: one launch per direction, one workgroup per 64-row block that
  * applies the earlier blocks' results as they are published (device flags,
  * bounded waits; a block that gives up waiting is written as NaN) and solves
- * its diagonal block with reciprocal pivots.  Otherwise one launch per
+ * its diagonal block with reciprocal pivots (3..16 right-hand sides: the
+ * block updates on FP64 MFMA).  Otherwise one launch per
  * 32-row block (16 complex) that solves the block in registers and updates
  * the remaining rows.  work holds n x nrhs values of the intermediate
  * L^-1 P B followed by 2 ceil(n/64) ints (the block flags).  B, X and work

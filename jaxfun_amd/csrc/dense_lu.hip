@@ -579,6 +579,9 @@ __global__ __launch_bounds__(256) void dl_trsv_sf_kernel(const double* __restric
   double acc[NR];
 #pragma unroll
   for (int c = 0; c < NR; ++c) acc[c] = 0.0;
+  v4d cacc[4];  // NR = 16: the wave's partial sums as four 16 x 16 MFMA tiles (rows 16 rt ..)
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt) cacc[rt] = v4d{0.0, 0.0, 0.0, 0.0};
   bool bad = false;
   const double* Lrow = LU + (long long)row * n;
   const int ndep = LOWER ? blk : nb - 1 - blk;
@@ -591,18 +594,43 @@ __global__ __launch_bounds__(256) void dl_trsv_sf_kernel(const double* __restric
     const int jr = j * BS + lane;
 #pragma unroll
     for (int c = 0; c < NR; ++c) xw[lane * NR + c] = (jr < n && c < nc) ? Xv[(long long)jr * nc + c] : 0.0;
-    double lv[16];
+    if constexpr (NR == 16) {
+      // -L[block rows, this wave's 16 columns] x_j on FP64 MFMA (16 x 16 x 4):
+      // A = -L (lane: row lane % 16 of tile rt, column lane / 16 of k-step
+      // s4), B = x_j (row lane / 16 of the k-step, right-hand side lane % 16)
+      const int ar = lane & 15, ak = lane >> 4;
 #pragma unroll
-    for (int kk = 0; kk < 16; ++kk) lv[kk] = Lrow[min(j * BS + 16 * w + kk, n - 1)];
+      for (int s4 = 0; s4 < 4; ++s4) {
+        const int kl = 16 * w + 4 * s4 + ak;
+        const int kc = min(j * BS + kl, n - 1);  // columns past n meet zero rows of x_j
+        const double bx = xw[kl * 16 + ar];
 #pragma unroll
-    for (int kk = 0; kk < 16; ++kk)
+        for (int rt = 0; rt < 4; ++rt) {
+          const double a = -LU[(long long)min(r0 + 16 * rt + ar, n - 1) * n + kc];
+          cacc[rt] = mfma_f64(a, bx, cacc[rt]);
+        }
+      }
+    } else {
+      double lv[16];
 #pragma unroll
-      for (int c = 0; c < NR; ++c) acc[c] = fma(-lv[kk], xw[(16 * w + kk) * NR + c], acc[c]);
+      for (int kk = 0; kk < 16; ++kk) lv[kk] = Lrow[min(j * BS + 16 * w + kk, n - 1)];
+#pragma unroll
+      for (int kk = 0; kk < 16; ++kk)
+#pragma unroll
+        for (int c = 0; c < NR; ++c) acc[c] = fma(-lv[kk], xw[(16 * w + kk) * NR + c], acc[c]);
+    }
   }
   if (bad) sbad = 1;
   // the wave's partial sums replace its staged block (same wave: in order)
+  if constexpr (NR == 16) {
 #pragma unroll
-  for (int c = 0; c < NR; ++c) xw[lane * NR + c] = acc[c];
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) xw[(16 * rt + mfma16_row_f64(lane, r)) * 16 + (lane & 15)] = cacc[rt][r];
+  } else {
+#pragma unroll
+    for (int c = 0; c < NR; ++c) xw[lane * NR + c] = acc[c];
+  }
   __syncthreads();
   if (NR <= 2 && w != 0) return;
   const int cb = NR <= 2 ? 0 : w * CW;  // this wave's first right-hand side
@@ -830,9 +858,7 @@ static int dl_solve(const T* LU, int n, const int* perm, const T* B, T* X, T* W,
       const int nc = (int)nrhs;
       if (nc == 1) return dl_trsv_sf<1>(LU, n, nc, X, W, flags, st);
       if (nc == 2) return dl_trsv_sf<2>(LU, n, nc, X, W, flags, st);
-      if (nc <= 4) return dl_trsv_sf<4>(LU, n, nc, X, W, flags, st);
-      if (nc <= 8) return dl_trsv_sf<8>(LU, n, nc, X, W, flags, st);
-      return dl_trsv_sf<16>(LU, n, nc, X, W, flags, st);
+      return dl_trsv_sf<16>(LU, n, nc, X, W, flags, st);  // 3..16 columns: the MFMA update
     }
   }
   const unsigned gx = (unsigned)((nrhs + 63) / 64);
