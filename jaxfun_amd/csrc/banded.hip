@@ -151,8 +151,21 @@ __global__ __launch_bounds__(64) void band_lu_factor_kernel(const double* __rest
 // consecutive lanes on consecutive addresses (the next chunk's input is
 // loaded into registers while lane 0 works).  A single lane's own loads and
 // stores would be bound by its memory counter, not by the recurrence.
+// FULL: p = q = B, so every window entry lies in the band (no shape masks).
+// The reciprocal is the hardware estimate refined by two Newton steps (within
+// an ulp of 1 / piv; a correctly rounded division costs about twice the
+// instructions on the serial chain).
 constexpr int FCH = 64;
-template <int B>
+
+__device__ __forceinline__ double band_rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+template <int B, bool FULL>
 __global__ __launch_bounds__(64) void band_lu_factor_wave_kernel(const double* __restrict__ band, int n, int p,
                                                                  int q, double* __restrict__ lu,
                                                                  int* __restrict__ status) {
@@ -225,26 +238,29 @@ __global__ __launch_bounds__(64) void band_lu_factor_wave_kernel(const double* _
         const bool colin = k + 1 + B < n;
 #pragma unroll
         for (int r = 0; r < B; ++r) {
-          const double v = pr[r * W + (B - r <= q ? B - r + p : 0)];
-          ecol[r] = (B - r <= q && colin) ? v : 0.0;
+          const bool inb_ = FULL || B - r <= q;
+          const double v = pr[r * W + (inb_ ? B - r + p : 0)];
+          ecol[r] = (inb_ && colin) ? v : 0.0;
         }
 #pragma unroll
         for (int cc = 0; cc <= B; ++cc) {
-          const double v = pr[B * W + (cc - B >= -p ? cc - B + p : 0)];
-          erow[cc] = cc - B >= -p ? v : 0.0;
+          const bool inb_ = FULL || cc - B >= -p;
+          const double v = pr[B * W + (inb_ ? cc - B + p : 0)];
+          erow[cc] = inb_ ? v : 0.0;
         }
-        const double rp = 1.0 / piv;  // one division per step; multipliers by the reciprocal
+        const double rp = band_rcp(piv);  // one reciprocal per step; multipliers by it
         double* ob = outb + (k & (2 * FCH - 1)) * WM;
+        // ring slots of rows past n are written but never flushed
 #pragma unroll
         for (int r = 1; r <= B; ++r) {
           const double l = w[r][0] * rp;
-          if (r <= p && k + r < n) outb[((k + r) & (2 * FCH - 1)) * WM + p - r] = l;
+          if (FULL || r <= p) outb[((k + r) & (2 * FCH - 1)) * WM + p - r] = l;
 #pragma unroll
           for (int cc = 1; cc <= B; ++cc) w[r][cc] -= l * w[0][cc];
         }
 #pragma unroll
         for (int cc = 0; cc <= B; ++cc)
-          if (cc <= q && k + cc < n) ob[p + cc] = w[0][cc];
+          if (FULL || cc <= q) ob[p + cc] = w[0][cc];
 #pragma unroll
         for (int r = 0; r < B; ++r) {
 #pragma unroll
@@ -1042,9 +1058,19 @@ int jfx_band_lu_factor(const double* band, int n, int p, int q, double* lu, int*
   if (!band || !lu || !status_dev || n <= 0 || p < 0 || q < 0)
     return set_error("band_lu_factor: bad arguments"), JFX_EARG;
   if (p <= 2 && q <= 2)
-    hipLaunchKernelGGL(band_lu_factor_wave_kernel<2>, dim3(1), dim3(64), 0, stream, band, n, p, q, lu, status_dev);
+    if (p == 2 && q == 2)
+      hipLaunchKernelGGL((band_lu_factor_wave_kernel<2, true>), dim3(1), dim3(64), 0, stream, band, n, p, q, lu,
+                         status_dev);
+    else
+      hipLaunchKernelGGL((band_lu_factor_wave_kernel<2, false>), dim3(1), dim3(64), 0, stream, band, n, p, q, lu,
+                         status_dev);
   else if (p <= 4 && q <= 4)
-    hipLaunchKernelGGL(band_lu_factor_wave_kernel<4>, dim3(1), dim3(64), 0, stream, band, n, p, q, lu, status_dev);
+    if (p == 4 && q == 4)
+      hipLaunchKernelGGL((band_lu_factor_wave_kernel<4, true>), dim3(1), dim3(64), 0, stream, band, n, p, q, lu,
+                         status_dev);
+    else
+      hipLaunchKernelGGL((band_lu_factor_wave_kernel<4, false>), dim3(1), dim3(64), 0, stream, band, n, p, q, lu,
+                         status_dev);
   else
     hipLaunchKernelGGL(band_lu_factor_kernel, dim3(1), dim3(64), 0, stream, band, n, p, q, lu, status_dev, 1);
   JFX_LAUNCH_CHECK();
