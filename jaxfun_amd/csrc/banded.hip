@@ -226,13 +226,16 @@ __global__ __launch_bounds__(64) void band_lu_factor_wave_kernel(const double* _
       // band are masked here
       const double* pr = inb[buf];
       const int kend = min(n, (c + 1) * FCH);
+      // no branch per step: the first zero / non-finite pivot of the chunk is
+      // recorded and reported after it (the steps after it only produce
+      // non-finite values that are never used), so the loop unrolls and the
+      // window's slide becomes register renaming
+      int firstbad = 0;
+#pragma unroll 3
       for (int k = c * FCH; k < kend; ++k, pr += W) {
         const double piv = w[0][0];
-        if (piv == 0.0 || !isfinite(piv)) {
-          *status = k + 1;
-          fail = 1;
-          break;
-        }
+        const bool badp = piv == 0.0 || !isfinite(piv);
+        firstbad = (badp && firstbad == 0) ? k + 1 : firstbad;
         // the entering entries, read before the arithmetic
         double ecol[B], erow[B + 1];
         const bool colin = k + 1 + B < n;
@@ -269,6 +272,10 @@ __global__ __launch_bounds__(64) void band_lu_factor_wave_kernel(const double* _
         }
 #pragma unroll
         for (int cc = 0; cc <= B; ++cc) w[B][cc] = erow[cc];
+      }
+      if (firstbad) {
+        *status = firstbad;
+        fail = 1;
       }
     }
     __syncthreads();
