@@ -344,7 +344,7 @@ def bench_band_solve(reps: int = 20):
 
     out = {}
     gen = torch.Generator(device="cuda").manual_seed(5)
-    for n, lines in ((1024, 1), (20000, 1), (20000, 16), (100000, 1), (1000000, 1)):
+    for n, lines in ((1024, 1), (20000, 1), (20000, 16), (100000, 1), (100000, 16), (1000000, 1)):
         k = np.arange(n, dtype=np.float64)
         band = np.zeros((n, 5))
         band[:, 2] = 2.0 / (2 * k + 1) + 2.0 / (2 * k + 5)

@@ -197,6 +197,9 @@ def test_band_solve_wave_vs_banded_lapack(dev, n, p, q, shape, axis):
         (16384, 3, 0, (16384,), 0),  # lower factor only
         (30000, 1, 3, (2, 30000, 3), 1),  # six lines of eight chunks
         (300000, 2, 2, (300000,), 0),  # 59 chunks of 5120 rows
+        # 40 lines x 25 chunks = 1000 workgroups, beyond the GPU's resident
+        # slots: waiting workgroups only wait for lower-numbered ones
+        (100000, 2, 2, (100000, 40), 0),
     ],
 )
 def test_band_solve_grid_vs_banded_lapack(dev, n, p, q, shape, axis):
