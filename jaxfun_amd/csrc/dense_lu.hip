@@ -166,17 +166,50 @@ __global__ __launch_bounds__(NT) void dl_panel_reg_kernel(T* __restrict__ A, int
   const int ld = n - k0;
   // (c - k0) * ld + (i - k0) < 32 n <= 2^19: 32-bit offsets
   auto SP = [&](int i, int c) -> T& { return S[(c - k0) * ld + (i - k0)]; };
-  for (int rb = k0 + 32 * w; rb < n; rb += 32 * (NT / 64)) {
+  // TB tiles' loads are issued before any of them is used (one memory round
+  // trip per TB tiles, not per tile; complex panels keep one tile, their
+  // registers are full)
+  constexpr int TB = sizeof(T) != sizeof(double) ? 1 : (NT >= 1024 ? 2 : 8), TV = 32 * TC / 64;
+  for (int rb0 = k0 + 32 * w; rb0 < n; rb0 += 32 * (NT / 64) * TB) {
     for (int cg = 0; cg < nb; cg += TC) {
+      if constexpr (TB == 1) {
 #pragma unroll
-      for (int t = 0; t < 32 * TC / 64; ++t) {
-        const int e = lane + 64 * t, rr = e / TC, cc = e % TC;
-        if (rb + rr < n && cg + cc < nb) tt[w][rr][cc] = A[(long long)(rb + rr) * n + k0 + cg + cc];
-      }
+        for (int t = 0; t < TV; ++t) {
+          const int e = lane + 64 * t, rr = e / TC, cc = e % TC;
+          if (rb0 + rr < n && cg + cc < nb) tt[w][rr][cc] = A[(long long)(rb0 + rr) * n + k0 + cg + cc];
+        }
 #pragma unroll
-      for (int t = 0; t < 32 * TC / 64; ++t) {
-        const int e = lane + 64 * t, cc = e / 32, rr = e % 32;
-        if (rb + rr < n && cg + cc < nb) SP(rb + rr, k0 + cg + cc) = tt[w][rr][cc];
+        for (int t = 0; t < TV; ++t) {
+          const int e = lane + 64 * t, cc = e / 32, rr = e % 32;
+          if (rb0 + rr < n && cg + cc < nb) SP(rb0 + rr, k0 + cg + cc) = tt[w][rr][cc];
+        }
+      } else {
+        T ld_[TB][TV];
+#pragma unroll
+        for (int tb = 0; tb < TB; ++tb) {
+          const int rb = rb0 + 32 * (NT / 64) * tb;
+#pragma unroll
+          for (int t = 0; t < TV; ++t) {
+            const int e = lane + 64 * t, rr = e / TC, cc = e % TC;
+            const bool in = rb + rr < n && cg + cc < nb;
+            ld_[tb][t] = A[in ? (long long)(rb + rr) * n + k0 + cg + cc : 0];
+          }
+        }
+#pragma unroll
+        for (int tb = 0; tb < TB; ++tb) {
+          const int rb = rb0 + 32 * (NT / 64) * tb;
+          if (rb >= n) break;  // uniform
+#pragma unroll
+          for (int t = 0; t < TV; ++t) {
+            const int e = lane + 64 * t, rr = e / TC, cc = e % TC;
+            tt[w][rr][cc] = ld_[tb][t];
+          }
+#pragma unroll
+          for (int t = 0; t < TV; ++t) {
+            const int e = lane + 64 * t, cc = e / 32, rr = e % 32;
+            if (rb + rr < n && cg + cc < nb) SP(rb + rr, k0 + cg + cc) = tt[w][rr][cc];
+          }
+        }
       }
     }
   }
@@ -364,17 +397,46 @@ __global__ __launch_bounds__(NT) void dl_panel_reg_kernel(T* __restrict__ A, int
     __syncthreads();
   }
   // the factored panel back to A (the barrier above ends the last sub-panel)
-  for (int rb = k0 + 32 * w; rb < n; rb += 32 * (NT / 64)) {
+  for (int rb0 = k0 + 32 * w; rb0 < n; rb0 += 32 * (NT / 64) * TB) {
     for (int cg = 0; cg < nb; cg += TC) {
+      if constexpr (TB == 1) {
 #pragma unroll
-      for (int t = 0; t < 32 * TC / 64; ++t) {
-        const int e = lane + 64 * t, cc = e / 32, rr = e % 32;
-        if (rb + rr < n && cg + cc < nb) tt[w][rr][cc] = SP(rb + rr, k0 + cg + cc);
-      }
+        for (int t = 0; t < TV; ++t) {
+          const int e = lane + 64 * t, cc = e / 32, rr = e % 32;
+          if (rb0 + rr < n && cg + cc < nb) tt[w][rr][cc] = SP(rb0 + rr, k0 + cg + cc);
+        }
 #pragma unroll
-      for (int t = 0; t < 32 * TC / 64; ++t) {
-        const int e = lane + 64 * t, rr = e / TC, cc = e % TC;
-        if (rb + rr < n && cg + cc < nb) A[(long long)(rb + rr) * n + k0 + cg + cc] = tt[w][rr][cc];
+        for (int t = 0; t < TV; ++t) {
+          const int e = lane + 64 * t, rr = e / TC, cc = e % TC;
+          if (rb0 + rr < n && cg + cc < nb) A[(long long)(rb0 + rr) * n + k0 + cg + cc] = tt[w][rr][cc];
+        }
+      } else {
+        T ld_[TB][TV];
+#pragma unroll
+        for (int tb = 0; tb < TB; ++tb) {
+          const int rb = rb0 + 32 * (NT / 64) * tb;
+#pragma unroll
+          for (int t = 0; t < TV; ++t) {
+            const int e = lane + 64 * t, cc = e / 32, rr = e % 32;
+            const bool in = rb + rr < n && cg + cc < nb;
+            ld_[tb][t] = in ? SP(rb + rr, k0 + cg + cc) : SP(k0, k0);
+          }
+        }
+#pragma unroll
+        for (int tb = 0; tb < TB; ++tb) {
+          const int rb = rb0 + 32 * (NT / 64) * tb;
+          if (rb >= n) break;  // uniform
+#pragma unroll
+          for (int t = 0; t < TV; ++t) {
+            const int e = lane + 64 * t, cc = e / 32, rr = e % 32;
+            tt[w][rr][cc] = ld_[tb][t];
+          }
+#pragma unroll
+          for (int t = 0; t < TV; ++t) {
+            const int e = lane + 64 * t, rr = e / TC, cc = e % TC;
+            if (rb + rr < n && cg + cc < nb) A[(long long)(rb + rr) * n + k0 + cg + cc] = tt[w][rr][cc];
+          }
+        }
       }
     }
   }
